@@ -1,0 +1,199 @@
+// oracle_capi.cpp — flat C entry points over the f64 oracle, for ctypes.
+//
+// TEST INFRASTRUCTURE: loaded only by tests/, the smoke check and bench.py's
+// CPU-baseline leg. Scene construction mirrors the reference's
+// test/parity/reference_main.cpp:45-130 (body / joint records) and
+// src/physics/rigid.js (box and sphere constructors).
+#include <chrono>
+#include <cstring>
+#include <limits>
+
+#include "avbd_oracle.hpp"
+
+using namespace orc;
+
+extern "C" {
+
+void *orc_create(void) { return new World(); }
+void orc_destroy(void *w) { delete static_cast<World *>(w); }
+
+// d: dt, gravity, betaLin, betaAng, alpha, gamma, penaltyMin, penaltyMax,
+//    collisionMargin, stickThreshold
+// i: iterations, postStabilize, rotatedInertia, cachedContactJacobians
+void orc_set_params(void *w, const double *d, const int *i) {
+  Params &P = static_cast<World *>(w)->prm;
+  P.dt = d[0]; P.gravity = d[1]; P.betaLin = d[2]; P.betaAng = d[3]; P.alpha = d[4];
+  P.gamma = d[5]; P.penaltyMin = d[6]; P.penaltyMax = d[7]; P.collisionMargin = d[8];
+  P.stickThreshold = d[9];
+  P.iterations = i[0]; P.postStabilize = i[1] != 0; P.rotatedInertia = i[2] != 0;
+  P.cachedContactJacobians = i[3] != 0;
+}
+
+static void setPose(Body &b, const double *q, const double *w) {
+  for (int k = 0; k < 4; k++) b.q[k] = b.q0[k] = q[k];
+  for (int k = 0; k < 3; k++) b.w[k] = w[k];
+}
+
+int orc_add_box(void *w, const double *size, double density, double friction, const double *pos,
+                const double *vel, const double *quat, const double *angvel) {
+  World &W = *static_cast<World *>(w);
+  const int id = W.addBox(size, density, friction, pos, vel);
+  setPose(W.bodies[id], quat, angvel);
+  return id;
+}
+
+int orc_add_sphere(void *w, double diameter, double density, double friction, const double *pos,
+                   const double *vel, const double *quat, const double *angvel) {
+  World &W = *static_cast<World *>(w);
+  const int id = W.addSphere(diameter, density, friction, pos, vel);
+  setPose(W.bodies[id], quat, angvel);
+  return id;
+}
+
+int orc_add_joint(void *w, int a, int b, const double *rA, const double *rB, double stiffLin, double stiffAng,
+                  double fracture) {
+  World &W = *static_cast<World *>(w);
+  if (b < 0 || b >= int(W.bodies.size()) || a >= int(W.bodies.size())) return -1;
+  return W.addJoint(a, b, rA, rB, stiffLin, stiffAng, fracture);
+}
+
+void orc_set_schedule(void *w, int mode, const int *order, int n) {
+  World &W = *static_cast<World *>(w);
+  W.schedule = Schedule(mode);
+  W.bodyOrder.assign(order ? order : nullptr, order ? order + n : nullptr);
+}
+
+void orc_set_broadphase(void *w, int mode) { static_cast<World *>(w)->broadphase = mode; }
+void orc_record_pairs(void *w, int on) { static_cast<World *>(w)->recordPairs = on != 0; }
+
+// Runs n steps; returns the wall time in seconds.
+double orc_step(void *w, int n) {
+  World &W = *static_cast<World *>(w);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int s = 0; s < n; s++) W.step();
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int orc_num_bodies(void *w) { return int(static_cast<World *>(w)->bodies.size()); }
+int orc_num_dynamic(void *w) {
+  int n = 0;
+  for (const Body &b : static_cast<World *>(w)->bodies) n += b.mass > 0;
+  return n;
+}
+
+// Per body, 16 doubles: p[3] q[4] v[3] w[3] vprev[3].
+void orc_get_state(void *w, double *out) {
+  const World &W = *static_cast<World *>(w);
+  for (size_t i = 0; i < W.bodies.size(); i++) {
+    const Body &b = W.bodies[i];
+    double *o = out + i * 16;
+    for (int k = 0; k < 3; k++) o[k] = b.p[k];
+    for (int k = 0; k < 4; k++) o[3 + k] = b.q[k];
+    for (int k = 0; k < 3; k++) { o[7 + k] = b.v[k]; o[10 + k] = b.w[k]; o[13 + k] = b.vprev[k]; }
+  }
+}
+
+// Per body, 8 doubles: mass moment[3] radius shape size[0..1]... (mass props
+// for tests): mass, moment[3], radius, shape, density, friction.
+void orc_get_mass(void *w, double *out) {
+  const World &W = *static_cast<World *>(w);
+  for (size_t i = 0; i < W.bodies.size(); i++) {
+    const Body &b = W.bodies[i];
+    double *o = out + i * 8;
+    o[0] = b.mass; o[1] = b.moment[0]; o[2] = b.moment[1]; o[3] = b.moment[2];
+    o[4] = b.radius; o[5] = b.shape; o[6] = b.density; o[7] = b.friction;
+  }
+}
+
+int orc_num_forces(void *w) { return int(static_cast<World *>(w)->forces.size()); }
+
+// Per live force in list order: kind, a, b, nc (ints) and seq (int64).
+void orc_get_forces(void *w, int *kab, long long *seq) {
+  const World &W = *static_cast<World *>(w);
+  for (size_t i = 0; i < W.forces.size(); i++) {
+    const Force &f = W.pool[W.forces[i]];
+    kab[i * 4] = f.kind; kab[i * 4 + 1] = f.a; kab[i * 4 + 2] = f.b;
+    kab[i * 4 + 3] = f.kind == MANIFOLD ? f.nc : 0;
+    seq[i] = f.seq;
+  }
+}
+
+// Manifold at list position `idx`: basis[9], mu, and per contact (19 doubles):
+// rA[3] rB[3] C0[3] k[3] lam[3] C[3] stick; features separately.
+int orc_get_manifold(void *w, int idx, double *basis_mu, unsigned *features, double *contacts) {
+  const World &W = *static_cast<World *>(w);
+  if (idx < 0 || idx >= int(W.forces.size())) return -1;
+  const Force &f = W.pool[W.forces[idx]];
+  if (f.kind != MANIFOLD) return -1;
+  for (int k = 0; k < 9; k++) basis_mu[k] = f.basis[k];
+  basis_mu[9] = f.mu;
+  for (int i = 0; i < f.nc; i++) {
+    const Contact &c = f.c[i];
+    features[i] = c.feature;
+    double *o = contacts + i * 19;
+    for (int k = 0; k < 3; k++) {
+      o[k] = c.rA[k]; o[3 + k] = c.rB[k]; o[6 + k] = c.C0[k];
+      o[9 + k] = c.k[k]; o[12 + k] = c.lam[k]; o[15 + k] = c.C[k];
+    }
+    o[18] = c.stick ? 1.0 : 0.0;
+  }
+  return f.nc;
+}
+
+// Joint at list position `idx`: C0Lin kLin lamLin CLin C0Ang kAng lamAng CAng (24) + broken.
+int orc_get_joint(void *w, int idx, double *out) {
+  const World &W = *static_cast<World *>(w);
+  if (idx < 0 || idx >= int(W.forces.size())) return -1;
+  const Force &f = W.pool[W.forces[idx]];
+  if (f.kind != JOINT) return -1;
+  const double *src[8] = {f.C0Lin, f.kLin, f.lamLin, f.CLin, f.C0Ang, f.kAng, f.lamAng, f.CAng};
+  for (int s = 0; s < 8; s++)
+    for (int k = 0; k < 3; k++) out[s * 3 + k] = src[s][k];
+  out[24] = f.broken ? 1.0 : 0.0;
+  return 0;
+}
+
+// Colouring of the live force graph as it stands (coloring.js:43-118).
+int orc_coloring(void *w, int *colorOf, int *offsets, int *entries) {
+  const World &W = *static_cast<World *>(w);
+  Coloring c;
+  W.buildColoring(c);
+  std::memcpy(colorOf, c.colorOf.data(), sizeof(int) * c.colorOf.size());
+  if (offsets) std::memcpy(offsets, c.offsets.data(), sizeof(int) * c.offsets.size());
+  if (entries) std::memcpy(entries, c.entries.data(), sizeof(int) * c.entries.size());
+  return c.numColors;
+}
+
+// Colouring used by the last step under the colour schedule.
+int orc_last_coloring(void *w, int *colorOf) {
+  const World &W = *static_cast<World *>(w);
+  std::memcpy(colorOf, W.coloring.colorOf.data(), sizeof(int) * W.coloring.colorOf.size());
+  return W.coloring.numColors;
+}
+
+int orc_num_pairs(void *w) { return int(static_cast<World *>(w)->lastPairs.size() / 2); }
+void orc_get_pairs(void *w, int *out) {
+  const World &W = *static_cast<World *>(w);
+  std::memcpy(out, W.lastPairs.data(), sizeof(int) * W.lastPairs.size());
+}
+
+// Narrow phase between bodies a and b of the world (collide.js:612-685).
+// contacts: 7 per contact (feature as double, rA[3], rB[3]).
+int orc_collide(void *w, int a, int b, double *basis, double *contacts) {
+  const World &W = *static_cast<World *>(w);
+  Contact c[MAX_CONTACTS];
+  for (auto &x : c) x.reset();
+  const int n = collide(W.bodies[a], W.bodies[b], c, basis);
+  for (int i = 0; i < n; i++) {
+    contacts[i * 7] = double(c[i].feature);
+    for (int k = 0; k < 3; k++) { contacts[i * 7 + 1 + k] = c[i].rA[k]; contacts[i * 7 + 4 + k] = c[i].rB[k]; }
+  }
+  return n;
+}
+
+void orc_solve6(const double *aLin, const double *aAng, const double *aCross, const double *bLin,
+                const double *bAng, double *xLin, double *xAng) {
+  solve6(aLin, aAng, aCross, bLin, bAng, xLin, xAng);
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+"""ctypes wrapper over liboracle.so — the f64 CPU oracle.
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker or the timed CPU
+reference, never as the product path.
+
+A scene is given as a dict with ``bodies`` (shape, size, density, friction,
+pos, vel, quat, angvel) and ``joints`` (a, b, rA, rB, kLin, kAng, fracture),
+the same description the HIP backend's ``pack`` takes.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+SCHED_REVERSE, SCHED_ORDER, SCHED_COLOR = 0, 1, 2
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        d, i, v = C.c_double, C.c_int, C.c_void_p
+        pd, pi = P(d), P(i)
+        sig = {
+            "orc_create": (v, []), "orc_destroy": (None, [v]),
+            "orc_set_params": (None, [v, pd, pi]),
+            "orc_add_box": (i, [v, pd, d, d, pd, pd, pd, pd]),
+            "orc_add_sphere": (i, [v, d, d, d, pd, pd, pd, pd]),
+            "orc_add_joint": (i, [v, i, i, pd, pd, d, d, d]),
+            "orc_set_schedule": (None, [v, i, pi, i]),
+            "orc_set_broadphase": (None, [v, i]),
+            "orc_record_pairs": (None, [v, i]),
+            "orc_step": (d, [v, i]),
+            "orc_num_bodies": (i, [v]), "orc_num_dynamic": (i, [v]),
+            "orc_get_state": (None, [v, pd]), "orc_get_mass": (None, [v, pd]),
+            "orc_num_forces": (i, [v]),
+            "orc_get_forces": (None, [v, pi, P(C.c_longlong)]),
+            "orc_get_manifold": (i, [v, i, pd, P(C.c_uint), pd]),
+            "orc_get_joint": (i, [v, i, pd]),
+            "orc_coloring": (i, [v, pi, pi, pi]),
+            "orc_last_coloring": (i, [v, pi]),
+            "orc_num_pairs": (i, [v]), "orc_get_pairs": (None, [v, pi]),
+            "orc_collide": (i, [v, i, i, pd, pd]),
+            "orc_solve6": (None, [pd, pd, pd, pd, pd, pd, pd]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _vec(x, n):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n))
+
+
+def _num(x):
+    if isinstance(x, str):
+        return {"inf": math.inf, "-inf": -math.inf}[x]
+    return float(x)
+
+
+DEFAULT_PARAMS = dict(dt=1.0 / 60.0, gravity=-10.0, iterations=10, betaLin=1e4, betaAng=100.0,
+                      alpha=0.95, gamma=0.999, penaltyMin=1.0, penaltyMax=1e10,
+                      collisionMargin=0.01, stickThreshold=1e-5, postStabilize=False,
+                      rotatedInertia=True, cachedContactJacobians=True)
+
+
+class Oracle:
+    """One f64 world. Mirrors the reference ``Solver`` (solver.js:59-236)."""
+
+    def __init__(self, scene, params=None, schedule=SCHED_COLOR, broadphase="grid"):
+        self.L = lib()
+        self.h = self.L.orc_create()
+        p = dict(DEFAULT_PARAMS)
+        p.update({k: v for k, v in (params or {}).items() if k in DEFAULT_PARAMS})
+        self.params = p
+        dv = np.array([p["dt"], p["gravity"], p["betaLin"], p["betaAng"], p["alpha"], p["gamma"],
+                       p["penaltyMin"], p["penaltyMax"], p["collisionMargin"], p["stickThreshold"]],
+                      dtype=np.float64)
+        iv = np.array([p["iterations"], int(p["postStabilize"]), int(p["rotatedInertia"]),
+                       int(p["cachedContactJacobians"])], dtype=np.int32)
+        self.L.orc_set_params(self.h, _dp(dv), _ip(iv))
+        for b in scene["bodies"]:
+            q = _vec(b.get("quat", (0, 0, 0, 1)), 4)
+            w = _vec(b.get("angvel", (0, 0, 0)), 3)
+            vel = _vec(b.get("vel", (0, 0, 0)), 3)
+            pos = _vec(b["pos"], 3)
+            if b["shape"] == "sphere":
+                self.L.orc_add_sphere(self.h, float(b["size"][0]), float(b["density"]), float(b["friction"]),
+                                      _dp(pos), _dp(vel), _dp(q), _dp(w))
+            else:
+                size = _vec(b["size"], 3)
+                self.L.orc_add_box(self.h, _dp(size), float(b["density"]), float(b["friction"]),
+                                   _dp(pos), _dp(vel), _dp(q), _dp(w))
+        for j in scene.get("joints", []):
+            rA, rB = _vec(j["rA"], 3), _vec(j["rB"], 3)
+            r = self.L.orc_add_joint(self.h, int(j["a"]), int(j["b"]), _dp(rA), _dp(rB), _num(j["kLin"]),
+                                     _num(j["kAng"]), _num(j["fracture"]))
+            if r < 0:
+                raise ValueError(f"bad joint {j}")
+        self.set_schedule(schedule)
+        self.L.orc_set_broadphase(self.h, 1 if broadphase == "brute" else 0)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def set_schedule(self, mode, order=None):
+        o = np.ascontiguousarray(np.asarray(order if order is not None else [], dtype=np.int32))
+        self.L.orc_set_schedule(self.h, int(mode), _ip(o), int(o.size))
+
+    def record_pairs(self, on=True):
+        self.L.orc_record_pairs(self.h, int(on))
+
+    def step(self, n=1):
+        """Run n steps; returns wall seconds."""
+        return self.L.orc_step(self.h, int(n))
+
+    @property
+    def n(self):
+        return self.L.orc_num_bodies(self.h)
+
+    @property
+    def n_dynamic(self):
+        return self.L.orc_num_dynamic(self.h)
+
+    def state(self):
+        """(n, 16): p[3] q[4] v[3] w[3] vprev[3]."""
+        out = np.zeros((self.n, 16), dtype=np.float64)
+        self.L.orc_get_state(self.h, _dp(out))
+        return out
+
+    def mass(self):
+        out = np.zeros((self.n, 8), dtype=np.float64)
+        self.L.orc_get_mass(self.h, _dp(out))
+        return out
+
+    def forces(self):
+        """(kind, a, b, nc) per live force in list order, and seq."""
+        m = self.L.orc_num_forces(self.h)
+        kab = np.zeros((m, 4), dtype=np.int32)
+        seq = np.zeros(m, dtype=np.int64)
+        self.L.orc_get_forces(self.h, _ip(kab), seq.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return kab, seq
+
+    def manifold(self, idx):
+        bm = np.zeros(10, dtype=np.float64)
+        feat = np.zeros(8, dtype=np.uint32)
+        con = np.zeros((8, 19), dtype=np.float64)
+        nc = self.L.orc_get_manifold(self.h, int(idx), _dp(bm), feat.ctypes.data_as(C.POINTER(C.c_uint)),
+                                     _dp(con))
+        if nc < 0:
+            raise IndexError(idx)
+        return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], features=feat[:nc], contacts=con[:nc])
+
+    def joint(self, idx):
+        out = np.zeros(25, dtype=np.float64)
+        if self.L.orc_get_joint(self.h, int(idx), _dp(out)) < 0:
+            raise IndexError(idx)
+        return out
+
+    def coloring(self):
+        """Greedy colouring of the live force graph: (colorOf, offsets, entries)."""
+        n = self.n
+        col = np.zeros(n, dtype=np.int32)
+        off = np.zeros(n + 2, dtype=np.int32)
+        ent = np.zeros(max(n, 1), dtype=np.int32)
+        k = self.L.orc_coloring(self.h, _ip(col), _ip(off), _ip(ent))
+        return col, off[:k + 1], ent[:off[k]]
+
+    def last_coloring(self):
+        col = np.zeros(self.n, dtype=np.int32)
+        k = self.L.orc_last_coloring(self.h, _ip(col))
+        return col, k
+
+    def pairs(self):
+        m = self.L.orc_num_pairs(self.h)
+        out = np.zeros((m, 2), dtype=np.int32)
+        if m:
+            self.L.orc_get_pairs(self.h, _ip(out))
+        return out
+
+    def collide(self, a, b):
+        basis = np.zeros(9, dtype=np.float64)
+        con = np.zeros((8, 7), dtype=np.float64)
+        nc = self.L.orc_collide(self.h, int(a), int(b), _dp(basis), _dp(con))
+        return nc, basis.reshape(3, 3), con[:nc]
+
+
+def solve6(aLin, aAng, aCross, bLin, bAng):
+    L = lib()
+    a1, a2, a3 = (_vec(x, 9) for x in (aLin, aAng, aCross))
+    b1, b2 = _vec(bLin, 3), _vec(bAng, 3)
+    x1, x2 = np.zeros(3), np.zeros(3)
+    L.orc_solve6(_dp(a1), _dp(a2), _dp(a3), _dp(b1), _dp(b2), _dp(x1), _dp(x2))
+    return x1, x2
