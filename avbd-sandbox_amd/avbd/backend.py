@@ -1,0 +1,203 @@
+"""HipBackend — the Python face of the C ABI, shaped like the reference's
+GpuBackend (src/physics/gpu/backend.js:243-2226): pack / step / idle /
+readState / readColoring / lastStats.
+
+A scene is a dict with ``bodies`` (each: shape 'box' | 'sphere', size (full
+widths; a sphere's diameter three times), density (0 = static), friction, pos,
+and optional vel, quat (xyzw), angvel, scene) and ``joints`` (each: a (-1 =
+world anchor), b, rA, rB, kLin, kAng, fracture; 'inf' or math.inf for
+infinite). Parameters use the names of the reference's Solver
+(solver.js:92-236).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+PARAM_NAMES = {
+    "dt": "dt", "gravity": "gravity", "betaLin": "beta_lin", "betaAng": "beta_ang", "alpha": "alpha",
+    "gamma": "gamma", "penaltyMin": "penalty_min", "penaltyMax": "penalty_max",
+    "collisionMargin": "collision_margin", "stickThreshold": "stick_threshold", "iterations": "iterations",
+    "postStabilize": "post_stabilize", "rotatedInertia": "rotated_inertia",
+    "cachedContactJacobians": "cached_contact_jacobians",
+}
+
+
+class AvbdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"avbd error {code}: {msg}")
+        self.code = code
+
+
+def _num(x):
+    if isinstance(x, str):
+        return {"inf": math.inf, "-inf": -math.inf}[x]
+    return float(x)
+
+
+def _dp(a):
+    return a.ctypes.data_as(L.pd) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(L.pi) if a is not None else None
+
+
+def scene_arrays(scene):
+    """Structure-of-arrays form of a scene dict (numpy, contiguous)."""
+    bodies = scene["bodies"]
+    n = len(bodies)
+    A = {
+        "shape": np.array([L.SHAPE_SPHERE if b["shape"] == "sphere" else L.SHAPE_BOX for b in bodies], np.int32),
+        "size": np.array([b["size"] for b in bodies], np.float64).reshape(n, 3),
+        "density": np.array([b["density"] for b in bodies], np.float64),
+        "friction": np.array([b["friction"] for b in bodies], np.float64),
+        "pos": np.array([b["pos"] for b in bodies], np.float64).reshape(n, 3),
+        "quat": np.array([b.get("quat", (0, 0, 0, 1)) for b in bodies], np.float64).reshape(n, 4),
+        "vel": np.array([b.get("vel", (0, 0, 0)) for b in bodies], np.float64).reshape(n, 3),
+        "angvel": np.array([b.get("angvel", (0, 0, 0)) for b in bodies], np.float64).reshape(n, 3),
+        "scene_of": np.array([b.get("scene", 0) for b in bodies], np.int32),
+    }
+    js = scene.get("joints", [])
+    m = len(js)
+    A.update({
+        "joint_a": np.array([j["a"] for j in js], np.int32),
+        "joint_b": np.array([j["b"] for j in js], np.int32),
+        "joint_ra": np.array([j["rA"] for j in js], np.float64).reshape(m, 3),
+        "joint_rb": np.array([j["rB"] for j in js], np.float64).reshape(m, 3),
+        "joint_stiff_lin": np.array([_num(j["kLin"]) for j in js], np.float64),
+        "joint_stiff_ang": np.array([_num(j["kAng"]) for j in js], np.float64),
+        "joint_fracture": np.array([_num(j["fracture"]) for j in js], np.float64),
+    })
+    return {k: np.ascontiguousarray(v) for k, v in A.items()}
+
+
+class HipBackend:
+    """One device context (stands in for GpuBackend; backend.js:243-335)."""
+
+    def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0):
+        self.L = L.lib()
+        self.ctx = C.c_void_p()
+        cfg = L.Config(device=device, precision=precision, team=team, manifold_cap=manifold_cap,
+                       pair_cap=pair_cap)
+        rc = self.L.avbd_create(C.byref(self.ctx), C.byref(cfg))
+        if rc != 0:
+            raise AvbdError(rc, "avbd_create failed (no GPU, or bad configuration)")
+        self.n = 0
+        self.precision = precision
+        self.params = L.Params()
+        self.L.avbd_default_params(C.byref(self.params))
+
+    def close(self):
+        if self.ctx:
+            self.L.avbd_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise AvbdError(rc, self.L.avbd_last_error(self.ctx).decode())
+
+    # --- topology (backend.js:662-1301)
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            name = PARAM_NAMES.get(k, k)
+            if not hasattr(self.params, name):
+                continue
+            setattr(self.params, name, type(getattr(self.params, name))(v))
+        self._ck(self.L.avbd_set_params(self.ctx, C.byref(self.params)))
+
+    def pack(self, scene, params=None):
+        if params:
+            self.set_params(**{k: v for k, v in params.items() if k in PARAM_NAMES})
+        else:
+            self._ck(self.L.avbd_set_params(self.ctx, C.byref(self.params)))
+        A = scene_arrays(scene)
+        self._arrays = A  # keep alive for the call
+        s = L.Scene()
+        s.num_bodies = len(A["shape"])
+        s.shape, s.size, s.density, s.friction = _ip(A["shape"]), _dp(A["size"]), _dp(A["density"]), _dp(A["friction"])
+        s.pos, s.quat, s.vel, s.angvel = _dp(A["pos"]), _dp(A["quat"]), _dp(A["vel"]), _dp(A["angvel"])
+        s.scene_of = _ip(A["scene_of"])
+        s.num_joints = len(A["joint_a"])
+        if s.num_joints:
+            s.joint_a, s.joint_b = _ip(A["joint_a"]), _ip(A["joint_b"])
+            s.joint_ra, s.joint_rb = _dp(A["joint_ra"]), _dp(A["joint_rb"])
+            s.joint_stiff_lin, s.joint_stiff_ang = _dp(A["joint_stiff_lin"]), _dp(A["joint_stiff_ang"])
+            s.joint_fracture = _dp(A["joint_fracture"])
+        self._ck(self.L.avbd_pack(self.ctx, C.byref(s)))
+        self.n = s.num_bodies
+
+    # --- stepping (backend.js:1393-1620, 1804-1818)
+    def step(self, n=1):
+        self._ck(self.L.avbd_step(self.ctx, int(n)))
+
+    def idle(self):
+        self._ck(self.L.avbd_sync(self.ctx))
+
+    # --- state (backend.js:2154-2192)
+    def read_state(self):
+        n = self.n
+        pos, quat = np.zeros((n, 3)), np.zeros((n, 4))
+        vel, angvel, prev = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._ck(self.L.avbd_read_state(self.ctx, _dp(pos), _dp(quat), _dp(vel), _dp(angvel), _dp(prev), n))
+        return dict(pos=pos, quat=quat, vel=vel, angvel=angvel, prev_vel=prev)
+
+    def write_state(self, pos=None, quat=None, vel=None, angvel=None, prev_vel=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in
+                (pos, quat, vel, angvel, prev_vel)]
+        self._ck(self.L.avbd_write_state(self.ctx, *[_dp(a) for a in arrs], self.n))
+
+    # --- verification (backend.js:2053-2151)
+    def read_pairs(self):
+        cnt = C.c_int32()
+        self._ck(self.L.avbd_read_pairs(self.ctx, None, 0, C.byref(cnt)))
+        out = np.zeros((cnt.value, 2), np.int32)
+        if cnt.value:
+            self._ck(self.L.avbd_read_pairs(self.ctx, _ip(out), cnt.value, C.byref(cnt)))
+        return out
+
+    def read_coloring(self):
+        col = np.zeros(self.n, np.int32)
+        k = C.c_int32()
+        self._ck(self.L.avbd_read_coloring(self.ctx, _ip(col), self.n, C.byref(k)))
+        return col, k.value
+
+    def read_forces(self):
+        cnt = C.c_int32()
+        self._ck(self.L.avbd_read_forces(self.ctx, None, None, 0, C.byref(cnt)))
+        kabn = np.zeros((cnt.value, 4), np.int32)
+        feat = np.zeros((cnt.value, 8), np.uint32)
+        if cnt.value:
+            self._ck(self.L.avbd_read_forces(self.ctx, _ip(kabn), feat.ctypes.data_as(L.pu), cnt.value,
+                                             C.byref(cnt)))
+        return kabn, feat
+
+    def read_manifold(self, index):
+        bm = np.zeros(10)
+        con = np.zeros((8, 16))
+        nc = C.c_int32()
+        self._ck(self.L.avbd_read_manifold(self.ctx, int(index), _dp(bm), _dp(con), C.byref(nc)))
+        return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], contacts=con[:nc.value])
+
+    # --- metrics / profiling (backend.js:309-324, 1836-1891)
+    def counters(self):
+        c = L.Counters()
+        self._ck(self.L.avbd_read_counters(self.ctx, C.byref(c)))
+        return {f: getattr(c, f) for f, _ in L.Counters._fields_}
+
+    def set_profiling(self, on=True):
+        self._ck(self.L.avbd_set_profiling(self.ctx, int(on)))
+
+    def profile(self):
+        ms = (C.c_float * L.NUM_PHASES)()
+        steps = C.c_int32()
+        self._ck(self.L.avbd_read_profile(self.ctx, ms, L.NUM_PHASES, C.byref(steps)))
+        return dict(zip(L.PHASES, [float(x) for x in ms])), steps.value

@@ -1,0 +1,914 @@
+// avbd_api.hip — context, step orchestration and the C ABI of include/avbd.h.
+//
+// One context owns one HIP stream and every device buffer. avbd_step runs the
+// kernels of avbd_kernels.hpp in the order of the reference's Solver.step
+// (solver.js:307-493), with the primal sweep scheduled by the greedy colouring
+// of the force graph (coloring.js:43-118) taken after force initialisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd.h"
+#include "avbd_kernels.hpp"
+
+using namespace avbd;
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                  \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess)                                                                            \
+      throw HipError(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +        \
+                     std::to_string(__LINE__) + ")");                                                \
+  } while (0)
+
+inline int blocks(long n, int b = 256) { return int((n + b - 1) / b); }
+
+template <class T> struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void reserve(size_t m) {
+    if (m <= n && p) return;
+    if (p) HIPCHECK(hipFree(p));
+    p = nullptr;
+    n = std::max<size_t>(m, 1);
+    HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct ManStore {
+  DBuf<int4> head;
+  DBuf<char> basis, hot, Cd;
+  DBuf<double> anchor;
+  DBuf<uint32_t> feat;
+  DBuf<int32_t> stick;
+  void reserve(size_t m, size_t rsize) {
+    head.reserve(m);
+    basis.reserve(m * BAS * rsize);
+    hot.reserve(m * MAXC * HOT * rsize);
+    Cd.reserve(m * MAXC * 4 * rsize);
+    anchor.reserve(m * MAXC * 6);
+    feat.reserve(m * MAXC);
+    stick.reserve(m * MAXC);
+  }
+  template <class R> ManBuf<R> view() {
+    return ManBuf<R>{head.p, (R *)basis.p, (R *)hot.p, anchor.p, feat.p, stick.p, (R *)Cd.p};
+  }
+  void release() {
+    head.release(); basis.release(); hot.release(); Cd.release(); anchor.release(); feat.release(); stick.release();
+  }
+};
+
+struct Phase {
+  hipEvent_t ev[AVBD_NUM_PHASES];
+};
+
+}  // namespace
+
+struct avbd_ctx {
+  std::string err;
+  int device = 0, prec = AVBD_PREC_MIXED, team = 16;
+  int manCapCfg = 0, pairCapCfg = 0;
+  hipStream_t stream = nullptr;
+  avbd_params prm{};
+  bool packed = false;
+
+  // sizes
+  int N = 0, J = 0, S = 1, nLarge = 0;
+  int manCap = 0, pairCap = 0;
+  uint32_t tableMask = 0;
+  double invh = 1.0;
+  int64_t steps = 0;
+
+  // bodies
+  DBuf<BodyProp> props;
+  DBuf<double> X, X0, Y, V;
+  DBuf<char> DQ;
+  std::vector<BodyProp> hprops;
+
+  // broad phase
+  DBuf<int4> cellOf;
+  DBuf<int> bucketCnt, bucketStart, bucketCursor, entries;
+  DBuf<int> sceneLargeStart, sceneLarges, sceneStart, largeList;
+  DBuf<int> countRev, startRev, pairNew, newPos;
+  DBuf<int2> pairs, newPairs;
+  int nPairs = 0, nNew = 0;
+
+  // manifolds (double-buffered)
+  ManStore store[2];
+  DBuf<int> live[2];
+  int cur = 0, nLive = 0;
+  DBuf<int> keep, keepPos;
+
+  // joints
+  DBuf<JointDev> joints;
+
+  // topology
+  DBuf<int> deg, adjStart, adjCursor, adjKey, adjOther;
+  DBuf<int> color[2];
+  int colorCur = 0;
+  DBuf<int> ccount, cstart, ccursor, cbodies;
+  std::vector<int> hcstart;
+  int numColors = 0, colorRounds = 0;
+  DBuf<int> itemCnt, itemStart;
+  DBuf<uint32_t> items;
+
+  // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
+  // 5 overflow, 6 items, 7 adj total
+  DBuf<int> scal;
+  int *hscal = nullptr;  // pinned mirror
+
+  // scan temporaries per level
+  std::vector<DBuf<int>> scanTmp = std::vector<DBuf<int>>(16);
+
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> evs;
+  double profMs[AVBD_NUM_PHASES] = {0};
+  int profSteps = 0;
+
+  ~avbd_ctx() { releaseAll(); }
+
+  void releaseAll() {
+    props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release();
+    cellOf.release(); bucketCnt.release(); bucketStart.release(); bucketCursor.release(); entries.release();
+    sceneLargeStart.release(); sceneLarges.release(); sceneStart.release(); largeList.release();
+    countRev.release(); startRev.release(); pairNew.release(); newPos.release(); pairs.release(); newPairs.release();
+    store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
+    joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
+    color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release();
+    itemCnt.release(); itemStart.release(); items.release(); scal.release();
+    for (auto &b : scanTmp) b.release();
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    evs.clear();
+    if (hscal) (void)hipHostFree(hscal);
+    hscal = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  size_t rsize() const { return prec == AVBD_PREC_MIXED ? sizeof(float) : sizeof(double); }
+
+  StepParams sp() const {
+    StepParams P;
+    P.dt = prm.dt; P.gravity = prm.gravity; P.betaLin = prm.beta_lin; P.betaAng = prm.beta_ang;
+    P.alpha = prm.alpha; P.gamma = prm.gamma; P.pmin = prm.penalty_min; P.pmax = prm.penalty_max;
+    P.margin = prm.collision_margin; P.stick = prm.stick_threshold; P.iterations = prm.iterations;
+    P.postStab = prm.post_stabilize; P.rotated = prm.rotated_inertia;
+    return P;
+  }
+
+  // ---------------------------------------------------------------- scan
+  // Exclusive scan of n ints; *total (device) receives the sum.
+  void scan(const int *in, int *out, int n, int *total, int level = 0) {
+    if (n <= 0) {
+      hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, total, 1, 0);
+      return;
+    }
+    const int nb = (n + 1023) / 1024;
+    if (level * 2 + 1 >= int(scanTmp.size())) throw ApiError(AVBD_E_CAPACITY, "scan too deep");
+    DBuf<int> &sums = scanTmp[level * 2];
+    DBuf<int> &offs = scanTmp[level * 2 + 1];
+    sums.reserve(nb);
+    offs.reserve(nb + 1);
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, stream, in, out, n, sums.p);
+    if (nb > 1) {
+      scan(sums.p, offs.p, nb, offs.p + nb, level + 1);
+      hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, stream, out, n, offs.p);
+    }
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, stream, in, out, n, total);
+  }
+
+  void readScal(int first, int count) {
+    HIPCHECK(hipMemcpyAsync(hscal + first, scal.p + first, sizeof(int) * count, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  void record(int phase) {
+    if (profiling) HIPCHECK(hipEventRecord(evs[phase], stream));
+  }
+
+  // ------------------------------------------------------------ adjacency
+  void buildAdjacency() {
+    const int nf = J + nLive;
+    HIPCHECK(hipMemsetAsync(deg.p, 0, sizeof(int) * N, stream));
+    const int4 *head = store[cur].head.p;
+    if (nf > 0)
+      hipLaunchKernelGGL(k_adj<0>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
+                         deg.p, adjCursor.p, adjKey.p, adjOther.p);
+    scan(deg.p, adjStart.p, N, adjStart.p + N);
+    HIPCHECK(hipMemcpyAsync(&hscal[7], adjStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const int tot = hscal[7];
+    adjKey.reserve(size_t(tot) + 1);
+    adjOther.reserve(size_t(tot) + 1);
+    HIPCHECK(hipMemcpyAsync(adjCursor.p, adjStart.p, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
+    if (nf > 0)
+      hipLaunchKernelGGL(k_adj<1>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
+                         deg.p, adjCursor.p, adjKey.p, adjOther.p);
+    hipLaunchKernelGGL(k_adj_sort, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjKey.p,
+                       adjOther.p);
+  }
+
+  // ------------------------------------------------------------ colouring
+  void colorize() {
+    int rounds = 0;
+    const int maxRounds = N + 2;
+    for (;;) {
+      HIPCHECK(hipMemsetAsync(scal.p + 3, 0, sizeof(int), stream));
+      const int batch = 4;
+      for (int r = 0; r < batch; r++) {
+        hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjOther.p,
+                           color[colorCur].p, color[1 - colorCur].p, scal.p + 3, scal.p + 5);
+        colorCur = 1 - colorCur;
+        rounds++;
+        if (r + 1 < batch) HIPCHECK(hipMemsetAsync(scal.p + 3, 0, sizeof(int), stream));
+      }
+      readScal(3, 1);
+      if (hscal[3] == 0) break;
+      if (rounds > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
+    }
+    colorRounds = rounds;
+    // colour buckets
+    HIPCHECK(hipMemsetAsync(ccount.p, 0, sizeof(int) * MAX_COLORS, stream));
+    HIPCHECK(hipMemsetAsync(scal.p + 4, 0, sizeof(int), stream));
+    hipLaunchKernelGGL(k_color_count, dim3(blocks(N)), dim3(256), 0, stream, N, color[colorCur].p, ccount.p,
+                       scal.p + 4);
+    readScal(4, 1);
+    numColors = hscal[4];
+    std::vector<int> cc(MAX_COLORS);
+    HIPCHECK(hipMemcpyAsync(cc.data(), ccount.p, sizeof(int) * MAX_COLORS, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    hcstart.assign(numColors + 1, 0);
+    for (int c = 0; c < numColors; c++) hcstart[c + 1] = hcstart[c] + cc[c];
+    if (numColors > 0) {
+      HIPCHECK(hipMemcpyAsync(ccursor.p, hcstart.data(), sizeof(int) * numColors, hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_color_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, color[colorCur].p, ccursor.p,
+                         cbodies.p);
+    }
+  }
+
+  void buildItems() {
+    const int4 *head = store[cur].head.p;
+    hipLaunchKernelGGL(k_items<0>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p);
+    scan(itemCnt.p, itemStart.p, N, itemStart.p + N);
+    HIPCHECK(hipMemcpyAsync(&hscal[6], itemStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    items.reserve(size_t(hscal[6]) + 1);
+    hipLaunchKernelGGL(k_items<1>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p);
+  }
+
+  template <class R, bool EXACT, int TEAM>
+  void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
+    const int cnt = hcstart[c + 1] - hcstart[c];
+    if (cnt <= 0) return;
+    const long threads = long(cnt) * TEAM;
+    hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(blocks(threads)), dim3(256), 0, stream, cnt,
+                       cbodies.p + hcstart[c], sa, mb, (R *)DQ.p, P, alpha);
+  }
+
+  template <class R, bool EXACT>
+  void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
+    if (EXACT) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha); return; }
+    switch (team) {
+      case 1: launchPrimal<R, false, 1>(c, sa, mb, P, alpha); break;
+      case 4: launchPrimal<R, false, 4>(c, sa, mb, P, alpha); break;
+      case 8: launchPrimal<R, false, 8>(c, sa, mb, P, alpha); break;
+      case 32: launchPrimal<R, false, 32>(c, sa, mb, P, alpha); break;
+      case 64: launchPrimal<R, false, 64>(c, sa, mb, P, alpha); break;
+      default: launchPrimal<R, false, 16>(c, sa, mb, P, alpha); break;
+    }
+  }
+
+  // ---------------------------------------------------------------- step
+  template <class R, bool EXACT>
+  void stepT() {
+    const StepParams P = sp();
+    record(0);
+    // 1. broad phase
+    HIPCHECK(hipMemsetAsync(bucketCnt.p, 0, sizeof(int) * (tableMask + 1), stream));
+    hipLaunchKernelGGL(k_bp_cells, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, invh, tableMask, cellOf.p,
+                       bucketCnt.p);
+    scan(bucketCnt.p, bucketStart.p, int(tableMask + 1), scal.p + 7);
+    HIPCHECK(hipMemcpyAsync(bucketCursor.p, bucketStart.p, sizeof(int) * (tableMask + 1), hipMemcpyDeviceToDevice,
+                            stream));
+    hipLaunchKernelGGL(k_bp_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, cellOf.p, tableMask, bucketCursor.p,
+                       entries.p);
+    BpArgs ba;
+    ba.n = N; ba.J = J; ba.props = props.p; ba.X = X.p; ba.cellOf = cellOf.p; ba.bucketStart = bucketStart.p;
+    ba.bucketCnt = bucketCnt.p; ba.entries = entries.p; ba.mask = tableMask; ba.sceneLargeStart = sceneLargeStart.p;
+    ba.sceneLarges = sceneLarges.p; ba.sceneStart = sceneStart.p; ba.adjStart = adjStart.p; ba.adjKey = adjKey.p;
+    ba.adjOther = adjOther.p; ba.joints = joints.p; ba.countRev = countRev.p; ba.startRev = startRev.p;
+    ba.pairs = pairs.p; ba.pairNew = pairNew.p; ba.pairCap = pairCap; ba.overflow = scal.p + 5;
+    HIPCHECK(hipMemsetAsync(countRev.p, 0, sizeof(int) * N, stream));
+    hipLaunchKernelGGL(k_bp_small<0>, dim3(blocks(N)), dim3(256), 0, stream, ba);
+    if (nLarge) hipLaunchKernelGGL(k_bp_large<0>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
+    scan(countRev.p, startRev.p, N, scal.p + 0);
+    readScal(0, 1);
+    nPairs = hscal[0];
+    if (nPairs > pairCap) {
+      HIPCHECK(hipMemsetAsync(scal.p + 5, 0, sizeof(int), stream));
+      throw ApiError(AVBD_E_CAPACITY, "broad-phase pairs " + std::to_string(nPairs) + " exceed capacity " +
+                                          std::to_string(pairCap));
+    }
+    hipLaunchKernelGGL(k_bp_small<1>, dim3(blocks(N)), dim3(256), 0, stream, ba);
+    if (nLarge) hipLaunchKernelGGL(k_bp_large<1>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
+    scan(pairNew.p, newPos.p, nPairs, scal.p + 1);
+    if (nPairs > 0)
+      hipLaunchKernelGGL(k_new_pairs, dim3(blocks(nPairs)), dim3(256), 0, stream, nPairs, pairs.p, pairNew.p,
+                         newPos.p, newPairs.p);
+    readScal(1, 1);
+    nNew = hscal[1];
+    record(1);
+
+    // 2. narrow phase and force initialisation
+    const int nOld = nLive, nTot = nOld + nNew;
+    if (nTot > manCap)
+      throw ApiError(AVBD_E_CAPACITY, "manifolds " + std::to_string(nTot) + " exceed capacity " +
+                                          std::to_string(manCap));
+    ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
+    if (nTot > 0)
+      hipLaunchKernelGGL(k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), 0, stream, nOld, nNew, live[cur].p, ob, nb,
+                         newPairs.p, props.p, X.p, P, keep.p);
+    if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
+    scan(keep.p, keepPos.p, nTot, scal.p + 2);
+    if (nTot > 0)
+      hipLaunchKernelGGL(k_live_compact, dim3(blocks(nTot)), dim3(256), 0, stream, nTot, keep.p, keepPos.p,
+                         live[1 - cur].p);
+    readScal(2, 1);
+    nLive = hscal[2];
+    cur = 1 - cur;
+    record(2);
+
+    // 3. topology: force lists, colouring, work lists
+    buildAdjacency();
+    record(3);
+    colorize();
+    buildItems();
+    record(4);
+
+    // 4. body initialisation and the iterations
+    ManBuf<R> mb = store[cur].view<R>();
+    hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
+                       (R *)DQ.p, P);
+    SolveArgs sa;
+    sa.props = props.p; sa.X = X.p; sa.X0 = X0.p; sa.Y = Y.p; sa.itemStart = itemStart.p; sa.items = items.p;
+    sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N;
+    const int total = prm.iterations + (prm.post_stabilize ? 1 : 0);
+    for (int it = 0; it < total; it++) {
+      const double alpha = prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha;
+      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha);
+      if (it < prm.iterations) {
+        if (nLive > 0)
+          hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(blocks(nLive, 128)), dim3(128), 0, stream, nLive, live[cur].p,
+                             mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
+        if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
+      }
+      if (it == prm.iterations - 1)
+        hipLaunchKernelGGL(k_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, V.p, prm.dt);
+    }
+    record(5);
+    HIPCHECK(hipGetLastError());
+    steps++;
+    if (profiling) {
+      HIPCHECK(hipEventSynchronize(evs[5]));
+      float ms;
+      for (int ph = 0; ph < 4; ph++) {
+        HIPCHECK(hipEventElapsedTime(&ms, evs[ph], evs[ph + 1]));
+        profMs[ph] += ms;
+      }
+      HIPCHECK(hipEventElapsedTime(&ms, evs[4], evs[5]));
+      profMs[4] += ms;
+      HIPCHECK(hipEventElapsedTime(&ms, evs[0], evs[5]));
+      profMs[5] += ms;
+      profSteps++;
+    }
+  }
+
+  void step() {
+    switch (prec) {
+      case AVBD_PREC_EXACT: stepT<double, true>(); break;
+      case AVBD_PREC_F64: stepT<double, false>(); break;
+      default: stepT<float, false>(); break;
+    }
+  }
+};
+
+namespace {
+
+int fail(avbd_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <class F> int guarded(avbd_ctx *c, F &&f) {
+  if (!c) return AVBD_E_INVALID;
+  try {
+    f();
+    return AVBD_OK;
+  } catch (const ApiError &e) {
+    return fail(c, e.code, e.what());
+  } catch (const HipError &e) {
+    return fail(c, AVBD_E_HIP, e.what());
+  } catch (const std::exception &e) {
+    return fail(c, AVBD_E_STATE, e.what());
+  }
+}
+
+void packScene(avbd_ctx &c, const avbd_scene &s) {
+  if (s.num_bodies <= 0) throw ApiError(AVBD_E_INVALID, "num_bodies must be positive");
+  if (!s.shape || !s.size || !s.density || !s.friction || !s.pos)
+    throw ApiError(AVBD_E_INVALID, "shape, size, density, friction and pos are required");
+  const int N = s.num_bodies;
+  c.releaseAll();
+  HIPCHECK(hipSetDevice(c.device));
+  HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  HIPCHECK(hipHostMalloc((void **)&c.hscal, sizeof(int) * 16, hipHostMallocDefault));
+  c.N = N;
+  c.J = s.num_joints;
+  c.steps = 0;
+  c.nLive = 0;
+  c.cur = 0;
+  c.nPairs = c.nNew = 0;
+  c.numColors = 0;
+
+  // scenes must be contiguous runs of bodies
+  std::vector<int> sceneOf(N, 0);
+  if (s.scene_of) {
+    for (int i = 0; i < N; i++) {
+      sceneOf[i] = s.scene_of[i];
+      if (sceneOf[i] < 0 || (i > 0 && sceneOf[i] < sceneOf[i - 1]))
+        throw ApiError(AVBD_E_INVALID, "scene_of must be non-negative and non-decreasing");
+    }
+  }
+  const int S = sceneOf[N - 1] + 1;
+  c.S = S;
+
+  // mass properties (rigid.js:29-133)
+  std::vector<BodyProp> P(N);
+  std::vector<double> X(size_t(N) * XS, 0.0), V(size_t(N) * VS, 0.0);
+  for (int i = 0; i < N; i++) {
+    BodyProp &b = P[i];
+    const double sx = s.size[i * 3], sy = s.size[i * 3 + 1], sz = s.size[i * 3 + 2];
+    b.sx = sx; b.sy = sy; b.sz = sz;
+    b.hx = sx * 0.5; b.hy = sy * 0.5; b.hz = sz * 0.5;
+    b.friction = s.friction[i];
+    b.shape = s.shape[i];
+    b.scene = sceneOf[i];
+    b.pad = 0;
+    const double rho = s.density[i];
+    if (b.shape == AVBD_SHAPE_SPHERE) {
+      const double r = sx * 0.5;
+      b.radius = r;
+      b.mass = (4.0 / 3.0) * M_PI * r * r * r * rho;
+      const double I = (2.0 / 5.0) * b.mass * r * r;
+      b.m0 = b.m1 = b.m2 = I;
+    } else if (b.shape == AVBD_SHAPE_BOX) {
+      b.mass = sx * sy * sz * rho;
+      b.m0 = ((sy * sy + sz * sz) / 12.0) * b.mass;
+      b.m1 = ((sx * sx + sz * sz) / 12.0) * b.mass;
+      b.m2 = ((sx * sx + sy * sy) / 12.0) * b.mass;
+      const double hx = sx * 0.5, hy = sy * 0.5, hz = sz * 0.5;
+      b.radius = std::sqrt(hx * hx + hy * hy + hz * hz);
+    } else {
+      throw ApiError(AVBD_E_INVALID, "unknown shape");
+    }
+    double *x = &X[size_t(i) * XS];
+    for (int k = 0; k < 3; k++) x[k] = s.pos[i * 3 + k];
+    if (s.quat) for (int k = 0; k < 4; k++) x[3 + k] = s.quat[i * 4 + k];
+    else { x[3] = x[4] = x[5] = 0; x[6] = 1; }
+    double *v = &V[size_t(i) * VS];
+    for (int k = 0; k < 3; k++) {
+      v[k] = s.vel ? s.vel[i * 3 + k] : 0.0;
+      v[3 + k] = s.angvel ? s.angvel[i * 3 + k] : 0.0;
+      v[6 + k] = v[k];  // prevVelocityLin starts equal to the velocity
+    }
+  }
+  // large bodies per scene (radius > 8x the scene's mean radius), grid cell
+  std::vector<double> rsum(S, 0.0);
+  std::vector<int> scnt(S, 0);
+  for (int i = 0; i < N; i++) { rsum[P[i].scene] += P[i].radius; scnt[P[i].scene]++; }
+  std::vector<int> sceneStart(S + 1, 0), sls(S + 1, 0), slist, largeList;
+  for (int i = 0; i < N; i++) sceneStart[P[i].scene + 1]++;
+  for (int k = 0; k < S; k++) sceneStart[k + 1] += sceneStart[k];
+  double rmax = 0;
+  for (int i = 0; i < N; i++) {
+    const int sc = P[i].scene;
+    const double thr = std::max(rsum[sc] / scnt[sc] * 8.0, 1e-6);
+    P[i].large = P[i].radius > thr ? 1 : 0;
+    if (!P[i].large) rmax = std::max(rmax, P[i].radius);
+  }
+  for (int sc = 0; sc < S; sc++) {
+    sls[sc] = int(slist.size());
+    for (int i = sceneStart[sc]; i < sceneStart[sc + 1]; i++)
+      if (P[i].large) { slist.push_back(i); largeList.push_back(i); }
+  }
+  sls[S] = int(slist.size());
+  c.nLarge = int(largeList.size());
+  const double h = std::max(2.0 * rmax * (1.0 + 1e-9), 1e-6);
+  c.invh = 1.0 / h;
+  uint32_t tsz = 64;
+  while (tsz < uint32_t(2 * N)) tsz <<= 1;
+  c.tableMask = tsz - 1;
+
+  c.hprops = P;
+  c.props.reserve(N);
+  HIPCHECK(hipMemcpy(c.props.p, P.data(), sizeof(BodyProp) * N, hipMemcpyHostToDevice));
+  c.X.reserve(size_t(N) * XS); c.X0.reserve(size_t(N) * XS); c.Y.reserve(size_t(N) * XS);
+  c.V.reserve(size_t(N) * VS);
+  HIPCHECK(hipMemcpy(c.X.p, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.X0.p, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.Y.p, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.V.p, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice));
+  c.DQ.reserve(size_t(N) * 8 * c.rsize());
+  HIPCHECK(hipMemset(c.DQ.p, 0, size_t(N) * 8 * c.rsize()));
+
+  c.cellOf.reserve(N); c.entries.reserve(N);
+  c.bucketCnt.reserve(tsz); c.bucketStart.reserve(tsz + 1); c.bucketCursor.reserve(tsz);
+  c.sceneLargeStart.reserve(S + 1); c.sceneLarges.reserve(slist.size() + 1);
+  c.sceneStart.reserve(S + 1); c.largeList.reserve(largeList.size() + 1);
+  HIPCHECK(hipMemcpy(c.sceneLargeStart.p, sls.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
+  if (!slist.empty()) HIPCHECK(hipMemcpy(c.sceneLarges.p, slist.data(), sizeof(int) * slist.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.sceneStart.p, sceneStart.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
+  if (!largeList.empty())
+    HIPCHECK(hipMemcpy(c.largeList.p, largeList.data(), sizeof(int) * largeList.size(), hipMemcpyHostToDevice));
+
+  c.pairCap = c.pairCapCfg > 0 ? c.pairCapCfg : std::max(32 * N, 4096);
+  c.manCap = c.manCapCfg > 0 ? c.manCapCfg : std::max(32 * N, 4096);
+  c.countRev.reserve(N); c.startRev.reserve(N + 1);
+  c.pairs.reserve(c.pairCap); c.pairNew.reserve(c.pairCap); c.newPos.reserve(c.pairCap + 1);
+  c.newPairs.reserve(c.pairCap);
+  c.store[0].reserve(c.manCap, c.rsize());
+  c.store[1].reserve(c.manCap, c.rsize());
+  c.live[0].reserve(c.manCap); c.live[1].reserve(c.manCap);
+  c.keep.reserve(c.manCap); c.keepPos.reserve(c.manCap + 1);
+
+  // joints (joint.js:78-113)
+  std::vector<JointDev> JD(c.J);
+  for (int j = 0; j < c.J; j++) {
+    JointDev &d = JD[j];
+    std::memset(&d, 0, sizeof(d));
+    d.a = s.joint_a[j];
+    d.b = s.joint_b[j];
+    if (d.b < 0 || d.b >= N || d.a < -1 || d.a >= N || d.a == d.b)
+      throw ApiError(AVBD_E_INVALID, "joint " + std::to_string(j) + " has invalid bodies");
+    if (d.a >= 0 && sceneOf[d.a] != sceneOf[d.b]) throw ApiError(AVBD_E_INVALID, "joint crosses scenes");
+    for (int k = 0; k < 3; k++) { d.rA[k] = s.joint_ra[j * 3 + k]; d.rB[k] = s.joint_rb[j * 3 + k]; }
+    d.stiffLin = s.joint_stiff_lin[j];
+    d.stiffAng = s.joint_stiff_ang[j];
+    d.fracture = s.joint_fracture[j];
+    const double ax = d.a >= 0 ? s.size[d.a * 3] : 0, ay = d.a >= 0 ? s.size[d.a * 3 + 1] : 0,
+                 az = d.a >= 0 ? s.size[d.a * 3 + 2] : 0;
+    const double sx = ax + s.size[d.b * 3], sy = ay + s.size[d.b * 3 + 1], sz = az + s.size[d.b * 3 + 2];
+    d.torqueArm = sx * sx + sy * sy + sz * sz;
+  }
+  c.joints.reserve(std::max(c.J, 1));
+  if (c.J) HIPCHECK(hipMemcpy(c.joints.p, JD.data(), sizeof(JointDev) * c.J, hipMemcpyHostToDevice));
+
+  c.deg.reserve(N); c.adjStart.reserve(N + 1); c.adjCursor.reserve(N);
+  c.adjKey.reserve(size_t(2 * c.J) + 1); c.adjOther.reserve(size_t(2 * c.J) + 1);
+  c.color[0].reserve(N); c.color[1].reserve(N);
+  std::vector<int> col(N);
+  for (int i = 0; i < N; i++) col[i] = P[i].mass > 0 ? 0 : -1;
+  HIPCHECK(hipMemcpy(c.color[0].p, col.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.color[1].p, col.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  c.colorCur = 0;
+  c.ccount.reserve(MAX_COLORS); c.cstart.reserve(MAX_COLORS + 1); c.ccursor.reserve(MAX_COLORS);
+  c.cbodies.reserve(N);
+  c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.items.reserve(size_t(N) * 16);
+  c.scal.reserve(16);
+  HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
+  c.evs.resize(AVBD_NUM_PHASES);
+  for (auto &e : c.evs) HIPCHECK(hipEventCreate(&e));
+  // force lists of the initial scene (joints only) for the first broad phase
+  c.buildAdjacency();
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  c.packed = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int avbd_abi_version(void) { return AVBD_ABI_VERSION; }
+
+void avbd_default_params(avbd_params *p) {
+  if (!p) return;
+  p->dt = 1.0 / 60.0;
+  p->gravity = -10.0;
+  p->beta_lin = 10000.0;
+  p->beta_ang = 100.0;
+  p->alpha = 0.95;
+  p->gamma = 0.999;
+  p->penalty_min = 1.0;
+  p->penalty_max = 1.0e10;
+  p->collision_margin = 0.01;
+  p->stick_threshold = 0.00001;
+  p->iterations = 10;
+  p->post_stabilize = 0;
+  p->rotated_inertia = 1;
+  p->cached_contact_jacobians = 1;
+}
+
+int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
+  if (!out) return AVBD_E_INVALID;
+  *out = nullptr;
+  avbd_ctx *c = new (std::nothrow) avbd_ctx();
+  if (!c) return AVBD_E_STATE;
+  avbd_default_params(&c->prm);
+  if (cfg) {
+    c->device = cfg->device;
+    c->prec = cfg->precision;
+    c->team = cfg->team > 0 ? cfg->team : 16;
+    c->manCapCfg = cfg->manifold_cap;
+    c->pairCapCfg = cfg->pair_cap;
+  }
+  if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {
+    delete c;
+    return AVBD_E_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device) {
+    delete c;
+    return AVBD_E_HIP;
+  }
+  *out = c;
+  return AVBD_OK;
+}
+
+int avbd_destroy(avbd_ctx *ctx) {
+  if (!ctx) return AVBD_E_INVALID;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return AVBD_OK;
+}
+
+const char *avbd_last_error(const avbd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int avbd_set_params(avbd_ctx *ctx, const avbd_params *p) {
+  return guarded(ctx, [&] {
+    if (!p) throw ApiError(AVBD_E_INVALID, "null params");
+    if (!(p->dt > 0) || p->iterations < 1) throw ApiError(AVBD_E_INVALID, "dt must be > 0 and iterations >= 1");
+    if (!p->cached_contact_jacobians)
+      throw ApiError(AVBD_E_INVALID, "only cached contact Jacobians are implemented on the GPU");
+    ctx->prm = *p;
+  });
+}
+
+int avbd_pack(avbd_ctx *ctx, const avbd_scene *scene) {
+  return guarded(ctx, [&] {
+    if (!scene) throw ApiError(AVBD_E_INVALID, "null scene");
+    ctx->packed = false;
+    packScene(*ctx, *scene);
+  });
+}
+
+int avbd_step(avbd_ctx *ctx, int32_t nsteps) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    HIPCHECK(hipSetDevice(ctx->device));
+    for (int s = 0; s < nsteps; s++) ctx->step();
+  });
+}
+
+int avbd_sync(avbd_ctx *ctx) {
+  return guarded(ctx, [&] {
+    if (ctx->stream) HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int avbd_read_state(avbd_ctx *ctx, double *pos, double *quat, double *vel, double *angvel, double *prev_vel,
+                    int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    std::vector<double> X(size_t(n) * XS), V(size_t(n) * VS);
+    HIPCHECK(hipMemcpyAsync(X.data(), ctx->X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(V.data(), ctx->V.p, sizeof(double) * V.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; i++) {
+      const double *x = &X[size_t(i) * XS], *v = &V[size_t(i) * VS];
+      for (int k = 0; k < 3; k++) {
+        if (pos) pos[i * 3 + k] = x[k];
+        if (vel) vel[i * 3 + k] = v[k];
+        if (angvel) angvel[i * 3 + k] = v[3 + k];
+        if (prev_vel) prev_vel[i * 3 + k] = v[6 + k];
+      }
+      if (quat) for (int k = 0; k < 4; k++) quat[i * 4 + k] = x[3 + k];
+    }
+  });
+}
+
+int avbd_write_state(avbd_ctx *ctx, const double *pos, const double *quat, const double *vel, const double *angvel,
+                     const double *prev_vel, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    std::vector<double> X(size_t(n) * XS), V(size_t(n) * VS);
+    HIPCHECK(hipMemcpyAsync(X.data(), ctx->X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(V.data(), ctx->V.p, sizeof(double) * V.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; i++) {
+      double *x = &X[size_t(i) * XS], *v = &V[size_t(i) * VS];
+      for (int k = 0; k < 3; k++) {
+        if (pos) x[k] = pos[i * 3 + k];
+        if (vel) v[k] = vel[i * 3 + k];
+        if (angvel) v[3 + k] = angvel[i * 3 + k];
+        if (prev_vel) v[6 + k] = prev_vel[i * 3 + k];
+      }
+      if (quat) for (int k = 0; k < 4; k++) x[3 + k] = quat[i * 4 + k];
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->X.p, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(ctx->V.p, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int avbd_read_pairs(avbd_ctx *ctx, int32_t *out, int32_t cap, int32_t *count) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (count) *count = ctx->nPairs;
+    const int n = std::min(cap, ctx->nPairs);
+    if (out && n > 0) {
+      HIPCHECK(hipMemcpyAsync(out, ctx->pairs.p, sizeof(int2) * n, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
+int avbd_read_coloring(avbd_ctx *ctx, int32_t *color_of, int32_t n, int32_t *num_colors) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    if (num_colors) *num_colors = ctx->numColors;
+    if (color_of) {
+      HIPCHECK(hipMemcpyAsync(color_of, ctx->color[ctx->colorCur].p, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
+static std::vector<JointDev> hostJoints(avbd_ctx *ctx) {
+  std::vector<JointDev> J(ctx->J);
+  if (ctx->J) {
+    HIPCHECK(hipMemcpyAsync(J.data(), ctx->joints.p, sizeof(JointDev) * ctx->J, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return J;
+}
+
+int avbd_read_forces(avbd_ctx *ctx, int32_t *kabn, uint32_t *features, int32_t cap, int32_t *count) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    std::vector<JointDev> J = hostJoints(ctx);
+    std::vector<int> live(ctx->nLive);
+    std::vector<int4> head(ctx->nLive);
+    if (ctx->nLive) {
+      HIPCHECK(hipMemcpyAsync(live.data(), ctx->live[ctx->cur].p, sizeof(int) * ctx->nLive, hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<int4> allHead(ctx->nLive ? size_t(*std::max_element(live.begin(), live.end())) + 1 : 0);
+    std::vector<uint32_t> allFeat(allHead.size() * MAXC);
+    if (!allHead.empty()) {
+      HIPCHECK(hipMemcpyAsync(allHead.data(), ctx->store[ctx->cur].head.p, sizeof(int4) * allHead.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipMemcpyAsync(allFeat.data(), ctx->store[ctx->cur].feat.p, sizeof(uint32_t) * allFeat.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    int k = 0;
+    auto put = [&](int kind, int a, int b, int nc, const uint32_t *f) {
+      if (k < cap) {
+        if (kabn) { kabn[k * 4] = kind; kabn[k * 4 + 1] = a; kabn[k * 4 + 2] = b; kabn[k * 4 + 3] = nc; }
+        if (features)
+          for (int s = 0; s < MAXC; s++) features[k * MAXC + s] = (f && s < nc) ? f[s] : 0u;
+      }
+      k++;
+    };
+    for (int j = 0; j < ctx->J; j++)
+      if (J[j].broken != 2) put(1, J[j].a, J[j].b, 0, nullptr);
+    for (int t = 0; t < ctx->nLive; t++) {
+      const int4 h = allHead[live[t]];
+      put(0, h.x, h.y, h.z, &allFeat[size_t(live[t]) * MAXC]);
+    }
+    if (count) *count = k;
+  });
+}
+
+int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *contacts, int32_t *num_contacts) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    std::vector<JointDev> J = hostJoints(ctx);
+    int nj = 0;
+    for (auto &j : J) nj += j.broken != 2;
+    const int t = index - nj;
+    if (t < 0 || t >= ctx->nLive) throw ApiError(AVBD_E_INVALID, "index is not a live manifold");
+    int m;
+    HIPCHECK(hipMemcpyAsync(&m, ctx->live[ctx->cur].p + t, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ManStore &st = ctx->store[ctx->cur];
+    const size_t rs = ctx->rsize();
+    int4 h;
+    std::vector<char> bas(BAS * rs), hot(MAXC * HOT * rs);
+    std::vector<double> anc(MAXC * 6);
+    std::vector<int32_t> stick(MAXC);
+    HIPCHECK(hipMemcpyAsync(&h, st.head.p + m, sizeof(int4), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(bas.data(), st.basis.p + size_t(m) * BAS * rs, BAS * rs, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(hot.data(), st.hot.p + size_t(m) * MAXC * HOT * rs, MAXC * HOT * rs, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HIPCHECK(hipMemcpyAsync(anc.data(), st.anchor.p + size_t(m) * MAXC * 6, sizeof(double) * MAXC * 6,
+                            hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(stick.data(), st.stick.p + size_t(m) * MAXC, sizeof(int32_t) * MAXC,
+                            hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto rd = [&](const std::vector<char> &v, size_t i) {
+      return rs == sizeof(float) ? double(((const float *)v.data())[i]) : ((const double *)v.data())[i];
+    };
+    if (basis_mu) for (int k = 0; k < 10; k++) basis_mu[k] = rd(bas, k);
+    if (num_contacts) *num_contacts = h.z;
+    if (contacts)
+      for (int s = 0; s < h.z; s++) {
+        double *o = contacts + s * 16;
+        for (int k = 0; k < 6; k++) o[k] = anc[s * 6 + k];
+        for (int k = 0; k < 9; k++) o[6 + k] = rd(hot, size_t(s) * HOT + k);
+        o[15] = stick[s];
+      }
+  });
+}
+
+int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out) {
+  return guarded(ctx, [&] {
+    if (!out) throw ApiError(AVBD_E_INVALID, "null output");
+    std::memset(out, 0, sizeof(*out));
+    out->steps = ctx->steps;
+    out->bodies = ctx->N;
+    int nd = 0;
+    for (auto &p : ctx->hprops) nd += p.mass > 0;
+    out->dynamic_bodies = nd;
+    out->pairs = ctx->nPairs;
+    out->new_pairs = ctx->nNew;
+    out->manifolds = ctx->nLive;
+    out->colors = ctx->numColors;
+    out->color_rounds = ctx->colorRounds;
+    if (ctx->packed) {
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      std::vector<JointDev> J = hostJoints(ctx);
+      int nj = 0;
+      for (auto &j : J) nj += j.broken != 2;
+      out->joints = nj;
+      int ov = 0;
+      HIPCHECK(hipMemcpy(&ov, ctx->scal.p + 5, sizeof(int), hipMemcpyDeviceToHost));
+      out->overflow = ov;
+      // contacts = sum of nc over live manifolds
+      if (ctx->nLive) {
+        std::vector<int> live(ctx->nLive);
+        HIPCHECK(hipMemcpy(live.data(), ctx->live[ctx->cur].p, sizeof(int) * ctx->nLive, hipMemcpyDeviceToHost));
+        const int mx = *std::max_element(live.begin(), live.end()) + 1;
+        std::vector<int4> head(mx);
+        HIPCHECK(hipMemcpy(head.data(), ctx->store[ctx->cur].head.p, sizeof(int4) * mx, hipMemcpyDeviceToHost));
+        int nc = 0;
+        for (int t : live) nc += head[t].z;
+        out->contacts = nc;
+      }
+    }
+  });
+}
+
+int avbd_set_profiling(avbd_ctx *ctx, int32_t on) {
+  return guarded(ctx, [&] {
+    ctx->profiling = on != 0;
+    for (double &m : ctx->profMs) m = 0;
+    ctx->profSteps = 0;
+  });
+}
+
+int avbd_read_profile(avbd_ctx *ctx, float *ms, int32_t n, int32_t *steps) {
+  return guarded(ctx, [&] {
+    if (steps) *steps = ctx->profSteps;
+    for (int k = 0; k < n && k < AVBD_NUM_PHASES; k++)
+      ms[k] = ctx->profSteps ? float(ctx->profMs[k] / ctx->profSteps) : 0.0f;
+  });
+}
+
+}  // extern "C"

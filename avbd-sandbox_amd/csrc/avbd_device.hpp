@@ -1,0 +1,511 @@
+// avbd_device.hpp — device data layout and the per-thread building blocks of
+// the AVBD step: narrow phase (collide.js), warm-start merge
+// (manifold.js:214-318), contact evaluation (manifold.js:358-409) and the
+// joint rows (joint.js). The operation order follows the reference's f64
+// engine; instantiated with double it reproduces it exactly.
+#pragma once
+#include <cstdint>
+
+#include "avbd_math.hpp"
+
+namespace avbd {
+
+constexpr int MAXC = 8;            // contacts per manifold (collide.js:22)
+
+// Static per-body data, read-only after pack.
+struct BodyProp {
+  double mass, m0, m1, m2;         // mass and body-frame principal moments
+  double hx, hy, hz, radius;       // half extents (size * 0.5), bounding radius
+  double sx, sy, sz, friction;     // full size, friction coefficient
+  int32_t shape, scene, large, pad;
+};
+
+// Body state rows, double, 8 per body: X = [p0 p1 p2 q0 q1 q2 q3 -].
+constexpr int XS = 8;
+// Velocities, 12 per body: v[3] w[3] vprev[3] -.
+constexpr int VS = 12;
+
+// Joint, always double (joint.js:78-129).
+struct JointDev {
+  int32_t a, b, broken, pad;
+  double rA[3], rB[3];
+  double stiffLin, stiffAng, fracture, torqueArm;
+  double C0Lin[3], C0Ang[3], kLin[3], kAng[3], lamLin[3], lamAng[3], CLin[3], CAng[3];
+};
+
+// One manifold buffer (two are kept: last step's and this step's).
+// Per manifold: head = {a, b, nc, flags}, basis[12] = row-major basis, mu.
+// Per contact slot s of manifold m (index m * MAXC + s):
+//   hot[16]: C0[3] k[3] lam[3] rAw[3] rBw[3] -  (rAw/rBw: anchors rotated
+//            by the x_t orientations, the cached Jacobian of manifold.js:338-349)
+//   anchor[6]: rA[3] rB[3] body-frame anchors (double)
+//   feat, stick, Cd[4] (last constraint value, diagnostics)
+template <class R> struct ManBuf {
+  int4 *head;
+  R *basis;
+  R *hot;
+  double *anchor;
+  uint32_t *feat;
+  int32_t *stick;
+  R *Cd;
+};
+constexpr int HOT = 16;
+constexpr int BAS = 12;
+constexpr uint32_t MF_SPHERE = 1u;  // head.w flag: a sphere is involved
+
+// ---------------------------------------------------------------------------
+// Narrow phase, double (collide.js).
+// ---------------------------------------------------------------------------
+constexpr int MAXPOLY = 16;
+constexpr double SAT_EPS = 1.0e-6;
+constexpr double SPHERE_EPS_SQ = 1.0e-20;
+constexpr double PLANE_EPS = 1.0e-5;
+constexpr double MERGE_DIST_SQ = 1.0e-6;
+
+struct Obb { double c[3], h[3], ax[3][3]; };
+struct NewContact { uint32_t feature; double rA[3], rB[3]; };
+
+struct BodyView { const double *p, *q; double hx, hy, hz, radius; int shape; };
+
+__device__ inline void makeObb(Obb &o, const BodyView &b) {
+  const double ux[3] = {1, 0, 0}, uy[3] = {0, 1, 0}, uz[3] = {0, 0, 1};
+  o.c[0] = b.p[0]; o.c[1] = b.p[1]; o.c[2] = b.p[2];
+  o.h[0] = b.hx; o.h[1] = b.hy; o.h[2] = b.hz;
+  qrot(o.ax[0], b.q, ux);
+  qrot(o.ax[1], b.q, uy);
+  qrot(o.ax[2], b.q, uz);
+}
+
+__device__ inline void supportPt(double *o, const Obb &b, const double *dir) {
+  const double sx = dot3(dir, b.ax[0]) >= 0 ? 1.0 : -1.0;
+  const double sy = dot3(dir, b.ax[1]) >= 0 ? 1.0 : -1.0;
+  const double sz = dot3(dir, b.ax[2]) >= 0 ? 1.0 : -1.0;
+  for (int k = 0; k < 3; k++) o[k] = b.c[k];
+  for (int k = 0; k < 3; k++) o[k] += b.ax[0][k] * (b.h[0] * sx);
+  for (int k = 0; k < 3; k++) o[k] += b.ax[1][k] * (b.h[1] * sy);
+  for (int k = 0; k < 3; k++) o[k] += b.ax[2][k] * (b.h[2] * sz);
+}
+
+__device__ inline void faceAx(const Obb &b, int ai, double *u, double *v, double &eu, double &ev) {
+  const int iu = ai == 0 ? 1 : 0, iv = ai == 2 ? 1 : 2;
+  for (int k = 0; k < 3; k++) { u[k] = b.ax[iu][k]; v[k] = b.ax[iv][k]; }
+  eu = b.h[iu]; ev = b.h[iv];
+}
+
+// Sutherland-Hodgman against the half-space n.x <= off (collide.js:202-247).
+__device__ inline int clipPoly(const double *in, int n, const double *pn, double off, double *out) {
+  if (n <= 0) return 0;
+  int cnt = 0;
+  double ax = in[(n - 1) * 3], ay = in[(n - 1) * 3 + 1], az = in[(n - 1) * 3 + 2];
+  double da = pn[0] * ax + pn[1] * ay + pn[2] * az - off;
+  for (int i = 0; i < n; i++) {
+    const double bx = in[i * 3], by = in[i * 3 + 1], bz = in[i * 3 + 2];
+    const double db = pn[0] * bx + pn[1] * by + pn[2] * bz - off;
+    const bool aIn = da <= PLANE_EPS, bIn = db <= PLANE_EPS;
+    if (aIn != bIn) {
+      double t = 0;
+      const double den = da - db;
+      if (xabs(den) > SAT_EPS) t = clampj(da / den, 0.0, 1.0);
+      if (cnt < MAXPOLY) {
+        out[cnt * 3] = ax + (bx - ax) * t;
+        out[cnt * 3 + 1] = ay + (by - ay) * t;
+        out[cnt * 3 + 2] = az + (bz - az) * t;
+        cnt++;
+      }
+    }
+    if (bIn && cnt < MAXPOLY) {
+      out[cnt * 3] = bx; out[cnt * 3 + 1] = by; out[cnt * 3 + 2] = bz;
+      cnt++;
+    }
+    ax = bx; ay = by; az = bz; da = db;
+  }
+  return cnt;
+}
+
+struct Sink {
+  const BodyView *A, *B;
+  NewContact *out;
+  int count;
+  double mid[MAXC * 3];
+  // collide.js:254-285
+  __device__ bool add(const double *xA, const double *xB, uint32_t key) {
+    const double mx = (xA[0] + xB[0]) * 0.5, my = (xA[1] + xB[1]) * 0.5, mz = (xA[2] + xB[2]) * 0.5;
+    for (int i = 0; i < count; i++) {
+      const double dx = mx - mid[i * 3], dy = my - mid[i * 3 + 1], dz = mz - mid[i * 3 + 2];
+      if (dx * dx + dy * dy + dz * dz < MERGE_DIST_SQ) return false;
+    }
+    if (count >= MAXC) return false;
+    NewContact &c = out[count];
+    c.feature = key;
+    double inv[4], t[3];
+    inv[0] = -A->q[0]; inv[1] = -A->q[1]; inv[2] = -A->q[2]; inv[3] = A->q[3];
+    t[0] = xA[0] - A->p[0]; t[1] = xA[1] - A->p[1]; t[2] = xA[2] - A->p[2];
+    qrot(c.rA, inv, t);
+    inv[0] = -B->q[0]; inv[1] = -B->q[1]; inv[2] = -B->q[2]; inv[3] = B->q[3];
+    t[0] = xB[0] - B->p[0]; t[1] = xB[1] - B->p[1]; t[2] = xB[2] - B->p[2];
+    qrot(c.rB, inv, t);
+    mid[count * 3] = mx; mid[count * 3 + 1] = my; mid[count * 3 + 2] = mz;
+    count++;
+    return true;
+  }
+};
+
+struct Sat { int type, ia, ib; double sep; double n[3]; bool valid; };
+
+// collide.js:291-324
+__device__ inline bool satAxis(const Obb &A, const Obb &B, const double *delta, const double *axis, int type,
+                               int ia, int ib, Sat &t) {
+  const double lenSq = dot3(axis, axis);
+  if (lenSq < SAT_EPS) return true;
+  const double invLen = 1.0 / xsqrt(lenSq);
+  double n[3] = {axis[0] * invLen, axis[1] * invLen, axis[2] * invLen};
+  if (dot3(n, delta) < 0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+  const double dist = xabs(dot3(delta, n));
+  const double rA = A.h[0] * xabs(dot3(n, A.ax[0])) + A.h[1] * xabs(dot3(n, A.ax[1])) + A.h[2] * xabs(dot3(n, A.ax[2]));
+  const double rB = B.h[0] * xabs(dot3(n, B.ax[0])) + B.h[1] * xabs(dot3(n, B.ax[1])) + B.h[2] * xabs(dot3(n, B.ax[2]));
+  const double sep = dist - (rA + rB);
+  if (sep > 0) return false;
+  if (!t.valid || sep > t.sep) {
+    t.valid = true; t.type = type; t.ia = ia; t.ib = ib; t.sep = sep;
+    t.n[0] = n[0]; t.n[1] = n[1]; t.n[2] = n[2];
+  }
+  return true;
+}
+
+// collide.js:327-343
+__device__ inline void supportEdge(const Obb &b, int ai, const double *dir, double *e0, double *e1) {
+  const int a1 = (ai + 1) % 3, a2 = (ai + 2) % 3;
+  const double s1 = dot3(dir, b.ax[a1]) >= 0 ? 1.0 : -1.0;
+  const double s2 = dot3(dir, b.ax[a2]) >= 0 ? 1.0 : -1.0;
+  double t[3] = {b.c[0], b.c[1], b.c[2]};
+  for (int k = 0; k < 3; k++) t[k] += b.ax[a1][k] * (b.h[a1] * s1);
+  for (int k = 0; k < 3; k++) t[k] += b.ax[a2][k] * (b.h[a2] * s2);
+  for (int k = 0; k < 3; k++) { e0[k] = t[k]; e0[k] += b.ax[ai][k] * -b.h[ai]; }
+  for (int k = 0; k < 3; k++) { e1[k] = t[k]; e1[k] += b.ax[ai][k] * b.h[ai]; }
+}
+
+// collide.js:345-393
+__device__ inline void closestSeg(const double *p0, const double *p1, const double *q0, const double *q1, double *c0,
+                                  double *c1) {
+  const double d1x = p1[0] - p0[0], d1y = p1[1] - p0[1], d1z = p1[2] - p0[2];
+  const double d2x = q1[0] - q0[0], d2y = q1[1] - q0[1], d2z = q1[2] - q0[2];
+  const double rx = p0[0] - q0[0], ry = p0[1] - q0[1], rz = p0[2] - q0[2];
+  const double a = d1x * d1x + d1y * d1y + d1z * d1z;
+  const double e = d2x * d2x + d2y * d2y + d2z * d2z;
+  const double f = d2x * rx + d2y * ry + d2z * rz;
+  double s = 0, t = 0;
+  if (a <= SAT_EPS && e <= SAT_EPS) {
+    for (int k = 0; k < 3; k++) { c0[k] = p0[k]; c1[k] = q0[k]; }
+    return;
+  }
+  if (a <= SAT_EPS) {
+    t = clampj(f / e, 0.0, 1.0);
+  } else {
+    const double c = d1x * rx + d1y * ry + d1z * rz;
+    if (e <= SAT_EPS) {
+      s = clampj(-c / a, 0.0, 1.0);
+    } else {
+      const double bb = d1x * d2x + d1y * d2y + d1z * d2z;
+      const double den = a * e - bb * bb;
+      if (xabs(den) > SAT_EPS) s = clampj((bb * f - c * e) / den, 0.0, 1.0);
+      t = (bb * s + f) / e;
+      if (t < 0) { t = 0; s = clampj(-c / a, 0.0, 1.0); }
+      else if (t > 1) { t = 1; s = clampj((bb - c) / a, 0.0, 1.0); }
+    }
+  }
+  c0[0] = p0[0] + d1x * s; c0[1] = p0[1] + d1y * s; c0[2] = p0[2] + d1z * s;
+  c1[0] = q0[0] + d2x * t; c1[1] = q0[1] + d2y * t; c1[2] = q0[2] + d2z * t;
+}
+
+// collide.js:403-472
+__device__ inline int faceManifold(const Obb &oA, const Obb &oB, Sink &sink, bool refIsA, int refAxis,
+                                   const double *nAB) {
+  const Obb &ref = refIsA ? oA : oB;
+  const Obb &inc = refIsA ? oB : oA;
+  double on[3];
+  for (int k = 0; k < 3; k++) on[k] = refIsA ? nAB[k] : -nAB[k];
+  double fn[3], fc[3], fu[3], fv[3], eu, ev;
+  const double s = dot3(on, ref.ax[refAxis]) >= 0 ? 1.0 : -1.0;
+  for (int k = 0; k < 3; k++) fn[k] = ref.ax[refAxis][k] * s;
+  for (int k = 0; k < 3; k++) { fc[k] = ref.c[k]; fc[k] += fn[k] * ref.h[refAxis]; }
+  faceAx(ref, refAxis, fu, fv, eu, ev);
+  int incAxis = 0;
+  double best = -__builtin_inf();
+  for (int i = 0; i < 3; i++) {
+    const double d = xabs(dot3(inc.ax[i], fn));
+    if (d > best) { best = d; incAxis = i; }
+  }
+  double c0[MAXPOLY * 3], c1[MAXPOLY * 3];
+  {
+    const double si = dot3(inc.ax[incAxis], fn) > 0 ? -1.0 : 1.0;
+    double nrm[3], ctr[3], u[3], v[3], iu, iv;
+    for (int k = 0; k < 3; k++) nrm[k] = inc.ax[incAxis][k] * si;
+    for (int k = 0; k < 3; k++) { ctr[k] = inc.c[k]; ctr[k] += nrm[k] * inc.h[incAxis]; }
+    faceAx(inc, incAxis, u, v, iu, iv);
+    for (int i = 0; i < 4; i++) {
+      const double su = (i == 0 || i == 3) ? 1.0 : -1.0;
+      const double sv = (i == 0 || i == 1) ? 1.0 : -1.0;
+      for (int k = 0; k < 3; k++) c0[i * 3 + k] = ctr[k] + u[k] * (iu * su) + v[k] * (iv * sv);
+    }
+  }
+  int count = 4;
+  double pn[3];
+  for (int k = 0; k < 3; k++) pn[k] = fu[k];
+  count = clipPoly(c0, count, pn, dot3(pn, fc) + eu, c1);
+  if (!count) return 0;
+  for (int k = 0; k < 3; k++) pn[k] = -fu[k];
+  count = clipPoly(c1, count, pn, dot3(pn, fc) + eu, c0);
+  if (!count) return 0;
+  for (int k = 0; k < 3; k++) pn[k] = fv[k];
+  count = clipPoly(c0, count, pn, dot3(pn, fc) + ev, c1);
+  if (!count) return 0;
+  for (int k = 0; k < 3; k++) pn[k] = -fv[k];
+  count = clipPoly(c1, count, pn, dot3(pn, fc) + ev, c0);
+  if (!count) return 0;
+
+  sink.count = 0;
+  uint32_t prefix = uint32_t(refIsA ? 0 : 1) << 24;
+  prefix |= uint32_t(refAxis & 0xff) << 16;
+  prefix |= uint32_t(incAxis & 0xff) << 8;
+  for (int i = 0; i < count && sink.count < MAXC; i++) {
+    double p[3], t[3];
+    for (int k = 0; k < 3; k++) p[k] = c0[i * 3 + k];
+    for (int k = 0; k < 3; k++) t[k] = p[k] - fc[k];
+    const double dist = dot3(t, fn);
+    if (dist > PLANE_EPS) continue;
+    for (int k = 0; k < 3; k++) { t[k] = p[k]; t[k] += fn[k] * -dist; }
+    if (refIsA) sink.add(t, p, prefix | uint32_t(i & 0xff));
+    else sink.add(p, t, prefix | uint32_t(i & 0xff));
+  }
+  if (!sink.count) {
+    double xA[3], xB[3], nn[3] = {-nAB[0], -nAB[1], -nAB[2]};
+    supportPt(xA, oA, nAB);
+    supportPt(xB, oB, nn);
+    sink.add(xA, xB, prefix);
+  }
+  return sink.count;
+}
+
+// collide.js:475-493
+__device__ inline int edgeContact(const Obb &oA, const Obb &oB, Sink &sink, int axA, int axB, const double *nAB) {
+  double nn[3] = {-nAB[0], -nAB[1], -nAB[2]}, a0[3], a1[3], b0[3], b1[3], xA[3], xB[3];
+  supportEdge(oA, axA, nAB, a0, a1);
+  supportEdge(oB, axB, nn, b0, b1);
+  closestSeg(a0, a1, b0, b1, xA, xB);
+  sink.count = 0;
+  const uint32_t key = (2u << 24) | (uint32_t(axA & 0xff) << 8) | uint32_t(axB & 0xff);
+  sink.add(xA, xB, key);
+  if (!sink.count) {
+    supportPt(xA, oA, nAB);
+    supportPt(xB, oB, nn);
+    sink.add(xA, xB, key);
+  }
+  return sink.count;
+}
+
+// collide.js:496-520
+__device__ inline int sphereSphere(const BodyView &A, const BodyView &B, Sink &sink, double *basis) {
+  double d[3] = {B.p[0] - A.p[0], B.p[1] - A.p[1], B.p[2] - A.p[2]}, n[3], t[3], xA[3], xB[3];
+  const double dsq = dot3(d, d);
+  const double rs = A.radius + B.radius;
+  if (dsq > rs * rs) return 0;
+  if (dsq > SPHERE_EPS_SQ) {
+    const double inv = 1 / xsqrt(dsq);
+    n[0] = d[0] * inv; n[1] = d[1] * inv; n[2] = d[2] * inv;
+  } else {
+    n[0] = 1; n[1] = 0; n[2] = 0;
+  }
+  t[0] = -n[0]; t[1] = -n[1]; t[2] = -n[2];
+  orthonormal(basis, t);
+  for (int k = 0; k < 3; k++) { xA[k] = A.p[k]; xA[k] += n[k] * A.radius; }
+  for (int k = 0; k < 3; k++) { xB[k] = B.p[k]; xB[k] += n[k] * -B.radius; }
+  sink.count = 0;
+  sink.add(xA, xB, 0x31000001u);
+  return sink.count;
+}
+
+// collide.js:526-600
+__device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink, double *basis, bool sphereIsA) {
+  const BodyView &sph = sphereIsA ? A : B;
+  const BodyView &bx = sphereIsA ? B : A;
+  Obb o;
+  makeObb(o, bx);
+  double d[3], cp[3], sv[3], n[3], xA[3], xB[3];
+  for (int k = 0; k < 3; k++) d[k] = sph.p[k] - o.c[k];
+  const double lx = dot3(d, o.ax[0]), ly = dot3(d, o.ax[1]), lz = dot3(d, o.ax[2]);
+  const double cx = clampj(lx, -o.h[0], o.h[0]);
+  const double cy = clampj(ly, -o.h[1], o.h[1]);
+  const double cz = clampj(lz, -o.h[2], o.h[2]);
+  for (int k = 0; k < 3; k++) cp[k] = o.c[k];
+  for (int k = 0; k < 3; k++) cp[k] += o.ax[0][k] * cx;
+  for (int k = 0; k < 3; k++) cp[k] += o.ax[1][k] * cy;
+  for (int k = 0; k < 3; k++) cp[k] += o.ax[2][k] * cz;
+  for (int k = 0; k < 3; k++) sv[k] = sph.p[k] - cp[k];
+  const double dsq = dot3(sv, sv);
+  int fa = 0;
+  if (dsq > SPHERE_EPS_SQ) {
+    const double dist = xsqrt(dsq);
+    if (dist > sph.radius) return 0;
+    const double inv = 1 / dist;
+    for (int k = 0; k < 3; k++) n[k] = sv[k] * inv;
+  } else {
+    const double gx = o.h[0] - xabs(lx), gy = o.h[1] - xabs(ly), gz = o.h[2] - xabs(lz);
+    fa = gx < gy ? (gx < gz ? 0 : 2) : (gy < gz ? 1 : 2);
+    const double local = fa == 0 ? lx : fa == 1 ? ly : lz;
+    const double s = local >= 0 ? 1.0 : -1.0;
+    for (int k = 0; k < 3; k++) n[k] = o.ax[fa][k] * s;
+    const double gap = fa == 0 ? gx : fa == 1 ? gy : gz;
+    for (int k = 0; k < 3; k++) { cp[k] = sph.p[k]; cp[k] += n[k] * gap; }
+  }
+  for (int k = 0; k < 3; k++) { sv[k] = sph.p[k]; sv[k] += n[k] * -sph.radius; }
+  if (sphereIsA) {
+    for (int k = 0; k < 3; k++) { xA[k] = sv[k]; xB[k] = cp[k]; }
+    orthonormal(basis, n);
+  } else {
+    double t[3] = {-n[0], -n[1], -n[2]};
+    for (int k = 0; k < 3; k++) { xA[k] = cp[k]; xB[k] = sv[k]; }
+    orthonormal(basis, t);
+  }
+  sink.count = 0;
+  int sX = lx < -o.h[0] ? 1 : lx > o.h[0] ? 2 : 0;
+  int sY = ly < -o.h[1] ? 1 : ly > o.h[1] ? 2 : 0;
+  int sZ = lz < -o.h[2] ? 1 : lz > o.h[2] ? 2 : 0;
+  if ((sX | sY | sZ) == 0) {
+    const double local = fa == 0 ? lx : fa == 1 ? ly : lz;
+    const int side = local >= 0 ? 2 : 1;
+    if (fa == 0) sX = side; else if (fa == 1) sY = side; else sZ = side;
+  }
+  const uint32_t region = uint32_t(sX | (sY << 2) | (sZ << 4));
+  sink.add(xA, xB, 0x32000000u | (region << 2) | (sphereIsA ? 1u : 2u));
+  return sink.count;
+}
+
+// collide.js:612-685. Writes the basis only where the reference does.
+__device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *out, double *basis) {
+  Sink sink;
+  sink.A = &A; sink.B = &B; sink.out = out; sink.count = 0;
+  const bool sA = A.shape == 1, sB = B.shape == 1;
+  if (sA && sB) return sphereSphere(A, B, sink, basis);
+  if (sA || sB) return sphereBox(A, B, sink, basis, sA);
+  Obb oA, oB;
+  makeObb(oA, A);
+  makeObb(oB, B);
+  double delta[3] = {oB.c[0] - oA.c[0], oB.c[1] - oA.c[1], oB.c[2] - oA.c[2]};
+  Sat face{0, -1, -1, -__builtin_inf(), {0, 0, 0}, false};
+  Sat edge = face;
+  for (int i = 0; i < 3; i++)
+    if (!satAxis(oA, oB, delta, oA.ax[i], 0, i, -1, face)) return 0;
+  for (int i = 0; i < 3; i++)
+    if (!satAxis(oA, oB, delta, oB.ax[i], 1, -1, i, face)) return 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double ax[3];
+      cross3(ax, oA.ax[i], oB.ax[j]);
+      if (!satAxis(oA, oB, delta, ax, 2, i, j, edge)) return 0;
+    }
+  if (!face.valid) return 0;
+  Sat best = face;
+  if (edge.valid) {
+    if (0.95 * edge.sep > face.sep + 0.01) best = edge;
+  }
+  double t[3] = {-best.n[0], -best.n[1], -best.n[2]};
+  orthonormal(basis, t);
+  if (best.type == 2) return edgeContact(oA, oB, sink, best.ia, best.ib, best.n);
+  if (best.type == 0) return faceManifold(oA, oB, sink, true, best.ia, best.n);
+  return faceManifold(oA, oB, sink, false, best.ib, best.n);
+}
+
+// ---------------------------------------------------------------------------
+// Contact evaluation (manifold.js:358-409), in the arithmetic type R.
+// jLinA = basis, jLinB = -basis; jAngA row r = rAw x basis_r, jAngB row r =
+// rBw x (-basis_r) (manifold.js:104-120).
+// ---------------------------------------------------------------------------
+template <class R> struct Eval {
+  R jA[9], jB[9];   // angular blocks of A and B
+  R C[3], F[3];
+  bool within;
+};
+
+template <class R, bool EXACT>
+__device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, const R *hot, const R *dqA,
+                                            const R *dqB, R alpha, Eval<R> &e) {
+  // angular blocks from the cached world anchors
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    R acc[3];
+    cross3(acc, hot + 9, bas + r * 3);
+    e.jA[r * 3] = acc[0]; e.jA[r * 3 + 1] = acc[1]; e.jA[r * 3 + 2] = acc[2];
+    cross3(acc, hot + 12, negb + r * 3);
+    e.jB[r * 3] = acc[0]; e.jB[r * 3 + 1] = acc[1]; e.jB[r * 3 + 2] = acc[2];
+  }
+  const R *C0 = hot, *k = hot + 3, *lam = hot + 6;
+  R acc[3];
+  const R oma = R(1) - alpha;
+  e.C[0] = C0[0] * oma; e.C[1] = C0[1] * oma; e.C[2] = C0[2] * oma;
+  mulv(acc, bas, dqA);
+  e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
+  mulv(acc, negb, dqB);
+  e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
+  mulv(acc, e.jA, dqA + 3);
+  e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
+  mulv(acc, e.jB, dqB + 3);
+  e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
+  if (EXACT) {
+    // K C with the diagonal K written out as the reference's full product
+    e.F[0] = k[0] * e.C[0] + R(0) * e.C[1] + R(0) * e.C[2];
+    e.F[1] = R(0) * e.C[0] + k[1] * e.C[1] + R(0) * e.C[2];
+    e.F[2] = R(0) * e.C[0] + R(0) * e.C[1] + k[2] * e.C[2];
+  } else {
+    e.F[0] = k[0] * e.C[0]; e.F[1] = k[1] * e.C[1]; e.F[2] = k[2] * e.C[2];
+  }
+  e.F[0] = e.F[0] + lam[0]; e.F[1] = e.F[1] + lam[1]; e.F[2] = e.F[2] + lam[2];
+  e.F[0] = jmin(e.F[0], R(0));
+  const R bounds = xabs(e.F[0]) * mu;
+  const R fs = xsqrt(e.F[1] * e.F[1] + e.F[2] * e.F[2]);
+  if (fs > bounds && fs > R(0)) {
+    e.F[1] *= bounds / fs;
+    e.F[2] *= bounds / fs;
+  }
+  e.within = fs <= bounds;
+}
+
+// Accumulated 6x6 system: lower triangles of the Lin and Ang blocks, the full
+// Cross block, and the right-hand side (the 27 numbers solve6 reads).
+// lin: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) 4:(2,1) 5:(2,2)  (row, col)
+template <class A> struct Sys {
+  A lin[6], ang[6], cross[9], rl[3], ra[3];
+  __device__ void zero() {
+    for (int i = 0; i < 6; i++) lin[i] = ang[i] = A(0);
+    for (int i = 0; i < 9; i++) cross[i] = A(0);
+    for (int i = 0; i < 3; i++) rl[i] = ra[i] = A(0);
+  }
+};
+__device__ constexpr int LT_R[6] = {0, 1, 1, 2, 2, 2};
+__device__ constexpr int LT_C[6] = {0, 0, 1, 0, 1, 2};
+
+// One contact's contribution to the side `isA` (manifold.js:411-445):
+//   Lin += Jl^T K Jl, Ang += Ja^T K Ja, Cross += Ja^T K Jl, rhs += J^T F.
+// Entry (r, c) of Jx^T K Jy is sum_j (Jx[j][r] k_j) Jy[j][c] in the reference's
+// association.
+template <class R, class A>
+__device__ __forceinline__ void stampContact(Sys<A> &s, const R *jl, const R *ja, const R *k, const R *F) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const int r = LT_R[i], c = LT_C[i];
+    s.lin[i] += A((jl[r] * k[0]) * jl[c] + (jl[3 + r] * k[1]) * jl[3 + c] + (jl[6 + r] * k[2]) * jl[6 + c]);
+    s.ang[i] += A((ja[r] * k[0]) * ja[c] + (ja[3 + r] * k[1]) * ja[3 + c] + (ja[6 + r] * k[2]) * ja[6 + c]);
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      s.cross[r * 3 + c] +=
+          A((ja[r] * k[0]) * jl[c] + (ja[3 + r] * k[1]) * jl[3 + c] + (ja[6 + r] * k[2]) * jl[6 + c]);
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    s.rl[r] += A(jl[r] * F[0] + jl[3 + r] * F[1] + jl[6 + r] * F[2]);
+    s.ra[r] += A(ja[r] * F[0] + ja[3 + r] * F[1] + ja[6 + r] * F[2]);
+  }
+}
+
+}  // namespace avbd

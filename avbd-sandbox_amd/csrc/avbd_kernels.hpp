@@ -1,0 +1,893 @@
+// avbd_kernels.hpp — the kernels of one AVBD step on gfx950.
+//
+// Step pipeline (one HIP stream):
+//   broad phase   bp_cells -> bp_scatter -> bp_small/bp_large (count, write)
+//   narrow phase  narrow (one thread per old manifold or new pair) + joint_init
+//   topology      live compaction, per-body force CSR (adj_*), colouring
+//                 rounds to the greedy fixed point, colour buckets, per-body
+//                 contact work lists (item_*)
+//   solve         body_init, then per iteration: primal per colour, dual,
+//                 and velocity after the last main iteration
+// References are file:line in the reference repository.
+#pragma once
+#include "avbd_device.hpp"
+
+namespace avbd {
+
+struct StepParams {
+  double dt, gravity, betaLin, betaAng, alpha, gamma, pmin, pmax, margin, stick;
+  int iterations, postStab, rotated;
+};
+
+// ------------------------------------------------------------------ utils
+__global__ void k_fill(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Block-level exclusive scan of 1024 ints (256 threads x 4).
+__global__ void k_scan_block(const int *in, int *out, int n, int *blockSums) {
+  __shared__ int sh[256];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int v[4], s = 0;
+  for (int k = 0; k < 4; k++) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    s += v[k];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - s;
+  for (int k = 0; k < 4; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) blockSums[blockIdx.x] = sh[255];
+}
+
+__global__ void k_scan_add(int *out, int n, const int *blockOffs) {
+  const int i = blockIdx.x * 1024 + threadIdx.x * 4;
+  const int add = blockOffs[blockIdx.x];
+  for (int k = 0; k < 4; k++)
+    if (i + k < n) out[i + k] += add;
+}
+
+// total = out[n-1] + in[n-1]
+__global__ void k_scan_total(const int *in, const int *out, int n, int *total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? out[n - 1] + in[n - 1] : 0;
+}
+
+// ------------------------------------------------------------- broad phase
+// Uniform grid over small bodies (radius <= 8x the scene's mean), cell edge h
+// >= 2 x the largest small radius, hashed by (scene, cell). Every pair whose
+// bounding spheres overlap has centres in neighbouring cells, so the 27-cell
+// query returns a superset of the brute-force pairs (broadphase.js:236-243)
+// and the exact predicate of solver.js:316-329 selects them.
+__device__ __forceinline__ uint32_t cellHash(int s, int x, int y, int z) {
+  uint32_t h = uint32_t(x) * 73856093u ^ uint32_t(y) * 19349663u ^ uint32_t(z) * 83492791u ^ uint32_t(s) * 2654435761u;
+  h ^= h >> 15;
+  h *= 0x2c1b3c6du;
+  h ^= h >> 12;
+  return h;
+}
+
+__global__ void k_bp_cells(int n, const BodyProp *props, const double *X, double invh, uint32_t mask,
+                           int4 *cellOf, int *bucketCnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BodyProp &b = props[i];
+  if (b.large) { cellOf[i] = make_int4(0, 0, 0, -1); return; }
+  const double *p = X + size_t(i) * XS;
+  const int cx = int(floor(p[0] * invh)), cy = int(floor(p[1] * invh)), cz = int(floor(p[2] * invh));
+  cellOf[i] = make_int4(cx, cy, cz, b.scene);
+  atomicAdd(&bucketCnt[cellHash(b.scene, cx, cy, cz) & mask], 1);
+}
+
+__global__ void k_bp_scatter(int n, const int4 *cellOf, uint32_t mask, int *cursor, int *entries) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 c = cellOf[i];
+  if (c.w < 0) return;
+  const int pos = atomicAdd(&cursor[cellHash(c.w, c.x, c.y, c.z) & mask], 1);
+  entries[pos] = i;
+}
+
+// Is there a force between x and y? kind: 1 live joint, 2 manifold, 0 none.
+// Looks at x's force list (the CSR of the last step).
+__device__ __forceinline__ int linkKind(int x, int y, const int *adjStart, const int *adjKey, const int *adjOther,
+                                        int J, const JointDev *joints) {
+  for (int e = adjStart[x]; e < adjStart[x + 1]; e++) {
+    if (adjOther[e] != y) continue;
+    const int key = adjKey[e];
+    if (key < J) {
+      if (!joints[key].broken) return 1;
+    } else {
+      return 2;
+    }
+  }
+  return 0;
+}
+
+struct BpArgs {
+  int n, J;
+  const BodyProp *props;
+  const double *X;
+  const int4 *cellOf;
+  const int *bucketStart, *bucketCnt, *entries;
+  uint32_t mask;
+  const int *sceneLargeStart, *sceneLarges;  // large bodies per scene
+  const int *sceneStart;                     // first body of each scene
+  const int *adjStart, *adjKey, *adjOther;
+  const JointDev *joints;
+  int *countRev;                             // [n] count for A stored at n-1-A
+  const int *startRev;                       // [n] exclusive scan of countRev
+  int2 *pairs;                               // output pairs (A, B)
+  int *pairNew;                              // 1 if the pair creates a manifold
+  int pairCap;
+  int *overflow;
+};
+
+// The predicate of solver.js:316-329 for A > B, plus the two flags: is the
+// pair recorded (a dynamic body, no live joint), does it create a manifold
+// (no manifold yet).
+__device__ __forceinline__ int pairTest(const BpArgs &a, int A, int B) {
+  const BodyProp &pa = a.props[A], &pb = a.props[B];
+  const double *xa = a.X + size_t(A) * XS, *xb = a.X + size_t(B) * XS;
+  const double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
+  const double r = pa.radius + pb.radius;
+  if (!(dx * dx + dy * dy + dz * dz <= r * r)) return 0;
+  if (!(pa.mass > 0 || pb.mass > 0)) return 0;
+  // look the link up from the side with the shorter force list
+  const int la = a.adjStart[A + 1] - a.adjStart[A], lb = a.adjStart[B + 1] - a.adjStart[B];
+  const int kind = la <= lb ? linkKind(A, B, a.adjStart, a.adjKey, a.adjOther, a.J, a.joints)
+                            : linkKind(B, A, a.adjStart, a.adjKey, a.adjOther, a.J, a.joints);
+  if (kind == 1) return 0;
+  return kind == 2 ? 1 : 2;  // 1: recorded, existing manifold; 2: recorded, new
+}
+
+// Small bodies: pairs (x, y) with y < x from the grid, and (x, L) with the
+// scene's large bodies L < x. PASS 0 counts, PASS 1 writes sorted by y desc.
+template <int PASS>
+__global__ void k_bp_small(BpArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const BodyProp &px = a.props[x];
+  if (px.large) return;
+  const int4 c = a.cellOf[x];
+  int cnt = 0;
+  int base = 0;
+  if (PASS == 1) base = a.startRev[a.n - 1 - x];
+  for (int dx = -1; dx <= 1; dx++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dz = -1; dz <= 1; dz++) {
+        const int qx = c.x + dx, qy = c.y + dy, qz = c.z + dz;
+        const uint32_t bkt = cellHash(c.w, qx, qy, qz) & a.mask;
+        const int s0 = a.bucketStart[bkt], s1 = s0 + a.bucketCnt[bkt];
+        for (int e = s0; e < s1; e++) {
+          const int y = a.entries[e];
+          if (y >= x) continue;
+          const int4 cy = a.cellOf[y];
+          if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
+          const int t = pairTest(a, x, y);
+          if (!t) continue;
+          if (PASS == 1) {
+            // insertion into the descending run written so far
+            const int pos = base + cnt;
+            if (pos < a.pairCap) {
+              int k = pos;
+              while (k > base && a.pairs[k - 1].y < y) {
+                a.pairs[k] = a.pairs[k - 1];
+                a.pairNew[k] = a.pairNew[k - 1];
+                k--;
+              }
+              a.pairs[k] = make_int2(x, y);
+              a.pairNew[k] = t == 2;
+            } else {
+              atomicOr(a.overflow, 1);
+            }
+          }
+          cnt++;
+        }
+      }
+  const int l0 = a.sceneLargeStart[px.scene], l1 = a.sceneLargeStart[px.scene + 1];
+  for (int e = l0; e < l1; e++) {
+    const int y = a.sceneLarges[e];
+    if (y >= x) continue;
+    const int t = pairTest(a, x, y);
+    if (!t) continue;
+    if (PASS == 1) {
+      const int pos = base + cnt;
+      if (pos < a.pairCap) {
+        int k = pos;
+        while (k > base && a.pairs[k - 1].y < y) {
+          a.pairs[k] = a.pairs[k - 1];
+          a.pairNew[k] = a.pairNew[k - 1];
+          k--;
+        }
+        a.pairs[k] = make_int2(x, y);
+        a.pairNew[k] = t == 2;
+      } else {
+        atomicOr(a.overflow, 1);
+      }
+    }
+    cnt++;
+  }
+  if (PASS == 0) a.countRev[a.n - 1 - x] = cnt;
+}
+
+// Large bodies: one 256-thread block per large body L tests every y < L of
+// its scene, in descending y, and compacts the accepted ones in that order.
+template <int PASS>
+__global__ void k_bp_large(BpArgs a, const int *largeList) {
+  const int L = largeList[blockIdx.x];
+  const int lo = a.sceneStart[a.props[L].scene];
+  __shared__ int warpCnt[4];
+  __shared__ int running;
+  if (threadIdx.x == 0) running = 0;
+  __syncthreads();
+  const int base = PASS == 1 ? a.startRev[a.n - 1 - L] : 0;
+  for (int top = L - 1; top >= lo; top -= 256) {
+    const int y = top - int(threadIdx.x);
+    const int t = y >= lo ? pairTest(a, L, y) : 0;
+    const unsigned long long m = __ballot(t != 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) warpCnt[w] = __popcll(m);
+    __syncthreads();
+    int off = running;
+    for (int k = 0; k < w; k++) off += warpCnt[k];
+    if (t && PASS == 1) {
+      const int pos = base + off + before;
+      if (pos < a.pairCap) {
+        a.pairs[pos] = make_int2(L, y);
+        a.pairNew[pos] = t == 2;
+      } else {
+        atomicOr(a.overflow, 1);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) running += warpCnt[0] + warpCnt[1] + warpCnt[2] + warpCnt[3];
+    __syncthreads();
+  }
+  if (PASS == 0 && threadIdx.x == 0) a.countRev[a.n - 1 - L] = running;
+}
+
+__global__ void k_new_pairs(int np, const int2 *pairs, const int *pairNew, const int *newPos, int2 *newPairs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np && pairNew[i]) newPairs[newPos[i]] = pairs[i];
+}
+
+// ------------------------------------------------------------ narrow phase
+// Manifold.initialize (manifold.js:214-318) for every old manifold (t < nOld)
+// and every new pair, written to slot t of the other buffer.
+template <class R>
+__global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
+                         const BodyProp *props, const double *X, StepParams P, int *keep) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nOld + nNew) return;
+  int a, b, m = -1, ncOld = 0;
+  if (t < nOld) {
+    m = liveOld[t];
+    const int4 h = ob.head[m];
+    a = h.x; b = h.y; ncOld = h.z;
+  } else {
+    const int2 pr = newPairs[t - nOld];
+    a = pr.x; b = pr.y;
+  }
+  const BodyProp &pa = props[a], &pb = props[b];
+  BodyView A{X + size_t(a) * XS, X + size_t(a) * XS + 3, pa.hx, pa.hy, pa.hz, pa.radius, pa.shape};
+  BodyView B{X + size_t(b) * XS, X + size_t(b) * XS + 3, pb.hx, pb.hy, pb.hz, pb.radius, pb.shape};
+  const bool hasSphere = pa.shape == 1 || pb.shape == 1;
+  const bool hadContacts = ncOld > 0;
+  double oldBasis[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double basis[9];
+  if (m >= 0)
+    for (int k = 0; k < 9; k++) basis[k] = double(ob.basis[size_t(m) * BAS + k]);
+  else
+    for (int k = 0; k < 9; k++) basis[k] = 0;
+  if (hasSphere && hadContacts)
+    for (int k = 0; k < 9; k++) oldBasis[k] = basis[k];
+  const double mu = xsqrt(pa.friction * pb.friction);
+
+  NewContact nc[MAXC];
+  const int n = collide(A, B, nc, basis);
+  const bool compatible =
+      !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
+
+  // merged per-contact state: anchors, k, lambda, C, stick
+  const size_t nbase = size_t(t) * MAXC;
+  for (int i = 0; i < n; i++) {
+    double rA[3] = {nc[i].rA[0], nc[i].rA[1], nc[i].rA[2]};
+    double rB[3] = {nc[i].rB[0], nc[i].rB[1], nc[i].rB[2]};
+    double kk[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
+    int stick = 0;
+    for (int j = 0; j < ncOld; j++) {
+      const size_t os = size_t(m) * MAXC + j;
+      if (compatible && ob.feat[os] == nc[i].feature) {
+        const R *oh = ob.hot + os * HOT;
+        for (int k = 0; k < 3; k++) {
+          kk[k] = double(oh[3 + k]);
+          lam[k] = double(oh[6 + k]);
+          Cc[k] = double(ob.Cd[os * 4 + k]);
+        }
+        const int oldStick = ob.stick[os];
+        stick = oldStick;
+        if (hasSphere) {
+          // transportComponents / transportPenalty (manifold.js:73-162)
+          const double x = lam[0], y = lam[1], z = lam[2];
+          const double px = kk[0], py = kk[1], pz = kk[2];
+          const double *o = oldBasis, *q = basis;
+          for (int r = 0; r < 3; r++) {
+            const int rr = r * 3;
+            lam[r] = x * (q[rr] * o[0] + q[rr + 1] * o[1] + q[rr + 2] * o[2]) +
+                     y * (q[rr] * o[3] + q[rr + 1] * o[4] + q[rr + 2] * o[5]) +
+                     z * (q[rr] * o[6] + q[rr + 1] * o[7] + q[rr + 2] * o[8]);
+          }
+          for (int r = 0; r < 3; r++) {
+            const int rr = r * 3;
+            const double ddx = q[rr] * o[0] + q[rr + 1] * o[1] + q[rr + 2] * o[2];
+            const double ddy = q[rr] * o[3] + q[rr + 1] * o[4] + q[rr + 2] * o[5];
+            const double ddz = q[rr] * o[6] + q[rr + 1] * o[7] + q[rr + 2] * o[8];
+            kk[r] = px * ddx * ddx + py * ddy * ddy + pz * ddz * ddz;
+          }
+        }
+        if (oldStick && !hasSphere) {
+          const double *an = ob.anchor + os * 6;
+          for (int k = 0; k < 3; k++) { rA[k] = an[k]; rB[k] = an[3 + k]; }
+        }
+        break;
+      }
+    }
+    // C0 at x_t and the warm start (manifold.js:290-315)
+    double xa[3], xb[3], d[3], C0[3];
+    xform(xa, A.p, A.q, rA);
+    xform(xb, B.p, B.q, rB);
+    d[0] = xa[0] - xb[0]; d[1] = xa[1] - xb[1]; d[2] = xa[2] - xb[2];
+    mulv(C0, basis, d);
+    C0[0] += P.margin;
+    if (!P.postStab) {
+      for (int k = 0; k < 3; k++) lam[k] = lam[k] * P.alpha;
+      for (int k = 0; k < 3; k++) lam[k] = lam[k] * P.gamma;
+    }
+    for (int k = 0; k < 3; k++) kk[k] = kk[k] * P.gamma;
+    for (int k = 0; k < 3; k++) kk[k] = clampj(kk[k], P.pmin, P.pmax);
+    // cached angular Jacobian anchors at x_t (manifold.js:338-349)
+    double rAw[3], rBw[3];
+    qrot(rAw, A.q, rA);
+    qrot(rBw, B.q, rB);
+    const size_t s = nbase + i;
+    R *h = nb.hot + s * HOT;
+    for (int k = 0; k < 3; k++) {
+      h[k] = R(C0[k]); h[3 + k] = R(kk[k]); h[6 + k] = R(lam[k]);
+      h[9 + k] = R(rAw[k]); h[12 + k] = R(rBw[k]);
+    }
+    h[15] = R(0);
+    double *an = nb.anchor + s * 6;
+    for (int k = 0; k < 3; k++) { an[k] = rA[k]; an[3 + k] = rB[k]; }
+    nb.feat[s] = nc[i].feature;
+    nb.stick[s] = stick;
+    for (int k = 0; k < 3; k++) nb.Cd[s * 4 + k] = R(Cc[k]);
+    nb.Cd[s * 4 + 3] = R(0);
+  }
+  nb.head[t] = make_int4(a, b, n, hasSphere ? int(MF_SPHERE) : 0);
+  for (int k = 0; k < 9; k++) nb.basis[size_t(t) * BAS + k] = R(basis[k]);
+  nb.basis[size_t(t) * BAS + 9] = R(mu);
+  nb.basis[size_t(t) * BAS + 10] = R(0);
+  nb.basis[size_t(t) * BAS + 11] = R(0);
+  keep[t] = n > 0 ? 1 : 0;
+}
+
+// ----------------------------------------------------------------- joints
+__device__ const double IDQ[4] = {0, 0, 0, 1};
+
+// joint.js:115-128
+__device__ __forceinline__ void jointLin(const JointDev &j, const double *X, double *o) {
+  double pa[3], pb[3];
+  if (j.a >= 0) xform(pa, X + size_t(j.a) * XS, X + size_t(j.a) * XS + 3, j.rA);
+  else { pa[0] = j.rA[0]; pa[1] = j.rA[1]; pa[2] = j.rA[2]; }
+  xform(pb, X + size_t(j.b) * XS, X + size_t(j.b) * XS + 3, j.rB);
+  o[0] = pa[0] - pb[0]; o[1] = pa[1] - pb[1]; o[2] = pa[2] - pb[2];
+}
+__device__ __forceinline__ void jointAng(const JointDev &j, const double *X, double *o) {
+  qsub(o, j.a >= 0 ? X + size_t(j.a) * XS + 3 : IDQ, X + size_t(j.b) * XS + 3);
+  o[0] = o[0] * j.torqueArm; o[1] = o[1] * j.torqueArm; o[2] = o[2] * j.torqueArm;
+}
+
+// joint.js:131-160; a broken joint leaves the force list.
+__global__ void k_joint_init(int J, JointDev *joints, const double *X, StepParams P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= J) return;
+  JointDev &j = joints[i];
+  if (j.broken) { j.broken = 2; return; }  // 2: removed
+  jointLin(j, X, j.C0Lin);
+  jointAng(j, X, j.C0Ang);
+  if (!P.postStab) {
+    for (int k = 0; k < 3; k++) { j.lamLin[k] = j.lamLin[k] * P.alpha; }
+    for (int k = 0; k < 3; k++) { j.lamLin[k] = j.lamLin[k] * P.gamma; }
+    for (int k = 0; k < 3; k++) { j.lamAng[k] = j.lamAng[k] * P.alpha; }
+    for (int k = 0; k < 3; k++) { j.lamAng[k] = j.lamAng[k] * P.gamma; }
+  }
+  for (int k = 0; k < 3; k++) j.kLin[k] = j.kLin[k] * P.gamma;
+  for (int k = 0; k < 3; k++) j.kLin[k] = clampj(j.kLin[k], P.pmin, P.pmax);
+  for (int k = 0; k < 3; k++) j.kAng[k] = j.kAng[k] * P.gamma;
+  for (int k = 0; k < 3; k++) j.kAng[k] = clampj(j.kAng[k], P.pmin, P.pmax);
+  for (int k = 0; k < 3; k++) j.kLin[k] = jmin(j.kLin[k], j.stiffLin);
+  for (int k = 0; k < 3; k++) j.kAng[k] = jmin(j.kAng[k], j.stiffAng);
+}
+
+// Full-matrix helpers for the joint rows (their products are written out as
+// the reference writes them).
+__device__ __forceinline__ void mT(double *o, const double *a) {
+  o[0] = a[0]; o[1] = a[3]; o[2] = a[6]; o[3] = a[1]; o[4] = a[4]; o[5] = a[7]; o[6] = a[2]; o[7] = a[5]; o[8] = a[8];
+}
+__device__ __forceinline__ void mDiagR(double *o, const double *a, const double *d) {
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3 + c] * d[c];
+}
+__device__ __forceinline__ void addLower(double *lt, const double *m) {
+  lt[0] += m[0]; lt[1] += m[3]; lt[2] += m[4]; lt[3] += m[6]; lt[4] += m[7]; lt[5] += m[8];
+}
+
+// Joint.updatePrimal (joint.js:162-253) into the body's system.
+__device__ void jointStamp(Sys<double> &s, const JointDev &j, bool isA, const double *X, double alpha) {
+  const double INF = __builtin_inf();
+  if (j.kLin[0] * j.kLin[0] + j.kLin[1] * j.kLin[1] + j.kLin[2] * j.kLin[2] > 0) {
+    double K[9] = {j.kLin[0], 0, 0, 0, j.kLin[1], 0, 0, 0, j.kLin[2]};
+    double C[3], F[3], jl[9], ja[9], r[3], rW[3];
+    jointLin(j, X, C);
+    if (j.stiffLin == INF)
+      for (int k = 0; k < 3; k++) C[k] += j.C0Lin[k] * -alpha;
+    mulv(F, K, C);
+    for (int k = 0; k < 3; k++) F[k] = F[k] + j.lamLin[k];
+    if (isA) {
+      for (int k = 0; k < 9; k++) jl[k] = (k % 4 == 0) ? 1.0 : 0.0;
+      qrot(rW, X + size_t(j.a) * XS + 3, j.rA);
+      r[0] = -rW[0]; r[1] = -rW[1]; r[2] = -rW[2];
+      ja[0] = 0; ja[1] = -r[2]; ja[2] = r[1]; ja[3] = r[2]; ja[4] = 0; ja[5] = -r[0]; ja[6] = -r[1]; ja[7] = r[0]; ja[8] = 0;
+      r[0] = rW[0]; r[1] = rW[1]; r[2] = rW[2];
+    } else {
+      for (int k = 0; k < 9; k++) jl[k] = (k % 4 == 0) ? -1.0 : 0.0;
+      qrot(rW, X + size_t(j.b) * XS + 3, j.rB);
+      ja[0] = 0; ja[1] = -rW[2]; ja[2] = rW[1]; ja[3] = rW[2]; ja[4] = 0; ja[5] = -rW[0]; ja[6] = -rW[1]; ja[7] = rW[0]; ja[8] = 0;
+      r[0] = -rW[0]; r[1] = -rW[1]; r[2] = -rW[2];
+    }
+    double jlT[9], jaT[9], jaTk[9], m0[9], acc[3];
+    mT(jlT, jl);
+    mT(jaT, ja);
+    mDiagR(jaTk, jaT, j.kLin);
+    mDiagR(m0, jlT, j.kLin);
+    mul33(m0, m0, jl);
+    addLower(s.lin, m0);
+    mul33(m0, jaTk, ja);
+    addLower(s.ang, m0);
+    mul33(m0, jaTk, jl);
+    for (int k = 0; k < 9; k++) s.cross[k] += m0[k];
+    // geometric stiffness, lumped by column norms (joint.js:224-234, maths.js:373-381)
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9];
+    for (int k = 0; k < 3; k++) {
+      for (int e = 0; e < 9; e++) G[e] = 0;
+      G[0] = -r[k]; G[4] = -r[k]; G[8] = -r[k];
+      G[0 * 3 + k] += r[0];
+      G[1 * 3 + k] += r[1];
+      G[2 * 3 + k] += r[2];
+      for (int e = 0; e < 9; e++) G[e] = G[e] * F[k];
+      for (int e = 0; e < 9; e++) H[e] += G[e];
+    }
+    s.ang[0] += xsqrt(H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+    s.ang[2] += xsqrt(H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+    s.ang[5] += xsqrt(H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+    mulv(acc, jlT, F);
+    for (int k = 0; k < 3; k++) s.rl[k] = s.rl[k] + acc[k];
+    mulv(acc, jaT, F);
+    for (int k = 0; k < 3; k++) s.ra[k] = s.ra[k] + acc[k];
+  }
+  if (j.kAng[0] * j.kAng[0] + j.kAng[1] * j.kAng[1] + j.kAng[2] * j.kAng[2] > 0) {
+    double K[9] = {j.kAng[0], 0, 0, 0, j.kAng[1], 0, 0, 0, j.kAng[2]};
+    double C[3], F[3], ja[9], jaT[9], m0[9], acc[3];
+    jointAng(j, X, C);
+    if (j.stiffAng == INF)
+      for (int k = 0; k < 3; k++) C[k] += j.C0Ang[k] * -alpha;
+    mulv(F, K, C);
+    for (int k = 0; k < 3; k++) F[k] = F[k] + j.lamAng[k];
+    const double sg = isA ? j.torqueArm : -j.torqueArm;
+    for (int k = 0; k < 9; k++) ja[k] = (k % 4 == 0) ? sg : 0.0;
+    mT(jaT, ja);
+    mul33(m0, jaT, K);
+    mul33(m0, m0, ja);
+    addLower(s.ang, m0);
+    mulv(acc, jaT, F);
+    for (int k = 0; k < 3; k++) s.ra[k] = s.ra[k] + acc[k];
+  }
+}
+
+// Joint.updateDual (joint.js:255-317)
+__global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParams P, double alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= J) return;
+  JointDev &j = joints[i];
+  if (j.broken == 2) return;
+  const double INF = __builtin_inf();
+  if (j.kLin[0] * j.kLin[0] + j.kLin[1] * j.kLin[1] + j.kLin[2] * j.kLin[2] > 0) {
+    double K[9] = {j.kLin[0], 0, 0, 0, j.kLin[1], 0, 0, 0, j.kLin[2]}, C[3], F[3];
+    jointLin(j, X, C);
+    if (j.stiffLin == INF) {
+      for (int k = 0; k < 3; k++) C[k] += j.C0Lin[k] * -alpha;
+      mulv(F, K, C);
+      for (int k = 0; k < 3; k++) j.lamLin[k] = F[k] + j.lamLin[k];
+    }
+    for (int k = 0; k < 3; k++) j.CLin[k] = C[k];
+    const double cap = jmin(j.stiffLin, P.pmax);
+    for (int k = 0; k < 3; k++) j.kLin[k] = jmin(j.kLin[k] + xabs(C[k]) * P.betaLin, cap);
+  }
+  if (j.kAng[0] * j.kAng[0] + j.kAng[1] * j.kAng[1] + j.kAng[2] * j.kAng[2] > 0) {
+    double K[9] = {j.kAng[0], 0, 0, 0, j.kAng[1], 0, 0, 0, j.kAng[2]}, C[3], F[3];
+    jointAng(j, X, C);
+    if (j.stiffAng == INF) {
+      for (int k = 0; k < 3; k++) C[k] += j.C0Ang[k] * -alpha;
+      mulv(F, K, C);
+      for (int k = 0; k < 3; k++) j.lamAng[k] = F[k] + j.lamAng[k];
+    }
+    for (int k = 0; k < 3; k++) j.CAng[k] = C[k];
+    const double cap = jmin(j.stiffAng, P.pmax);
+    for (int k = 0; k < 3; k++) j.kAng[k] = jmin(j.kAng[k] + xabs(C[k]) * P.betaAng, cap);
+  }
+  const double l2 = j.lamAng[0] * j.lamAng[0] + j.lamAng[1] * j.lamAng[1] + j.lamAng[2] * j.lamAng[2];
+  if (l2 > j.fracture * j.fracture) {
+    for (int k = 0; k < 3; k++) j.kLin[k] = j.kAng[k] = j.lamLin[k] = j.lamAng[k] = 0.0;
+    j.broken = 1;
+  }
+}
+
+// ---------------------------------------------------------------- topology
+__global__ void k_live_compact(int nTot, const int *keep, const int *pos, int *live) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nTot && keep[t]) live[pos[t]] = t;
+}
+
+// Per-body force lists. Key = j for joint j, J + k for live manifold k, so
+// ascending keys are the force list's creation order.
+template <int PASS>
+__global__ void k_adj(int nLive, int J, const int *live, const int4 *head, const JointDev *joints, int *deg,
+                      int *cursor, int *adjKey, int *adjOther) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int a, b, key;
+  if (t < J) {
+    if (joints[t].broken == 2) return;
+    a = joints[t].a; b = joints[t].b; key = t;
+  } else if (t < J + nLive) {
+    const int4 h = head[live[t - J]];
+    a = h.x; b = h.y; key = t;
+  } else {
+    return;
+  }
+  if (PASS == 0) {
+    if (a >= 0) atomicAdd(&deg[a], 1);
+    atomicAdd(&deg[b], 1);
+  } else {
+    if (a >= 0) {
+      const int p = atomicAdd(&cursor[a], 1);
+      adjKey[p] = key; adjOther[p] = b;
+    }
+    const int p = atomicAdd(&cursor[b], 1);
+    adjKey[p] = key; adjOther[p] = a;
+  }
+}
+
+// Sort each dynamic body's list by key (insertion sort; lists are short).
+__global__ void k_adj_sort(int n, const BodyProp *props, const int *adjStart, int *adjKey, int *adjOther) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(props[i].mass > 0)) return;
+  const int s0 = adjStart[i], s1 = adjStart[i + 1];
+  for (int e = s0 + 1; e < s1; e++) {
+    const int k = adjKey[e], o = adjOther[e];
+    int f = e;
+    while (f > s0 && adjKey[f - 1] > k) {
+      adjKey[f] = adjKey[f - 1];
+      adjOther[f] = adjOther[f - 1];
+      f--;
+    }
+    adjKey[f] = k;
+    adjOther[f] = o;
+  }
+}
+
+// Greedy colouring in index order (coloring.js:75-92): colour(i) is the
+// smallest colour not used by a lower-index dynamic neighbour. One Jacobi
+// round; repeated rounds reach the unique fixed point, which is exactly the
+// sequential greedy result.
+constexpr int MAX_COLORS = 256;
+__global__ void k_color_round(int n, const BodyProp *props, const int *adjStart, const int *adjOther, const int *cin,
+                              int *cout, int *changed, int *overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!(props[i].mass > 0)) { cout[i] = -1; return; }
+  unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
+  for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
+    const int j = adjOther[e];
+    if (j < 0 || j >= i) continue;
+    const int c = cin[j];
+    if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
+  }
+  int c = 0;
+  for (int w = 0; w < MAX_COLORS / 64; w++) {
+    if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
+    c = MAX_COLORS;
+  }
+  if (c >= MAX_COLORS) { atomicOr(overflow, 4); c = MAX_COLORS - 1; }
+  cout[i] = c;
+  if (c != cin[i]) atomicAdd(changed, 1);
+}
+
+__global__ void k_color_count(int n, const int *color, int *ccount, int *numColors) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = color[i];
+  if (c < 0) return;
+  atomicAdd(&ccount[c], 1);
+  atomicMax(numColors, c + 1);
+}
+
+__global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodies) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = color[i];
+  if (c < 0) return;
+  cbodies[atomicAdd(&cursor[c], 1)] = i;
+}
+
+// Per dynamic body: its contact/joint work items in the order the reference
+// visits them (forces newest first, contacts in order; solver.js:443-448).
+// Item: manifold slot (m << 4 | s << 1 | isA), or 0x80000000 | j << 1 | isA.
+template <int PASS>
+__global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart, const int *adjKey, const int *live,
+                        const int4 *head, const JointDev *joints, int *itemCnt, const int *itemStart,
+                        uint32_t *items) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!(props[i].mass > 0)) { if (PASS == 0) itemCnt[i] = 0; return; }
+  int cnt = 0;
+  const int base = PASS == 1 ? itemStart[i] : 0;
+  for (int e = adjStart[i + 1] - 1; e >= adjStart[i]; e--) {
+    const int key = adjKey[e];
+    if (key < J) {
+      if (PASS == 1) items[base + cnt] = 0x80000000u | (uint32_t(key) << 1) | (joints[key].a == i ? 1u : 0u);
+      cnt++;
+    } else {
+      const int m = live[key - J];
+      const int4 h = head[m];
+      if (PASS == 1)
+        for (int s = 0; s < h.z; s++)
+          items[base + cnt + s] = (uint32_t(m) << 4) | (uint32_t(s) << 1) | (h.x == i ? 1u : 0u);
+      cnt += h.z;
+    }
+  }
+  if (PASS == 0) itemCnt[i] = cnt;
+}
+
+// ------------------------------------------------------------------ solve
+// Body initialisation (solver.js:366-401), plus the per-body delta record
+// dq = (p - p0, q (-) q0) that neighbours read.
+template <class R>
+__global__ void k_body_init(int n, const BodyProp *props, double *X, double *X0, double *Y, const double *V, R *DQ,
+                            StepParams P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt = P.dt;
+  const BodyProp &b = props[i];
+  double *x = X + size_t(i) * XS, *x0 = X0 + size_t(i) * XS, *y = Y + size_t(i) * XS;
+  const double *v = V + size_t(i) * VS;
+  const double g[3] = {0, 0, P.gravity};
+  double py[3] = {x[0], x[1], x[2]};
+  for (int k = 0; k < 3; k++) py[k] += v[k] * dt;
+  if (b.mass > 0)
+    for (int k = 0; k < 3; k++) py[k] += g[k] * (dt * dt);
+  double wdt[3] = {v[3] * dt, v[4] * dt, v[5] * dt}, qy[4];
+  qint(qy, x + 3, wdt);
+  const double accelZ = (v[2] - v[8]) / dt;
+  const double accelExt = accelZ * sgn(P.gravity);
+  double wgt = clampj(accelExt / xabs(P.gravity), 0.0, 1.0);
+  if (!__builtin_isfinite(wgt)) wgt = 0.0;
+  for (int k = 0; k < 7; k++) x0[k] = x[k];
+  for (int k = 0; k < 3; k++) y[k] = py[k];
+  for (int k = 0; k < 4; k++) y[3 + k] = qy[k];
+  if (b.mass > 0) {
+    for (int k = 0; k < 3; k++) x[k] += v[k] * dt;
+    const double s = wgt * dt * dt;
+    for (int k = 0; k < 3; k++) x[k] += g[k] * s;
+    double q[4];
+    qint(q, x + 3, wdt);
+    for (int k = 0; k < 4; k++) x[3 + k] = q[k];
+  }
+  double dq[3];
+  qsub(dq, x + 3, x0 + 3);
+  R *d = DQ + size_t(i) * 8;
+  d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
+  d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
+  d[6] = R(0); d[7] = R(0);
+}
+
+struct SolveArgs {
+  const BodyProp *props;
+  double *X;
+  const double *X0, *Y;
+  const int *itemStart;
+  const uint32_t *items;
+  const int4 *head;
+  const JointDev *joints;
+  int nBodies;
+};
+
+template <class R>
+__device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
+#pragma unroll
+  for (int k = 0; k < n; k++) o[k] = p[k];
+}
+
+// Primal update of every body of one colour (solver.js:419-458): assemble
+// H = M/dt^2 + sum J^T K J, f = M/dt^2 (x - y) + sum J^T lambda+, solve the
+// 6x6 system in f64 and apply Eq. 21. TEAM lanes share a body: each stamps a
+// strided share of its items and the team tree-reduces; TEAM = 1 with EXACT
+// stamps in the reference's order onto the inertial terms.
+template <class R, bool EXACT, int TEAM>
+__global__ void __launch_bounds__(256) k_primal(int cnt, const int *bodies, SolveArgs S, ManBuf<R> mb, R *DQ,
+                                                StepParams P, double alpha) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int team = gt / TEAM, lane = gt % TEAM;
+  const bool active = team < cnt;
+  const int i = active ? bodies[team] : 0;
+  const BodyProp &bp = S.props[i];
+  double *x = S.X + size_t(i) * XS;
+  const double *y = S.Y + size_t(i) * XS;
+  const double dt2 = P.dt * P.dt;
+  Sys<double> s;
+  s.zero();
+  double Ml[9], Ma[9];
+  if (EXACT && active) {
+    // inertial terms first, as the reference assembles them
+    double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9];
+    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+    for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
+    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
+    mulv(s.rl, Ml, t);
+    qsub(dq, x + 3, y + 3);
+    mulv(s.ra, Ma, dq);
+    s.lin[0] = Ml[0]; s.lin[1] = Ml[3]; s.lin[2] = Ml[4]; s.lin[3] = Ml[6]; s.lin[4] = Ml[7]; s.lin[5] = Ml[8];
+    s.ang[0] = Ma[0]; s.ang[1] = Ma[3]; s.ang[2] = Ma[4]; s.ang[3] = Ma[6]; s.ang[4] = Ma[7]; s.ang[5] = Ma[8];
+    for (int k = 0; k < 9; k++) s.cross[k] = 0.0;
+  }
+  if (active) {
+    const int b0 = S.itemStart[i], b1 = S.itemStart[i + 1];
+    const R ralpha = R(alpha);
+    for (int t = b0 + lane; t < b1; t += TEAM) {
+      const uint32_t it = S.items[t];
+      const bool isA = it & 1u;
+      if (it & 0x80000000u) {
+        const int j = int((it & 0x7fffffffu) >> 1);
+        jointStamp(s, S.joints[j], isA, S.X, alpha);
+        continue;
+      }
+      const int m = int(it >> 4), sl = int((it >> 1) & 7u);
+      const int4 h = mb.head[m];
+      R bas[9], negb[9], dqA[6], dqB[6], hot[15];
+      const R *bsrc = mb.basis + size_t(m) * BAS;
+      loadRow(bas, bsrc, 9);
+      const R mu = bsrc[9];
+      for (int k = 0; k < 9; k++) negb[k] = -bas[k];
+      loadRow(dqA, DQ + size_t(h.x) * 8, 6);
+      loadRow(dqB, DQ + size_t(h.y) * 8, 6);
+      loadRow(hot, mb.hot + (size_t(m) * MAXC + sl) * HOT, 15);
+      Eval<R> e;
+      evalContact<R, EXACT>(bas, negb, mu, hot, dqA, dqB, ralpha, e);
+      stampContact<R, double>(s, isA ? bas : negb, isA ? e.jA : e.jB, hot + 3, e.F);
+    }
+  }
+  if (TEAM > 1) {
+#pragma unroll
+    for (int off = TEAM / 2; off > 0; off >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        s.lin[k] += __shfl_xor(s.lin[k], off, TEAM);
+        s.ang[k] += __shfl_xor(s.ang[k], off, TEAM);
+      }
+#pragma unroll
+      for (int k = 0; k < 9; k++) s.cross[k] += __shfl_xor(s.cross[k], off, TEAM);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        s.rl[k] += __shfl_xor(s.rl[k], off, TEAM);
+        s.ra[k] += __shfl_xor(s.ra[k], off, TEAM);
+      }
+    }
+  }
+  if (!active || lane != 0) return;
+  if (!EXACT) {
+    double MAng[9];
+    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+    const double ml = bp.mass / dt2;
+    double Ma9[9];
+    for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] / dt2;
+    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], ra[3];
+    qsub(dq, x + 3, y + 3);
+    mulv(ra, Ma9, dq);
+    s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
+    s.ang[0] += Ma9[0]; s.ang[1] += Ma9[3]; s.ang[2] += Ma9[4];
+    s.ang[3] += Ma9[6]; s.ang[4] += Ma9[7]; s.ang[5] += Ma9[8];
+    for (int k = 0; k < 3; k++) { s.rl[k] += ml * t[k]; s.ra[k] += ra[k]; }
+  }
+  double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
+  double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
+  double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
+  double dxl[3], dxa[3];
+  solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+  for (int k = 0; k < 3; k++) x[k] = x[k] + dxl[k];
+  double q[4];
+  qint(q, x + 3, dxa);
+  for (int k = 0; k < 4; k++) x[3 + k] = q[k];
+  const double *x0 = S.X0 + size_t(i) * XS;
+  double dq[3];
+  qsub(dq, x + 3, x0 + 3);
+  R *d = DQ + size_t(i) * 8;
+  d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
+  d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
+}
+
+// Dual update of every live manifold (manifold.js:447-491).
+template <class R, bool EXACT>
+__global__ void k_dual(int nLive, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
+                       int writeC) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nLive) return;
+  const int m = live[k];
+  const int4 h = mb.head[m];
+  R bas[9], negb[9], dqA[6], dqB[6];
+  const R *bsrc = mb.basis + size_t(m) * BAS;
+  loadRow(bas, bsrc, 9);
+  const R mu = bsrc[9];
+  for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+  loadRow(dqA, DQ + size_t(h.x) * 8, 6);
+  loadRow(dqB, DQ + size_t(h.y) * 8, 6);
+  const R ralpha = R(alpha), beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
+  const bool hasSphere = h.w & MF_SPHERE;
+  for (int sl = 0; sl < h.z; sl++) {
+    const size_t si = size_t(m) * MAXC + sl;
+    R *hp = mb.hot + si * HOT;
+    R hot[15];
+    loadRow(hot, hp, 15);
+    Eval<R> e;
+    evalContact<R, EXACT>(bas, negb, mu, hot, dqA, dqB, ralpha, e);
+    hp[6] = e.F[0]; hp[7] = e.F[1]; hp[8] = e.F[2];
+    if (writeC) {
+      R *cd = mb.Cd + si * 4;
+      cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];
+    }
+    if (e.F[0] < R(0)) hp[3] = jmin(hot[3] + beta * xabs(e.C[0]), pmax);
+    if (e.within) {
+      hp[4] = jmin(hot[4] + beta * xabs(e.C[1]), pmax);
+      hp[5] = jmin(hot[5] + beta * xabs(e.C[2]), pmax);
+      mb.stick[si] = xsqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < stickT ? 1 : 0;
+    } else if (hasSphere) {
+      mb.stick[si] = 0;
+    }
+  }
+}
+
+// BDF1 velocities (solver.js:476-488).
+__global__ void k_velocity(int n, const BodyProp *props, const double *X, const double *X0, double *V, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double *v = V + size_t(i) * VS;
+  v[6] = v[0]; v[7] = v[1]; v[8] = v[2];
+  if (props[i].mass > 0) {
+    const double *x = X + size_t(i) * XS, *x0 = X0 + size_t(i) * XS;
+    for (int k = 0; k < 3; k++) v[k] = (x[k] - x0[k]) / dt;
+    double w[3];
+    qsub(w, x + 3, x0 + 3);
+    for (int k = 0; k < 3; k++) v[3 + k] = w[k] / dt;
+  }
+}
+
+}  // namespace avbd

@@ -1,0 +1,189 @@
+/*
+ * avbd.h — C ABI of the MI355X (gfx950) AVBD rigid-body step.
+ *
+ * This library is a third backend beside the reference sandbox's f64 CPU
+ * engine (src/physics/solver.js) and its WebGPU backend
+ * (src/physics/gpu/backend.js, class GpuBackend). Each entry point below names
+ * the GpuBackend member it stands in for. The Node-API addon
+ * (avbd-sandbox_amd/js/) and the Python binding (avbd-sandbox_amd/avbd/) both
+ * call exactly these functions.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative AVBD_E_* code on failure;
+ *    avbd_last_error() then holds a message. No exception crosses the ABI.
+ *  - Host arrays are caller-owned and copied in or out; device memory is owned
+ *    by the context. One context per host thread; calls are not re-entrant.
+ *  - Units and frames follow the reference: z-up, metres, kilograms, seconds,
+ *    quaternions as x, y, z, w (scalar last), 3x3 matrices row-major.
+ *  - avbd_step() only enqueues work on the context's HIP stream (the
+ *    reference's step() never waits either, backend.js:1606-1611); the read
+ *    functions synchronise.
+ */
+#ifndef AVBD_H
+#define AVBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_ABI_VERSION 1
+
+/* Error codes. */
+#define AVBD_OK 0
+#define AVBD_E_INVALID (-1)   /* bad argument                               */
+#define AVBD_E_HIP (-2)       /* a HIP runtime call failed                  */
+#define AVBD_E_CAPACITY (-3)  /* a capacity was exceeded (see counters)     */
+#define AVBD_E_STATE (-4)     /* call not valid in the context's state      */
+
+/* Precision of the step.
+ *  EXACT: every quantity in f64, each body's system assembled sequentially in
+ *         the reference's force order: results equal the f64 CPU engine's.
+ *  F64:   f64 throughout, lanes of a team assemble a body's system in
+ *         parallel (tree-reduced sums).
+ *  MIXED: contact records and contact arithmetic in f32, body state, world
+ *         positions and the 6x6 LDL^T in f64. The throughput mode. */
+#define AVBD_PREC_EXACT 0
+#define AVBD_PREC_F64 1
+#define AVBD_PREC_MIXED 2
+
+/* Body shapes (rigid.js:34 `shape`). */
+#define AVBD_SHAPE_BOX 0
+#define AVBD_SHAPE_SPHERE 1
+
+typedef struct avbd_ctx avbd_ctx;
+
+/* Context configuration (stands in for GpuBackend's constructor and
+ * capacity plan, backend.js:243-335 and _allocate 499-659). Zero fields take
+ * defaults. */
+typedef struct avbd_config {
+  int32_t device;          /* HIP device ordinal                            */
+  int32_t precision;       /* AVBD_PREC_*                                   */
+  int32_t team;            /* lanes per body in the primal solve (0: auto)  */
+  int32_t manifold_cap;    /* max live + new manifolds (0: 32 per body)     */
+  int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body)        */
+  int32_t reserved[3];
+} avbd_config;
+
+/* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform
+ * block GU_FIELDS, layout.js:223-252). */
+typedef struct avbd_params {
+  double dt;               /* 1/60                                          */
+  double gravity;          /* along z; -10 in the reference default         */
+  double beta_lin;         /* 1e4                                           */
+  double beta_ang;         /* 100                                           */
+  double alpha;            /* 0.95                                          */
+  double gamma;            /* 0.999                                         */
+  double penalty_min;      /* 1                                             */
+  double penalty_max;      /* 1e10                                          */
+  double collision_margin; /* 0.01                                          */
+  double stick_threshold;  /* 1e-5                                          */
+  int32_t iterations;      /* 10                                            */
+  int32_t post_stabilize;  /* 0                                             */
+  int32_t rotated_inertia; /* 1                                             */
+  int32_t cached_contact_jacobians; /* 1 (the only mode the GPU implements) */
+} avbd_params;
+
+/* Fills *p with the reference defaults (solver.js:92-236). */
+void avbd_default_params(avbd_params *p);
+
+/* Scene, structure of arrays (stands in for the arenas GpuBackend.pack builds
+ * from the Solver scene graph, backend.js:662-1301; the record layout of
+ * test/parity/reference_main.cpp:45-130). Bodies are numbered in creation
+ * order; the step's Gauss-Seidel schedule depends on it. */
+typedef struct avbd_scene {
+  int32_t num_bodies;
+  const int32_t *shape;       /* [n] AVBD_SHAPE_*                            */
+  const double *size;         /* [n*3] full widths; spheres: diameter x3     */
+  const double *density;      /* [n] 0 = static                             */
+  const double *friction;     /* [n]                                        */
+  const double *pos;          /* [n*3]                                      */
+  const double *quat;         /* [n*4] xyzw, or NULL for identity           */
+  const double *vel;          /* [n*3] or NULL                              */
+  const double *angvel;       /* [n*3] or NULL                              */
+  const int32_t *scene_of;    /* [n] batch scene id (bodies of different
+                                 scenes never interact), or NULL: one scene */
+  int32_t num_joints;         /* joint.js:78-113                            */
+  const int32_t *joint_a;     /* [J] body index, -1 = world anchor          */
+  const int32_t *joint_b;     /* [J]                                        */
+  const double *joint_ra;     /* [J*3] anchor in A (world point if a = -1)  */
+  const double *joint_rb;     /* [J*3] anchor in B                          */
+  const double *joint_stiff_lin; /* [J] INFINITY = hard                     */
+  const double *joint_stiff_ang; /* [J] 0 = no angular rows                 */
+  const double *joint_fracture;  /* [J] INFINITY = unbreakable              */
+} avbd_scene;
+
+/* Step counters (layout.js:171-178 CTR_*; backend.js:1622-1792 lastStats). */
+typedef struct avbd_counters {
+  int64_t steps;
+  int32_t bodies, dynamic_bodies;
+  int32_t pairs;           /* broad-phase pairs of the last step           */
+  int32_t new_pairs;       /* of which created a manifold                   */
+  int32_t manifolds;       /* live manifolds after force initialisation     */
+  int32_t contacts;        /* contacts in them                              */
+  int32_t joints;          /* live joints                                   */
+  int32_t colors;          /* colours of the last step                      */
+  int32_t color_rounds;    /* fixed-point rounds the colouring took         */
+  int32_t overflow;        /* bit 1 pairs, 2 manifolds (layout.js:174)      */
+} avbd_counters;
+
+/* --- lifecycle: GpuBackend.init / initDevice / initPipelines (backend.js:337-469) */
+int avbd_create(avbd_ctx **out, const avbd_config *cfg);
+int avbd_destroy(avbd_ctx *ctx);
+const char *avbd_last_error(const avbd_ctx *ctx);
+int avbd_abi_version(void);
+
+/* --- topology: GpuBackend.pack / flushTopology (backend.js:662-1301, 1898-1915).
+ * Replaces the whole scene; warm-start state starts empty, as for a fresh
+ * Solver. */
+int avbd_set_params(avbd_ctx *ctx, const avbd_params *p);
+int avbd_pack(avbd_ctx *ctx, const avbd_scene *scene);
+
+/* --- stepping: GpuBackend.step (backend.js:1393-1620); avbd_sync is idle()
+ * (backend.js:1804-1818). */
+int avbd_step(avbd_ctx *ctx, int32_t nsteps);
+int avbd_sync(avbd_ctx *ctx);
+
+/* --- state: GpuBackend.readState / syncToCPU (backend.js:2154-2192, 1984-2039).
+ * Any pointer may be NULL. n must equal the packed body count. */
+int avbd_read_state(avbd_ctx *ctx, double *pos, double *quat, double *vel, double *angvel,
+                    double *prev_vel, int32_t n);
+/* Overwrites body state (the device-authoritative half of syncToCPU's
+ * inverse, backend.js:662-736). */
+int avbd_write_state(avbd_ctx *ctx, const double *pos, const double *quat, const double *vel,
+                     const double *angvel, const double *prev_vel, int32_t n);
+
+/* --- verification: GpuBackend.readColoring (backend.js:2053-2151).
+ * Broad-phase pairs of the last step as (A, B) index pairs, A > B, in the
+ * order A descending then B descending (broadphase.js:236-243). */
+int avbd_read_pairs(avbd_ctx *ctx, int32_t *pairs, int32_t cap, int32_t *count);
+/* Colouring used by the last step: colour per body (-1 static), and the
+ * colour count (coloring.js:43-118). */
+int avbd_read_coloring(avbd_ctx *ctx, int32_t *color_of, int32_t n, int32_t *num_colors);
+/* Live force list in the order the step holds it (solver.js forces[]):
+ * per force kind (0 manifold, 1 joint), body A (-1 world), body B and contact
+ * count; `features` receives 8 words per force (contact feature keys,
+ * collide.js:254-285). Either output may be NULL. */
+int avbd_read_forces(avbd_ctx *ctx, int32_t *kind_a_b_nc, uint32_t *features, int32_t cap,
+                     int32_t *count);
+/* Contact records of force `index` of that list: per contact 16 doubles
+ * rA[3] rB[3] C0[3] penalty[3] lambda[3] stick (manifold.js:34-70), and the
+ * basis (9 doubles, row 0 the normal, B to A) and friction coefficient. */
+int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *contacts,
+                       int32_t *num_contacts);
+
+/* --- metrics: GpuBackend.lastStats (backend.js:309-311, 1622-1792) */
+int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out);
+
+/* --- profiling: GpuBackend.setProfiling / lastProfile (backend.js:1836-1891).
+ * When on, each step records HIP-event times of its phases; ms receives
+ * AVBD_NUM_PHASES averages over the steps since profiling was turned on. */
+#define AVBD_NUM_PHASES 6 /* broad, narrow, topology, colour, solve, total */
+int avbd_set_profiling(avbd_ctx *ctx, int32_t on);
+int avbd_read_profile(avbd_ctx *ctx, float *ms, int32_t n, int32_t *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AVBD_H */
