@@ -20,8 +20,9 @@ SYMBOLS = (
     "avbd_abi_version", "avbd_default_params", "avbd_create", "avbd_destroy", "avbd_last_error",
     "avbd_set_params", "avbd_pack", "avbd_step", "avbd_sync", "avbd_read_state", "avbd_write_state",
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
-    "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile",
+    "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
 )
+K_PRIMAL, K_DUAL, K_NARROW = 0, 1, 2
 
 P = C.POINTER
 d, i32, u32, vp = C.c_double, C.c_int32, C.c_uint32, C.c_void_p
@@ -85,6 +86,7 @@ def lib(path=None):
         "avbd_read_counters": (C.c_int, [ctx, P(Counters)]),
         "avbd_set_profiling": (C.c_int, [ctx, i32]),
         "avbd_read_profile": (C.c_int, [ctx, P(C.c_float), i32, pi]),
+        "avbd_read_kernel_stats": (C.c_int, [ctx, i32, pd, P(C.c_int64), pd]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -201,3 +201,9 @@ class HipBackend:
         steps = C.c_int32()
         self._ck(self.L.avbd_read_profile(self.ctx, ms, L.NUM_PHASES, C.byref(steps)))
         return dict(zip(L.PHASES, [float(x) for x in ms])), steps.value
+
+    def kernel_stats(self, kernel):
+        """(total ms, launches, algorithmic bytes) of a kernel while profiling."""
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        self._ck(self.L.avbd_read_kernel_stats(self.ctx, int(kernel), C.byref(ms), C.byref(n), C.byref(b)))
+        return ms.value, n.value, b.value

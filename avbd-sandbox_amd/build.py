@@ -18,6 +18,26 @@ FLAGS = [
 ]
 
 
+ADDON_SRC = os.path.join(HERE, "js", "avbd_addon.c")
+ADDON_OUT = os.path.join(HERE, "js", "avbd.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_addon():
+    """Node-API addon for the JavaScript host (needs the Node headers)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    cmd = ["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", "-I", NODE_INCLUDE,
+           "-o", ADDON_OUT + ".tmp", ADDON_SRC, "-L", HERE, "-l:libavbd_hip.so",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("addon build failed")
+    os.replace(ADDON_OUT + ".tmp", ADDON_OUT)
+    return ADDON_OUT
+
+
 def build(verbose=False):
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp", SRC]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -27,6 +47,7 @@ def build(verbose=False):
     if verbose:
         sys.stderr.write(r.stderr)
     os.replace(OUT + ".tmp", OUT)
+    build_addon()
     return OUT
 
 

@@ -146,11 +146,72 @@ struct avbd_ctx {
   std::vector<hipEvent_t> evs;
   double profMs[AVBD_NUM_PHASES] = {0};
   int profSteps = 0;
+  // per-kernel event pairs awaiting harvest, and the totals
+  struct Pending { int kid; hipEvent_t a, b; double bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> evPool;
+  double kMs[AVBD_NUM_KERNELS] = {0}, kBytes[AVBD_NUM_KERNELS] = {0};
+  int64_t kLaunches[AVBD_NUM_KERNELS] = {0};
+  DBuf<unsigned long long> cbytes;
+  std::vector<unsigned long long> hcbytes;
+  int contactsTotal = 0;
+
+  hipEvent_t takeEvent() {
+    if (evPool.empty()) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = evPool.back();
+    evPool.pop_back();
+    return e;
+  }
+  // Bracket the launches issued by f with events when profiling.
+  template <class F> void timed(int kid, double bytes, F &&f) {
+    if (!profiling) { f(); return; }
+    Pending p{kid, takeEvent(), takeEvent(), bytes};
+    HIPCHECK(hipEventRecord(p.a, stream));
+    f();
+    HIPCHECK(hipEventRecord(p.b, stream));
+    pending.push_back(p);
+  }
+  void harvest() {
+    for (auto &p : pending) {
+      HIPCHECK(hipEventSynchronize(p.b));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+      kMs[p.kid] += ms;
+      kBytes[p.kid] += p.bytes;
+      kLaunches[p.kid]++;
+      evPool.push_back(p.a);
+      evPool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  // Algorithmic bytes per unit of work for the record layout in use.
+  ByteModel primalModel() const {
+    const unsigned long long rs = rsize();
+    ByteModel m;
+    m.own = 7 * 8 * 3 + 40 + 8 + 7 * 8 + 6 * rs;  // X, Y, X0, props, itemStart; X and delta written
+    m.man = 16 + 10 * rs + 6 * rs;                 // head, basis + mu, other body's delta
+    m.item = 4 + 15 * rs;                          // item word, hot contact record
+    m.joint = sizeof(JointDev) + 2 * 7 * 8 + 4;    // joint record, both bodies' poses
+    return m;
+  }
+  double dualBytes() const {
+    const double rs = double(rsize());
+    return double(nLive) * (4 + 16 + 10 * rs + 12 * rs) + double(contactsTotal) * (15 * rs + 6 * rs + 4);
+  }
 
   ~avbd_ctx() { releaseAll(); }
 
   void releaseAll() {
     props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release();
+    cbytes.release();
+    for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    pending.clear();
+    for (auto &e : evPool) (void)hipEventDestroy(e);
+    evPool.clear();
     cellOf.release(); bucketCnt.release(); bucketStart.release(); bucketCursor.release(); entries.release();
     sceneLargeStart.release(); sceneLarges.release(); sceneStart.release(); largeList.release();
     countRev.release(); startRev.release(); pairNew.release(); newPos.release(); pairs.release(); newPairs.release();
@@ -285,8 +346,10 @@ struct avbd_ctx {
     const int cnt = hcstart[c + 1] - hcstart[c];
     if (cnt <= 0) return;
     const long threads = long(cnt) * TEAM;
-    hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(blocks(threads)), dim3(256), 0, stream, cnt,
-                       cbodies.p + hcstart[c], sa, mb, (R *)DQ.p, P, alpha);
+    timed(AVBD_K_PRIMAL, profiling ? double(hcbytes[c]) : 0.0, [&] {
+      hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(blocks(threads)), dim3(256), 0, stream, cnt,
+                         cbodies.p + hcstart[c], sa, mb, (R *)DQ.p, P, alpha);
+    });
   }
 
   template <class R, bool EXACT>
@@ -349,16 +412,20 @@ struct avbd_ctx {
       throw ApiError(AVBD_E_CAPACITY, "manifolds " + std::to_string(nTot) + " exceed capacity " +
                                           std::to_string(manCap));
     ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
+    HIPCHECK(hipMemsetAsync(scal.p + 8, 0, sizeof(int), stream));
     if (nTot > 0)
-      hipLaunchKernelGGL(k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), 0, stream, nOld, nNew, live[cur].p, ob, nb,
-                         newPairs.p, props.p, X.p, P, keep.p);
+      timed(AVBD_K_NARROW, 0.0, [&] {
+        hipLaunchKernelGGL(k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), 0, stream, nOld, nNew, live[cur].p, ob, nb,
+                           newPairs.p, props.p, X.p, P, keep.p, scal.p + 8);
+      });
     if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
     scan(keep.p, keepPos.p, nTot, scal.p + 2);
     if (nTot > 0)
       hipLaunchKernelGGL(k_live_compact, dim3(blocks(nTot)), dim3(256), 0, stream, nTot, keep.p, keepPos.p,
                          live[1 - cur].p);
-    readScal(2, 1);
+    readScal(2, 7);
     nLive = hscal[2];
+    contactsTotal = hscal[8];
     cur = 1 - cur;
     record(2);
 
@@ -367,6 +434,15 @@ struct avbd_ctx {
     record(3);
     colorize();
     buildItems();
+    if (profiling && numColors > 0) {
+      HIPCHECK(hipMemsetAsync(cbytes.p, 0, sizeof(unsigned long long) * MAX_COLORS, stream));
+      hipLaunchKernelGGL(k_color_bytes, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, color[colorCur].p,
+                         adjStart.p, adjKey.p, itemCnt.p, primalModel(), cbytes.p);
+      hcbytes.resize(MAX_COLORS);
+      HIPCHECK(hipMemcpyAsync(hcbytes.data(), cbytes.p, sizeof(unsigned long long) * MAX_COLORS,
+                              hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
     record(4);
 
     // 4. body initialisation and the iterations
@@ -382,8 +458,10 @@ struct avbd_ctx {
       for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha);
       if (it < prm.iterations) {
         if (nLive > 0)
-          hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(blocks(nLive, 128)), dim3(128), 0, stream, nLive, live[cur].p,
-                             mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
+          timed(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, [&] {
+            hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(blocks(nLive, 128)), dim3(128), 0, stream, nLive,
+                               live[cur].p, mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
+          });
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
       if (it == prm.iterations - 1)
@@ -404,6 +482,7 @@ struct avbd_ctx {
       HIPCHECK(hipEventElapsedTime(&ms, evs[0], evs[5]));
       profMs[5] += ms;
       profSteps++;
+      harvest();
     }
   }
 
@@ -599,6 +678,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.cbodies.reserve(N);
   c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.items.reserve(size_t(N) * 16);
   c.scal.reserve(16);
+  c.cbytes.reserve(MAX_COLORS);
   HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
   c.evs.resize(AVBD_NUM_PHASES);
   for (auto &e : c.evs) HIPCHECK(hipEventCreate(&e));
@@ -897,9 +977,23 @@ int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out) {
 
 int avbd_set_profiling(avbd_ctx *ctx, int32_t on) {
   return guarded(ctx, [&] {
+    if (ctx->stream) HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->harvest();
     ctx->profiling = on != 0;
     for (double &m : ctx->profMs) m = 0;
     ctx->profSteps = 0;
+    for (int k = 0; k < AVBD_NUM_KERNELS; k++) ctx->kMs[k] = ctx->kBytes[k] = 0, ctx->kLaunches[k] = 0;
+  });
+}
+
+int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int64_t *launches, double *bytes_total) {
+  return guarded(ctx, [&] {
+    if (kernel < 0 || kernel >= AVBD_NUM_KERNELS) throw ApiError(AVBD_E_INVALID, "unknown kernel id");
+    if (ctx->stream) HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->harvest();
+    if (ms_total) *ms_total = ctx->kMs[kernel];
+    if (launches) *launches = ctx->kLaunches[kernel];
+    if (bytes_total) *bytes_total = ctx->kBytes[kernel];
   });
 }
 

@@ -267,7 +267,7 @@ __global__ void k_new_pairs(int np, const int2 *pairs, const int *pairNew, const
 // and every new pair, written to slot t of the other buffer.
 template <class R>
 __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
-                         const BodyProp *props, const double *X, StepParams P, int *keep) {
+                         const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nOld + nNew) return;
   int a, b, m = -1, ncOld = 0;
@@ -380,6 +380,7 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
   nb.basis[size_t(t) * BAS + 10] = R(0);
   nb.basis[size_t(t) * BAS + 11] = R(0);
   keep[t] = n > 0 ? 1 : 0;
+  if (n > 0) atomicAdd(contactTotal, n);
 }
 
 // ----------------------------------------------------------------- joints
@@ -667,6 +668,20 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
     }
   }
   if (PASS == 0) itemCnt[i] = cnt;
+}
+
+// Algorithmic bytes of one primal visit of each dynamic body, summed per
+// colour: own records, plus per manifold touched (head, basis, the other
+// body's delta record) and per contact (item word, hot record).
+struct ByteModel { unsigned long long own, man, item, joint; };
+__global__ void k_color_bytes(int n, int J, const BodyProp *props, const int *color, const int *adjStart,
+                              const int *adjKey, const int *itemCnt, ByteModel bm, unsigned long long *cbytes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(props[i].mass > 0)) return;
+  int nm = 0, nj = 0;
+  for (int e = adjStart[i]; e < adjStart[i + 1]; e++) (adjKey[e] < J ? nj : nm)++;
+  const unsigned long long b = bm.own + bm.man * nm + bm.item * (itemCnt[i] - nj) + bm.joint * nj;
+  atomicAdd(&cbytes[color[i]], b);
 }
 
 // ------------------------------------------------------------------ solve

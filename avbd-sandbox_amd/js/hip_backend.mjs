@@ -1,0 +1,173 @@
+// HipBackend — the MI355X backend for a JavaScript host, with the method
+// surface of the reference's GpuBackend (src/physics/gpu/backend.js:243-2226)
+// so a host can switch backends at the same call sites (boot.js:2145-2156
+// doStep, 1774-1813 backend switch). Compute runs in libavbd_hip.so through
+// the Node-API addon avbd.node; nothing here falls back to the CPU.
+import { createRequire } from 'module';
+import { dirname, join } from 'path';
+import { fileURLToPath } from 'url';
+
+const here = dirname(fileURLToPath(import.meta.url));
+const require = createRequire(import.meta.url);
+const addon = require(join(here, 'avbd.node'));
+
+export const PRECISION = { exact: 0, f64: 1, mixed: 2 };
+const PHASE_NAMES = ['broad', 'narrow', 'topology', 'colour', 'solve', 'total'];
+
+export class HipBackend {
+  constructor({ precision = 'mixed', team = 0, device = 0 } = {}) {
+    this.precision = precision;
+    this.team = team;
+    this.device = device;
+    this.ctx = null;
+    this.ready = false;
+    this.adapterInfo = null;
+    this.gpuErrors = [];
+    this.topologyDirty = true;
+    this.bodyIndex = new Map();
+    this.numBodies = 0;
+    this.lastProfile = null;
+    this._profiling = false;
+    this._queued = 0;
+  }
+
+  // --- lifecycle (backend.js:337-469)
+  async init() {
+    this.ctx = addon.create({ device: this.device, precision: PRECISION[this.precision], team: this.team });
+    this.adapterInfo = 'AMD Instinct MI355X (gfx950, HIP)';
+    this.ready = true;
+    return this;
+  }
+
+  // --- topology (backend.js:662-1301, 1898-1915)
+  pack(solver) {
+    const bodies = solver.bodies;
+    const n = bodies.length;
+    const A = {
+      shape: new Int32Array(n), size: new Float64Array(n * 3), density: new Float64Array(n),
+      friction: new Float64Array(n), pos: new Float64Array(n * 3), quat: new Float64Array(n * 4),
+      vel: new Float64Array(n * 3), angvel: new Float64Array(n * 3),
+    };
+    this.bodyIndex = new Map();
+    bodies.forEach((b, i) => {
+      this.bodyIndex.set(b, i);
+      A.shape[i] = b.shape === 'sphere' ? 1 : 0;
+      A.size.set(b.size, i * 3);
+      A.density[i] = b.density;
+      A.friction[i] = b.friction;
+      A.pos.set(b.positionLin, i * 3);
+      A.quat.set(b.positionAng, i * 4);
+      A.vel.set(b.velocityLin, i * 3);
+      A.angvel.set(b.velocityAng, i * 3);
+    });
+    const joints = solver.forces.filter((f) => f.rA && f.rB && 'stiffnessLin' in f);
+    const m = joints.length;
+    if (m) {
+      A.jointA = new Int32Array(m);
+      A.jointB = new Int32Array(m);
+      A.jointRA = new Float64Array(m * 3);
+      A.jointRB = new Float64Array(m * 3);
+      A.jointStiffLin = new Float64Array(m);
+      A.jointStiffAng = new Float64Array(m);
+      A.jointFracture = new Float64Array(m);
+      joints.forEach((j, k) => {
+        A.jointA[k] = j.bodyA ? this.bodyIndex.get(j.bodyA) : -1;
+        A.jointB[k] = this.bodyIndex.get(j.bodyB);
+        A.jointRA.set(j.rA, k * 3);
+        A.jointRB.set(j.rB, k * 3);
+        A.jointStiffLin[k] = j.stiffnessLin;
+        A.jointStiffAng[k] = j.stiffnessAng;
+        A.jointFracture[k] = j.fracture;
+      });
+    }
+    addon.setParams(this.ctx, {
+      dt: solver.dt, gravity: solver.gravity, iterations: solver.iterations, betaLin: solver.betaLin,
+      betaAng: solver.betaAng, alpha: solver.alpha, gamma: solver.gamma, penaltyMin: solver.penaltyMin,
+      penaltyMax: solver.penaltyMax, collisionMargin: solver.collisionMargin,
+      stickThreshold: solver.stickThreshold, postStabilize: solver.postStabilize ? 1 : 0,
+      rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
+    });
+    addon.pack(this.ctx, A);
+    this.numBodies = n;
+    this.topologyDirty = false;
+  }
+
+  prepareTopologyChange() {
+    this.topologyDirty = true;
+  }
+
+  flushTopology(solver) {
+    if (this.topologyDirty) this.pack(solver);
+  }
+
+  // --- stepping (backend.js:1393-1620); returns the host encode time in ms
+  step(solver) {
+    const t0 = Date.now();
+    this.flushTopology(solver);
+    addon.step(this.ctx, 1);
+    this._queued++;
+    if (this._profiling) this._readProfile();
+    return Date.now() - t0;
+  }
+
+  get queuedSteps() {
+    return this._queued;
+  }
+
+  async idle() {
+    await addon.sync(this.ctx);
+    this._queued = 0;
+  }
+
+  // --- state (backend.js:2154-2192, 1984-2039)
+  async readState() {
+    await this.idle();
+    const n = this.numBodies;
+    const data = {
+      pos: new Float64Array(n * 3), quat: new Float64Array(n * 4), vel: new Float64Array(n * 3),
+      angvel: new Float64Array(n * 3), prevVel: new Float64Array(n * 3),
+    };
+    addon.readState(this.ctx, data.pos, data.quat, data.vel, data.angvel, data.prevVel);
+    return { data, numBodies: n, counters: addon.readCounters(this.ctx) };
+  }
+
+  async syncToCPU(solver) {
+    const { data } = await this.readState();
+    solver.bodies.forEach((b, i) => {
+      b.positionLin.set(data.pos.subarray(i * 3, i * 3 + 3));
+      b.positionAng.set(data.quat.subarray(i * 4, i * 4 + 4));
+      b.velocityLin.set(data.vel.subarray(i * 3, i * 3 + 3));
+      b.velocityAng.set(data.angvel.subarray(i * 3, i * 3 + 3));
+      b.prevVelocityLin.set(data.prevVel.subarray(i * 3, i * 3 + 3));
+    });
+  }
+
+  // --- verification (backend.js:2053-2151)
+  async readColoring() {
+    await this.idle();
+    const colorOf = new Int32Array(this.numBodies);
+    const numColors = addon.readColoring(this.ctx, colorOf);
+    const count = addon.readPairs(this.ctx, null);
+    const contactPairs = new Int32Array(count * 2);
+    addon.readPairs(this.ctx, contactPairs);
+    return { colorOf, numColors, contactPairs };
+  }
+
+  // --- metrics and profiling (backend.js:309-324, 1836-1891)
+  get lastStats() {
+    return this.ctx ? addon.readCounters(this.ctx) : null;
+  }
+
+  setProfiling(on) {
+    addon.setProfiling(this.ctx, !!on);
+    this._profiling = !!on;
+    return true;
+  }
+
+  _readProfile() {
+    const ms = addon.readProfile(this.ctx);
+    this.lastProfile = Object.fromEntries(PHASE_NAMES.map((k, i) => [k, ms[i]]));
+  }
+}
+
+export const abiVersion = () => addon.abiVersion();

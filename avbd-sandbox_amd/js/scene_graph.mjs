@@ -1,0 +1,100 @@
+// Minimal scene graph with the reference's Solver / Rigid / Joint shapes
+// (src/physics/solver.js:59-236, rigid.js:29-133, joint.js:78-113), so that a
+// host written against the reference can hand its solver to HipBackend.pack()
+// and .step() unchanged. Only the fields the backend reads are modelled; the
+// dynamics run on the GPU.
+
+export class Solver {
+  constructor() {
+    this.bodies = [];
+    this.forces = [];
+    this.defaultParams();
+  }
+
+  defaultParams() {
+    this.dt = 1.0 / 60.0;
+    this.gravity = -10.0;
+    this.iterations = 10;
+    this.betaLin = 10000.0;
+    this.betaAng = 100.0;
+    this.alpha = 0.95;
+    this.postStabilize = false;
+    this.gamma = 0.999;
+    this.penaltyMin = 1.0;
+    this.penaltyMax = 1.0e10;
+    this.collisionMargin = 0.01;
+    this.stickThreshold = 0.00001;
+    this.rotatedInertia = true;
+    this.cachedContactJacobians = true;
+  }
+}
+
+export class Rigid {
+  constructor(solver, size, density, friction, position, velocity = [0, 0, 0]) {
+    this.solver = solver;
+    this.shape = 'box';
+    this.forces = [];
+    this.size = Float64Array.from(size);
+    this.density = density;
+    this.friction = friction;
+    this.positionLin = Float64Array.from(position);
+    this.positionAng = Float64Array.from([0, 0, 0, 1]);
+    this.velocityLin = Float64Array.from(velocity);
+    this.velocityAng = new Float64Array(3);
+    this.prevVelocityLin = Float64Array.from(velocity);
+    this.mass = size[0] * size[1] * size[2] * density;
+    solver.bodies.push(this);
+  }
+
+  static sphere(solver, diameter, density, friction, position, velocity = [0, 0, 0]) {
+    if (!Number.isFinite(diameter) || diameter <= 0) {
+      throw new RangeError('sphere diameter must be a finite positive number');
+    }
+    const b = new Rigid(solver, [diameter, diameter, diameter], density, friction, position, velocity);
+    b.shape = 'sphere';
+    const r = diameter * 0.5;
+    b.mass = (4 / 3) * Math.PI * r * r * r * density;
+    return b;
+  }
+
+  get isStatic() {
+    return this.mass <= 0;
+  }
+}
+
+export class Joint {
+  constructor(solver, bodyA, bodyB, rA, rB, stiffnessLin = Infinity, stiffnessAng = 0, fracture = Infinity) {
+    this.solver = solver;
+    this.bodyA = bodyA;
+    this.bodyB = bodyB;
+    this.rA = Float64Array.from(rA);
+    this.rB = Float64Array.from(rB);
+    this.stiffnessLin = stiffnessLin;
+    this.stiffnessAng = stiffnessAng;
+    this.fracture = fracture;
+    this.broken = false;
+    solver.forces.push(this);
+    if (bodyA) bodyA.forces.push(this);
+    bodyB.forces.push(this);
+  }
+}
+
+/** Build a scene graph from the JSON scene description the Python side uses. */
+export function fromDescription(desc, params = {}) {
+  const solver = new Solver();
+  Object.assign(solver, params);
+  const num = (x) => (x === 'inf' ? Infinity : x === '-inf' ? -Infinity : x);
+  const bodies = desc.bodies.map((d) => {
+    const b = d.shape === 'sphere'
+      ? Rigid.sphere(solver, d.size[0], d.density, d.friction, d.pos, d.vel || [0, 0, 0])
+      : new Rigid(solver, d.size, d.density, d.friction, d.pos, d.vel || [0, 0, 0]);
+    if (d.quat) b.positionAng.set(d.quat);
+    if (d.angvel) b.velocityAng.set(d.angvel);
+    return b;
+  });
+  for (const j of desc.joints || []) {
+    new Joint(solver, j.a < 0 ? null : bodies[j.a], bodies[j.b], j.rA, j.rB, num(j.kLin), num(j.kAng),
+      num(j.fracture));
+  }
+  return solver;
+}

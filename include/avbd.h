@@ -183,6 +183,18 @@ int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out);
 int avbd_set_profiling(avbd_ctx *ctx, int32_t on);
 int avbd_read_profile(avbd_ctx *ctx, float *ms, int32_t n, int32_t *steps);
 
+/* Per-kernel timing while profiling is on (HIP events around each launch of
+ * the kernel on the context's stream), with the algorithmic bytes each launch
+ * moves: the unique body, manifold and contact records it must read or write
+ * under the library's data layout (per-unit figures documented with the
+ * layout). No counterpart in the reference beyond lastProfile. */
+#define AVBD_K_PRIMAL 0
+#define AVBD_K_DUAL 1
+#define AVBD_K_NARROW 2
+#define AVBD_NUM_KERNELS 3
+int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int64_t *launches,
+                           double *bytes_total);
+
 #ifdef __cplusplus
 }
 #endif

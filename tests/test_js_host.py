@@ -1,0 +1,64 @@
+"""The JavaScript host: Node-API addon + HipBackend (GpuBackend's surface).
+
+CPU: the addon loads under node and exports the binding; the ES module that
+wraps it imports. GPU: the JS host steps a scene in exact precision and the
+state it syncs back equals the oracle's bit for bit.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "avbd-sandbox_amd", "js", "avbd.node")
+NODE = shutil.which("node")
+
+needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+EXPORTS = ["abiVersion", "create", "setParams", "pack", "step", "sync", "readState", "writeState",
+           "readColoring", "readPairs", "readCounters", "setProfiling", "readProfile"]
+
+
+@needs_node
+def test_addon_loads_and_exports():
+    assert os.path.exists(ADDON), "avbd.node not built; run __graft_entry__.build()"
+    out = subprocess.run([NODE, "-e", f"const a=require({json.dumps(ADDON)});"
+                          "console.log(JSON.stringify({k:Object.keys(a),v:a.abiVersion()}))"],
+                         capture_output=True, text=True, check=True).stdout
+    d = json.loads(out)
+    assert d["v"] == 1
+    assert set(EXPORTS) <= set(d["k"])
+
+
+@needs_node
+def test_backend_module_imports():
+    js = ("import('./avbd-sandbox_amd/js/hip_backend.mjs').then(m => { const b = new m.HipBackend();"
+          " console.log(typeof b.pack, typeof b.step, typeof b.idle, typeof b.readState, typeof b.syncToCPU,"
+          " typeof b.readColoring, typeof b.setProfiling, m.abiVersion()); })")
+    r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["function"] * 7 + ["1"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_host_exact_matches_oracle(tmp_path):
+    import avbd
+    import pyoracle
+    scene, params, _ = avbd.scenes.stack10()
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": scene, "params": params}))
+    r = subprocess.run([NODE, "tests/js/run_host.mjs", str(path), "30", "exact"], capture_output=True, text=True,
+                       cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    ref.step(30)
+    g = np.array(d["state"])
+    o = ref.state()[:, :13]
+    assert np.array_equal(g, o), f"max |diff| {np.abs(g - o).max():.3e}"
+    col, k = ref.last_coloring()
+    assert d["numColors"] == k and d["colorOf"] == col.tolist()
