@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "libavbd_hip.so")
+# AVBD_LIB may name another build of the same library (for instance one built
+# with extra instrumentation); the default is the in-tree build.
+LIB_PATH = os.environ.get("AVBD_LIB") or os.path.join(PKG_ROOT, "libavbd_hip.so")
 
 AVBD_OK, AVBD_E_INVALID, AVBD_E_HIP, AVBD_E_CAPACITY, AVBD_E_STATE = 0, -1, -2, -3, -4
 PREC_EXACT, PREC_F64, PREC_MIXED = 0, 1, 2
@@ -38,7 +40,10 @@ class Params(C.Structure):
     _fields_ = [("dt", d), ("gravity", d), ("beta_lin", d), ("beta_ang", d), ("alpha", d), ("gamma", d),
                 ("penalty_min", d), ("penalty_max", d), ("collision_margin", d), ("stick_threshold", d),
                 ("iterations", i32), ("post_stabilize", i32), ("rotated_inertia", i32),
-                ("cached_contact_jacobians", i32)]
+                ("cached_contact_jacobians", i32), ("color_order", i32), ("reserved", i32)]
+
+
+ORDER_INDEX, ORDER_PRIORITY = 0, 1
 
 
 class Scene(C.Structure):

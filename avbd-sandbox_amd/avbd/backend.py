@@ -21,7 +21,7 @@ PARAM_NAMES = {
     "gamma": "gamma", "penaltyMin": "penalty_min", "penaltyMax": "penalty_max",
     "collisionMargin": "collision_margin", "stickThreshold": "stick_threshold", "iterations": "iterations",
     "postStabilize": "post_stabilize", "rotatedInertia": "rotated_inertia",
-    "cachedContactJacobians": "cached_contact_jacobians",
+    "cachedContactJacobians": "cached_contact_jacobians", "colorOrder": "color_order",
 }
 
 

@@ -110,7 +110,7 @@ struct avbd_ctx {
   DBuf<int4> cellOf;
   DBuf<int> bucketCnt, bucketStart, bucketCursor, entries;
   DBuf<int> sceneLargeStart, sceneLarges, sceneStart, largeList;
-  DBuf<int> countRev, startRev, pairNew, newPos;
+  DBuf<int> countRev, startRev, countNewRev, startNewRev;
   DBuf<int2> pairs, newPairs;
   int nPairs = 0, nNew = 0;
 
@@ -127,8 +127,9 @@ struct avbd_ctx {
   DBuf<int> deg, adjStart, adjCursor, adjKey, adjOther;
   DBuf<int> color[2];
   int colorCur = 0;
-  DBuf<int> ccount, cstart, ccursor, cbodies;
-  std::vector<int> hcstart;
+  DBuf<int> ccount, cstart, ccursor, cbodies, cmeta;
+  std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
+  int colorBatch = 16;  // relaxation rounds launched between convergence checks
   int numColors = 0, colorRounds = 0;
   DBuf<int> itemCnt, itemStart;
   DBuf<uint32_t> items;
@@ -214,7 +215,7 @@ struct avbd_ctx {
     evPool.clear();
     cellOf.release(); bucketCnt.release(); bucketStart.release(); bucketCursor.release(); entries.release();
     sceneLargeStart.release(); sceneLarges.release(); sceneStart.release(); largeList.release();
-    countRev.release(); startRev.release(); pairNew.release(); newPos.release(); pairs.release(); newPairs.release();
+    countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
     color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release();
@@ -270,6 +271,9 @@ struct avbd_ctx {
   }
 
   // ------------------------------------------------------------ adjacency
+  // Force lists (CSR) and per-body work lists. Their totals are bounded by
+  // 2 (manifolds + joints) and 2 (8 manifolds + joints), for which the
+  // buffers are sized at pack, so neither needs a host round trip.
   void buildAdjacency() {
     const int nf = J + nLive;
     HIPCHECK(hipMemsetAsync(deg.p, 0, sizeof(int) * N, stream));
@@ -278,11 +282,6 @@ struct avbd_ctx {
       hipLaunchKernelGGL(k_adj<0>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
                          deg.p, adjCursor.p, adjKey.p, adjOther.p);
     scan(deg.p, adjStart.p, N, adjStart.p + N);
-    HIPCHECK(hipMemcpyAsync(&hscal[7], adjStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    const int tot = hscal[7];
-    adjKey.reserve(size_t(tot) + 1);
-    adjOther.reserve(size_t(tot) + 1);
     HIPCHECK(hipMemcpyAsync(adjCursor.p, adjStart.p, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
     if (nf > 0)
       hipLaunchKernelGGL(k_adj<1>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
@@ -291,54 +290,48 @@ struct avbd_ctx {
                        adjOther.p);
   }
 
-  // ------------------------------------------------------------ colouring
-  void colorize() {
-    int rounds = 0;
-    const int maxRounds = N + 2;
-    for (;;) {
-      HIPCHECK(hipMemsetAsync(scal.p + 3, 0, sizeof(int), stream));
-      const int batch = 4;
-      for (int r = 0; r < batch; r++) {
-        hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjOther.p,
-                           color[colorCur].p, color[1 - colorCur].p, scal.p + 3, scal.p + 5);
-        colorCur = 1 - colorCur;
-        rounds++;
-        if (r + 1 < batch) HIPCHECK(hipMemsetAsync(scal.p + 3, 0, sizeof(int), stream));
-      }
-      readScal(3, 1);
-      if (hscal[3] == 0) break;
-      if (rounds > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
-    }
-    colorRounds = rounds;
-    // colour buckets
-    HIPCHECK(hipMemsetAsync(ccount.p, 0, sizeof(int) * MAX_COLORS, stream));
-    HIPCHECK(hipMemsetAsync(scal.p + 4, 0, sizeof(int), stream));
-    hipLaunchKernelGGL(k_color_count, dim3(blocks(N)), dim3(256), 0, stream, N, color[colorCur].p, ccount.p,
-                       scal.p + 4);
-    readScal(4, 1);
-    numColors = hscal[4];
-    std::vector<int> cc(MAX_COLORS);
-    HIPCHECK(hipMemcpyAsync(cc.data(), ccount.p, sizeof(int) * MAX_COLORS, hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    hcstart.assign(numColors + 1, 0);
-    for (int c = 0; c < numColors; c++) hcstart[c + 1] = hcstart[c] + cc[c];
-    if (numColors > 0) {
-      HIPCHECK(hipMemcpyAsync(ccursor.p, hcstart.data(), sizeof(int) * numColors, hipMemcpyHostToDevice, stream));
-      hipLaunchKernelGGL(k_color_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, color[colorCur].p, ccursor.p,
-                         cbodies.p);
-    }
-  }
-
   void buildItems() {
     const int4 *head = store[cur].head.p;
+    const int cap = int(items.n);
     hipLaunchKernelGGL(k_items<0>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
-                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p);
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5);
     scan(itemCnt.p, itemStart.p, N, itemStart.p + N);
-    HIPCHECK(hipMemcpyAsync(&hscal[6], itemStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    items.reserve(size_t(hscal[6]) + 1);
     hipLaunchKernelGGL(k_items<1>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
-                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p);
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5);
+  }
+
+  // ------------------------------------------------------------ colouring
+  // cmeta: [0..2] change counters, [3] rounds, [4] colours, [8..8+257) bucket
+  // starts. One cooperative launch relaxes to the greedy fixed point; one
+  // readback brings the colour count and buckets to the host.
+  // Rounds go out in batches (colorBatch round + check pairs); rounds after
+  // convergence return at once. The bucket kernels follow the batch, so when
+  // it converged a single readback serves both.
+  void colorize() {
+    HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
+    const int maxRounds = N + 2;
+    for (int launched = 0;;) {
+      for (int r = 0; r < colorBatch; r++) {
+        hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
+                           adjOther.p, color[0].p, cmeta.p, prm.color_order, sceneStart.p, scal.p + 5);
+        hipLaunchKernelGGL(k_color_check, dim3(1), dim3(64), 0, stream, cmeta.p);
+      }
+      launched += colorBatch;
+      HIPCHECK(hipMemsetAsync(ccount.p, 0, sizeof(int) * MAX_COLORS, stream));
+      HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
+      hipLaunchKernelGGL(k_color_count, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, ccount.p,
+                         cmeta.p + 4);
+      hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ccount.p, cmeta.p + 8, ccursor.p);
+      hipLaunchKernelGGL(k_color_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, ccursor.p,
+                         cbodies.p);
+      HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (hmeta[1]) break;
+      if (launched > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
+    }
+    colorRounds = hmeta[3];
+    numColors = hmeta[4];
+    hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
 
   template <class R, bool EXACT, int TEAM>
@@ -384,26 +377,26 @@ struct avbd_ctx {
     ba.bucketCnt = bucketCnt.p; ba.entries = entries.p; ba.mask = tableMask; ba.sceneLargeStart = sceneLargeStart.p;
     ba.sceneLarges = sceneLarges.p; ba.sceneStart = sceneStart.p; ba.adjStart = adjStart.p; ba.adjKey = adjKey.p;
     ba.adjOther = adjOther.p; ba.joints = joints.p; ba.countRev = countRev.p; ba.startRev = startRev.p;
-    ba.pairs = pairs.p; ba.pairNew = pairNew.p; ba.pairCap = pairCap; ba.overflow = scal.p + 5;
+    ba.countNewRev = countNewRev.p; ba.startNewRev = startNewRev.p;
+    ba.pairs = pairs.p; ba.newPairs = newPairs.p; ba.pairCap = pairCap;
     HIPCHECK(hipMemsetAsync(countRev.p, 0, sizeof(int) * N, stream));
+    HIPCHECK(hipMemsetAsync(countNewRev.p, 0, sizeof(int) * N, stream));
     hipLaunchKernelGGL(k_bp_small<0>, dim3(blocks(N)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<0>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
     scan(countRev.p, startRev.p, N, scal.p + 0);
-    readScal(0, 1);
-    nPairs = hscal[0];
-    if (nPairs > pairCap) {
-      HIPCHECK(hipMemsetAsync(scal.p + 5, 0, sizeof(int), stream));
-      throw ApiError(AVBD_E_CAPACITY, "broad-phase pairs " + std::to_string(nPairs) + " exceed capacity " +
-                                          std::to_string(pairCap));
-    }
+    scan(countNewRev.p, startNewRev.p, N, scal.p + 1);
+    // the write pass keeps every write below pairCap; an excess is reported
+    // below, before anything reads the lists
     hipLaunchKernelGGL(k_bp_small<1>, dim3(blocks(N)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<1>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
-    scan(pairNew.p, newPos.p, nPairs, scal.p + 1);
-    if (nPairs > 0)
-      hipLaunchKernelGGL(k_new_pairs, dim3(blocks(nPairs)), dim3(256), 0, stream, nPairs, pairs.p, pairNew.p,
-                         newPos.p, newPairs.p);
-    readScal(1, 1);
+    readScal(0, 2);
+    nPairs = hscal[0];
     nNew = hscal[1];
+    if (nPairs > pairCap) {
+      nPairs = 0;
+      throw ApiError(AVBD_E_CAPACITY, "broad-phase pairs " + std::to_string(hscal[0]) + " exceed capacity " +
+                                          std::to_string(pairCap));
+    }
     record(1);
 
     // 2. narrow phase and force initialisation
@@ -431,9 +424,9 @@ struct avbd_ctx {
 
     // 3. topology: force lists, colouring, work lists
     buildAdjacency();
+    buildItems();
     record(3);
     colorize();
-    buildItems();
     if (profiling && numColors > 0) {
       HIPCHECK(hipMemsetAsync(cbytes.p, 0, sizeof(unsigned long long) * MAX_COLORS, stream));
       hipLaunchKernelGGL(k_color_bytes, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, color[colorCur].p,
@@ -637,7 +630,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.pairCap = c.pairCapCfg > 0 ? c.pairCapCfg : std::max(32 * N, 4096);
   c.manCap = c.manCapCfg > 0 ? c.manCapCfg : std::max(32 * N, 4096);
   c.countRev.reserve(N); c.startRev.reserve(N + 1);
-  c.pairs.reserve(c.pairCap); c.pairNew.reserve(c.pairCap); c.newPos.reserve(c.pairCap + 1);
+  c.countNewRev.reserve(N); c.startNewRev.reserve(N + 1);
+  c.pairs.reserve(c.pairCap);
   c.newPairs.reserve(c.pairCap);
   c.store[0].reserve(c.manCap, c.rsize());
   c.store[1].reserve(c.manCap, c.rsize());
@@ -667,7 +661,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   if (c.J) HIPCHECK(hipMemcpy(c.joints.p, JD.data(), sizeof(JointDev) * c.J, hipMemcpyHostToDevice));
 
   c.deg.reserve(N); c.adjStart.reserve(N + 1); c.adjCursor.reserve(N);
-  c.adjKey.reserve(size_t(2 * c.J) + 1); c.adjOther.reserve(size_t(2 * c.J) + 1);
+  c.adjKey.reserve(2 * (size_t(c.manCap) + c.J) + 1);
+  c.adjOther.reserve(2 * (size_t(c.manCap) + c.J) + 1);
   c.color[0].reserve(N); c.color[1].reserve(N);
   std::vector<int> col(N);
   for (int i = 0; i < N; i++) col[i] = P[i].mass > 0 ? 0 : -1;
@@ -676,9 +671,11 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.colorCur = 0;
   c.ccount.reserve(MAX_COLORS); c.cstart.reserve(MAX_COLORS + 1); c.ccursor.reserve(MAX_COLORS);
   c.cbodies.reserve(N);
-  c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.items.reserve(size_t(N) * 16);
+  c.itemCnt.reserve(N); c.itemStart.reserve(N + 1);
+  c.items.reserve(2 * (size_t(MAXC) * c.manCap + c.J) + 1);
   c.scal.reserve(16);
   c.cbytes.reserve(MAX_COLORS);
+  c.cmeta.reserve(8 + MAX_COLORS + 1);
   HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
   c.evs.resize(AVBD_NUM_PHASES);
   for (auto &e : c.evs) HIPCHECK(hipEventCreate(&e));
@@ -710,6 +707,8 @@ void avbd_default_params(avbd_params *p) {
   p->post_stabilize = 0;
   p->rotated_inertia = 1;
   p->cached_contact_jacobians = 1;
+  p->color_order = AVBD_ORDER_INDEX;
+  p->reserved = 0;
 }
 
 int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
@@ -751,6 +750,8 @@ int avbd_set_params(avbd_ctx *ctx, const avbd_params *p) {
   return guarded(ctx, [&] {
     if (!p) throw ApiError(AVBD_E_INVALID, "null params");
     if (!(p->dt > 0) || p->iterations < 1) throw ApiError(AVBD_E_INVALID, "dt must be > 0 and iterations >= 1");
+    if (p->color_order != AVBD_ORDER_INDEX && p->color_order != AVBD_ORDER_PRIORITY)
+      throw ApiError(AVBD_E_INVALID, "color_order must be AVBD_ORDER_INDEX or AVBD_ORDER_PRIORITY");
     if (!p->cached_contact_jacobians)
       throw ApiError(AVBD_E_INVALID, "only cached contact Jacobians are implemented on the GPU");
     ctx->prm = *p;

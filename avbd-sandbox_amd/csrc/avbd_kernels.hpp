@@ -124,12 +124,10 @@ struct BpArgs {
   const int *sceneStart;                     // first body of each scene
   const int *adjStart, *adjKey, *adjOther;
   const JointDev *joints;
-  int *countRev;                             // [n] count for A stored at n-1-A
-  const int *startRev;                       // [n] exclusive scan of countRev
-  int2 *pairs;                               // output pairs (A, B)
-  int *pairNew;                              // 1 if the pair creates a manifold
+  int *countRev, *countNewRev;               // [n] counts for A stored at n-1-A
+  const int *startRev, *startNewRev;         // [n] exclusive scans of the counts
+  int2 *pairs, *newPairs;                    // output lists (A, B)
   int pairCap;
-  int *overflow;
 };
 
 // The predicate of solver.js:316-329 for A > B, plus the two flags: is the
@@ -150,8 +148,20 @@ __device__ __forceinline__ int pairTest(const BpArgs &a, int A, int B) {
   return kind == 2 ? 1 : 2;  // 1: recorded, existing manifold; 2: recorded, new
 }
 
+// Insert (x, y) into the run list[base .. base+cnt) kept sorted by y descending.
+__device__ __forceinline__ void insertDesc(int2 *list, int base, int cnt, int cap, int x, int y) {
+  int k = base + cnt;
+  if (k >= cap) return;
+  while (k > base && list[k - 1].y < y) {
+    list[k] = list[k - 1];
+    k--;
+  }
+  list[k] = make_int2(x, y);
+}
+
 // Small bodies: pairs (x, y) with y < x from the grid, and (x, L) with the
-// scene's large bodies L < x. PASS 0 counts, PASS 1 writes sorted by y desc.
+// scene's large bodies L < x. PASS 0 counts, PASS 1 writes both lists sorted
+// by y descending.
 template <int PASS>
 __global__ void k_bp_small(BpArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,9 +169,19 @@ __global__ void k_bp_small(BpArgs a) {
   const BodyProp &px = a.props[x];
   if (px.large) return;
   const int4 c = a.cellOf[x];
-  int cnt = 0;
-  int base = 0;
-  if (PASS == 1) base = a.startRev[a.n - 1 - x];
+  int cnt = 0, ncnt = 0, base = 0, nbase = 0;
+  if (PASS == 1) {
+    base = a.startRev[a.n - 1 - x];
+    nbase = a.startNewRev[a.n - 1 - x];
+  }
+  auto accept = [&](int y, int t) {
+    if (PASS == 1) {
+      insertDesc(a.pairs, base, cnt, a.pairCap, x, y);
+      if (t == 2) insertDesc(a.newPairs, nbase, ncnt, a.pairCap, x, y);
+    }
+    cnt++;
+    ncnt += t == 2;
+  };
   for (int dx = -1; dx <= 1; dx++)
     for (int dy = -1; dy <= 1; dy++)
       for (int dz = -1; dz <= 1; dz++) {
@@ -174,24 +194,7 @@ __global__ void k_bp_small(BpArgs a) {
           const int4 cy = a.cellOf[y];
           if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
           const int t = pairTest(a, x, y);
-          if (!t) continue;
-          if (PASS == 1) {
-            // insertion into the descending run written so far
-            const int pos = base + cnt;
-            if (pos < a.pairCap) {
-              int k = pos;
-              while (k > base && a.pairs[k - 1].y < y) {
-                a.pairs[k] = a.pairs[k - 1];
-                a.pairNew[k] = a.pairNew[k - 1];
-                k--;
-              }
-              a.pairs[k] = make_int2(x, y);
-              a.pairNew[k] = t == 2;
-            } else {
-              atomicOr(a.overflow, 1);
-            }
-          }
-          cnt++;
+          if (t) accept(y, t);
         }
       }
   const int l0 = a.sceneLargeStart[px.scene], l1 = a.sceneLargeStart[px.scene + 1];
@@ -199,25 +202,12 @@ __global__ void k_bp_small(BpArgs a) {
     const int y = a.sceneLarges[e];
     if (y >= x) continue;
     const int t = pairTest(a, x, y);
-    if (!t) continue;
-    if (PASS == 1) {
-      const int pos = base + cnt;
-      if (pos < a.pairCap) {
-        int k = pos;
-        while (k > base && a.pairs[k - 1].y < y) {
-          a.pairs[k] = a.pairs[k - 1];
-          a.pairNew[k] = a.pairNew[k - 1];
-          k--;
-        }
-        a.pairs[k] = make_int2(x, y);
-        a.pairNew[k] = t == 2;
-      } else {
-        atomicOr(a.overflow, 1);
-      }
-    }
-    cnt++;
+    if (t) accept(y, t);
   }
-  if (PASS == 0) a.countRev[a.n - 1 - x] = cnt;
+  if (PASS == 0) {
+    a.countRev[a.n - 1 - x] = cnt;
+    a.countNewRev[a.n - 1 - x] = ncnt;
+  }
 }
 
 // Large bodies: one 256-thread block per large body L tests every y < L of
@@ -226,40 +216,41 @@ template <int PASS>
 __global__ void k_bp_large(BpArgs a, const int *largeList) {
   const int L = largeList[blockIdx.x];
   const int lo = a.sceneStart[a.props[L].scene];
-  __shared__ int warpCnt[4];
-  __shared__ int running;
-  if (threadIdx.x == 0) running = 0;
+  __shared__ int wc[4], wn[4];
+  __shared__ int running, runningNew;
+  if (threadIdx.x == 0) running = runningNew = 0;
   __syncthreads();
   const int base = PASS == 1 ? a.startRev[a.n - 1 - L] : 0;
+  const int nbase = PASS == 1 ? a.startNewRev[a.n - 1 - L] : 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int top = L - 1; top >= lo; top -= 256) {
     const int y = top - int(threadIdx.x);
     const int t = y >= lo ? pairTest(a, L, y) : 0;
-    const unsigned long long m = __ballot(t != 0);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) warpCnt[w] = __popcll(m);
+    const unsigned long long m = __ballot(t != 0), mn = __ballot(t == 2);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (lane == 0) { wc[w] = __popcll(m); wn[w] = __popcll(mn); }
     __syncthreads();
-    int off = running;
-    for (int k = 0; k < w; k++) off += warpCnt[k];
+    int off = running, noff = runningNew;
+    for (int k = 0; k < w; k++) { off += wc[k]; noff += wn[k]; }
     if (t && PASS == 1) {
-      const int pos = base + off + before;
-      if (pos < a.pairCap) {
-        a.pairs[pos] = make_int2(L, y);
-        a.pairNew[pos] = t == 2;
-      } else {
-        atomicOr(a.overflow, 1);
+      const int pos = base + off + __popcll(m & below);
+      if (pos < a.pairCap) a.pairs[pos] = make_int2(L, y);
+      if (t == 2) {
+        const int np = nbase + noff + __popcll(mn & below);
+        if (np < a.pairCap) a.newPairs[np] = make_int2(L, y);
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) running += warpCnt[0] + warpCnt[1] + warpCnt[2] + warpCnt[3];
+    if (threadIdx.x == 0) {
+      running += wc[0] + wc[1] + wc[2] + wc[3];
+      runningNew += wn[0] + wn[1] + wn[2] + wn[3];
+    }
     __syncthreads();
   }
-  if (PASS == 0 && threadIdx.x == 0) a.countRev[a.n - 1 - L] = running;
-}
-
-__global__ void k_new_pairs(int np, const int2 *pairs, const int *pairNew, const int *newPos, int2 *newPairs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np && pairNew[i]) newPairs[newPos[i]] = pairs[i];
+  if (PASS == 0 && threadIdx.x == 0) {
+    a.countRev[a.n - 1 - L] = running;
+    a.countNewRev[a.n - 1 - L] = runningNew;
+  }
 }
 
 // ------------------------------------------------------------ narrow phase
@@ -597,31 +588,67 @@ __global__ void k_adj_sort(int n, const BodyProp *props, const int *adjStart, in
   }
 }
 
-// Greedy colouring in index order (coloring.js:75-92): colour(i) is the
-// smallest colour not used by a lower-index dynamic neighbour. One Jacobi
-// round; repeated rounds reach the unique fixed point, which is exactly the
-// sequential greedy result.
+// Greedy colouring (coloring.js:75-92): colour(i) is the smallest colour not
+// used by a dynamic neighbour that claims before i. Claim order: body index
+// (order 0, the CPU engine), or lowbias32 priority with ties to the lower
+// index (order 1, layout.js:50-57 and shaders.js:1571-1585). The greedy
+// result is the unique fixed point of colour(i) = mex(colours of earlier
+// neighbours); relaxation reaches it from any start, including last step's.
 constexpr int MAX_COLORS = 256;
-__global__ void k_color_round(int n, const BodyProp *props, const int *adjStart, const int *adjOther, const int *cin,
-                              int *cout, int *changed, int *overflow) {
+
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// a and b belong to one scene whose first body is `base`; the priority hashes
+// the index within the scene, so a scene colours the same alone or batched.
+__device__ __forceinline__ bool claimsBefore(int a, int b, int order, int base) {
+  if (order == 0) return a < b;
+  const uint32_t pa = lowbias32(uint32_t(a - base)), pb = lowbias32(uint32_t(b - base));
+  return pa < pb || (pa == pb && a < b);
+}
+
+// One relaxation round, in place: each body takes the mex of its earlier
+// neighbours' current colours. Rounds repeat until one changes nothing; that
+// round saw only final colours, so every body equals the mex of its earlier
+// neighbours: the fixed point. ctl[0] counts this round's changes, ctl[1] is
+// set once a round changed nothing (later rounds return at once), ctl[3]
+// receives the number of rounds.
+__global__ void k_color_round(int n, const BodyProp *props, const int *adjStart, const int *adjOther, int *color,
+                              int *ctl, int order, const int *sceneStart, int *overflow) {
+  if (ctl[1]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (!(props[i].mass > 0)) { cout[i] = -1; return; }
+  if (i >= n || !(props[i].mass > 0)) return;
+  volatile int *vc = color;
+  const int base = sceneStart[props[i].scene];
   unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
   for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
     const int j = adjOther[e];
-    if (j < 0 || j >= i) continue;
-    const int c = cin[j];
+    if (j < 0 || !claimsBefore(j, i, order, base)) continue;
+    const int c = vc[j];
     if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
   }
-  int c = 0;
-  for (int w = 0; w < MAX_COLORS / 64; w++) {
+  int c = MAX_COLORS;
+  for (int w = 0; w < MAX_COLORS / 64; w++)
     if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
-    c = MAX_COLORS;
-  }
   if (c >= MAX_COLORS) { atomicOr(overflow, 4); c = MAX_COLORS - 1; }
-  cout[i] = c;
-  if (c != cin[i]) atomicAdd(changed, 1);
+  if (c != vc[i]) {
+    vc[i] = c;
+    atomicAdd(&ctl[0], 1);
+  }
+}
+
+// After each round: stop once a round changed nothing.
+__global__ void k_color_check(int *ctl) {
+  if (threadIdx.x || blockIdx.x || ctl[1]) return;
+  ctl[3] += 1;
+  if (ctl[0] == 0) ctl[1] = 1;
+  ctl[0] = 0;
 }
 
 __global__ void k_color_count(int n, const int *color, int *ccount, int *numColors) {
@@ -631,6 +658,25 @@ __global__ void k_color_count(int n, const int *color, int *ccount, int *numColo
   if (c < 0) return;
   atomicAdd(&ccount[c], 1);
   atomicMax(numColors, c + 1);
+}
+
+// Exclusive scan of the MAX_COLORS bucket counts (one 256-thread block):
+// cstart[0..MAX_COLORS], and the scatter cursors.
+__global__ void k_color_starts(const int *ccount, int *cstart, int *cursor) {
+  __shared__ int sh[MAX_COLORS];
+  const int t = threadIdx.x;
+  const int v = ccount[t];
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < MAX_COLORS; off <<= 1) {
+    const int u = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += u;
+    __syncthreads();
+  }
+  cstart[t] = sh[t] - v;
+  cursor[t] = sh[t] - v;
+  if (t == MAX_COLORS - 1) cstart[MAX_COLORS] = sh[t];
 }
 
 __global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodies) {
@@ -647,12 +693,13 @@ __global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodi
 template <int PASS>
 __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart, const int *adjKey, const int *live,
                         const int4 *head, const JointDev *joints, int *itemCnt, const int *itemStart,
-                        uint32_t *items) {
+                        uint32_t *items, int cap, int *overflow) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (!(props[i].mass > 0)) { if (PASS == 0) itemCnt[i] = 0; return; }
   int cnt = 0;
   const int base = PASS == 1 ? itemStart[i] : 0;
+  if (PASS == 1 && itemStart[i + 1] > cap) { atomicOr(overflow, 8); return; }
   for (int e = adjStart[i + 1] - 1; e >= adjStart[i]; e--) {
     const int key = adjKey[e];
     if (key < J) {

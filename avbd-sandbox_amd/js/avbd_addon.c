@@ -118,6 +118,7 @@ static napi_value js_set_params(napi_env env, napi_callback_info info) {
   p.post_stabilize = (int32_t)get_num(env, o, "postStabilize", p.post_stabilize);
   p.rotated_inertia = (int32_t)get_num(env, o, "rotatedInertia", p.rotated_inertia);
   p.cached_contact_jacobians = (int32_t)get_num(env, o, "cachedContactJacobians", p.cached_contact_jacobians);
+  p.color_order = (int32_t)get_num(env, o, "colorOrder", p.color_order);
   int rc = avbd_set_params(ctx, &p);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;

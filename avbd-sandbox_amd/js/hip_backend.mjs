@@ -86,6 +86,7 @@ export class HipBackend {
       penaltyMax: solver.penaltyMax, collisionMargin: solver.collisionMargin,
       stickThreshold: solver.stickThreshold, postStabilize: solver.postStabilize ? 1 : 0,
       rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
+      colorOrder: solver.colorOrder === 'priority' || solver.colorOrder === 1 ? 1 : 0,
     });
     addon.pack(this.ctx, A);
     this.numBodies = n;

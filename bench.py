@@ -33,6 +33,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f64", "exact"])
     ap.add_argument("--team", type=int, default=16)
+    ap.add_argument("--order", default="priority", choices=["priority", "index"])
     ap.add_argument("--layers", type=int, default=53)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -47,6 +48,7 @@ def main():
 
     prec = {"mixed": avbd.PREC_MIXED, "f64": avbd.PREC_F64, "exact": avbd.PREC_EXACT}[args.precision]
     scene, params, meta = avbd.scenes.pyramid(layers=args.layers)
+    params = dict(params, colorOrder=1 if args.order == "priority" else 0)
     gpu = avbd.HipBackend(precision=prec, team=args.team)
     gpu.pack(scene, params)
     iters = params["iterations"]
@@ -90,7 +92,8 @@ def main():
                         f"on a static ground, dt 1/60, {iters} iterations, steps {args.warmup + 1}-"
                         f"{args.warmup + args.steps} after pack",
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
-            "precision": args.precision, "team": args.team,
+            "precision": args.precision, "team": args.team, "color_order": args.order,
+            "color_rounds": c["color_rounds"],
             "manifolds": c["manifolds"], "contacts": c["contacts"], "colors": c["colors"],
         },
         "roofline": {
@@ -109,7 +112,7 @@ def main():
     if not args.no_cpu:
         import pyoracle
         t0 = time.perf_counter()
-        orc = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+        orc = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
         build_s = time.perf_counter() - t0
         cpu_secs = orc.step(args.cpu_steps)
         cpu_rate = orc.n_dynamic * iters * args.cpu_steps / cpu_secs

@@ -83,7 +83,18 @@ typedef struct avbd_params {
   int32_t post_stabilize;  /* 0                                             */
   int32_t rotated_inertia; /* 1                                             */
   int32_t cached_contact_jacobians; /* 1 (the only mode the GPU implements) */
+  /* Order in which bodies claim colours in the greedy colouring that
+   * schedules the primal sweep. AVBD_ORDER_INDEX: body index, the CPU
+   * engine's Coloring.build (coloring.js:75-92). AVBD_ORDER_PRIORITY: the
+   * lowbias32 claim order of the reference GPU backend (layout.js:50-66,
+   * shaders.js:1571-1585), whose dependency chains are short. Either way the
+   * colouring is the exact greedy fixed point. */
+  int32_t color_order;
+  int32_t reserved;
 } avbd_params;
+
+#define AVBD_ORDER_INDEX 0
+#define AVBD_ORDER_PRIORITY 1
 
 /* Fills *p with the reference defaults (solver.js:92-236). */
 void avbd_default_params(avbd_params *p);

@@ -1123,7 +1123,17 @@ int World::addJoint(int a, int b, const double rA[3], const double rB[3], double
   return id;
 }
 
-// coloring.js:43-118 — greedy, index order, statics uncoloured
+// layout.js:50-57 — lowbias32 integer hash
+uint32_t colorPriority(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// coloring.js:43-118 — greedy, statics uncoloured
 void World::buildColoring(Coloring &out) const {
   const int n = int(bodies.size());
   out.colorOf.assign(n, -1);
@@ -1140,10 +1150,21 @@ void World::buildColoring(Coloring &out) const {
     const Force &f = pool[id];
     if (f.a >= 0 && f.b >= 0) { adj[cur[f.a]++] = f.b; adj[cur[f.b]++] = f.a; }
   }
-  std::vector<int> used;
+  // Visiting order: body index (coloring.js:75-92), or the reference GPU's
+  // claim order, lowbias32 priority with ties to the lower index
+  // (layout.js:50-66, shaders.js:1571-1585); greedy in that order is the
+  // fixed point its Jacobi rounds converge to.
+  std::vector<int> visit(n);
+  for (int i = 0; i < n; i++) visit[i] = i;
+  if (colorOrder == 1)
+    std::stable_sort(visit.begin(), visit.end(), [](int a, int b) {
+      const uint32_t pa = colorPriority(uint32_t(a)), pb = colorPriority(uint32_t(b));
+      return pa < pb || (pa == pb && a < b);
+    });
   int numColors = 0, stamp = 0;
   std::vector<int> stampOf;
-  for (int i = 0; i < n; i++) {
+  for (int v = 0; v < n; v++) {
+    const int i = visit[v];
     if (bodies[i].mass <= 0) continue;
     stamp++;
     for (int k = start[i]; k < start[i + 1]; k++) {

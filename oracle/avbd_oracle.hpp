@@ -107,6 +107,7 @@ struct World {
   std::vector<int> lastPairs;    // broad-phase pairs of the last step (i, j)...
   bool recordPairs = false;
   int broadphase = 0;            // 0 grid (exactly the brute-force pair set), 1 brute force
+  int colorOrder = 0;            // 0 body index, 1 lowbias32 priority
 
   int addBox(const double size[3], double density, double friction,
              const double pos[3], const double vel[3]);
@@ -117,6 +118,9 @@ struct World {
   void step();
   void buildColoring(Coloring &out) const;
 };
+
+// lowbias32, the colour-claim priority of the reference GPU (layout.js:50-57).
+uint32_t colorPriority(uint32_t x);
 
 // Narrow phase on its own, for tests: fills up to MAX_CONTACTS contacts
 // (feature, rA, rB) and the basis; returns the contact count.

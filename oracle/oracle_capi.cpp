@@ -64,6 +64,19 @@ void orc_set_schedule(void *w, int mode, const int *order, int n) {
 }
 
 void orc_set_broadphase(void *w, int mode) { static_cast<World *>(w)->broadphase = mode; }
+void orc_set_color_order(void *w, int order) { static_cast<World *>(w)->colorOrder = order; }
+unsigned orc_color_priority(unsigned i) { return colorPriority(i); }
+
+// Overwrite one body's pose and velocities (a launch or a pick-and-throw).
+void orc_set_body(void *w, int i, const double *pos, const double *quat, const double *vel, const double *angvel) {
+  Body &b = static_cast<World *>(w)->bodies[i];
+  for (int k = 0; k < 3; k++) {
+    if (pos) b.p[k] = pos[k];
+    if (vel) b.v[k] = b.vprev[k] = vel[k];
+    if (angvel) b.w[k] = angvel[k];
+  }
+  if (quat) for (int k = 0; k < 4; k++) b.q[k] = quat[k];
+}
 void orc_record_pairs(void *w, int on) { static_cast<World *>(w)->recordPairs = on != 0; }
 
 // Runs n steps; returns the wall time in seconds.

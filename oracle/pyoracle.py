@@ -39,6 +39,9 @@ def lib():
             "orc_add_joint": (i, [v, i, i, pd, pd, d, d, d]),
             "orc_set_schedule": (None, [v, i, pi, i]),
             "orc_set_broadphase": (None, [v, i]),
+            "orc_set_color_order": (None, [v, i]),
+            "orc_color_priority": (C.c_uint, [C.c_uint]),
+            "orc_set_body": (None, [v, i, pd, pd, pd, pd]),
             "orc_record_pairs": (None, [v, i]),
             "orc_step": (d, [v, i]),
             "orc_num_bodies": (i, [v]), "orc_num_dynamic": (i, [v]),
@@ -88,7 +91,7 @@ DEFAULT_PARAMS = dict(dt=1.0 / 60.0, gravity=-10.0, iterations=10, betaLin=1e4, 
 class Oracle:
     """One f64 world. Mirrors the reference ``Solver`` (solver.js:59-236)."""
 
-    def __init__(self, scene, params=None, schedule=SCHED_COLOR, broadphase="grid"):
+    def __init__(self, scene, params=None, schedule=SCHED_COLOR, broadphase="grid", color_order=0):
         self.L = lib()
         self.h = self.L.orc_create()
         p = dict(DEFAULT_PARAMS)
@@ -120,6 +123,7 @@ class Oracle:
                 raise ValueError(f"bad joint {j}")
         self.set_schedule(schedule)
         self.L.orc_set_broadphase(self.h, 1 if broadphase == "brute" else 0)
+        self.L.orc_set_color_order(self.h, int(color_order))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -129,6 +133,10 @@ class Oracle:
     def set_schedule(self, mode, order=None):
         o = np.ascontiguousarray(np.asarray(order if order is not None else [], dtype=np.int32))
         self.L.orc_set_schedule(self.h, int(mode), _ip(o), int(o.size))
+
+    def set_body(self, i, pos=None, quat=None, vel=None, angvel=None):
+        a = [None if x is None else _vec(x, n) for x, n in ((pos, 3), (quat, 4), (vel, 3), (angvel, 3))]
+        self.L.orc_set_body(self.h, int(i), *[None if x is None else _dp(x) for x in a])
 
     def record_pairs(self, on=True):
         self.L.orc_record_pairs(self.h, int(on))

@@ -21,7 +21,7 @@ import { pathToFileURL } from 'url';
 const REF = process.argv[2] || '/root/reference';
 const OUT = process.argv[3] || 'tests/golden';
 const imp = (p) => import(pathToFileURL(join(REF, p)).href);
-let Solver, Rigid, Joint, Manifold, Coloring;
+let Solver, Rigid, Joint, Manifold, Coloring, colorOrder;
 
 // Seeded LCG (Numerical Recipes constants), uniform in [0, 1).
 function lcg(seed) {
@@ -179,7 +179,37 @@ function build(def) {
   return solver;
 }
 
-function installColorSchedule(solver, sink) {
+// Greedy colouring visiting bodies in the reference GPU's claim order
+// (layout.js colorOrder: lowbias32 priority, ties to the lower index), the
+// fixed point its Jacobi rounds converge to; bodies grouped by colour in index
+// order, as Coloring.flatOrder() groups them.
+function priorityColoring(solver) {
+  const n = solver.bodies.length;
+  const index = new Map();
+  solver.bodies.forEach((b, i) => index.set(b, i));
+  const adj = Array.from({ length: n }, () => []);
+  for (const f of solver.forces) {
+    const a = f.bodyA ? index.get(f.bodyA) : -1;
+    const b = f.bodyB ? index.get(f.bodyB) : -1;
+    if (a >= 0 && b >= 0) { adj[a].push(b); adj[b].push(a); }
+  }
+  const colorOf = new Array(n).fill(-1);
+  let numColors = 0;
+  for (const i of colorOrder(n)) {
+    if (solver.bodies[i].mass <= 0) continue;
+    const used = new Set();
+    for (const j of adj[i]) if (colorOf[j] >= 0) used.add(colorOf[j]);
+    let c = 0;
+    while (used.has(c)) c++;
+    colorOf[i] = c;
+    numColors = Math.max(numColors, c + 1);
+  }
+  const order = [];
+  for (let c = 0; c < numColors; c++) for (let i = 0; i < n; i++) if (colorOf[i] === c) order.push(i);
+  return { colorOf, numColors, order };
+}
+
+function installColorSchedule(solver, sink, priority) {
   const coloring = new Coloring();
   let stepNo = 0, builtFor = -1, order = null;
   const step = solver.step.bind(solver);
@@ -188,12 +218,19 @@ function installColorSchedule(solver, sink) {
     configurable: true,
     get() {
       if (builtFor !== stepNo) {
-        const index = new Map();
-        solver.bodies.forEach((b, i) => index.set(b, i));
-        coloring.build(solver.bodies, solver.forces, index);
-        order = Array.from(coloring.flatOrder());
-        sink.colorOf = Array.from(coloring.colorOf.subarray(0, solver.bodies.length));
-        sink.numColors = coloring.numColors;
+        if (priority) {
+          const pc = priorityColoring(solver);
+          order = pc.order;
+          sink.colorOf = pc.colorOf;
+          sink.numColors = pc.numColors;
+        } else {
+          const index = new Map();
+          solver.bodies.forEach((b, i) => index.set(b, i));
+          coloring.build(solver.bodies, solver.forces, index);
+          order = Array.from(coloring.flatOrder());
+          sink.colorOf = Array.from(coloring.colorOf.subarray(0, solver.bodies.length));
+          sink.numColors = coloring.numColors;
+        }
         builtFor = stepNo;
       }
       return order;
@@ -230,7 +267,7 @@ function forceListOf(solver) {
 function run(name, def, schedule) {
   const solver = build(def);
   const sink = {};
-  if (schedule === 'color') installColorSchedule(solver, sink);
+  if (schedule === 'color' || schedule === 'prio') installColorSchedule(solver, sink, schedule === 'prio');
   const sampleAt = new Set([1, 2, 3, 5, 10, 20, 50, 100, 200, def.steps]);
   const steps = [];
   const samples = {};
@@ -239,7 +276,7 @@ function run(name, def, schedule) {
     const st = stateOf(solver);
     const fl = forceListOf(solver);
     const rec = { state: sha(st), forces: sha(fl), nforces: solver.forces.length };
-    if (schedule === 'color') { rec.colorOf = sink.colorOf; rec.numColors = sink.numColors; }
+    if (schedule !== 'reverse') { rec.colorOf = sink.colorOf; rec.numColors = sink.numColors; }
     if (s <= 3) rec.forceList = Array.from(fl);
     steps.push(rec);
     if (sampleAt.has(s)) samples[s] = Array.from(st);
@@ -260,6 +297,7 @@ async function main() {
   ({ Joint } = await imp('src/physics/joint.js'));
   ({ Manifold } = await imp('src/physics/manifold.js'));
   ({ Coloring } = await imp('src/physics/coloring.js'));
+  ({ colorOrder } = await imp('src/physics/gpu/layout.js'));
 run('stack10_reverse', sceneStack10(), 'reverse');
 run('stack10_color', sceneStack10(), 'color');
 run('drop_color', sceneDrop(), 'color');
@@ -272,6 +310,10 @@ run('rain_color', sceneRain(), 'color');
 run('rain_reverse', sceneRain(), 'reverse');
 run('pyramid5_color', scenePyramid(5, 'bruteforce', 60), 'color');
 run('pyramid8_hash_reverse', scenePyramid(8, 'auto', 20), 'reverse');
+run('pyramid5_prio', scenePyramid(5, 'bruteforce', 60), 'prio');
+run('rain_prio', sceneRain(), 'prio');
+run('pendulum_prio', scenePendulum(), 'prio');
+run('chain_prio', sceneChain(), 'prio');
 }
 
 main().catch((e) => { console.error(e); process.exit(1); });

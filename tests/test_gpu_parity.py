@@ -23,7 +23,8 @@ import pyoracle
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-COLOR_SCENES = sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*_color.json")))
+COLOR_SCENES = sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*_color.json")) +
+                      glob.glob(os.path.join(GOLDEN, "*_prio.json")))
 MAX_STEPS = 120
 
 
@@ -51,9 +52,10 @@ def oracle_forces(o):
 def test_exact_bit_identical_to_reference(name):
     doc = json.load(open(os.path.join(GOLDEN, name + ".json")))
     print(f"[parity] {name}: {len(doc['scene']['bodies'])} bodies", flush=True)
+    order = 1 if doc["schedule"] == "prio" else 0
     gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
-    gpu.pack(doc["scene"], doc["params"])
-    ref = pyoracle.Oracle(doc["scene"], doc["params"], schedule=pyoracle.SCHED_COLOR)
+    gpu.pack(doc["scene"], dict(doc["params"], colorOrder=order))
+    ref = pyoracle.Oracle(doc["scene"], doc["params"], schedule=pyoracle.SCHED_COLOR, color_order=order)
     ref.record_pairs(True)
     for s, rec in enumerate(doc["perStep"][:MAX_STEPS], start=1):
         gpu.step(1)
@@ -75,6 +77,31 @@ def test_exact_bit_identical_to_reference(name):
             i, k = np.unravel_index(np.argmax(diff), diff.shape)
             pytest.fail(f"step {s}: state differs from the reference; max |diff| {diff.max():.3e} "
                         f"at body {i} component {k}")
+
+
+@pytest.mark.parametrize("order", [0, 1], ids=["index", "priority"])
+def test_exact_multi_workgroup_scene(order):
+    # enough bodies that every kernel runs many workgroups (the golden scenes
+    # fit in one); pairs, force lists, colouring and state against the oracle
+    scene, params, _ = avbd.scenes.pyramid(layers=12)
+    params = dict(params, colorOrder=order)
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    gpu.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=order)
+    ref.record_pairs(True)
+    for s in range(1, 7):
+        gpu.step(1)
+        ref.step(1)
+        assert np.array_equal(gpu.read_pairs(), ref.pairs()), f"step {s}: pairs"
+        gk, gf = gpu.read_forces()
+        rk, rf = oracle_forces(ref)
+        assert np.array_equal(gk, rk) and np.array_equal(gf, rf), f"step {s}: forces"
+        gc, gn = gpu.read_coloring()
+        rc, rn = ref.last_coloring()
+        assert gn == rn and np.array_equal(gc, rc), f"step {s}: colouring"
+        st = gpu.read_state()
+        o = ref.state()
+        assert np.array_equal(st["pos"], o[:, :3]) and np.array_equal(st["quat"], o[:, 3:7]), f"step {s}: state"
 
 
 def test_exact_counters_and_manifold_records():

@@ -41,9 +41,9 @@ def _force_words(o):
 
 
 def _run(doc):
-    sched = pyoracle.SCHED_COLOR if doc["schedule"] == "color" else pyoracle.SCHED_REVERSE
-    o = pyoracle.Oracle(doc["scene"], doc["params"], schedule=sched)
-    return o
+    sched = pyoracle.SCHED_REVERSE if doc["schedule"] == "reverse" else pyoracle.SCHED_COLOR
+    order = 1 if doc["schedule"] == "prio" else 0
+    return pyoracle.Oracle(doc["scene"], doc["params"], schedule=sched, color_order=order)
 
 
 @pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f)[:-5] for f in FILES])
