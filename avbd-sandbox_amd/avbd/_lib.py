@@ -23,6 +23,7 @@ SYMBOLS = (
     "avbd_set_params", "avbd_pack", "avbd_step", "avbd_sync", "avbd_read_state", "avbd_write_state",
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
+    "avbd_autotune",
 )
 K_PRIMAL, K_DUAL, K_NARROW = 0, 1, 2
 
@@ -33,7 +34,7 @@ pd, pi, pu = P(d), P(i32), P(u32)
 
 class Config(C.Structure):
     _fields_ = [("device", i32), ("precision", i32), ("team", i32), ("manifold_cap", i32),
-                ("pair_cap", i32), ("reserved", i32 * 3)]
+                ("pair_cap", i32), ("graphs", i32), ("reserved", i32 * 2)]
 
 
 class Params(C.Structure):
@@ -92,6 +93,7 @@ def lib(path=None):
         "avbd_set_profiling": (C.c_int, [ctx, i32]),
         "avbd_read_profile": (C.c_int, [ctx, P(C.c_float), i32, pi]),
         "avbd_read_kernel_stats": (C.c_int, [ctx, i32, pd, P(C.c_int64), pd]),
+        "avbd_autotune": (C.c_int, [ctx, pi, P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -77,11 +77,11 @@ def scene_arrays(scene):
 class HipBackend:
     """One device context (stands in for GpuBackend; backend.js:243-335)."""
 
-    def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0):
+    def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0, graphs=0):
         self.L = L.lib()
         self.ctx = C.c_void_p()
         cfg = L.Config(device=device, precision=precision, team=team, manifold_cap=manifold_cap,
-                       pair_cap=pair_cap)
+                       pair_cap=pair_cap, graphs=graphs)
         rc = self.L.avbd_create(C.byref(self.ctx), C.byref(cfg))
         if rc != 0:
             raise AvbdError(rc, "avbd_create failed (no GPU, or bad configuration)")
@@ -207,3 +207,11 @@ class HipBackend:
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()
         self._ck(self.L.avbd_read_kernel_stats(self.ctx, int(kernel), C.byref(ms), C.byref(n), C.byref(b)))
         return ms.value, n.value, b.value
+
+    def autotune(self):
+        """One step that also times the primal sweep for each team shape and
+        keeps the fastest; returns (team, {team: ms})."""
+        t = C.c_int32()
+        ms = (C.c_float * 6)()
+        self._ck(self.L.avbd_autotune(self.ctx, C.byref(t), ms))
+        return t.value, dict(zip((1, 4, 8, 16, 32, 64), [float(x) for x in ms]))

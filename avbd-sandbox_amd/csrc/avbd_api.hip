@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -130,6 +131,16 @@ struct avbd_ctx {
   DBuf<int> ccount, cstart, ccursor, cbodies, cmeta;
   std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
+  int nDynamic = 0;
+
+  // solve-loop graphs, keyed by colour count and manifold buffer parity
+  std::map<long long, hipGraphExec_t> graphs;
+  bool graphCapturing = false;
+  int graphsCfg = 0;
+  // autotuner
+  bool tuneRequest = false;
+  float tuneMs[6] = {0};
+  DBuf<char> Xbk, DQbk;
   int numColors = 0, colorRounds = 0;
   DBuf<int> itemCnt, itemStart;
   DBuf<uint32_t> items;
@@ -207,6 +218,8 @@ struct avbd_ctx {
   ~avbd_ctx() { releaseAll(); }
 
   void releaseAll() {
+    dropGraphs();
+    Xbk.release(); DQbk.release();
     props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release();
     cbytes.release();
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -334,21 +347,28 @@ struct avbd_ctx {
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
 
+  // Launch shapes of the solve loop depend only on the body count, the colour
+  // count and the team: the kernels take their counts from device memory and
+  // stride, so one captured graph serves every step with that colour count.
+  int primalBlocks(int tm) const {
+    const long teams = long(1.5 * double(std::max(nDynamic, 1)) / std::max(numColors, 1)) + 1;
+    return std::max(1, std::min(blocks(teams * tm), 4096));
+  }
+  int dualBlocks() const { return std::max(1, std::min(blocks(manCap, 128), 2048)); }
+
   template <class R, bool EXACT, int TEAM>
   void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
-    const int cnt = hcstart[c + 1] - hcstart[c];
-    if (cnt <= 0) return;
-    const long threads = long(cnt) * TEAM;
+    if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
     timed(AVBD_K_PRIMAL, profiling ? double(hcbytes[c]) : 0.0, [&] {
-      hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(blocks(threads)), dim3(256), 0, stream, cnt,
-                         cbodies.p + hcstart[c], sa, mb, (R *)DQ.p, P, alpha);
+      hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(primalBlocks(TEAM)), dim3(256), 0, stream, c,
+                         cmeta.p + 8, cbodies.p, sa, mb, (R *)DQ.p, P, alpha);
     });
   }
 
   template <class R, bool EXACT>
-  void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
+  void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int tm) {
     if (EXACT) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha); return; }
-    switch (team) {
+    switch (tm) {
       case 1: launchPrimal<R, false, 1>(c, sa, mb, P, alpha); break;
       case 4: launchPrimal<R, false, 4>(c, sa, mb, P, alpha); break;
       case 8: launchPrimal<R, false, 8>(c, sa, mb, P, alpha); break;
@@ -356,6 +376,98 @@ struct avbd_ctx {
       case 64: launchPrimal<R, false, 64>(c, sa, mb, P, alpha); break;
       default: launchPrimal<R, false, 16>(c, sa, mb, P, alpha); break;
     }
+  }
+
+  // body initialisation, then the iterations (solver.js:366-489)
+  template <class R, bool EXACT>
+  void enqueueSolve(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, bool withInit) {
+    if (withInit)
+      hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
+                         (R *)DQ.p, P);
+    const int total = prm.iterations + (prm.post_stabilize ? 1 : 0);
+    for (int it = 0; it < total; it++) {
+      const double alpha = prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha;
+      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team);
+      if (it < prm.iterations) {
+        if (graphCapturing || nLive > 0)
+          timed(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, [&] {
+            hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(dualBlocks()), dim3(128), 0, stream, scal.p + 2,
+                               live[cur].p, mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
+          });
+        if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
+      }
+      if (it == prm.iterations - 1)
+        hipLaunchKernelGGL(k_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, V.p, prm.dt);
+    }
+  }
+
+  void dropGraphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+
+  // Replay (capturing on first use) the solve loop for this colour count and
+  // manifold buffer parity.
+  template <class R, bool EXACT>
+  void solveGraph(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
+    const long long key = (long long)numColors * 4 + cur;
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      graphCapturing = true;
+      try {
+        enqueueSolve<R, EXACT>(sa, mb, P, true);
+      } catch (...) {
+        graphCapturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      graphCapturing = false;
+      HIPCHECK(hipStreamEndCapture(stream, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHECK(e);
+      it = graphs.emplace(key, ex).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, stream));
+  }
+
+  // One primal sweep (every colour, first iteration's alpha) per team shape,
+  // timed with events; the body state it changes (X, DQ) is restored after
+  // each trial. Keeps the fastest shape.
+  template <class R>
+  void autotuneT(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
+    static const int cands[6] = {1, 4, 8, 16, 32, 64};
+    const size_t xb = sizeof(double) * size_t(N) * XS, qb = size_t(N) * 8 * rsize();
+    Xbk.reserve(xb);
+    DQbk.reserve(qb);
+    HIPCHECK(hipMemcpyAsync(Xbk.p, X.p, xb, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(DQbk.p, DQ.p, qb, hipMemcpyDeviceToDevice, stream));
+    hipEvent_t a = takeEvent(), b = takeEvent();
+    const bool prof = profiling;
+    profiling = false;
+    int best = team;
+    float bestMs = 1e30f;
+    for (int k = 0; k < 6; k++) {
+      HIPCHECK(hipEventRecord(a, stream));
+      for (int c = 0; c < numColors; c++) primal<R, false>(c, sa, mb, P, prm.alpha, cands[k]);
+      HIPCHECK(hipEventRecord(b, stream));
+      HIPCHECK(hipEventSynchronize(b));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, a, b));
+      tuneMs[k] = ms;
+      if (ms < bestMs) { bestMs = ms; best = cands[k]; }
+      HIPCHECK(hipMemcpyAsync(X.p, Xbk.p, xb, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(DQ.p, DQbk.p, qb, hipMemcpyDeviceToDevice, stream));
+    }
+    profiling = prof;
+    evPool.push_back(a);
+    evPool.push_back(b);
+    if (best != team) dropGraphs();
+    team = best;
   }
 
   // ---------------------------------------------------------------- step
@@ -440,25 +552,19 @@ struct avbd_ctx {
 
     // 4. body initialisation and the iterations
     ManBuf<R> mb = store[cur].view<R>();
-    hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
-                       (R *)DQ.p, P);
     SolveArgs sa;
     sa.props = props.p; sa.X = X.p; sa.X0 = X0.p; sa.Y = Y.p; sa.itemStart = itemStart.p; sa.items = items.p;
     sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N;
-    const int total = prm.iterations + (prm.post_stabilize ? 1 : 0);
-    for (int it = 0; it < total; it++) {
-      const double alpha = prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha;
-      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha);
-      if (it < prm.iterations) {
-        if (nLive > 0)
-          timed(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, [&] {
-            hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(blocks(nLive, 128)), dim3(128), 0, stream, nLive,
-                               live[cur].p, mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
-          });
-        if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
-      }
-      if (it == prm.iterations - 1)
-        hipLaunchKernelGGL(k_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, V.p, prm.dt);
+    if (tuneRequest && !EXACT) {
+      hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
+                         (R *)DQ.p, P);
+      autotuneT<R>(sa, mb, P);
+      tuneRequest = false;
+      enqueueSolve<R, EXACT>(sa, mb, P, false);
+    } else if (graphsCfg >= 0 && !profiling) {
+      solveGraph<R, EXACT>(sa, mb, P);
+    } else {
+      enqueueSolve<R, EXACT>(sa, mb, P, true);
     }
     record(5);
     HIPCHECK(hipGetLastError());
@@ -606,6 +712,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.tableMask = tsz - 1;
 
   c.hprops = P;
+  c.nDynamic = 0;
+  for (const BodyProp &b : P) c.nDynamic += b.mass > 0;
   c.props.reserve(N);
   HIPCHECK(hipMemcpy(c.props.p, P.data(), sizeof(BodyProp) * N, hipMemcpyHostToDevice));
   c.X.reserve(size_t(N) * XS); c.X0.reserve(size_t(N) * XS); c.Y.reserve(size_t(N) * XS);
@@ -722,6 +830,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->prec = cfg->precision;
     c->team = cfg->team > 0 ? cfg->team : 16;
     c->manCapCfg = cfg->manifold_cap;
+    c->graphsCfg = cfg->graphs;
     c->pairCapCfg = cfg->pair_cap;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {
@@ -755,6 +864,7 @@ int avbd_set_params(avbd_ctx *ctx, const avbd_params *p) {
     if (!p->cached_contact_jacobians)
       throw ApiError(AVBD_E_INVALID, "only cached contact Jacobians are implemented on the GPU");
     ctx->prm = *p;
+    ctx->dropGraphs();  // parameters are baked into the captured launches
   });
 }
 
@@ -932,7 +1042,11 @@ int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *c
       for (int s = 0; s < h.z; s++) {
         double *o = contacts + s * 16;
         for (int k = 0; k < 6; k++) o[k] = anc[s * 6 + k];
-        for (int k = 0; k < 9; k++) o[6 + k] = rd(hot, size_t(s) * HOT + k);
+        for (int k = 0; k < 3; k++) {
+          o[6 + k] = rd(hot, size_t(s) * HOT + HC0 + k);
+          o[9 + k] = rd(hot, size_t(s) * HOT + HK + k);
+          o[12 + k] = rd(hot, size_t(s) * HOT + HL + k);
+        }
         o[15] = stick[s];
       }
   });
@@ -984,6 +1098,18 @@ int avbd_set_profiling(avbd_ctx *ctx, int32_t on) {
     for (double &m : ctx->profMs) m = 0;
     ctx->profSteps = 0;
     for (int k = 0; k < AVBD_NUM_KERNELS; k++) ctx->kMs[k] = ctx->kBytes[k] = 0, ctx->kLaunches[k] = 0;
+  });
+}
+
+int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (ctx->prec == AVBD_PREC_EXACT) throw ApiError(AVBD_E_STATE, "the exact precision has a fixed team of 1");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->tuneRequest = true;
+    ctx->step();
+    if (team) *team = ctx->team;
+    if (ms) for (int k = 0; k < 6; k++) ms[k] = ctx->tuneMs[k];
   });
 }
 

@@ -51,6 +51,39 @@ template <class R> struct ManBuf {
 };
 constexpr int HOT = 16;
 constexpr int BAS = 12;
+// Offsets in a hot contact record. k and lambda lead so that the dual update
+// writes them as aligned vectors.
+constexpr int HK = 0, HL = 3, HC0 = 6, HRA = 9, HRB = 12;
+
+// Aligned 16-byte loads of N consecutive numbers (N a multiple of 4 floats or
+// 2 doubles; the records are 16-byte aligned).
+template <int N> __device__ __forceinline__ void vload(float *o, const float *p) {
+  const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+  for (int k = 0; k < N / 4; k++) {
+    const float4 v = q[k];
+    o[4 * k] = v.x; o[4 * k + 1] = v.y; o[4 * k + 2] = v.z; o[4 * k + 3] = v.w;
+  }
+}
+template <int N> __device__ __forceinline__ void vload(double *o, const double *p) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+  for (int k = 0; k < N / 2; k++) {
+    const double2 v = q[k];
+    o[2 * k] = v.x; o[2 * k + 1] = v.y;
+  }
+}
+// Store the six numbers k[3], lambda[3] at the head of a hot record.
+__device__ __forceinline__ void vstoreKL(float *p, const float *kl) {
+  reinterpret_cast<float4 *>(p)[0] = make_float4(kl[0], kl[1], kl[2], kl[3]);
+  reinterpret_cast<float2 *>(p)[2] = make_float2(kl[4], kl[5]);
+}
+__device__ __forceinline__ void vstoreKL(double *p, const double *kl) {
+  double2 *q = reinterpret_cast<double2 *>(p);
+  q[0] = make_double2(kl[0], kl[1]);
+  q[1] = make_double2(kl[2], kl[3]);
+  q[2] = make_double2(kl[4], kl[5]);
+}
 constexpr uint32_t MF_SPHERE = 1u;  // head.w flag: a sphere is involved
 
 // ---------------------------------------------------------------------------
@@ -433,12 +466,12 @@ __device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, c
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     R acc[3];
-    cross3(acc, hot + 9, bas + r * 3);
+    cross3(acc, hot + HRA, bas + r * 3);
     e.jA[r * 3] = acc[0]; e.jA[r * 3 + 1] = acc[1]; e.jA[r * 3 + 2] = acc[2];
-    cross3(acc, hot + 12, negb + r * 3);
+    cross3(acc, hot + HRB, negb + r * 3);
     e.jB[r * 3] = acc[0]; e.jB[r * 3 + 1] = acc[1]; e.jB[r * 3 + 2] = acc[2];
   }
-  const R *C0 = hot, *k = hot + 3, *lam = hot + 6;
+  const R *C0 = hot + HC0, *k = hot + HK, *lam = hot + HL;
   R acc[3];
   const R oma = R(1) - alpha;
   e.C[0] = C0[0] * oma; e.C[1] = C0[1] * oma; e.C[2] = C0[2] * oma;

@@ -302,8 +302,8 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
       if (compatible && ob.feat[os] == nc[i].feature) {
         const R *oh = ob.hot + os * HOT;
         for (int k = 0; k < 3; k++) {
-          kk[k] = double(oh[3 + k]);
-          lam[k] = double(oh[6 + k]);
+          kk[k] = double(oh[HK + k]);
+          lam[k] = double(oh[HL + k]);
           Cc[k] = double(ob.Cd[os * 4 + k]);
         }
         const int oldStick = ob.stick[os];
@@ -354,8 +354,8 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
     const size_t s = nbase + i;
     R *h = nb.hot + s * HOT;
     for (int k = 0; k < 3; k++) {
-      h[k] = R(C0[k]); h[3 + k] = R(kk[k]); h[6 + k] = R(lam[k]);
-      h[9 + k] = R(rAw[k]); h[12 + k] = R(rBw[k]);
+      h[HC0 + k] = R(C0[k]); h[HK + k] = R(kk[k]); h[HL + k] = R(lam[k]);
+      h[HRA + k] = R(rAw[k]); h[HRB + k] = R(rBw[k]);
     }
     h[15] = R(0);
     double *an = nb.anchor + s * 6;
@@ -794,145 +794,141 @@ __device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
 // H = M/dt^2 + sum J^T K J, f = M/dt^2 (x - y) + sum J^T lambda+, solve the
 // 6x6 system in f64 and apply Eq. 21. TEAM lanes share a body: each stamps a
 // strided share of its items and the team tree-reduces; TEAM = 1 with EXACT
-// stamps in the reference's order onto the inertial terms.
+// stamps in the reference's order onto the inertial terms. The colour's body
+// range comes from device memory and teams stride over it, so the launch
+// shape does not depend on the step's colouring.
 template <class R, bool EXACT, int TEAM>
-__global__ void __launch_bounds__(256) k_primal(int cnt, const int *bodies, SolveArgs S, ManBuf<R> mb, R *DQ,
-                                                StepParams P, double alpha) {
+__global__ void __launch_bounds__(256) k_primal(int c, const int *cstart, const int *cbodies, SolveArgs S,
+                                                ManBuf<R> mb, R *DQ, StepParams P, double alpha) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int team = gt / TEAM, lane = gt % TEAM;
-  const bool active = team < cnt;
-  const int i = active ? bodies[team] : 0;
-  const BodyProp &bp = S.props[i];
-  double *x = S.X + size_t(i) * XS;
-  const double *y = S.Y + size_t(i) * XS;
+  const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
+  const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
   const double dt2 = P.dt * P.dt;
-  Sys<double> s;
-  s.zero();
-  double Ml[9], Ma[9];
-  if (EXACT && active) {
-    // inertial terms first, as the reference assembles them
-    double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9];
-    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-    for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
-    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
-    mulv(s.rl, Ml, t);
-    qsub(dq, x + 3, y + 3);
-    mulv(s.ra, Ma, dq);
-    s.lin[0] = Ml[0]; s.lin[1] = Ml[3]; s.lin[2] = Ml[4]; s.lin[3] = Ml[6]; s.lin[4] = Ml[7]; s.lin[5] = Ml[8];
-    s.ang[0] = Ma[0]; s.ang[1] = Ma[3]; s.ang[2] = Ma[4]; s.ang[3] = Ma[6]; s.ang[4] = Ma[7]; s.ang[5] = Ma[8];
-    for (int k = 0; k < 9; k++) s.cross[k] = 0.0;
-  }
-  if (active) {
+  for (int team = gt / TEAM; team < cnt; team += teams) {
+    const int i = cbodies[c0 + team];
+    const BodyProp &bp = S.props[i];
+    double *x = S.X + size_t(i) * XS;
+    const double *y = S.Y + size_t(i) * XS;
+    Sys<double> s;
+    s.zero();
+    if (EXACT) {
+      // inertial terms first, as the reference assembles them
+      double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
+      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+      for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
+      double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
+      mulv(s.rl, Ml, t);
+      qsub(dq, x + 3, y + 3);
+      mulv(s.ra, Ma, dq);
+      s.lin[0] = Ml[0]; s.lin[1] = Ml[3]; s.lin[2] = Ml[4]; s.lin[3] = Ml[6]; s.lin[4] = Ml[7]; s.lin[5] = Ml[8];
+      s.ang[0] = Ma[0]; s.ang[1] = Ma[3]; s.ang[2] = Ma[4]; s.ang[3] = Ma[6]; s.ang[4] = Ma[7]; s.ang[5] = Ma[8];
+    }
     const int b0 = S.itemStart[i], b1 = S.itemStart[i + 1];
     const R ralpha = R(alpha);
     for (int t = b0 + lane; t < b1; t += TEAM) {
       const uint32_t it = S.items[t];
       const bool isA = it & 1u;
       if (it & 0x80000000u) {
-        const int j = int((it & 0x7fffffffu) >> 1);
-        jointStamp(s, S.joints[j], isA, S.X, alpha);
+        jointStamp(s, S.joints[int((it & 0x7fffffffu) >> 1)], isA, S.X, alpha);
         continue;
       }
       const int m = int(it >> 4), sl = int((it >> 1) & 7u);
       const int4 h = mb.head[m];
-      R bas[9], negb[9], dqA[6], dqB[6], hot[15];
-      const R *bsrc = mb.basis + size_t(m) * BAS;
-      loadRow(bas, bsrc, 9);
-      const R mu = bsrc[9];
+      R bas[BAS], negb[9], dqA[8], dqB[8], hot[HOT];
+      vload<BAS>(bas, mb.basis + size_t(m) * BAS);
+      vload<8>(dqA, DQ + size_t(h.x) * 8);
+      vload<8>(dqB, DQ + size_t(h.y) * 8);
+      vload<HOT>(hot, mb.hot + (size_t(m) * MAXC + sl) * HOT);
       for (int k = 0; k < 9; k++) negb[k] = -bas[k];
-      loadRow(dqA, DQ + size_t(h.x) * 8, 6);
-      loadRow(dqB, DQ + size_t(h.y) * 8, 6);
-      loadRow(hot, mb.hot + (size_t(m) * MAXC + sl) * HOT, 15);
       Eval<R> e;
-      evalContact<R, EXACT>(bas, negb, mu, hot, dqA, dqB, ralpha, e);
-      stampContact<R, double>(s, isA ? bas : negb, isA ? e.jA : e.jB, hot + 3, e.F);
+      evalContact<R, EXACT>(bas, negb, bas[9], hot, dqA, dqB, ralpha, e);
+      stampContact<R, double>(s, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
     }
-  }
-  if (TEAM > 1) {
+    if (TEAM > 1) {
 #pragma unroll
-    for (int off = TEAM / 2; off > 0; off >>= 1) {
+      for (int off = TEAM / 2; off > 0; off >>= 1) {
 #pragma unroll
-      for (int k = 0; k < 6; k++) {
-        s.lin[k] += __shfl_xor(s.lin[k], off, TEAM);
-        s.ang[k] += __shfl_xor(s.ang[k], off, TEAM);
-      }
+        for (int k = 0; k < 6; k++) {
+          s.lin[k] += __shfl_xor(s.lin[k], off, TEAM);
+          s.ang[k] += __shfl_xor(s.ang[k], off, TEAM);
+        }
 #pragma unroll
-      for (int k = 0; k < 9; k++) s.cross[k] += __shfl_xor(s.cross[k], off, TEAM);
+        for (int k = 0; k < 9; k++) s.cross[k] += __shfl_xor(s.cross[k], off, TEAM);
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        s.rl[k] += __shfl_xor(s.rl[k], off, TEAM);
-        s.ra[k] += __shfl_xor(s.ra[k], off, TEAM);
+        for (int k = 0; k < 3; k++) {
+          s.rl[k] += __shfl_xor(s.rl[k], off, TEAM);
+          s.ra[k] += __shfl_xor(s.ra[k], off, TEAM);
+        }
       }
     }
+    if (lane != 0) continue;
+    if (!EXACT) {
+      double MAng[9], Ma9[9], t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], ra[3];
+      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+      const double ml = bp.mass / dt2;
+      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] / dt2;
+      qsub(dq, x + 3, y + 3);
+      mulv(ra, Ma9, dq);
+      s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
+      s.ang[0] += Ma9[0]; s.ang[1] += Ma9[3]; s.ang[2] += Ma9[4];
+      s.ang[3] += Ma9[6]; s.ang[4] += Ma9[7]; s.ang[5] += Ma9[8];
+      for (int k = 0; k < 3; k++) { s.rl[k] += ml * t[k]; s.ra[k] += ra[k]; }
+    }
+    double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
+    double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
+    double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
+    double dxl[3], dxa[3];
+    solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+    for (int k = 0; k < 3; k++) x[k] = x[k] + dxl[k];
+    double q[4];
+    qint(q, x + 3, dxa);
+    for (int k = 0; k < 4; k++) x[3 + k] = q[k];
+    const double *x0 = S.X0 + size_t(i) * XS;
+    double dq[3];
+    qsub(dq, x + 3, x0 + 3);
+    R *d = DQ + size_t(i) * 8;
+    d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
+    d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
   }
-  if (!active || lane != 0) return;
-  if (!EXACT) {
-    double MAng[9];
-    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-    const double ml = bp.mass / dt2;
-    double Ma9[9];
-    for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] / dt2;
-    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], ra[3];
-    qsub(dq, x + 3, y + 3);
-    mulv(ra, Ma9, dq);
-    s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
-    s.ang[0] += Ma9[0]; s.ang[1] += Ma9[3]; s.ang[2] += Ma9[4];
-    s.ang[3] += Ma9[6]; s.ang[4] += Ma9[7]; s.ang[5] += Ma9[8];
-    for (int k = 0; k < 3; k++) { s.rl[k] += ml * t[k]; s.ra[k] += ra[k]; }
-  }
-  double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
-  double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
-  double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
-  double dxl[3], dxa[3];
-  solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
-  for (int k = 0; k < 3; k++) x[k] = x[k] + dxl[k];
-  double q[4];
-  qint(q, x + 3, dxa);
-  for (int k = 0; k < 4; k++) x[3 + k] = q[k];
-  const double *x0 = S.X0 + size_t(i) * XS;
-  double dq[3];
-  qsub(dq, x + 3, x0 + 3);
-  R *d = DQ + size_t(i) * 8;
-  d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
-  d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
 }
 
-// Dual update of every live manifold (manifold.js:447-491).
+// Dual update of every live manifold (manifold.js:447-491); the live count
+// comes from device memory, threads stride over it.
 template <class R, bool EXACT>
-__global__ void k_dual(int nLive, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
+__global__ void k_dual(const int *nLivePtr, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
                        int writeC) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nLive) return;
-  const int m = live[k];
-  const int4 h = mb.head[m];
-  R bas[9], negb[9], dqA[6], dqB[6];
-  const R *bsrc = mb.basis + size_t(m) * BAS;
-  loadRow(bas, bsrc, 9);
-  const R mu = bsrc[9];
-  for (int e = 0; e < 9; e++) negb[e] = -bas[e];
-  loadRow(dqA, DQ + size_t(h.x) * 8, 6);
-  loadRow(dqB, DQ + size_t(h.y) * 8, 6);
+  const int nLive = *nLivePtr;
   const R ralpha = R(alpha), beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
-  const bool hasSphere = h.w & MF_SPHERE;
-  for (int sl = 0; sl < h.z; sl++) {
-    const size_t si = size_t(m) * MAXC + sl;
-    R *hp = mb.hot + si * HOT;
-    R hot[15];
-    loadRow(hot, hp, 15);
-    Eval<R> e;
-    evalContact<R, EXACT>(bas, negb, mu, hot, dqA, dqB, ralpha, e);
-    hp[6] = e.F[0]; hp[7] = e.F[1]; hp[8] = e.F[2];
-    if (writeC) {
-      R *cd = mb.Cd + si * 4;
-      cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];
-    }
-    if (e.F[0] < R(0)) hp[3] = jmin(hot[3] + beta * xabs(e.C[0]), pmax);
-    if (e.within) {
-      hp[4] = jmin(hot[4] + beta * xabs(e.C[1]), pmax);
-      hp[5] = jmin(hot[5] + beta * xabs(e.C[2]), pmax);
-      mb.stick[si] = xsqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < stickT ? 1 : 0;
-    } else if (hasSphere) {
-      mb.stick[si] = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nLive; k += gridDim.x * blockDim.x) {
+    const int m = live[k];
+    const int4 h = mb.head[m];
+    R bas[BAS], negb[9], dqA[8], dqB[8];
+    vload<BAS>(bas, mb.basis + size_t(m) * BAS);
+    for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+    vload<8>(dqA, DQ + size_t(h.x) * 8);
+    vload<8>(dqB, DQ + size_t(h.y) * 8);
+    const bool hasSphere = h.w & MF_SPHERE;
+    for (int sl = 0; sl < h.z; sl++) {
+      const size_t si = size_t(m) * MAXC + sl;
+      R *hp = mb.hot + si * HOT;
+      R hot[HOT];
+      vload<HOT>(hot, hp);
+      Eval<R> e;
+      evalContact<R, EXACT>(bas, negb, bas[9], hot, dqA, dqB, ralpha, e);
+      if (writeC) {
+        R *cd = mb.Cd + si * 4;
+        cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];
+      }
+      R kl[6] = {hot[HK], hot[HK + 1], hot[HK + 2], e.F[0], e.F[1], e.F[2]};
+      if (e.F[0] < R(0)) kl[0] = jmin(hot[HK] + beta * xabs(e.C[0]), pmax);
+      if (e.within) {
+        kl[1] = jmin(hot[HK + 1] + beta * xabs(e.C[1]), pmax);
+        kl[2] = jmin(hot[HK + 2] + beta * xabs(e.C[2]), pmax);
+        mb.stick[si] = xsqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < stickT ? 1 : 0;
+      } else if (hasSphere) {
+        mb.stick[si] = 0;
+      }
+      vstoreKL(hp, kl);
     }
   }
 }

@@ -32,7 +32,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f64", "exact"])
-    ap.add_argument("--team", type=int, default=16)
+    ap.add_argument("--team", type=int, default=0, help="lanes per body; 0 = autotune")
     ap.add_argument("--order", default="priority", choices=["priority", "index"])
     ap.add_argument("--layers", type=int, default=53)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -53,17 +53,31 @@ def main():
     gpu.pack(scene, params)
     iters = params["iterations"]
     t0 = time.perf_counter()
-    gpu.step(args.warmup)
+    gpu.step(max(args.warmup - 1, 0))
+    tune = None
+    if args.team == 0 and prec != avbd.PREC_EXACT:
+        tune = gpu.autotune()  # one warm-up step that also picks the team shape
+    elif args.warmup > 0:
+        gpu.step(1)
     gpu.idle()
-    print(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, team {tune}", file=sys.stderr,
+          flush=True)
     c = gpu.counters()
     nd = c["dynamic_bodies"]
 
-    gpu.set_profiling(True)
+    # timed region: plain stepping (the solve loop replays a captured graph)
     t0 = time.perf_counter()
     gpu.step(args.steps)
     gpu.idle()
     secs = time.perf_counter() - t0
+
+    # the same number of further steps with per-kernel HIP events, for the
+    # roofline and the phase split (direct launches, so each is bracketed)
+    gpu.set_profiling(True)
+    t1 = time.perf_counter()
+    gpu.step(args.steps)
+    gpu.idle()
+    psecs = time.perf_counter() - t1
     phases, psteps = gpu.profile()
     pms, plaunch, pbytes = gpu.kernel_stats(L.K_PRIMAL)
     dms, dlaunch, dbytes = gpu.kernel_stats(L.K_DUAL)
@@ -92,7 +106,8 @@ def main():
                         f"on a static ground, dt 1/60, {iters} iterations, steps {args.warmup + 1}-"
                         f"{args.warmup + args.steps} after pack",
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
-            "precision": args.precision, "team": args.team, "color_order": args.order,
+            "precision": args.precision, "team": tune[0] if tune else args.team, "color_order": args.order,
+            "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],
             "manifolds": c["manifolds"], "contacts": c["contacts"], "colors": c["colors"],
         },
@@ -108,6 +123,7 @@ def main():
             "narrow_ms_per_step": nms / args.steps,
         },
         "phases_ms": phases,
+        "profiled_ms_per_step": psecs / args.steps * 1e3,
     }
     if not args.no_cpu:
         import pyoracle

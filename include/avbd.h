@@ -63,7 +63,9 @@ typedef struct avbd_config {
   int32_t team;            /* lanes per body in the primal solve (0: auto)  */
   int32_t manifold_cap;    /* max live + new manifolds (0: 32 per body)     */
   int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body)        */
-  int32_t reserved[3];
+  int32_t graphs;          /* solve loop as a replayed hipGraph: 0 auto (on
+                              unless profiling), -1 never                   */
+  int32_t reserved[2];
 } avbd_config;
 
 /* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform
@@ -205,6 +207,15 @@ int avbd_read_profile(avbd_ctx *ctx, float *ms, int32_t n, int32_t *steps);
 #define AVBD_NUM_KERNELS 3
 int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int64_t *launches,
                            double *bytes_total);
+
+/* Autotuner (no counterpart in the reference). Runs one step during which
+ * the primal sweep of the first iteration is timed for every team shape
+ * (lanes per body: 1, 4, 8, 16, 32, 64) on the step's own data, restoring the
+ * body state between trials; the step then completes with the fastest shape,
+ * which the context keeps. `team` receives it and `ms` (may be NULL) the six
+ * sweep times in that order. Not available in AVBD_PREC_EXACT, whose
+ * assembly order is fixed. */
+int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
 
 #ifdef __cplusplus
 }

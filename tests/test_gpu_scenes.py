@@ -174,3 +174,34 @@ def test_launch_through_write_state_matches_oracle():
     o = ref.state()
     assert np.array_equal(g["pos"], o[:, :3]) and np.array_equal(g["quat"], o[:, 3:7])
     assert g["pos"][L["body"], 2] < L["pos"][2] - 1.0  # the sphere came down onto the apex
+
+
+def test_autotune_keeps_results_and_picks_a_team():
+    scene, params, _ = avbd.scenes.pyramid(layers=10)
+    a = avbd.HipBackend(precision=avbd.PREC_MIXED)
+    a.pack(scene, params)
+    a.step(3)
+    team, ms = a.autotune()
+    assert team in (1, 4, 8, 16, 32, 64) and all(v > 0 for v in ms.values())
+    assert ms[team] == min(ms.values())
+    a.step(5)
+    b = avbd.HipBackend(precision=avbd.PREC_MIXED, team=team)
+    b.pack(scene, params)
+    b.step(9)
+    sa, sb = a.read_state(), b.read_state()
+    # the trials restore the state they touch: same result as a run with that team
+    assert np.array_equal(sa["pos"], sb["pos"]) and np.array_equal(sa["quat"], sb["quat"])
+
+
+@pytest.mark.parametrize("graphs", [0, -1], ids=["graph", "direct"])
+def test_graph_replay_equals_direct_launches(graphs):
+    scene, params, _ = avbd.scenes.pyramid(layers=8)
+    params = dict(params, colorOrder=1)
+    g = avbd.HipBackend(precision=avbd.PREC_EXACT, graphs=graphs)
+    g.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=1)
+    g.step(12)
+    ref.step(12)
+    st = g.read_state()
+    o = ref.state()
+    assert np.array_equal(st["pos"], o[:, :3]) and np.array_equal(st["quat"], o[:, 3:7])
