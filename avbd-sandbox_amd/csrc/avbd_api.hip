@@ -5,6 +5,7 @@
 // (solver.js:307-493), with the primal sweep scheduled by the greedy colouring
 // of the force graph (coloring.js:43-118) taken after force initialisation.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -178,13 +179,19 @@ struct avbd_ctx {
     evPool.pop_back();
     return e;
   }
-  // Bracket the launches issued by f with events when profiling.
-  template <class F> void timed(int kid, double bytes, F &&f) {
-    if (!profiling) { f(); return; }
+  // Launch a kernel on the stream. When profiling, the launch carries a pair of
+  // events that the dispatch itself stamps at its begin and end
+  // (hipExtLaunchKernel), so the time is the kernel's own duration, the same
+  // interval a kernel trace reports, without event packets in between.
+  template <class... KArgs, class... Args>
+  void timedLaunch(int kid, double bytes, void (*kernel)(KArgs...), dim3 grid, dim3 block, Args... args) {
+    if (!profiling) {
+      hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+      return;
+    }
     Pending p{kid, takeEvent(), takeEvent(), bytes};
-    HIPCHECK(hipEventRecord(p.a, stream));
-    f();
-    HIPCHECK(hipEventRecord(p.b, stream));
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, p.a, p.b, 0, args...);
+    HIPCHECK(hipGetLastError());
     pending.push_back(p);
   }
   void harvest() {
@@ -359,10 +366,9 @@ struct avbd_ctx {
   template <class R, bool EXACT, int TEAM>
   void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
     if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
-    timed(AVBD_K_PRIMAL, profiling ? double(hcbytes[c]) : 0.0, [&] {
-      hipLaunchKernelGGL((k_primal<R, EXACT, TEAM>), dim3(primalBlocks(TEAM)), dim3(256), 0, stream, c,
-                         cmeta.p + 8, cbodies.p, sa, mb, (R *)DQ.p, P, alpha);
-    });
+    timedLaunch(AVBD_K_PRIMAL, profiling ? double(hcbytes[c]) : 0.0, k_primal<R, EXACT, TEAM>,
+                dim3(primalBlocks(TEAM)), dim3(256), c, (const int *)(cmeta.p + 8), (const int *)cbodies.p, sa, mb,
+                (R *)DQ.p, P, alpha);
   }
 
   template <class R, bool EXACT>
@@ -390,10 +396,9 @@ struct avbd_ctx {
       for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team);
       if (it < prm.iterations) {
         if (graphCapturing || nLive > 0)
-          timed(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, [&] {
-            hipLaunchKernelGGL((k_dual<R, EXACT>), dim3(dualBlocks()), dim3(128), 0, stream, scal.p + 2,
-                               live[cur].p, mb, (const R *)DQ.p, P, alpha, it == prm.iterations - 1 ? 1 : 0);
-          });
+          timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
+                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha,
+                      it == prm.iterations - 1 ? 1 : 0);
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
       if (it == prm.iterations - 1)
@@ -519,10 +524,9 @@ struct avbd_ctx {
     ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
     HIPCHECK(hipMemsetAsync(scal.p + 8, 0, sizeof(int), stream));
     if (nTot > 0)
-      timed(AVBD_K_NARROW, 0.0, [&] {
-        hipLaunchKernelGGL(k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), 0, stream, nOld, nNew, live[cur].p, ob, nb,
-                           newPairs.p, props.p, X.p, P, keep.p, scal.p + 8);
-      });
+      timedLaunch(AVBD_K_NARROW, 0.0, k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), nOld, nNew,
+                  (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p, (const BodyProp *)props.p,
+                  (const double *)X.p, P, keep.p, scal.p + 8);
     if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
     scan(keep.p, keepPos.p, nTot, scal.p + 2);
     if (nTot > 0)

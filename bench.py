@@ -7,8 +7,10 @@ seconds, over exactly --steps steps after --warmup untimed steps.
 
 Also reported:
   roofline      the primal kernel (the dominant kernel): algorithmic bytes per
-                launch / average launch duration (HIP events on the library's
-                stream, over the timed region) against 8 TB/s HBM;
+                launch / average launch duration (events stamped by each
+                dispatch on the library's stream, over a second region of the
+                same length) against 8 TB/s HBM; traffic = HBM bytes per launch
+                from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary;
   cpu_baseline  the f64 CPU oracle (a single-threaded restatement of the
                 reference engine) on a bounded sample of the same workload.
 """
@@ -26,6 +28,36 @@ METRIC = "body-iterations/s on 51k-body pyramid vs host f64 CPU; adjoint/forward
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(precision, team):
+    """HBM bytes per k_primal launch (read + written) from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the same workload,
+    scaled by the copy-probe calibration; None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"k_primal_pyramid51k_{precision}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        doc = json.load(f)
+    k = doc["kernels"].get(f"k_primal<{'float' if precision == 'mixed' else 'double'}, "
+                           f"{'true' if precision == 'exact' else 'false'}, {team}>")
+    if doc.get("team") != team or k is None:
+        return None, None
+    return k["read_bytes"] + k["write_bytes"], os.path.relpath(path, ROOT)
+
+
+def copy_probe(mib):
+    """Three launches of one elementwise kernel that reads MIB MiB and writes
+    MIB MiB: what a byte counter reads for it calibrates the counter."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,11 +69,17 @@ def main():
     ap.add_argument("--layers", type=int, default=53)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the known-size reference "
+                         "for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import avbd
     from avbd import _lib as L
@@ -88,6 +126,8 @@ def main():
     value = nd * iters * args.steps / secs
     avg_ms = pms / plaunch
     achieved = (pbytes / plaunch) / (avg_ms * 1e-3) / 1e9
+    team = tune[0] if tune else args.team
+    traffic, traffic_src = measured_traffic(args.precision, team)
     line = {
         "metric": METRIC,
         "value": value,
@@ -106,14 +146,14 @@ def main():
                         f"on a static ground, dt 1/60, {iters} iterations, steps {args.warmup + 1}-"
                         f"{args.warmup + args.steps} after pack",
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
-            "precision": args.precision, "team": tune[0] if tune else args.team, "color_order": args.order,
+            "precision": args.precision, "team": team, "color_order": args.order,
             "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],
             "manifolds": c["manifolds"], "contacts": c["contacts"], "colors": c["colors"],
         },
         "roofline": {
             "kernel": "k_primal", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "avg_launch_us": avg_ms * 1e3, "launches": plaunch,
             "alg_bytes_per_launch": pbytes / plaunch,
         },
