@@ -130,6 +130,7 @@ struct avbd_ctx {
   DBuf<int> color[2];
   int colorCur = 0;
   DBuf<int> ccount, cstart, ccursor, cbodies, cmeta;
+  DBuf<int4> cwork;
   std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
   int nDynamic = 0;
@@ -144,7 +145,7 @@ struct avbd_ctx {
   DBuf<char> Xbk, DQbk;
   int numColors = 0, colorRounds = 0;
   DBuf<int> itemCnt, itemStart;
-  DBuf<uint32_t> items;
+  DBuf<uint2> items;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
   // 5 overflow, 6 items, 7 adj total
@@ -211,10 +212,11 @@ struct avbd_ctx {
   ByteModel primalModel() const {
     const unsigned long long rs = rsize();
     ByteModel m;
-    m.own = 7 * 8 * 3 + 40 + 8 + 7 * 8 + 6 * rs;  // X, Y, X0, props, itemStart; X and delta written
-    m.man = 16 + 10 * rs + 6 * rs;                 // head, basis + mu, other body's delta
-    m.item = 4 + 15 * rs;                          // item word, hot contact record
+    m.own = 7 * 8 * 3 + 40 + 16 + 6 * rs + 7 * 8 + 6 * rs;  // X, Y, X0, props, work entry, delta; X and delta written
+    m.man = 10 * rs + 6 * rs;                               // basis + mu, other body's delta
+    m.item = 8 + 15 * rs;                                   // item, hot contact record
     m.joint = sizeof(JointDev) + 2 * 7 * 8 + 4;    // joint record, both bodies' poses
+    m.dualw = 6 * rs + 4;                           // k, lambda and stick written by a first visitor
     return m;
   }
   double dualBytes() const {
@@ -238,7 +240,7 @@ struct avbd_ctx {
     countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
-    color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release();
+    color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release(); cwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     for (auto &b : scanTmp) b.release();
     for (auto &e : evs) (void)hipEventDestroy(e);
@@ -343,7 +345,7 @@ struct avbd_ctx {
                          cmeta.p + 4);
       hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ccount.p, cmeta.p + 8, ccursor.p);
       hipLaunchKernelGGL(k_color_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, ccursor.p,
-                         cbodies.p);
+                         cbodies.p, itemStart.p, items.p, cwork.p);
       HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
       if (hmeta[1]) break;
@@ -364,23 +366,27 @@ struct avbd_ctx {
   int dualBlocks() const { return std::max(1, std::min(blocks(manCap, 128), 2048)); }
 
   template <class R, bool EXACT, int TEAM>
-  void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha) {
+  void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int dual,
+                    double dualAlpha) {
     if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
-    timedLaunch(AVBD_K_PRIMAL, profiling ? double(hcbytes[c]) : 0.0, k_primal<R, EXACT, TEAM>,
-                dim3(primalBlocks(TEAM)), dim3(256), c, (const int *)(cmeta.p + 8), (const int *)cbodies.p, sa, mb,
-                (R *)DQ.p, P, alpha);
+    const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
+    timedLaunch(AVBD_K_PRIMAL, bytes, k_primal<R, EXACT, TEAM>, dim3(primalBlocks(TEAM)), dim3(256), c,
+                (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
   }
 
+  // One colour of the sweep; dual != 0 folds in the previous iteration's
+  // contact dual update (evaluated with dualAlpha).
   template <class R, bool EXACT>
-  void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int tm) {
-    if (EXACT) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha); return; }
+  void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int tm, int dual,
+              double dualAlpha) {
+    if (EXACT) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha, dual, dualAlpha); return; }
     switch (tm) {
-      case 1: launchPrimal<R, false, 1>(c, sa, mb, P, alpha); break;
-      case 4: launchPrimal<R, false, 4>(c, sa, mb, P, alpha); break;
-      case 8: launchPrimal<R, false, 8>(c, sa, mb, P, alpha); break;
-      case 32: launchPrimal<R, false, 32>(c, sa, mb, P, alpha); break;
-      case 64: launchPrimal<R, false, 64>(c, sa, mb, P, alpha); break;
-      default: launchPrimal<R, false, 16>(c, sa, mb, P, alpha); break;
+      case 1: launchPrimal<R, false, 1>(c, sa, mb, P, alpha, dual, dualAlpha); break;
+      case 4: launchPrimal<R, false, 4>(c, sa, mb, P, alpha, dual, dualAlpha); break;
+      case 8: launchPrimal<R, false, 8>(c, sa, mb, P, alpha, dual, dualAlpha); break;
+      case 32: launchPrimal<R, false, 32>(c, sa, mb, P, alpha, dual, dualAlpha); break;
+      case 64: launchPrimal<R, false, 64>(c, sa, mb, P, alpha, dual, dualAlpha); break;
+      default: launchPrimal<R, false, 16>(c, sa, mb, P, alpha, dual, dualAlpha); break;
     }
   }
 
@@ -390,15 +396,20 @@ struct avbd_ctx {
     if (withInit)
       hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
                          (R *)DQ.p, P);
+    // The contact dual update of iteration it runs inside the primal sweep of
+    // iteration it + 1 (first visitors apply it); after the last iteration it
+    // runs as its own pass, which also records C for the next step.
     const int total = prm.iterations + (prm.post_stabilize ? 1 : 0);
+    auto alphaOf = [&](int it) { return prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha; };
     for (int it = 0; it < total; it++) {
-      const double alpha = prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha;
-      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team);
+      const double alpha = alphaOf(it);
+      const int fused = it >= 1 && it < prm.iterations ? 1 : 0;
+      const double dualAlpha = fused ? alphaOf(it - 1) : alpha;
+      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team, fused, dualAlpha);
       if (it < prm.iterations) {
-        if (graphCapturing || nLive > 0)
+        if (it == prm.iterations - 1 && (graphCapturing || nLive > 0))
           timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
-                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha,
-                      it == prm.iterations - 1 ? 1 : 0);
+                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha, 1);
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
       if (it == prm.iterations - 1)
@@ -458,7 +469,7 @@ struct avbd_ctx {
     float bestMs = 1e30f;
     for (int k = 0; k < 6; k++) {
       HIPCHECK(hipEventRecord(a, stream));
-      for (int c = 0; c < numColors; c++) primal<R, false>(c, sa, mb, P, prm.alpha, cands[k]);
+      for (int c = 0; c < numColors; c++) primal<R, false>(c, sa, mb, P, prm.alpha, cands[k], 0, prm.alpha);
       HIPCHECK(hipEventRecord(b, stream));
       HIPCHECK(hipEventSynchronize(b));
       float ms = 0;
@@ -544,11 +555,11 @@ struct avbd_ctx {
     record(3);
     colorize();
     if (profiling && numColors > 0) {
-      HIPCHECK(hipMemsetAsync(cbytes.p, 0, sizeof(unsigned long long) * MAX_COLORS, stream));
+      HIPCHECK(hipMemsetAsync(cbytes.p, 0, sizeof(unsigned long long) * 2 * MAX_COLORS, stream));
       hipLaunchKernelGGL(k_color_bytes, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, color[colorCur].p,
-                         adjStart.p, adjKey.p, itemCnt.p, primalModel(), cbytes.p);
-      hcbytes.resize(MAX_COLORS);
-      HIPCHECK(hipMemcpyAsync(hcbytes.data(), cbytes.p, sizeof(unsigned long long) * MAX_COLORS,
+                         adjStart.p, adjKey.p, itemStart.p, items.p, primalModel(), cbytes.p);
+      hcbytes.resize(2 * MAX_COLORS);
+      HIPCHECK(hipMemcpyAsync(hcbytes.data(), cbytes.p, sizeof(unsigned long long) * 2 * MAX_COLORS,
                               hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
     }
@@ -783,10 +794,11 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.colorCur = 0;
   c.ccount.reserve(MAX_COLORS); c.cstart.reserve(MAX_COLORS + 1); c.ccursor.reserve(MAX_COLORS);
   c.cbodies.reserve(N);
+  c.cwork.reserve(N);
   c.itemCnt.reserve(N); c.itemStart.reserve(N + 1);
   c.items.reserve(2 * (size_t(MAXC) * c.manCap + c.J) + 1);
   c.scal.reserve(16);
-  c.cbytes.reserve(MAX_COLORS);
+  c.cbytes.reserve(2 * MAX_COLORS);
   c.cmeta.reserve(8 + MAX_COLORS + 1);
   HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
   c.evs.resize(AVBD_NUM_PHASES);

@@ -459,9 +459,11 @@ template <class R> struct Eval {
   bool within;
 };
 
+// Constraint value C = (1 - alpha) C0 + J dq of a contact from the cached
+// Jacobian at x_t (manifold.js:325-410), and the angular Jacobian blocks.
 template <class R, bool EXACT>
-__device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, const R *hot, const R *dqA,
-                                            const R *dqB, R alpha, Eval<R> &e) {
+__device__ __forceinline__ void evalC(const R *bas, const R *negb, const R *hot, const R *dqA, const R *dqB, R alpha,
+                                      Eval<R> &e) {
   // angular blocks from the cached world anchors
 #pragma unroll
   for (int r = 0; r < 3; r++) {
@@ -471,7 +473,7 @@ __device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, c
     cross3(acc, hot + HRB, negb + r * 3);
     e.jB[r * 3] = acc[0]; e.jB[r * 3 + 1] = acc[1]; e.jB[r * 3 + 2] = acc[2];
   }
-  const R *C0 = hot + HC0, *k = hot + HK, *lam = hot + HL;
+  const R *C0 = hot + HC0;
   R acc[3];
   const R oma = R(1) - alpha;
   e.C[0] = C0[0] * oma; e.C[1] = C0[1] * oma; e.C[2] = C0[2] * oma;
@@ -483,6 +485,11 @@ __device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, c
   e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
   mulv(acc, e.jB, dqB + 3);
   e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
+}
+
+// Force F = clamp(K C + lambda) into the friction cone (manifold.js:411-430).
+template <class R, bool EXACT>
+__device__ __forceinline__ void evalF(const R *k, const R *lam, R mu, Eval<R> &e) {
   if (EXACT) {
     // K C with the diagonal K written out as the reference's full product
     e.F[0] = k[0] * e.C[0] + R(0) * e.C[1] + R(0) * e.C[2];
@@ -500,6 +507,33 @@ __device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, c
     e.F[2] *= bounds / fs;
   }
   e.within = fs <= bounds;
+}
+
+template <class R, bool EXACT>
+__device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, const R *hot, const R *dqA,
+                                            const R *dqB, R alpha, Eval<R> &e) {
+  evalC<R, EXACT>(bas, negb, hot, dqA, dqB, alpha, e);
+  evalF<R, EXACT>(hot + HK, hot + HL, mu, e);
+}
+
+// Manifold.updateDual for one contact (manifold.js:447-491) from its
+// evaluation e with the current k: the new k and lambda into kl[6] (k then
+// lambda), and the stick flag to store, or -1 to leave it as it is.
+template <class R>
+__device__ __forceinline__ int dualUpdate(const R *k, const Eval<R> &e, R beta, R pmax, R stickT, bool hasSphere,
+                                          R *kl) {
+  kl[0] = k[0]; kl[1] = k[1]; kl[2] = k[2];
+  kl[3] = e.F[0]; kl[4] = e.F[1]; kl[5] = e.F[2];
+  if (e.F[0] < R(0)) kl[0] = jmin(k[0] + beta * xabs(e.C[0]), pmax);
+  int stick = -1;
+  if (e.within) {
+    kl[1] = jmin(k[1] + beta * xabs(e.C[1]), pmax);
+    kl[2] = jmin(k[2] + beta * xabs(e.C[2]), pmax);
+    stick = xsqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < stickT ? 1 : 0;
+  } else if (hasSphere) {
+    stick = 0;
+  }
+  return stick;
 }
 
 // Accumulated 6x6 system: lower triangles of the Lin and Ang blocks, the full

@@ -10,6 +10,8 @@
 //                 and velocity after the last main iteration
 // References are file:line in the reference repository.
 #pragma once
+#include <type_traits>
+
 #include "avbd_device.hpp"
 
 namespace avbd {
@@ -651,13 +653,36 @@ __global__ void k_color_check(int *ctl) {
   ctl[0] = 0;
 }
 
+// Flags in the y word of a contact work item (see k_items).
+constexpr uint32_t ITEM_FIRST = 0x80000000u, ITEM_SPHERE = 0x40000000u, ITEM_BODY = 0x3fffffffu;
+
+// Wave-aggregated bucket counting: the lanes of a wave that hold the same
+// colour add their number once (one atomic per colour present in the wave
+// rather than one per body).
+__device__ __forceinline__ int waveBucketAdd(int c, int *counter) {
+  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+  unsigned long long todo = __ballot(c >= 0);
+  int pos = -1;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cl = __shfl(c, leader);
+    const unsigned long long same = __ballot(c == cl);
+    int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&counter[cl], __popcll(same));
+    base = __shfl(base, leader);
+    if (c == cl) pos = base + __popcll(same & lt);
+    todo &= ~same;
+  }
+  return pos;
+}
+
 __global__ void k_color_count(int n, const int *color, int *ccount, int *numColors) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = color[i];
-  if (c < 0) return;
-  atomicAdd(&ccount[c], 1);
-  atomicMax(numColors, c + 1);
+  const int c = i < n ? color[i] : -1;
+  waveBucketAdd(c, ccount);
+  int top = c + 1;
+  for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
+  if (__lane_id() == 0 && top > 0) atomicMax(numColors, top);
 }
 
 // Exclusive scan of the MAX_COLORS bucket counts (one 256-thread block):
@@ -679,21 +704,41 @@ __global__ void k_color_starts(const int *ccount, int *cstart, int *cursor) {
   if (t == MAX_COLORS - 1) cstart[MAX_COLORS] = sh[t];
 }
 
-__global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodies) {
+// Bucket the dynamic bodies by colour: cbodies (the body list of each colour)
+// and cwork, the same list with each body's work-item range and its number of
+// joint items, which is all a primal team needs to find its work.
+// It also marks each contact item whose body comes first in the colour order
+// (the other body is static or of a later colour).
+__global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodies, const int *itemStart,
+                                uint2 *items, int4 *cwork) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = color[i];
+  const int c = i < n ? color[i] : -1;
+  const int pos = waveBucketAdd(c, cursor);
   if (c < 0) return;
-  cbodies[atomicAdd(&cursor[c], 1)] = i;
+  const int b0 = itemStart[i], b1 = itemStart[i + 1];
+  int nj = 0;
+  for (int t = b0; t < b1; t++) {
+    const uint2 w = items[t];
+    if (w.x & 0x80000000u) { nj++; continue; }
+    const int co = color[w.y & ITEM_BODY];
+    items[t].y = (w.y & ~ITEM_FIRST) | ((co < 0 || co > c) ? ITEM_FIRST : 0u);
+  }
+  cbodies[pos] = i;
+  cwork[pos] = make_int4(i, b0, b1, nj);
 }
 
 // Per dynamic body: its contact/joint work items in the order the reference
 // visits them (forces newest first, contacts in order; solver.js:443-448).
-// Item: manifold slot (m << 4 | s << 1 | isA), or 0x80000000 | j << 1 | isA.
+// Item: x = manifold slot (m << 4 | s << 1 | isA), or 0x80000000 | j << 1 |
+// isA; y = the other body (-1 for a joint's world anchor), so that its delta
+// record can be fetched together with the contact record. For a contact, y
+// also carries two flags: a sphere is involved, and (set after colouring) this
+// body is the contact's first visitor in a sweep, the one that applies the
+// contact's dual update of the previous iteration.
 template <int PASS>
 __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart, const int *adjKey, const int *live,
                         const int4 *head, const JointDev *joints, int *itemCnt, const int *itemStart,
-                        uint32_t *items, int cap, int *overflow) {
+                        uint2 *items, int cap, int *overflow) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (!(props[i].mass > 0)) { if (PASS == 0) itemCnt[i] = 0; return; }
@@ -703,14 +748,20 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
   for (int e = adjStart[i + 1] - 1; e >= adjStart[i]; e--) {
     const int key = adjKey[e];
     if (key < J) {
-      if (PASS == 1) items[base + cnt] = 0x80000000u | (uint32_t(key) << 1) | (joints[key].a == i ? 1u : 0u);
+      if (PASS == 1) {
+        const bool isA = joints[key].a == i;
+        items[base + cnt] = make_uint2(0x80000000u | (uint32_t(key) << 1) | (isA ? 1u : 0u),
+                                       uint32_t(isA ? joints[key].b : joints[key].a));
+      }
       cnt++;
     } else {
       const int m = live[key - J];
       const int4 h = head[m];
       if (PASS == 1)
         for (int s = 0; s < h.z; s++)
-          items[base + cnt + s] = (uint32_t(m) << 4) | (uint32_t(s) << 1) | (h.x == i ? 1u : 0u);
+          items[base + cnt + s] =
+              make_uint2((uint32_t(m) << 4) | (uint32_t(s) << 1) | (h.x == i ? 1u : 0u),
+                         uint32_t(h.x == i ? h.y : h.x) | ((h.w & MF_SPHERE) ? ITEM_SPHERE : 0u));
       cnt += h.z;
     }
   }
@@ -720,15 +771,23 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
 // Algorithmic bytes of one primal visit of each dynamic body, summed per
 // colour: own records, plus per manifold touched (head, basis, the other
 // body's delta record) and per contact (item word, hot record).
-struct ByteModel { unsigned long long own, man, item, joint; };
+// cbytes[c]: a sweep without the folded dual update; cbytes[MAX_COLORS + c]:
+// what the folded update adds (k, lambda and the stick flag written by first
+// visitors).
+struct ByteModel { unsigned long long own, man, item, joint, dualw; };
 __global__ void k_color_bytes(int n, int J, const BodyProp *props, const int *color, const int *adjStart,
-                              const int *adjKey, const int *itemCnt, ByteModel bm, unsigned long long *cbytes) {
+                              const int *adjKey, const int *itemStart, const uint2 *items, ByteModel bm,
+                              unsigned long long *cbytes) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !(props[i].mass > 0)) return;
-  int nm = 0, nj = 0;
+  int nm = 0, nj = 0, nf = 0;
   for (int e = adjStart[i]; e < adjStart[i + 1]; e++) (adjKey[e] < J ? nj : nm)++;
-  const unsigned long long b = bm.own + bm.man * nm + bm.item * (itemCnt[i] - nj) + bm.joint * nj;
+  for (int t = itemStart[i]; t < itemStart[i + 1]; t++)
+    if (!(items[t].x & 0x80000000u) && (items[t].y & ITEM_FIRST)) nf++;
+  const int nc = itemStart[i + 1] - itemStart[i] - nj;
+  const unsigned long long b = bm.own + bm.man * nm + bm.item * nc + bm.joint * nj;
   atomicAdd(&cbytes[color[i]], b);
+  atomicAdd(&cbytes[MAX_COLORS + color[i]], bm.dualw * nf);
 }
 
 // ------------------------------------------------------------------ solve
@@ -778,7 +837,7 @@ struct SolveArgs {
   double *X;
   const double *X0, *Y;
   const int *itemStart;
-  const uint32_t *items;
+  const uint2 *items;
   const int4 *head;
   const JointDev *joints;
   int nBodies;
@@ -797,53 +856,113 @@ __device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
 // stamps in the reference's order onto the inertial terms. The colour's body
 // range comes from device memory and teams stride over it, so the launch
 // shape does not depend on the step's colouring.
+// Work items a lane has in flight at once: their records are all requested
+// before any is evaluated, so a team waits for memory once per round rather
+// than once per contact.
+constexpr int PRIMAL_UNROLL = 1;
+
+// One colour of the primal sweep (solver.js:419-458): a team of TEAM lanes per
+// body gathers its contacts and joints, reduces the 6x6 system and solves it.
+// With dual set, the sweep also applies the previous iteration's dual update
+// (manifold.js:447-491, evaluated with dualAlpha) to each contact whose first
+// visitor this body is: both of its bodies are still where the previous
+// iteration left them, so this is the update the separate dual pass would
+// make, and the later visitor reads the updated k and lambda.
+//
+// In the exact mode (TEAM 1, f64) every term is added in the reference's
+// order. Otherwise each lane sums its contacts in the record precision R (the
+// contact terms are computed in R), the team reduces in f64, and lane 0 adds
+// the inertial terms and the joints (cwork.w of them) in f64: joint rows can
+// be stiff enough to swamp a light body's inertia in f32.
 template <class R, bool EXACT, int TEAM>
-__global__ void __launch_bounds__(256) k_primal(int c, const int *cstart, const int *cbodies, SolveArgs S,
-                                                ManBuf<R> mb, R *DQ, StepParams P, double alpha) {
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
+                                                ManBuf<R> mb, R *DQ, StepParams P, double alpha, int dual,
+                                                double dualAlpha) {
+  constexpr int U = PRIMAL_UNROLL;
+  using Acc = typename std::conditional<EXACT, double, R>::type;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
   const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
   const double dt2 = P.dt * P.dt;
   for (int team = gt / TEAM; team < cnt; team += teams) {
-    const int i = cbodies[c0 + team];
+    const int4 w = cwork[c0 + team];
+    const int i = w.x, b0 = w.y, b1 = w.z;
     const BodyProp &bp = S.props[i];
     double *x = S.X + size_t(i) * XS;
     const double *y = S.Y + size_t(i) * XS;
-    Sys<double> s;
-    s.zero();
-    if (EXACT) {
+    const double *x0 = S.X0 + size_t(i) * XS;
+    R dqi[8];
+    vload<8>(dqi, DQ + size_t(i) * 8);
+    Sys<Acc> sl;
+    sl.zero();
+    if constexpr (EXACT) {
       // inertial terms first, as the reference assembles them
       double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
       worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
       for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
       double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
-      mulv(s.rl, Ml, t);
+      mulv(sl.rl, Ml, t);
       qsub(dq, x + 3, y + 3);
-      mulv(s.ra, Ma, dq);
-      s.lin[0] = Ml[0]; s.lin[1] = Ml[3]; s.lin[2] = Ml[4]; s.lin[3] = Ml[6]; s.lin[4] = Ml[7]; s.lin[5] = Ml[8];
-      s.ang[0] = Ma[0]; s.ang[1] = Ma[3]; s.ang[2] = Ma[4]; s.ang[3] = Ma[6]; s.ang[4] = Ma[7]; s.ang[5] = Ma[8];
+      mulv(sl.ra, Ma, dq);
+      sl.lin[0] = Ml[0]; sl.lin[1] = Ml[3]; sl.lin[2] = Ml[4]; sl.lin[3] = Ml[6]; sl.lin[4] = Ml[7]; sl.lin[5] = Ml[8];
+      sl.ang[0] = Ma[0]; sl.ang[1] = Ma[3]; sl.ang[2] = Ma[4]; sl.ang[3] = Ma[6]; sl.ang[4] = Ma[7]; sl.ang[5] = Ma[8];
     }
-    const int b0 = S.itemStart[i], b1 = S.itemStart[i + 1];
-    const R ralpha = R(alpha);
-    for (int t = b0 + lane; t < b1; t += TEAM) {
-      const uint32_t it = S.items[t];
-      const bool isA = it & 1u;
-      if (it & 0x80000000u) {
-        jointStamp(s, S.joints[int((it & 0x7fffffffu) >> 1)], isA, S.X, alpha);
-        continue;
+    const R ralpha = R(alpha), rdalpha = R(dualAlpha);
+    const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
+    for (int t = b0 + lane; t < b1; t += TEAM * U) {
+      uint2 it[U];
+      R bas[U][BAS], hot[U][HOT], dqo[U][8];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int tu = t + u * TEAM;
+        it[u] = tu < b1 ? S.items[tu] : make_uint2(0xffffffffu, 0u);
       }
-      const int m = int(it >> 4), sl = int((it >> 1) & 7u);
-      const int4 h = mb.head[m];
-      R bas[BAS], negb[9], dqA[8], dqB[8], hot[HOT];
-      vload<BAS>(bas, mb.basis + size_t(m) * BAS);
-      vload<8>(dqA, DQ + size_t(h.x) * 8);
-      vload<8>(dqB, DQ + size_t(h.y) * 8);
-      vload<HOT>(hot, mb.hot + (size_t(m) * MAXC + sl) * HOT);
-      for (int k = 0; k < 9; k++) negb[k] = -bas[k];
-      Eval<R> e;
-      evalContact<R, EXACT>(bas, negb, bas[9], hot, dqA, dqB, ralpha, e);
-      stampContact<R, double>(s, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (it[u].x & 0x80000000u) continue;  // joint, or no item
+        const int m = int(it[u].x >> 4), sl = int((it[u].x >> 1) & 7u);
+        vload<BAS>(bas[u], mb.basis + size_t(m) * BAS);
+        vload<HOT>(hot[u], mb.hot + (size_t(m) * MAXC + sl) * HOT);
+        vload<8>(dqo[u], DQ + size_t(it[u].y & ITEM_BODY) * 8);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (it[u].x == 0xffffffffu) continue;
+        const bool isA = it[u].x & 1u;
+        if (it[u].x & 0x80000000u) {
+          if constexpr (EXACT) jointStamp(sl, S.joints[int((it[u].x & 0x7fffffffu) >> 1)], isA, S.X, alpha);
+          continue;
+        }
+        R negb[9];
+        for (int k = 0; k < 9; k++) negb[k] = -bas[u][k];
+        const R *dA = isA ? dqi : dqo[u], *dB = isA ? dqo[u] : dqi;
+        Eval<R> e;
+        if (dual && (it[u].y & ITEM_FIRST)) {
+          evalContact<R, EXACT>(bas[u], negb, bas[u][9], hot[u], dA, dB, rdalpha, e);
+          R kl[6];
+          const int st = dualUpdate(hot[u] + HK, e, beta, pmax, stickT, it[u].y & ITEM_SPHERE, kl);
+          const size_t si = size_t(it[u].x >> 4) * MAXC + ((it[u].x >> 1) & 7u);
+          vstoreKL(mb.hot + si * HOT, kl);
+          if (st >= 0) mb.stick[si] = st;
+#pragma unroll
+          for (int k = 0; k < 6; k++) hot[u][HK + k] = kl[k];  // k then lambda
+          if (rdalpha != ralpha) evalC<R, EXACT>(bas[u], negb, hot[u], dA, dB, ralpha, e);
+          evalF<R, EXACT>(hot[u] + HK, hot[u] + HL, bas[u][9], e);
+        } else {
+          evalContact<R, EXACT>(bas[u], negb, bas[u][9], hot[u], dA, dB, ralpha, e);
+        }
+        stampContact<R, Acc>(sl, isA ? bas[u] : negb, isA ? e.jA : e.jB, hot[u] + HK, e.F);
+      }
     }
+    // own pose and the step's start and inertial target, for lane 0's solve
+    // (requested before the reduction so that they arrive meanwhile)
+    double xs[7], ys[7], x0s[7];
+    if (lane == 0)
+      for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
+    Sys<double> s;
+    for (int k = 0; k < 6; k++) { s.lin[k] = double(sl.lin[k]); s.ang[k] = double(sl.ang[k]); }
+    for (int k = 0; k < 9; k++) s.cross[k] = double(sl.cross[k]);
+    for (int k = 0; k < 3; k++) { s.rl[k] = double(sl.rl[k]); s.ra[k] = double(sl.ra[k]); }
     if (TEAM > 1) {
 #pragma unroll
       for (int off = TEAM / 2; off > 0; off >>= 1) {
@@ -863,31 +982,35 @@ __global__ void __launch_bounds__(256) k_primal(int c, const int *cstart, const 
     }
     if (lane != 0) continue;
     if (!EXACT) {
-      double MAng[9], Ma9[9], t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], ra[3];
-      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+      double MAng[9], Ma9[9], t[3] = {xs[0] - ys[0], xs[1] - ys[1], xs[2] - ys[2]}, dq[3], ra[3];
+      worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
       const double ml = bp.mass / dt2;
       for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] / dt2;
-      qsub(dq, x + 3, y + 3);
+      qsub(dq, xs + 3, ys + 3);
       mulv(ra, Ma9, dq);
       s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
       s.ang[0] += Ma9[0]; s.ang[1] += Ma9[3]; s.ang[2] += Ma9[4];
       s.ang[3] += Ma9[6]; s.ang[4] += Ma9[7]; s.ang[5] += Ma9[8];
       for (int k = 0; k < 3; k++) { s.rl[k] += ml * t[k]; s.ra[k] += ra[k]; }
+      if (w.w > 0)
+        for (int t = b0; t < b1; t++) {
+          const uint2 itw = S.items[t];
+          if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
+        }
     }
     double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
     double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
     double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
     double dxl[3], dxa[3];
     solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
-    for (int k = 0; k < 3; k++) x[k] = x[k] + dxl[k];
-    double q[4];
-    qint(q, x + 3, dxa);
-    for (int k = 0; k < 4; k++) x[3 + k] = q[k];
-    const double *x0 = S.X0 + size_t(i) * XS;
+    double xn[7];
+    for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
+    qint(xn + 3, xs + 3, dxa);
+    for (int k = 0; k < 7; k++) x[k] = xn[k];
     double dq[3];
-    qsub(dq, x + 3, x0 + 3);
+    qsub(dq, xn + 3, x0s + 3);
     R *d = DQ + size_t(i) * 8;
-    d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
+    d[0] = R(xn[0] - x0s[0]); d[1] = R(xn[1] - x0s[1]); d[2] = R(xn[2] - x0s[2]);
     d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
   }
 }
@@ -919,15 +1042,9 @@ __global__ void k_dual(const int *nLivePtr, const int *live, ManBuf<R> mb, const
         R *cd = mb.Cd + si * 4;
         cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];
       }
-      R kl[6] = {hot[HK], hot[HK + 1], hot[HK + 2], e.F[0], e.F[1], e.F[2]};
-      if (e.F[0] < R(0)) kl[0] = jmin(hot[HK] + beta * xabs(e.C[0]), pmax);
-      if (e.within) {
-        kl[1] = jmin(hot[HK + 1] + beta * xabs(e.C[1]), pmax);
-        kl[2] = jmin(hot[HK + 2] + beta * xabs(e.C[2]), pmax);
-        mb.stick[si] = xsqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < stickT ? 1 : 0;
-      } else if (hasSphere) {
-        mb.stick[si] = 0;
-      }
+      R kl[6];
+      const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, hasSphere, kl);
+      if (st >= 0) mb.stick[si] = st;
       vstoreKL(hp, kl);
     }
   }
