@@ -536,6 +536,95 @@ __device__ __forceinline__ int dualUpdate(const R *k, const Eval<R> &e, R beta, 
   return stick;
 }
 
+// ---------------------------------------------------------------- fast f64
+// Reciprocal to about 1 ulp: the hardware estimate refined by two
+// Newton-Raphson steps, r <- r + r (1 - d r).
+__device__ __forceinline__ double frcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// solve6 with one reciprocal per pivot instead of a division per entry (the
+// f32/f64 team modes; the exact mode keeps the reference's divisions).
+__device__ __forceinline__ void solve6f(const double *aLin, const double *aAng, const double *aCross,
+                                        const double *bLin, const double *bAng, double *xLin, double *xAng) {
+  const double A11 = aLin[0];
+  const double A21 = aLin[3], A22 = aLin[4];
+  const double A31 = aLin[6], A32 = aLin[7], A33 = aLin[8];
+  const double A41 = aCross[0], A42 = aCross[1], A43 = aCross[2], A44 = aAng[0];
+  const double A51 = aCross[3], A52 = aCross[4], A53 = aCross[5], A54 = aAng[3], A55 = aAng[4];
+  const double A61 = aCross[6], A62 = aCross[7], A63 = aCross[8], A64 = aAng[6], A65 = aAng[7], A66 = aAng[8];
+  const double D1 = A11, i1 = frcp(D1);
+  const double L21 = A21 * i1, L31 = A31 * i1, L41 = A41 * i1, L51 = A51 * i1, L61 = A61 * i1;
+  const double D2 = A22 - L21 * L21 * D1, i2 = frcp(D2);
+  const double L32 = (A32 - L21 * L31 * D1) * i2;
+  const double L42 = (A42 - L21 * L41 * D1) * i2;
+  const double L52 = (A52 - L21 * L51 * D1) * i2;
+  const double L62 = (A62 - L21 * L61 * D1) * i2;
+  const double D3 = A33 - (L31 * L31 * D1 + L32 * L32 * D2), i3 = frcp(D3);
+  const double L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) * i3;
+  const double L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) * i3;
+  const double L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) * i3;
+  const double D4 = A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), i4 = frcp(D4);
+  const double L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) * i4;
+  const double L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) * i4;
+  const double D5 = A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), i5 = frcp(D5);
+  const double L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) * i5;
+  const double D6 = A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5);
+  const double i6 = frcp(D6);
+  const double y1 = bLin[0];
+  const double y2 = bLin[1] - L21 * y1;
+  const double y3 = bLin[2] - L31 * y1 - L32 * y2;
+  const double y4 = bAng[0] - L41 * y1 - L42 * y2 - L43 * y3;
+  const double y5 = bAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+  const double y6 = bAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+  xAng[2] = y6 * i6;
+  xAng[1] = y5 * i5 - L65 * xAng[2];
+  xAng[0] = y4 * i4 - L54 * xAng[1] - L64 * xAng[2];
+  xLin[2] = y3 * i3 - L43 * xAng[0] - L53 * xAng[1] - L63 * xAng[2];
+  xLin[1] = y2 * i2 - L32 * xLin[2] - L42 * xAng[0] - L52 * xAng[1] - L62 * xAng[2];
+  xLin[0] = y1 * i1 - L21 * xLin[1] - L31 * xLin[2] - L41 * xAng[0] - L51 * xAng[1] - L61 * xAng[2];
+}
+
+// Eq. 21 quaternion update with the normalisation as one reciprocal.
+__device__ __forceinline__ void qintf(double *o, const double *q, const double *dw) {
+  const double qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+  const double wx = dw[0], wy = dw[1], wz = dw[2];
+  const double rx = qx + (wx * qw + wy * qz - wz * qy) * 0.5;
+  const double ry = qy + (-wx * qz + wy * qw + wz * qx) * 0.5;
+  const double rz = qz + (wx * qy - wy * qx + wz * qw) * 0.5;
+  const double rw = qw + (-wx * qx - wy * qy - wz * qz) * 0.5;
+  const double il = frcp(xsqrt(rx * rx + ry * ry + rz * rz + rw * rw));
+  o[0] = rx * il; o[1] = ry * il; o[2] = rz * il; o[3] = rw * il;
+}
+
+// ------------------------------------------------------------ team reductions
+// A 64-bit lane move under a DPP pattern (quad permutes and row mirrors stay
+// inside a row of 16 lanes, so every source lane is one of the team's).
+template <int CTRL> __device__ __forceinline__ double dppMove(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)unsigned(hi) << 32) | unsigned(lo)));
+}
+
+// Sum over the TEAM lanes of an aligned team; every lane gets the total. The
+// pairs are those of an xor butterfly (lane ^ 1, ^ 2, ^ 4, ^ 8, ...): after
+// the quad steps all lanes of a quad hold the quad's sum, so mirroring within
+// 8 and then 16 lanes pairs each group with the other half.
+template <int TEAM> __device__ __forceinline__ double teamSum(double v) {
+  if constexpr (TEAM >= 2) v += dppMove<0xB1>(v);   // quad_perm [1,0,3,2]
+  if constexpr (TEAM >= 4) v += dppMove<0x4E>(v);   // quad_perm [2,3,0,1]
+  if constexpr (TEAM >= 8) v += dppMove<0x141>(v);  // row_half_mirror
+  if constexpr (TEAM >= 16) v += dppMove<0x140>(v); // row_mirror
+  if constexpr (TEAM >= 32) v += __shfl_xor(v, 16, TEAM);
+  if constexpr (TEAM >= 64) v += __shfl_xor(v, 32, TEAM);
+  return v;
+}
+
 // Accumulated 6x6 system: lower triangles of the Lin and Ang blocks, the full
 // Cross block, and the right-hand side (the 27 numbers solve6 reads).
 // lin: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) 4:(2,1) 5:(2,2)  (row, col)

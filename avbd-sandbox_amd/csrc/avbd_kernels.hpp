@@ -856,10 +856,9 @@ __device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
 // stamps in the reference's order onto the inertial terms. The colour's body
 // range comes from device memory and teams stride over it, so the launch
 // shape does not depend on the step's colouring.
-// Work items a lane has in flight at once: their records are all requested
-// before any is evaluated, so a team waits for memory once per round rather
-// than once per contact.
-constexpr int PRIMAL_UNROLL = 1;
+// Work-item words a lane fetches at once (one memory round trip for up to
+// PRIMAL_BATCH of its items).
+constexpr int PRIMAL_BATCH = 4;
 
 // One colour of the primal sweep (solver.js:419-458): a team of TEAM lanes per
 // body gathers its contacts and joints, reduces the 6x6 system and solves it.
@@ -875,10 +874,9 @@ constexpr int PRIMAL_UNROLL = 1;
 // the inertial terms and the joints (cwork.w of them) in f64: joint rows can
 // be stiff enough to swamp a light body's inertia in f32.
 template <class R, bool EXACT, int TEAM>
-__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
+__global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                 ManBuf<R> mb, R *DQ, StepParams P, double alpha, int dual,
                                                 double dualAlpha) {
-  constexpr int U = PRIMAL_UNROLL;
   using Acc = typename std::conditional<EXACT, double, R>::type;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
@@ -909,49 +907,46 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1) k_primal(int c, c
     }
     const R ralpha = R(alpha), rdalpha = R(dualAlpha);
     const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
-    for (int t = b0 + lane; t < b1; t += TEAM * U) {
-      uint2 it[U];
-      R bas[U][BAS], hot[U][HOT], dqo[U][8];
+    constexpr int IB = PRIMAL_BATCH;
+    for (int t0 = b0 + lane; t0 < b1; t0 += TEAM * IB) {
+      uint2 itw[IB];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int tu = t + u * TEAM;
-        it[u] = tu < b1 ? S.items[tu] : make_uint2(0xffffffffu, 0u);
+      for (int k = 0; k < IB; k++) {
+        const int tk = t0 + k * TEAM;
+        itw[k] = tk < b1 ? S.items[tk] : make_uint2(0xffffffffu, 0u);
       }
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (it[u].x & 0x80000000u) continue;  // joint, or no item
-        const int m = int(it[u].x >> 4), sl = int((it[u].x >> 1) & 7u);
-        vload<BAS>(bas[u], mb.basis + size_t(m) * BAS);
-        vload<HOT>(hot[u], mb.hot + (size_t(m) * MAXC + sl) * HOT);
-        vload<8>(dqo[u], DQ + size_t(it[u].y & ITEM_BODY) * 8);
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (it[u].x == 0xffffffffu) continue;
-        const bool isA = it[u].x & 1u;
-        if (it[u].x & 0x80000000u) {
-          if constexpr (EXACT) jointStamp(sl, S.joints[int((it[u].x & 0x7fffffffu) >> 1)], isA, S.X, alpha);
+      for (int k = 0; k < IB; k++) {
+        const uint2 iw = itw[k];
+        if (iw.x == 0xffffffffu) break;
+        const bool isA = iw.x & 1u;
+        if (iw.x & 0x80000000u) {
+          if constexpr (EXACT) jointStamp(sl, S.joints[int((iw.x & 0x7fffffffu) >> 1)], isA, S.X, alpha);
           continue;
         }
+        const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
+        R bas[BAS], hot[HOT], dqo[8];
+        vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
+        vload<HOT>(hot, mb.hot + si * HOT);
+        vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
         R negb[9];
-        for (int k = 0; k < 9; k++) negb[k] = -bas[u][k];
-        const R *dA = isA ? dqi : dqo[u], *dB = isA ? dqo[u] : dqi;
+        for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+        const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
         Eval<R> e;
-        if (dual && (it[u].y & ITEM_FIRST)) {
-          evalContact<R, EXACT>(bas[u], negb, bas[u][9], hot[u], dA, dB, rdalpha, e);
+        if (dual && (iw.y & ITEM_FIRST)) {
+          evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, rdalpha, e);
           R kl[6];
-          const int st = dualUpdate(hot[u] + HK, e, beta, pmax, stickT, it[u].y & ITEM_SPHERE, kl);
-          const size_t si = size_t(it[u].x >> 4) * MAXC + ((it[u].x >> 1) & 7u);
+          const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
           vstoreKL(mb.hot + si * HOT, kl);
           if (st >= 0) mb.stick[si] = st;
 #pragma unroll
-          for (int k = 0; k < 6; k++) hot[u][HK + k] = kl[k];  // k then lambda
-          if (rdalpha != ralpha) evalC<R, EXACT>(bas[u], negb, hot[u], dA, dB, ralpha, e);
-          evalF<R, EXACT>(hot[u] + HK, hot[u] + HL, bas[u][9], e);
+          for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];  // k then lambda
+          if (rdalpha != ralpha) evalC<R, EXACT>(bas, negb, hot, dA, dB, ralpha, e);
+          evalF<R, EXACT>(hot + HK, hot + HL, bas[9], e);
         } else {
-          evalContact<R, EXACT>(bas[u], negb, bas[u][9], hot[u], dA, dB, ralpha, e);
+          evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, ralpha, e);
         }
-        stampContact<R, Acc>(sl, isA ? bas[u] : negb, isA ? e.jA : e.jB, hot[u] + HK, e.F);
+        stampContact<R, Acc>(sl, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
       }
     }
     // own pose and the step's start and inertial target, for lane 0's solve
@@ -963,29 +958,21 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1) k_primal(int c, c
     for (int k = 0; k < 6; k++) { s.lin[k] = double(sl.lin[k]); s.ang[k] = double(sl.ang[k]); }
     for (int k = 0; k < 9; k++) s.cross[k] = double(sl.cross[k]);
     for (int k = 0; k < 3; k++) { s.rl[k] = double(sl.rl[k]); s.ra[k] = double(sl.ra[k]); }
-    if (TEAM > 1) {
+    if constexpr (TEAM > 1) {
 #pragma unroll
-      for (int off = TEAM / 2; off > 0; off >>= 1) {
+      for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-          s.lin[k] += __shfl_xor(s.lin[k], off, TEAM);
-          s.ang[k] += __shfl_xor(s.ang[k], off, TEAM);
-        }
+      for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
 #pragma unroll
-        for (int k = 0; k < 9; k++) s.cross[k] += __shfl_xor(s.cross[k], off, TEAM);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          s.rl[k] += __shfl_xor(s.rl[k], off, TEAM);
-          s.ra[k] += __shfl_xor(s.ra[k], off, TEAM);
-        }
-      }
+      for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
     }
     if (lane != 0) continue;
-    if (!EXACT) {
+    if constexpr (!EXACT) {
       double MAng[9], Ma9[9], t[3] = {xs[0] - ys[0], xs[1] - ys[1], xs[2] - ys[2]}, dq[3], ra[3];
       worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-      const double ml = bp.mass / dt2;
-      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] / dt2;
+      const double idt2 = frcp(dt2);
+      const double ml = bp.mass * idt2;
+      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] * idt2;
       qsub(dq, xs + 3, ys + 3);
       mulv(ra, Ma9, dq);
       s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
@@ -1002,10 +989,15 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1) k_primal(int c, c
     double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
     double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
     double dxl[3], dxa[3];
-    solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
     double xn[7];
+    if constexpr (EXACT) {
+      solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+      qint(xn + 3, xs + 3, dxa);
+    } else {
+      solve6f(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+      qintf(xn + 3, xs + 3, dxa);
+    }
     for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
-    qint(xn + 3, xs + 3, dxa);
     for (int k = 0; k < 7; k++) x[k] = xn[k];
     double dq[3];
     qsub(dq, xn + 3, x0s + 3);
