@@ -23,7 +23,7 @@ SYMBOLS = (
     "avbd_set_params", "avbd_pack", "avbd_step", "avbd_sync", "avbd_read_state", "avbd_write_state",
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
-    "avbd_autotune",
+    "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward",
 )
 K_PRIMAL, K_DUAL, K_NARROW = 0, 1, 2
 
@@ -94,6 +94,8 @@ def lib(path=None):
         "avbd_read_profile": (C.c_int, [ctx, P(C.c_float), i32, pi]),
         "avbd_read_kernel_stats": (C.c_int, [ctx, i32, pd, P(C.c_int64), pd]),
         "avbd_autotune": (C.c_int, [ctx, pi, P(C.c_float)]),
+        "avbd_adjoint_forward": (C.c_int, [ctx, i32, i32]),
+        "avbd_adjoint_backward": (C.c_int, [ctx, pd, pd, pd]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -215,3 +215,16 @@ class HipBackend:
         ms = (C.c_float * 6)()
         self._ck(self.L.avbd_autotune(self.ctx, C.byref(t), ms))
         return t.value, dict(zip((1, 4, 8, 16, 32, 64), [float(x) for x in ms]))
+
+    # --- adjoint (no counterpart in the reference)
+    def adjoint_forward(self, steps, ckpt_every=10):
+        """Step `steps` times keeping checkpoints every `ckpt_every` steps."""
+        self._ck(self.L.avbd_adjoint_forward(self.ctx, int(steps), int(ckpt_every)))
+
+    def adjoint_backward(self, dL_dpos):
+        """Gradients of a loss of the final positions (dL_dpos: n x 3) with
+        respect to each body's friction and initial linear velocity."""
+        g = np.ascontiguousarray(dL_dpos, dtype=np.float64).reshape(self.n, 3)
+        mu, v0 = np.zeros(self.n), np.zeros((self.n, 3))
+        self._ck(self.L.avbd_adjoint_backward(self.ctx, _dp(g), _dp(mu), _dp(v0)))
+        return mu, v0

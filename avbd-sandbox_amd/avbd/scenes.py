@@ -181,4 +181,24 @@ def ramp(n=64, seed=5, angle_deg=30.0):
     return dict(bodies=bodies, joints=[]), dict(gravity=-9.81, iterations=10), dict(steps=120, seed=seed)
 
 
+def ramp_lanes(n=8, seed=3, angle_deg=30.0):
+    """n boxes (0.5 m, rho 500) side by side across the 30-degree ramp, one per
+    lane 2.5 m apart so that no two touch, friction U[0.1, 0.4] (all below
+    tan 30 deg: they keep sliding) and initial speed U[0, 2] m/s down-slope.
+    The smooth sliding regime in which finite differences check gradients."""
+    r = lcg(seed)
+    a = math.radians(angle_deg)
+    q = axis_angle(0, 1, 0, a)
+    c, s = math.cos(a), math.sin(a)
+    bodies = [ground(0.5), box([40, 24, 1], 0, 1.0, [0, 0, 10], quat=q)]
+    for i in range(n):
+        u, w = -8.0, -9 + i * 2.5
+        hgt = 0.5 + 0.25 + 0.001
+        pos = [c * u + s * hgt, w, 10 - s * u + c * hgt]
+        mu = 0.1 + 0.3 * r()
+        v0 = 2 * r()
+        bodies.append(box([0.5, 0.5, 0.5], 500, mu, pos, quat=q, vel=[c * v0, 0, -s * v0]))
+    return dict(bodies=bodies, joints=[]), dict(gravity=-9.81, iterations=10), dict(slope=[c, 0.0, -s])
+
+
 CONFIGS = {"stack10": stack10, "pyramid": pyramid, "rain": rain, "batched": batched, "ramp": ramp}

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/avbd.h"
+#include "avbd_adjoint.hpp"
 #include "avbd_kernels.hpp"
 
 using namespace avbd;
@@ -55,6 +56,11 @@ template <class T> struct DBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
+  }
+  // Device copy of the first count elements of src (grows as needed).
+  void copyFrom(const T *src, size_t count, hipStream_t st) {
+    reserve(count);
+    if (count) HIPCHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyDeviceToDevice, st));
   }
 };
 
@@ -146,6 +152,46 @@ struct avbd_ctx {
   int numColors = 0, colorRounds = 0;
   DBuf<int> itemCnt, itemStart;
   DBuf<uint2> items;
+
+  // ------------------------------------------------------------- adjoint
+  // One recorded step (what the reverse kernels read) and a checkpoint of the
+  // state one step hands to the next.
+  struct TapeStep {
+    int nTot = 0, nLive = 0, numColors = 0;
+    DBuf<int4> head, cwork;
+    DBuf<double> basis, anchor, hot, X, X0, Y, V;
+    DBuf<int> src, live, color, cstart;
+    DBuf<uint2> items;
+    void release() {
+      head.release(); cwork.release(); basis.release(); anchor.release(); hot.release(); X.release();
+      X0.release(); Y.release(); V.release(); src.release(); live.release(); color.release(); cstart.release();
+      items.release();
+    }
+  };
+  struct Checkpoint {
+    DBuf<double> X, V, anchor;
+    DBuf<int4> head;
+    DBuf<char> basis, hot, Cd;
+    DBuf<uint32_t> feat;
+    DBuf<int32_t> stick;
+    DBuf<int> live, adjStart, adjKey, adjOther, color;
+    int cur = 0, nLive = 0, lastNTot = 0;
+    int64_t steps = 0;
+    void release() {
+      X.release(); V.release(); anchor.release(); head.release(); basis.release(); hot.release(); Cd.release();
+      feat.release(); stick.release(); live.release(); adjStart.release(); adjKey.release(); adjOther.release();
+      color.release();
+    }
+  };
+  std::vector<TapeStep> tape;
+  TapeStep *rec = nullptr;  // the step being recorded, if any
+  bool fuseDual = true;     // fold the dual update into the next sweep
+  DBuf<int> srcSlot;        // per new contact: matched old slot | kept-anchor bit, or -1
+  int lastNTot = 0;
+  std::vector<Checkpoint> ckpts;
+  Checkpoint endState;
+  int adjSteps = 0, adjEvery = 0;
+  DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
   // 5 overflow, 6 items, 7 adj total
@@ -242,6 +288,14 @@ struct avbd_ctx {
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
     color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release(); cwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
+    for (auto &t : tape) t.release();
+    tape.clear();
+    for (auto &c : ckpts) c.release();
+    ckpts.clear();
+    endState.release();
+    srcSlot.release();
+    aXb.release(); aX0b.release(); aYb.release(); aVb.release(); aVbS.release(); aLK[0].release(); aLK[1].release();
+    aC0b.release(); aRWb.release(); aMUb.release(); aMu.release();
     for (auto &b : scanTmp) b.release();
     for (auto &e : evs) (void)hipEventDestroy(e);
     evs.clear();
@@ -396,6 +450,7 @@ struct avbd_ctx {
     if (withInit)
       hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
                          (R *)DQ.p, P);
+    if (rec) recordSnapshot(0);
     // The contact dual update of iteration it runs inside the primal sweep of
     // iteration it + 1 (first visitors apply it); after the last iteration it
     // runs as its own pass, which also records C for the next step.
@@ -403,15 +458,18 @@ struct avbd_ctx {
     auto alphaOf = [&](int it) { return prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha; };
     for (int it = 0; it < total; it++) {
       const double alpha = alphaOf(it);
-      const int fused = it >= 1 && it < prm.iterations ? 1 : 0;
+      const int fused = fuseDual && it >= 1 && it < prm.iterations ? 1 : 0;
       const double dualAlpha = fused ? alphaOf(it - 1) : alpha;
       for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team, fused, dualAlpha);
       if (it < prm.iterations) {
-        if (it == prm.iterations - 1 && (graphCapturing || nLive > 0))
+        const bool last = it == prm.iterations - 1;
+        if ((last || !fuseDual) && (graphCapturing || nLive > 0))
           timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
-                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha, 1);
+                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha,
+                      last ? 1 : 0);
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
+      if (rec && it < prm.iterations) recordSnapshot(it + 1);
       if (it == prm.iterations - 1)
         hipLaunchKernelGGL(k_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, V.p, prm.dt);
     }
@@ -537,7 +595,7 @@ struct avbd_ctx {
     if (nTot > 0)
       timedLaunch(AVBD_K_NARROW, 0.0, k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), nOld, nNew,
                   (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p, (const BodyProp *)props.p,
-                  (const double *)X.p, P, keep.p, scal.p + 8);
+                  (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p);
     if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
     scan(keep.p, keepPos.p, nTot, scal.p + 2);
     if (nTot > 0)
@@ -547,6 +605,8 @@ struct avbd_ctx {
     nLive = hscal[2];
     contactsTotal = hscal[8];
     cur = 1 - cur;
+    lastNTot = nTot;
+    if (rec) recordContacts(nTot);
     record(2);
 
     // 3. topology: force lists, colouring, work lists
@@ -564,6 +624,7 @@ struct avbd_ctx {
       HIPCHECK(hipStreamSynchronize(stream));
     }
     record(4);
+    if (rec) recordTopology();
 
     // 4. body initialisation and the iterations
     ManBuf<R> mb = store[cur].view<R>();
@@ -576,7 +637,7 @@ struct avbd_ctx {
       autotuneT<R>(sa, mb, P);
       tuneRequest = false;
       enqueueSolve<R, EXACT>(sa, mb, P, false);
-    } else if (graphsCfg >= 0 && !profiling) {
+    } else if (graphsCfg >= 0 && !profiling && !rec) {
       solveGraph<R, EXACT>(sa, mb, P);
     } else {
       enqueueSolve<R, EXACT>(sa, mb, P, true);
@@ -605,6 +666,229 @@ struct avbd_ctx {
       case AVBD_PREC_EXACT: stepT<double, true>(); break;
       case AVBD_PREC_F64: stepT<double, false>(); break;
       default: stepT<float, false>(); break;
+    }
+  }
+
+  // ------------------------------------------------------------- adjoint
+  // Recording: the step's contacts (after the narrow phase), its colouring and
+  // work lists (after colouring), and a snapshot of the poses and hot records
+  // after body initialisation (l = 0) and after each iteration's dual update.
+  void recordContacts(int nTot) {
+    TapeStep &T = *rec;
+    T.nTot = nTot;
+    T.nLive = nLive;
+    ManStore &m = store[cur];
+    T.head.copyFrom(m.head.p, size_t(nTot), stream);
+    T.basis.copyFrom((const double *)m.basis.p, size_t(nTot) * BAS, stream);
+    T.anchor.copyFrom(m.anchor.p, size_t(nTot) * MAXC * 6, stream);
+    T.src.copyFrom(srcSlot.p, size_t(nTot) * MAXC, stream);
+    T.live.copyFrom(live[cur].p, size_t(nLive), stream);
+  }
+  int tapeItems = 0;
+  void recordTopology() {
+    TapeStep &T = *rec;
+    T.numColors = numColors;
+    T.color.copyFrom(color[0].p, size_t(N), stream);
+    T.cstart.copyFrom(cmeta.p + 8, size_t(numColors + 1), stream);
+    T.cwork.copyFrom(cwork.p, size_t(N), stream);
+    HIPCHECK(hipMemcpyAsync(&tapeItems, itemStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    T.items.copyFrom(items.p, size_t(tapeItems), stream);
+    T.V.copyFrom(V.p, size_t(N) * VS, stream);
+    T.X.reserve(size_t(prm.iterations + 1) * N * XS);
+    T.hot.reserve(size_t(prm.iterations + 1) * std::max(T.nTot, 1) * MAXC * HOT);
+  }
+  void recordSnapshot(int l) {
+    TapeStep &T = *rec;
+    if (l == 0) {
+      T.X0.copyFrom(X0.p, size_t(N) * XS, stream);
+      T.Y.copyFrom(Y.p, size_t(N) * XS, stream);
+    }
+    HIPCHECK(hipMemcpyAsync(T.X.p + size_t(l) * N * XS, X.p, sizeof(double) * N * XS, hipMemcpyDeviceToDevice,
+                            stream));
+    if (T.nTot)
+      HIPCHECK(hipMemcpyAsync(T.hot.p + size_t(l) * T.nTot * MAXC * HOT, store[cur].hot.p,
+                              sizeof(double) * T.nTot * MAXC * HOT, hipMemcpyDeviceToDevice, stream));
+  }
+
+  // Everything one step hands to the next: poses, velocities, the current
+  // contact store and live list, the force lists the broad phase consults,
+  // and the colouring.
+  void saveCheckpoint(Checkpoint &c) {
+    const size_t rs = rsize(), nT = size_t(std::max(lastNTot, 1));
+    ManStore &m = store[cur];
+    c.X.copyFrom(X.p, size_t(N) * XS, stream);
+    c.V.copyFrom(V.p, size_t(N) * VS, stream);
+    c.head.copyFrom(m.head.p, nT, stream);
+    c.basis.copyFrom(m.basis.p, nT * BAS * rs, stream);
+    c.hot.copyFrom(m.hot.p, nT * MAXC * HOT * rs, stream);
+    c.Cd.copyFrom(m.Cd.p, nT * MAXC * 4 * rs, stream);
+    c.anchor.copyFrom(m.anchor.p, nT * MAXC * 6, stream);
+    c.feat.copyFrom(m.feat.p, nT * MAXC, stream);
+    c.stick.copyFrom(m.stick.p, nT * MAXC, stream);
+    c.live.copyFrom(live[cur].p, size_t(std::max(nLive, 1)), stream);
+    c.adjStart.copyFrom(adjStart.p, size_t(N + 1), stream);
+    c.adjKey.copyFrom(adjKey.p, adjKey.n, stream);
+    c.adjOther.copyFrom(adjOther.p, adjOther.n, stream);
+    c.color.copyFrom(color[0].p, size_t(N), stream);
+    c.cur = cur;
+    c.nLive = nLive;
+    c.lastNTot = lastNTot;
+    c.steps = steps;
+  }
+  template <class T> void copyBack(T *dst, const DBuf<T> &src, size_t count) {
+    if (count) HIPCHECK(hipMemcpyAsync(dst, src.p, count * sizeof(T), hipMemcpyDeviceToDevice, stream));
+  }
+  void restoreCheckpoint(const Checkpoint &c) {
+    cur = c.cur;
+    nLive = c.nLive;
+    lastNTot = c.lastNTot;
+    steps = c.steps;
+    const size_t rs = rsize(), nT = size_t(std::max(lastNTot, 1));
+    ManStore &m = store[cur];
+    copyBack(X.p, c.X, size_t(N) * XS);
+    copyBack(V.p, c.V, size_t(N) * VS);
+    copyBack(m.head.p, c.head, nT);
+    copyBack(m.basis.p, c.basis, nT * BAS * rs);
+    copyBack(m.hot.p, c.hot, nT * MAXC * HOT * rs);
+    copyBack(m.Cd.p, c.Cd, nT * MAXC * 4 * rs);
+    copyBack(m.anchor.p, c.anchor, nT * MAXC * 6);
+    copyBack(m.feat.p, c.feat, nT * MAXC);
+    copyBack(m.stick.p, c.stick, nT * MAXC);
+    copyBack(live[cur].p, c.live, size_t(std::max(nLive, 1)));
+    copyBack(adjStart.p, c.adjStart, size_t(N + 1));
+    copyBack(adjKey.p, c.adjKey, adjKey.n);
+    copyBack(adjOther.p, c.adjOther, adjOther.n);
+    copyBack(color[0].p, c.color, size_t(N));
+  }
+
+  void checkAdjointScene() const {
+    if (prec == AVBD_PREC_MIXED)
+      throw ApiError(AVBD_E_STATE, "the adjoint needs f64 records (precision exact or f64)");
+    if (J > 0) throw ApiError(AVBD_E_STATE, "the adjoint covers contacts only; the scene has joints");
+    if (prm.post_stabilize) throw ApiError(AVBD_E_STATE, "the adjoint does not cover post-stabilisation");
+    for (const auto &b : hprops)
+      if (b.shape == 1) throw ApiError(AVBD_E_STATE, "the adjoint covers box contacts only; the scene has spheres");
+  }
+
+  void adjointForward(int nsteps, int every) {
+    for (auto &c : ckpts) c.release();
+    ckpts.clear();
+    ckpts.resize(size_t((nsteps + every - 1) / every));
+    for (int s = 0; s < nsteps; s++) {
+      if (s % every == 0) saveCheckpoint(ckpts[size_t(s / every)]);
+      step();
+    }
+    saveCheckpoint(endState);
+    adjSteps = nsteps;
+    adjEvery = every;
+  }
+
+  AdjStep adjView(const TapeStep &T) const {
+    AdjStep S;
+    S.nLive = T.nLive;
+    S.numColors = T.numColors;
+    S.live = T.live.p; S.head = T.head.p; S.basis = T.basis.p; S.anchor = T.anchor.p; S.src = T.src.p;
+    S.hot = T.hot.p; S.X = T.X.p; S.X0 = T.X0.p; S.Y = T.Y.p; S.V = T.V.p;
+    S.color = T.color.p; S.cstart = T.cstart.p; S.cwork = T.cwork.p; S.items = T.items.p;
+    S.hotStride = size_t(T.nTot) * MAXC * HOT;
+    S.xStride = size_t(N) * XS;
+    return S;
+  }
+  AdjBufs adjBufs(int lk) {
+    AdjBufs A;
+    A.Xb = aXb.p; A.X0b = aX0b.p; A.Yb = aYb.p; A.Vb = aVb.p; A.VbS = aVbS.p;
+    A.LK = aLK[lk].p; A.LKprev = aLK[1 - lk].p;
+    A.C0b = aC0b.p; A.RWb = aRWb.p; A.MUb = aMUb.p; A.muGrad = aMu.p;
+    return A;
+  }
+  template <class T> void zero(DBuf<T> &b, size_t count) {
+    b.reserve(count);
+    HIPCHECK(hipMemsetAsync(b.p, 0, sizeof(T) * count, stream));
+  }
+
+  // The reverse of one recorded step; lk selects which contact-adjoint buffer
+  // holds this step's slots.
+  void reverseStep(const TapeStep &T, int nItems, int &lk) {
+    const StepParams P = sp();
+    const int I = prm.iterations;
+    const AdjStep S = adjView(T);
+    const AdjBufs A = adjBufs(lk);
+    hipLaunchKernelGGL(k_adj_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, I,
+                       prm.dt);
+    const bool wide = nItems > 12 * std::max(nDynamic, 1);
+    const int gb = std::max(1, std::min(blocks(long(std::max(nDynamic, 1)) * (wide ? 16 : 4)), 2048));
+    for (int l = I - 1; l >= 0; l--) {
+      if (T.nLive > 0)
+        hipLaunchKernelGGL(k_adj_dual, dim3(blocks(T.nLive, 128)), dim3(128), 0, stream, S, A, l, P);
+      for (int c = T.numColors - 1; c >= 0; c--) {
+        if (wide)
+          hipLaunchKernelGGL(k_adj_primal<16>, dim3(gb), dim3(256), 0, stream, c, S, A, (const BodyProp *)props.p, l,
+                             P);
+        else
+          hipLaunchKernelGGL(k_adj_primal<4>, dim3(gb), dim3(256), 0, stream, c, S, A, (const BodyProp *)props.p, l,
+                             P);
+      }
+    }
+    if (T.nLive > 0)
+      hipLaunchKernelGGL(k_adj_narrow, dim3(blocks(T.nLive, 128)), dim3(128), 0, stream, S, A,
+                         (const BodyProp *)props.p, P);
+    hipLaunchKernelGGL(k_adj_init, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, P);
+    if (T.nTot) {
+      const size_t slots = size_t(T.nTot) * MAXC;
+      HIPCHECK(hipMemsetAsync(aLK[lk].p, 0, sizeof(double) * slots * 6, stream));
+      HIPCHECK(hipMemsetAsync(aC0b.p, 0, sizeof(double) * slots * 3, stream));
+      HIPCHECK(hipMemsetAsync(aRWb.p, 0, sizeof(double) * slots * 6, stream));
+      HIPCHECK(hipMemsetAsync(aMUb.p, 0, sizeof(double) * T.nTot, stream));
+    }
+    HIPCHECK(hipGetLastError());
+    lk = 1 - lk;
+  }
+
+  void adjointBackward(const double *dLdPos, double *dLdMu, double *dLdV0) {
+    if (adjSteps <= 0) throw ApiError(AVBD_E_STATE, "no adjoint forward recorded");
+    const size_t slots = size_t(manCap) * MAXC;
+    zero(aXb, size_t(N) * XS); zero(aX0b, size_t(N) * XS); zero(aYb, size_t(N) * XS);
+    zero(aVb, size_t(N) * VS); zero(aVbS, size_t(N) * VS);
+    zero(aLK[0], slots * 6); zero(aLK[1], slots * 6); zero(aC0b, slots * 3); zero(aRWb, slots * 6);
+    zero(aMUb, size_t(manCap)); zero(aMu, size_t(N));
+    {
+      std::vector<double> h(size_t(N) * XS, 0.0);
+      for (int i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) h[size_t(i) * XS + k] = dLdPos[i * 3 + k];
+      HIPCHECK(hipMemcpyAsync(aXb.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
+    struct Reset {
+      avbd_ctx *c;
+      ~Reset() { c->rec = nullptr; c->fuseDual = true; }
+    } reset{this};
+    fuseDual = false;
+    if (tape.size() < size_t(adjEvery)) tape.resize(size_t(adjEvery));
+    std::vector<int> nItems(static_cast<size_t>(adjEvery));
+    int lk = 0;
+    const int nseg = int(ckpts.size());
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+      restoreCheckpoint(ckpts[size_t(seg)]);
+      const int len = std::min(adjEvery, adjSteps - seg * adjEvery);
+      for (int k = 0; k < len; k++) {
+        rec = &tape[size_t(k)];
+        step();
+        nItems[size_t(k)] = tapeItems;
+      }
+      rec = nullptr;
+      for (int k = len - 1; k >= 0; k--) reverseStep(tape[size_t(k)], nItems[size_t(k)], lk);
+    }
+    restoreCheckpoint(endState);
+    std::vector<double> mu(static_cast<size_t>(N)), vb(static_cast<size_t>(N) * VS);
+    HIPCHECK(hipMemcpyAsync(mu.data(), aMu.p, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(vb.data(), aVb.p, sizeof(double) * N * VS, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int i = 0; i < N; i++) {
+      if (dLdMu) dLdMu[i] = mu[size_t(i)];
+      // pack sets the previous velocity equal to the velocity
+      if (dLdV0)
+        for (int k = 0; k < 3; k++) dLdV0[i * 3 + k] = vb[size_t(i) * VS + k] + vb[size_t(i) * VS + 6 + k];
     }
   }
 };
@@ -760,6 +1044,9 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.store[1].reserve(c.manCap, c.rsize());
   c.live[0].reserve(c.manCap); c.live[1].reserve(c.manCap);
   c.keep.reserve(c.manCap); c.keepPos.reserve(c.manCap + 1);
+  c.srcSlot.reserve(size_t(c.manCap) * MAXC);
+  c.lastNTot = 0;
+  c.adjSteps = 0;
 
   // joints (joint.js:78-113)
   std::vector<JointDev> JD(c.J);
@@ -1126,6 +1413,25 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
     ctx->step();
     if (team) *team = ctx->team;
     if (ms) for (int k = 0; k < 6; k++) ms[k] = ctx->tuneMs[k];
+  });
+}
+
+int avbd_adjoint_forward(avbd_ctx *ctx, int32_t nsteps, int32_t ckpt_every) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (nsteps <= 0 || ckpt_every <= 0) throw ApiError(AVBD_E_INVALID, "nsteps and ckpt_every must be positive");
+    ctx->checkAdjointScene();
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->adjointForward(nsteps, ckpt_every);
+  });
+}
+
+int avbd_adjoint_backward(avbd_ctx *ctx, const double *dL_dpos, double *dL_dmu, double *dL_dv0) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (!dL_dpos) throw ApiError(AVBD_E_INVALID, "dL_dpos is required");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->adjointBackward(dL_dpos, dL_dmu, dL_dv0);
   });
 }
 

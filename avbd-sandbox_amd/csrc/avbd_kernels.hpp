@@ -260,7 +260,8 @@ __global__ void k_bp_large(BpArgs a, const int *largeList) {
 // and every new pair, written to slot t of the other buffer.
 template <class R>
 __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
-                         const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal) {
+                         const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal,
+                         int *srcSlot) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nOld + nNew) return;
   int a, b, m = -1, ncOld = 0;
@@ -298,10 +299,11 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
     double rA[3] = {nc[i].rA[0], nc[i].rA[1], nc[i].rA[2]};
     double rB[3] = {nc[i].rB[0], nc[i].rB[1], nc[i].rB[2]};
     double kk[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
-    int stick = 0;
+    int stick = 0, src = -1;
     for (int j = 0; j < ncOld; j++) {
       const size_t os = size_t(m) * MAXC + j;
       if (compatible && ob.feat[os] == nc[i].feature) {
+        src = int(os);
         const R *oh = ob.hot + os * HOT;
         for (int k = 0; k < 3; k++) {
           kk[k] = double(oh[HK + k]);
@@ -332,6 +334,7 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
         if (oldStick && !hasSphere) {
           const double *an = ob.anchor + os * 6;
           for (int k = 0; k < 3; k++) { rA[k] = an[k]; rB[k] = an[3 + k]; }
+          src |= 0x40000000;  // anchors kept
         }
         break;
       }
@@ -364,6 +367,7 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
     for (int k = 0; k < 3; k++) { an[k] = rA[k]; an[3 + k] = rB[k]; }
     nb.feat[s] = nc[i].feature;
     nb.stick[s] = stick;
+    srcSlot[s] = src;
     for (int k = 0; k < 3; k++) nb.Cd[s * 4 + k] = R(Cc[k]);
     nb.Cd[s * 4 + 3] = R(0);
   }

@@ -217,6 +217,23 @@ int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int6
  * assembly order is fixed. */
 int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
 
+/* Adjoint of the step (no counterpart in the reference; the build plan's
+ * adjoint boundary). avbd_adjoint_forward steps the scene nsteps times, as
+ * avbd_step does, keeping a checkpoint of the carried state every ckpt_every
+ * steps. avbd_adjoint_backward then returns the gradient of a loss L(final
+ * positions), given dL_dpos [n*3], with respect to each body's friction
+ * coefficient (dL_dmu [n]) and initial linear velocity (dL_dv0 [n*3]; pack
+ * sets the velocity and the previous velocity to it). It re-runs each segment
+ * from its checkpoint recording every iteration and reverses it; the context
+ * ends at the forward's final state. The derivative is that of the computed
+ * trajectory (a fixed iteration count), with the contact geometry (normals,
+ * features, anchors) held piecewise constant and only the normal row of C0
+ * differentiated for contacts whose anchors were re-derived that step.
+ * Requires AVBD_PREC_EXACT or AVBD_PREC_F64, box contacts, no joints and no
+ * post-stabilisation (AVBD_E_STATE otherwise). */
+int avbd_adjoint_forward(avbd_ctx *ctx, int32_t nsteps, int32_t ckpt_every);
+int avbd_adjoint_backward(avbd_ctx *ctx, const double *dL_dpos, double *dL_dmu, double *dL_dv0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,93 @@
+"""Adjoint of the step against f64 central finite differences of the oracle.
+
+The scene is the sliding regime of the ramp benchmark: boxes side by side on a
+30-degree ramp with friction below tan 30 deg, so every contact keeps sliding
+and the trajectory is a smooth function of each box's friction and initial
+speed (the differences agree to 7 digits for h from 1e-4 to 1e-7). The GPU
+forward runs in the exact precision, bit-identical to the oracle, so the two
+sides differentiate the same computed trajectory. Tolerance: 1e-4 relative to
+the largest gradient of the set (the adjoint holds the contact geometry
+piecewise constant; in this regime that is exact up to rounding).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import avbd
+import pyoracle
+from avbd import AvbdError
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 60
+
+
+def final_positions(scene, params):
+    o = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    o.step(STEPS)
+    return o.state()[:, :3]
+
+
+def per_body_loss(x, target):
+    return ((x - target) ** 2).sum(axis=1)
+
+
+@pytest.mark.parametrize("precision", [avbd.PREC_EXACT, avbd.PREC_F64], ids=["exact", "f64"])
+def test_adjoint_matches_finite_differences(precision):
+    scene, params, meta = avbd.scenes.ramp_lanes(n=8)
+    slope = np.array(meta["slope"])
+    n = len(scene["bodies"])
+    target = np.array([b["pos"] for b in scene["bodies"]]) + np.array([2.0, 0.0, -1.0])
+    gpu = avbd.HipBackend(precision=precision)
+    gpu.pack(scene, params)
+    gpu.adjoint_forward(STEPS, 10)
+    xg = gpu.read_state()["pos"]
+    if precision == avbd.PREC_EXACT:
+        assert np.array_equal(xg, final_positions(scene, params))
+    dL = 2 * (xg - target)
+    dL[:2] = 0  # static ground and ramp
+    gmu, gv0 = gpu.adjoint_backward(dL)
+    # the backward pass leaves the context at the forward's final state
+    assert np.array_equal(gpu.read_state()["pos"], xg)
+
+    h = 1e-6
+    boxes = range(2, n)
+    fd_mu, fd_v = [], []
+    for i in boxes:
+        sp, sm = copy.deepcopy(scene), copy.deepcopy(scene)
+        sp["bodies"][i]["friction"] += h
+        sm["bodies"][i]["friction"] -= h
+        fd_mu.append((per_body_loss(final_positions(sp, params), target)[i] -
+                      per_body_loss(final_positions(sm, params), target)[i]) / (2 * h))
+        sp, sm = copy.deepcopy(scene), copy.deepcopy(scene)
+        sp["bodies"][i]["vel"] = list(np.array(scene["bodies"][i]["vel"]) + h * slope)
+        sm["bodies"][i]["vel"] = list(np.array(scene["bodies"][i]["vel"]) - h * slope)
+        fd_v.append((per_body_loss(final_positions(sp, params), target)[i] -
+                     per_body_loss(final_positions(sm, params), target)[i]) / (2 * h))
+    fd_mu, fd_v = np.array(fd_mu), np.array(fd_v)
+    ad_mu = gmu[2:]
+    ad_v = gv0[2:] @ slope
+    print("dL/dmu  adjoint", np.round(ad_mu, 6), "\n        fd     ", np.round(fd_mu, 6))
+    print("dL/dv0  adjoint", np.round(ad_v, 6), "\n        fd     ", np.round(fd_v, 6))
+    assert np.abs(ad_mu - fd_mu).max() <= 1e-4 * np.abs(fd_mu).max()
+    assert np.abs(ad_v - fd_v).max() <= 1e-4 * np.abs(fd_v).max()
+
+
+def test_adjoint_forward_equals_stepping_and_rejects_unsupported():
+    scene, params, _ = avbd.scenes.ramp_lanes(n=4)
+    a = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    a.pack(scene, params)
+    a.adjoint_forward(25, 7)
+    b = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    b.pack(scene, params)
+    b.step(25)
+    assert np.array_equal(a.read_state()["pos"], b.read_state()["pos"])
+    m = avbd.HipBackend(precision=avbd.PREC_MIXED)
+    m.pack(scene, params)
+    with pytest.raises(AvbdError) as e:
+        m.adjoint_forward(5, 5)
+    assert e.value.code == -4
+    with pytest.raises(AvbdError) as e:
+        b.adjoint_backward(np.zeros((b.n, 3)))
+    assert e.value.code == -4  # no adjoint forward recorded
