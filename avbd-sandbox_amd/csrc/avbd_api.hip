@@ -116,7 +116,8 @@ struct avbd_ctx {
 
   // broad phase
   DBuf<int4> cellOf;
-  DBuf<int> bucketCnt, bucketStart, bucketCursor, entries;
+  DBuf<int> bucketCnt, bucketStart, bucketCursor, entries, bpScratch;
+  DBuf<int4> entryCell;
   DBuf<int> sceneLargeStart, sceneLarges, sceneStart, largeList;
   DBuf<int> countRev, startRev, countNewRev, startNewRev;
   DBuf<int2> pairs, newPairs;
@@ -282,6 +283,7 @@ struct avbd_ctx {
     for (auto &e : evPool) (void)hipEventDestroy(e);
     evPool.clear();
     cellOf.release(); bucketCnt.release(); bucketStart.release(); bucketCursor.release(); entries.release();
+    bpScratch.release(); entryCell.release();
     sceneLargeStart.release(); sceneLarges.release(); sceneStart.release(); largeList.release();
     countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
@@ -557,7 +559,7 @@ struct avbd_ctx {
     HIPCHECK(hipMemcpyAsync(bucketCursor.p, bucketStart.p, sizeof(int) * (tableMask + 1), hipMemcpyDeviceToDevice,
                             stream));
     hipLaunchKernelGGL(k_bp_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, cellOf.p, tableMask, bucketCursor.p,
-                       entries.p);
+                       entries.p, entryCell.p);
     BpArgs ba;
     ba.n = N; ba.J = J; ba.props = props.p; ba.X = X.p; ba.cellOf = cellOf.p; ba.bucketStart = bucketStart.p;
     ba.bucketCnt = bucketCnt.p; ba.entries = entries.p; ba.mask = tableMask; ba.sceneLargeStart = sceneLargeStart.p;
@@ -565,15 +567,16 @@ struct avbd_ctx {
     ba.adjOther = adjOther.p; ba.joints = joints.p; ba.countRev = countRev.p; ba.startRev = startRev.p;
     ba.countNewRev = countNewRev.p; ba.startNewRev = startNewRev.p;
     ba.pairs = pairs.p; ba.newPairs = newPairs.p; ba.pairCap = pairCap;
+    ba.entryCell = entryCell.p; ba.scratch = bpScratch.p;
     HIPCHECK(hipMemsetAsync(countRev.p, 0, sizeof(int) * N, stream));
     HIPCHECK(hipMemsetAsync(countNewRev.p, 0, sizeof(int) * N, stream));
-    hipLaunchKernelGGL(k_bp_small<0>, dim3(blocks(N)), dim3(256), 0, stream, ba);
+    hipLaunchKernelGGL(k_bp_team, dim3(blocks(N, 256 / BP_TEAM)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<0>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
     scan(countRev.p, startRev.p, N, scal.p + 0);
     scan(countNewRev.p, startNewRev.p, N, scal.p + 1);
     // the write pass keeps every write below pairCap; an excess is reported
     // below, before anything reads the lists
-    hipLaunchKernelGGL(k_bp_small<1>, dim3(blocks(N)), dim3(256), 0, stream, ba);
+    hipLaunchKernelGGL(k_bp_emit, dim3(blocks(N)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<1>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
     readScal(0, 2);
     nPairs = hscal[0];
@@ -1024,7 +1027,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.DQ.reserve(size_t(N) * 8 * c.rsize());
   HIPCHECK(hipMemset(c.DQ.p, 0, size_t(N) * 8 * c.rsize()));
 
-  c.cellOf.reserve(N); c.entries.reserve(N);
+  c.cellOf.reserve(N); c.entries.reserve(N); c.entryCell.reserve(N);
+  c.bpScratch.reserve(size_t(N) * BP_CAP);
   c.bucketCnt.reserve(tsz); c.bucketStart.reserve(tsz + 1); c.bucketCursor.reserve(tsz);
   c.sceneLargeStart.reserve(S + 1); c.sceneLarges.reserve(slist.size() + 1);
   c.sceneStart.reserve(S + 1); c.largeList.reserve(largeList.size() + 1);

@@ -90,13 +90,16 @@ __global__ void k_bp_cells(int n, const BodyProp *props, const double *X, double
   atomicAdd(&bucketCnt[cellHash(b.scene, cx, cy, cz) & mask], 1);
 }
 
-__global__ void k_bp_scatter(int n, const int4 *cellOf, uint32_t mask, int *cursor, int *entries) {
+// Bucket the small bodies; each entry carries its cell so that a bucket scan
+// reads it in sequence.
+__global__ void k_bp_scatter(int n, const int4 *cellOf, uint32_t mask, int *cursor, int *entries, int4 *entryCell) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int4 c = cellOf[i];
   if (c.w < 0) return;
   const int pos = atomicAdd(&cursor[cellHash(c.w, c.x, c.y, c.z) & mask], 1);
   entries[pos] = i;
+  entryCell[pos] = c;
 }
 
 // Is there a force between x and y? kind: 1 live joint, 2 manifold, 0 none.
@@ -130,6 +133,8 @@ struct BpArgs {
   const int *startRev, *startNewRev;         // [n] exclusive scans of the counts
   int2 *pairs, *newPairs;                    // output lists (A, B)
   int pairCap;
+  const int4 *entryCell;                     // cell of each bucket entry
+  int *scratch;                              // [n * BP_CAP] per-body sorted lists
 };
 
 // The predicate of solver.js:316-329 for A > B, plus the two flags: is the
@@ -162,25 +167,81 @@ __device__ __forceinline__ void insertDesc(int2 *list, int base, int cnt, int ca
 }
 
 // Small bodies: pairs (x, y) with y < x from the grid, and (x, L) with the
-// scene's large bodies L < x. PASS 0 counts, PASS 1 writes both lists sorted
-// by y descending.
-template <int PASS>
-__global__ void k_bp_small(BpArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n) return;
-  const BodyProp &px = a.props[x];
-  if (px.large) return;
-  const int4 c = a.cellOf[x];
-  int cnt = 0, ncnt = 0, base = 0, nbase = 0;
-  if (PASS == 1) {
-    base = a.startRev[a.n - 1 - x];
-    nbase = a.startNewRev[a.n - 1 - x];
-  }
-  auto accept = [&](int y, int t) {
-    if (PASS == 1) {
-      insertDesc(a.pairs, base, cnt, a.pairCap, x, y);
-      if (t == 2) insertDesc(a.newPairs, nbase, ncnt, a.pairCap, x, y);
+// scene's large bodies L < x, sorted by y descending (the brute-force order
+// i = N-1..1, j = i-1..0 of broadphase.js:236-243 restricted to one i).
+//
+// One pass with a team of 32 lanes per body: lanes 0-26 scan one neighbour
+// cell each, lanes 27-31 the scene's large bodies. The accepted pairs are
+// ranked by y in LDS and written to the body's slot of a scratch list (at most
+// BP_CAP entries; a body with more is listed again, serially, when the lists
+// are emitted). The counts go to countRev / countNewRev (indexed n-1-x so an
+// exclusive scan yields the descending-x layout).
+constexpr int BP_TEAM = 32, BP_CAP = 64, BP_NEW = 0x40000000;
+
+__global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
+  constexpr int TEAMS = 256 / BP_TEAM;
+  __shared__ int sy[TEAMS][BP_CAP];
+  __shared__ int scount[TEAMS], snew[TEAMS];
+  const int tm = threadIdx.x / BP_TEAM, lane = threadIdx.x % BP_TEAM;
+  const int x = blockIdx.x * TEAMS + tm;
+  if (lane == 0) { scount[tm] = 0; snew[tm] = 0; }
+  __syncthreads();
+  const bool active = x < a.n && !a.props[x].large;
+  if (active) {
+    auto accept = [&](int y, int t) {
+      const int k = atomicAdd(&scount[tm], 1);
+      if (t == 2) atomicAdd(&snew[tm], 1);
+      if (k < BP_CAP) sy[tm][k] = y | (t == 2 ? BP_NEW : 0);
+    };
+    const int4 c = a.cellOf[x];
+    if (lane < 27) {
+      const int qx = c.x + lane % 3 - 1, qy = c.y + (lane / 3) % 3 - 1, qz = c.z + lane / 9 - 1;
+      const uint32_t bkt = cellHash(c.w, qx, qy, qz) & a.mask;
+      const int s0 = a.bucketStart[bkt], s1 = s0 + a.bucketCnt[bkt];
+      for (int e = s0; e < s1; e++) {
+        const int y = a.entries[e];
+        if (y >= x) continue;
+        const int4 cy = a.entryCell[e];
+        if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
+        const int t = pairTest(a, x, y);
+        if (t) accept(y, t);
+      }
+    } else {
+      const int sc = a.props[x].scene;
+      const int l0 = a.sceneLargeStart[sc], l1 = a.sceneLargeStart[sc + 1];
+      for (int e = l0 + lane - 27; e < l1; e += BP_TEAM - 27) {
+        const int y = a.sceneLarges[e];
+        if (y >= x) continue;
+        const int t = pairTest(a, x, y);
+        if (t) accept(y, t);
+      }
     }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int cnt = scount[tm], m = min(cnt, BP_CAP);
+  int *out = a.scratch + size_t(x) * BP_CAP;
+  for (int j = lane; j < m; j += BP_TEAM) {
+    const int v = sy[tm][j], y = v & ~BP_NEW;
+    int rank = 0;
+    for (int k = 0; k < m; k++) rank += (sy[tm][k] & ~BP_NEW) > y;
+    out[rank] = v;
+  }
+  if (lane == 0) {
+    a.countRev[a.n - 1 - x] = cnt;
+    a.countNewRev[a.n - 1 - x] = snew[tm];
+  }
+}
+
+// Serial listing of one small body straight into the output (insertion into
+// the descending run): the path for a body with more than BP_CAP pairs.
+__device__ void bpSerial(const BpArgs &a, int x) {
+  const int4 c = a.cellOf[x];
+  const int base = a.startRev[a.n - 1 - x], nbase = a.startNewRev[a.n - 1 - x];
+  int cnt = 0, ncnt = 0;
+  auto accept = [&](int y, int t) {
+    insertDesc(a.pairs, base, cnt, a.pairCap, x, y);
+    if (t == 2) insertDesc(a.newPairs, nbase, ncnt, a.pairCap, x, y);
     cnt++;
     ncnt += t == 2;
   };
@@ -193,22 +254,42 @@ __global__ void k_bp_small(BpArgs a) {
         for (int e = s0; e < s1; e++) {
           const int y = a.entries[e];
           if (y >= x) continue;
-          const int4 cy = a.cellOf[y];
+          const int4 cy = a.entryCell[e];
           if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
           const int t = pairTest(a, x, y);
           if (t) accept(y, t);
         }
       }
-  const int l0 = a.sceneLargeStart[px.scene], l1 = a.sceneLargeStart[px.scene + 1];
-  for (int e = l0; e < l1; e++) {
+  const int sc = a.props[x].scene;
+  for (int e = a.sceneLargeStart[sc]; e < a.sceneLargeStart[sc + 1]; e++) {
     const int y = a.sceneLarges[e];
     if (y >= x) continue;
     const int t = pairTest(a, x, y);
     if (t) accept(y, t);
   }
-  if (PASS == 0) {
-    a.countRev[a.n - 1 - x] = cnt;
-    a.countNewRev[a.n - 1 - x] = ncnt;
+}
+
+// Emit each small body's sorted list at its place in the pair lists (all
+// pairs, and the subsequence that opens a new manifold).
+__global__ void k_bp_emit(BpArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n || a.props[x].large) return;
+  const int cnt = a.countRev[a.n - 1 - x];
+  if (cnt == 0) return;
+  if (cnt > BP_CAP) {
+    bpSerial(a, x);
+    return;
+  }
+  const int base = a.startRev[a.n - 1 - x], nbase = a.startNewRev[a.n - 1 - x];
+  const int *in = a.scratch + size_t(x) * BP_CAP;
+  int k = 0;
+  for (int j = 0; j < cnt; j++) {
+    const int v = in[j], y = v & ~BP_NEW;
+    if (base + j < a.pairCap) a.pairs[base + j] = make_int2(x, y);
+    if (v & BP_NEW) {
+      if (nbase + k < a.pairCap) a.newPairs[nbase + k] = make_int2(x, y);
+      k++;
+    }
   }
 }
 

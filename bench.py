@@ -12,7 +12,12 @@ Also reported:
                 same length) against 8 TB/s HBM; traffic = HBM bytes per launch
                 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary;
   cpu_baseline  the f64 CPU oracle (a single-threaded restatement of the
-                reference engine) on a bounded sample of the same workload.
+                reference engine) on a bounded sample of the same workload;
+  adjoint       (pyramid runs) forward and reverse time of the 64-box ramp
+                gradient benchmark.
+
+--workload rain / batched measure the 50,000-body mixed-shape rain and the 256
+independent chain and stack scenes the same way (not the headline line).
 """
 import argparse
 import json
@@ -58,17 +63,106 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
 
 
+WORKLOADS = ("pyramid", "rain", "batched")
+
+
+def build_workload(args):
+    """(scene, params, description, cpu sample runner) of the chosen workload."""
+    import avbd
+    if args.workload == "pyramid":
+        scene, params, meta = avbd.scenes.pyramid(layers=args.layers)
+        desc = (f"pyramid of {meta['cubes']} unit cubes ({args.layers} layers, 1 m pitch, mu 0.6) on a static ground, "
+                f"dt 1/60, {params['iterations']} iterations")
+    elif args.workload == "rain":
+        scene, params, meta = avbd.scenes.rain(n=args.bodies)
+        desc = (f"rain of {args.bodies} bodies (70% boxes 0.2-1 m, 30% spheres r 0.1-0.5 m, density log-uniform "
+                f"1-1000, mu 0.2-0.9) into a 40 x 40 m walled column, dt 1/60, {params['iterations']} iterations")
+    else:
+        scene, params, meta = avbd.scenes.batched(num_scenes=args.scenes)
+        desc = (f"{args.scenes} independent scenes (chains of 10-200 hard links with a 26,000:1 end mass, stacks of "
+                f"5-50 cubes), dt 1/60, {params['iterations']} iterations")
+    params = dict(params, colorOrder=1 if args.order == "priority" else 0)
+    return scene, params, meta, desc
+
+
+def cpu_sample(args, scene, params, meta):
+    """The f64 oracle (one thread) on a bounded sample of the same workload:
+    returns (body-iterations/s, description)."""
+    import pyoracle
+    iters = params["iterations"]
+    if args.workload == "batched":
+        # scene by scene (the oracle has no notion of independent scenes)
+        bi, secs, k = 0, 0.0, 0
+        for part in meta["parts"]:
+            o = pyoracle.Oracle(part, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
+            secs += o.step(args.cpu_steps)
+            bi += o.n_dynamic * iters * args.cpu_steps
+            k += 1
+            if secs > 15:
+                break
+        return bi / secs, (f"steps 1-{args.cpu_steps} of the first {k} scenes, f64 oracle (C++ -O2, single thread); "
+                           f"{secs:.1f} s")
+    t0 = time.perf_counter()
+    o = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
+    build_s = time.perf_counter() - t0
+    secs = o.step(args.cpu_steps)
+    return (o.n_dynamic * iters * args.cpu_steps / secs,
+            f"steps 1-{args.cpu_steps} of the same scene, {iters} iterations, f64 oracle (C++ -O2, single thread); "
+            f"{secs:.1f} s (+{build_s:.1f} s scene build)")
+
+
+def adjoint_leg():
+    """Forward and reverse timing of the ramp benchmark: 64 boxes (0.5 m, rho
+    500) on a 30-degree ramp, 120 steps, 10 iterations, f64, checkpoint every
+    10 steps; gradients of a loss of the final positions with respect to each
+    box's friction and initial velocity."""
+    import avbd
+    import numpy as np
+    scene, params, meta = avbd.scenes.ramp(64)
+    out = None
+    for _ in range(2):  # the first round warms up the code paths
+        f = avbd.HipBackend(precision=avbd.PREC_F64)
+        f.pack(scene, params)
+        t0 = time.perf_counter()
+        f.step(meta["steps"])
+        f.idle()
+        fwd = time.perf_counter() - t0
+        a = avbd.HipBackend(precision=avbd.PREC_F64)
+        a.pack(scene, params)
+        t0 = time.perf_counter()
+        a.adjoint_forward(meta["steps"], 10)
+        a.idle()
+        cfwd = time.perf_counter() - t0
+        x = a.read_state()["pos"]
+        target = np.array([b["pos"] for b in scene["bodies"]])
+        dL = 2 * (x - target)
+        dL[:2] = 0
+        t0 = time.perf_counter()
+        a.adjoint_backward(dL)
+        bwd = time.perf_counter() - t0
+        out = {"workload": "ramp: 64 boxes (0.5 m, rho 500, mu 0.1-0.9, v0 0-2 m/s) on a 30-degree ramp, 120 steps, "
+                           "10 iterations, f64, checkpoint every 10 steps; d loss / d mu and d loss / d v0",
+               "forward_ms": fwd * 1e3, "checkpointed_forward_ms": cfwd * 1e3, "backward_ms": bwd * 1e3,
+               "backward_over_forward": bwd / fwd, "adjoint_total_over_forward": (cfwd + bwd) / fwd}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="pyramid", choices=WORKLOADS,
+                    help="pyramid (the headline), rain (50k mixed shapes) or batched (256 scenes)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f64", "exact"])
     ap.add_argument("--team", type=int, default=0, help="lanes per body; 0 = autotune")
     ap.add_argument("--order", default="priority", choices=["priority", "index"])
     ap.add_argument("--layers", type=int, default=53)
+    ap.add_argument("--bodies", type=int, default=50000, help="rain: body count")
+    ap.add_argument("--scenes", type=int, default=256, help="batched: scene count")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -85,8 +179,7 @@ def main():
     from avbd import _lib as L
 
     prec = {"mixed": avbd.PREC_MIXED, "f64": avbd.PREC_F64, "exact": avbd.PREC_EXACT}[args.precision]
-    scene, params, meta = avbd.scenes.pyramid(layers=args.layers)
-    params = dict(params, colorOrder=1 if args.order == "priority" else 0)
+    scene, params, meta, desc = build_workload(args)
     gpu = avbd.HipBackend(precision=prec, team=args.team)
     gpu.pack(scene, params)
     iters = params["iterations"]
@@ -109,8 +202,8 @@ def main():
     gpu.idle()
     secs = time.perf_counter() - t0
 
-    # the same number of further steps with per-kernel HIP events, for the
-    # roofline and the phase split (direct launches, so each is bracketed)
+    # the same number of further steps with per-kernel timing, for the
+    # roofline and the phase split (direct launches)
     gpu.set_profiling(True)
     t1 = time.perf_counter()
     gpu.step(args.steps)
@@ -124,10 +217,10 @@ def main():
     gpu.set_profiling(False)
 
     value = nd * iters * args.steps / secs
-    avg_ms = pms / plaunch
-    achieved = (pbytes / plaunch) / (avg_ms * 1e-3) / 1e9
+    avg_ms = pms / max(plaunch, 1)
+    achieved = (pbytes / max(plaunch, 1)) / (avg_ms * 1e-3) / 1e9 if pms > 0 else 0.0
     team = tune[0] if tune else args.team
-    traffic, traffic_src = measured_traffic(args.precision, team)
+    traffic, traffic_src = measured_traffic(args.precision, team) if args.workload == "pyramid" else (None, None)
     line = {
         "metric": METRIC,
         "value": value,
@@ -142,20 +235,18 @@ def main():
         "dtype": {"mixed": "f32+f64", "f64": "f64", "exact": "f64"}[args.precision],
         "data": "synthetic",
         "config": {
-            "workload": f"pyramid of {meta['cubes']} unit cubes ({args.layers} layers, 1 m pitch, mu 0.6) "
-                        f"on a static ground, dt 1/60, {iters} iterations, steps {args.warmup + 1}-"
-                        f"{args.warmup + args.steps} after pack",
+            "workload": f"{desc}, steps {args.warmup + 1}-{args.warmup + args.steps} after pack",
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
             "precision": args.precision, "team": team, "color_order": args.order,
             "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],
-            "manifolds": c["manifolds"], "contacts": c["contacts"], "colors": c["colors"],
+            "manifolds": c["manifolds"], "contacts": c["contacts"], "joints": c["joints"], "colors": c["colors"],
         },
         "roofline": {
             "kernel": "k_primal", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "avg_launch_us": avg_ms * 1e3, "launches": plaunch,
-            "alg_bytes_per_launch": pbytes / plaunch,
+            "alg_bytes_per_launch": pbytes / max(plaunch, 1),
         },
         "kernels": {
             "primal_ms_per_step": pms / args.steps, "dual_ms_per_step": dms / args.steps,
@@ -165,18 +256,12 @@ def main():
         "phases_ms": phases,
         "profiled_ms_per_step": psecs / args.steps * 1e3,
     }
+    if not args.no_adjoint and args.workload == "pyramid":
+        line["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        import pyoracle
-        t0 = time.perf_counter()
-        orc = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
-        build_s = time.perf_counter() - t0
-        cpu_secs = orc.step(args.cpu_steps)
-        cpu_rate = orc.n_dynamic * iters * args.cpu_steps / cpu_secs
-        line["cpu_baseline"] = {
-            "value": cpu_rate, "unit": "body-iterations/s", "cores": 1, "kind": "port",
-            "sample": f"steps 1-{args.cpu_steps} of the same {meta['cubes']}-cube pyramid, {iters} iterations, "
-                      f"f64 oracle (C++ -O2, single thread); {cpu_secs:.1f} s (+{build_s:.1f} s scene build)",
-        }
+        cpu_rate, sample = cpu_sample(args, scene, params, meta)
+        line["cpu_baseline"] = {"value": cpu_rate, "unit": "body-iterations/s", "cores": 1, "kind": "port",
+                                "sample": sample}
         line["vs_cpu"] = value / cpu_rate
     print(json.dumps(line))
 

@@ -104,6 +104,30 @@ def test_exact_multi_workgroup_scene(order):
         assert np.array_equal(st["pos"], o[:, :3]) and np.array_equal(st["quat"], o[:, 3:7]), f"step {s}: state"
 
 
+def test_broad_phase_crowded_bodies_take_the_serial_path():
+    # 150 overlapping spheres in a 1 m box: the later ones have more than the
+    # 64 pairs a body's ranked list holds, so their lists come from the serial
+    # path; the pair lists and the step still equal the oracle's
+    r = avbd.scenes.lcg(17)
+    bodies = [avbd.scenes.ground()]
+    for _ in range(150):
+        bodies.append(avbd.scenes.sphere(0.6, 1.0, 0.5, [r(), r(), 1.0 + r()]))
+    scene = dict(bodies=bodies, joints=[])
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    gpu.pack(scene)
+    ref = pyoracle.Oracle(scene, {}, schedule=pyoracle.SCHED_COLOR)
+    ref.record_pairs(True)
+    for s in range(1, 4):
+        gpu.step(1)
+        ref.step(1)
+        gp, rp = gpu.read_pairs(), ref.pairs()
+        assert np.array_equal(gp, rp), f"step {s}: pairs"
+        if s == 1:
+            assert np.bincount(gp[:, 0]).max() > 64  # the serial path was taken
+        st = gpu.read_state()
+        assert np.array_equal(st["pos"], ref.state()[:, :3]), f"step {s}: state"
+
+
 def test_exact_counters_and_manifold_records():
     doc = json.load(open(os.path.join(GOLDEN, "stack10_color.json")))
     gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
