@@ -136,14 +136,15 @@ struct avbd_ctx {
   DBuf<int> deg, adjStart, adjCursor, adjKey, adjOther;
   DBuf<int> color[2];
   int colorCur = 0;
-  DBuf<int> ccount, cstart, ccursor, cbodies, cmeta;
+  DBuf<int> cstart, cbodies, cmeta;
   DBuf<int4> cwork;
   std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
   int nDynamic = 0;
 
   // solve-loop graphs, keyed by colour count and manifold buffer parity
-  std::map<long long, hipGraphExec_t> graphs;
+  // (the captured graph is kept with its executable until both are dropped)
+  std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   bool graphCapturing = false;
   int graphsCfg = 0;
   // autotuner
@@ -151,7 +152,7 @@ struct avbd_ctx {
   float tuneMs[6] = {0};
   DBuf<char> Xbk, DQbk;
   int numColors = 0, colorRounds = 0;
-  DBuf<int> itemCnt, itemStart;
+  DBuf<int> itemCnt, itemStart, itemJoints, chist, coffs, ctotal;
   DBuf<uint2> items;
 
   // ------------------------------------------------------------- adjoint
@@ -288,8 +289,9 @@ struct avbd_ctx {
     countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
-    color[0].release(); color[1].release(); ccount.release(); cstart.release(); ccursor.release(); cbodies.release(); cwork.release();
+    color[0].release(); color[1].release(); cstart.release(); cbodies.release(); cwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
+    itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
     for (auto &t : tape) t.release();
     tape.clear();
     for (auto &c : ckpts) c.release();
@@ -372,19 +374,19 @@ struct avbd_ctx {
     const int4 *head = store[cur].head.p;
     const int cap = int(items.n);
     hipLaunchKernelGGL(k_items<0>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
-                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5);
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5, itemJoints.p);
     scan(itemCnt.p, itemStart.p, N, itemStart.p + N);
     hipLaunchKernelGGL(k_items<1>, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, adjStart.p, adjKey.p,
-                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5);
+                       live[cur].p, head, joints.p, itemCnt.p, itemStart.p, items.p, cap, scal.p + 5, itemJoints.p);
   }
 
   // ------------------------------------------------------------ colouring
   // cmeta: [0..2] change counters, [3] rounds, [4] colours, [8..8+257) bucket
-  // starts. One cooperative launch relaxes to the greedy fixed point; one
-  // readback brings the colour count and buckets to the host.
-  // Rounds go out in batches (colorBatch round + check pairs); rounds after
-  // convergence return at once. The bucket kernels follow the batch, so when
-  // it converged a single readback serves both.
+  // starts. Relaxation rounds go out in batches (colorBatch round + check
+  // pairs) and reach the greedy fixed point; rounds after convergence return
+  // at once. The bucket kernels follow the batch, so when it converged a single
+  // readback brings the colour count and buckets to the host. The batch is
+  // sized from the previous step's round count.
   void colorize() {
     HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
     const int maxRounds = N + 2;
@@ -395,19 +397,22 @@ struct avbd_ctx {
         hipLaunchKernelGGL(k_color_check, dim3(1), dim3(64), 0, stream, cmeta.p);
       }
       launched += colorBatch;
-      HIPCHECK(hipMemsetAsync(ccount.p, 0, sizeof(int) * MAX_COLORS, stream));
       HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
-      hipLaunchKernelGGL(k_color_count, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, ccount.p,
-                         cmeta.p + 4);
-      hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ccount.p, cmeta.p + 8, ccursor.p);
-      hipLaunchKernelGGL(k_color_scatter, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, ccursor.p,
-                         cbodies.p, itemStart.p, items.p, cwork.p);
+      const int nb = blocks(N);
+      hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
+      hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
+      hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
+      hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
+                         cbodies.p, itemStart.p, itemJoints.p, cwork.p);
+      hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
+                         items.p);
       HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
       if (hmeta[1]) break;
       if (launched > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
     }
     colorRounds = hmeta[3];
+    colorBatch = std::min(64, std::max(4, colorRounds + 2));
     numColors = hmeta[4];
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
@@ -478,7 +483,10 @@ struct avbd_ctx {
   }
 
   void dropGraphs() {
-    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto &g : graphs) {
+      (void)hipGraphExecDestroy(g.second.second);
+      (void)hipGraphDestroy(g.second.first);
+    }
     graphs.clear();
   }
 
@@ -504,11 +512,11 @@ struct avbd_ctx {
       HIPCHECK(hipStreamEndCapture(stream, &g));
       hipGraphExec_t ex = nullptr;
       const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) (void)hipGraphDestroy(g);
       HIPCHECK(e);
-      it = graphs.emplace(key, ex).first;
+      it = graphs.emplace(key, std::make_pair(g, ex)).first;
     }
-    HIPCHECK(hipGraphLaunch(it->second, stream));
+    HIPCHECK(hipGraphLaunch(it->second.second, stream));
   }
 
   // One primal sweep (every colour, first iteration's alpha) per team shape,
@@ -1083,10 +1091,12 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   HIPCHECK(hipMemcpy(c.color[0].p, col.data(), sizeof(int) * N, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c.color[1].p, col.data(), sizeof(int) * N, hipMemcpyHostToDevice));
   c.colorCur = 0;
-  c.ccount.reserve(MAX_COLORS); c.cstart.reserve(MAX_COLORS + 1); c.ccursor.reserve(MAX_COLORS);
+  c.cstart.reserve(MAX_COLORS + 1);
   c.cbodies.reserve(N);
   c.cwork.reserve(N);
-  c.itemCnt.reserve(N); c.itemStart.reserve(N + 1);
+  c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.itemJoints.reserve(N);
+  c.chist.reserve(size_t(blocks(N)) * MAX_COLORS); c.coffs.reserve(size_t(blocks(N)) * MAX_COLORS);
+  c.ctotal.reserve(MAX_COLORS);
   c.items.reserve(2 * (size_t(MAXC) * c.manCap + c.J) + 1);
   c.scal.reserve(16);
   c.cbytes.reserve(2 * MAX_COLORS);

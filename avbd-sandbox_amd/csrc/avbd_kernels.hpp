@@ -741,75 +741,103 @@ __global__ void k_color_check(int *ctl) {
 // Flags in the y word of a contact work item (see k_items).
 constexpr uint32_t ITEM_FIRST = 0x80000000u, ITEM_SPHERE = 0x40000000u, ITEM_BODY = 0x3fffffffu;
 
-// Wave-aggregated bucket counting: the lanes of a wave that hold the same
-// colour add their number once (one atomic per colour present in the wave
-// rather than one per body).
-__device__ __forceinline__ int waveBucketAdd(int c, int *counter) {
-  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
-  unsigned long long todo = __ballot(c >= 0);
-  int pos = -1;
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int cl = __shfl(c, leader);
-    const unsigned long long same = __ballot(c == cl);
-    int base = 0;
-    if (__lane_id() == leader) base = atomicAdd(&counter[cl], __popcll(same));
-    base = __shfl(base, leader);
-    if (c == cl) pos = base + __popcll(same & lt);
-    todo &= ~same;
-  }
-  return pos;
-}
-
-__global__ void k_color_count(int n, const int *color, int *ccount, int *numColors) {
+// Bucketing the dynamic bodies by colour without contended atomics: each
+// 256-body block writes its colour histogram (hist[block][colour]), one block
+// turns the histograms into per-block offsets and the colour starts, and each
+// block then places its bodies (order within a colour and block is free: a
+// colour's bodies share no force).
+__global__ void k_color_hist(int n, const int *color, int *hist, int *numColors) {
+  __shared__ int h[MAX_COLORS];
+  __shared__ int top;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = threadIdx.x; k < MAX_COLORS; k += blockDim.x) h[k] = 0;
+  if (threadIdx.x == 0) top = 0;
+  __syncthreads();
   const int c = i < n ? color[i] : -1;
-  waveBucketAdd(c, ccount);
-  int top = c + 1;
-  for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
-  if (__lane_id() == 0 && top > 0) atomicMax(numColors, top);
+  if (c >= 0) {
+    atomicAdd(&h[c], 1);
+    atomicMax(&top, c + 1);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < MAX_COLORS; k += blockDim.x) hist[size_t(blockIdx.x) * MAX_COLORS + k] = h[k];
+  if (threadIdx.x == 0 && top > 0) atomicMax(numColors, top);
 }
 
-// Exclusive scan of the MAX_COLORS bucket counts (one 256-thread block):
-// cstart[0..MAX_COLORS], and the scatter cursors.
-__global__ void k_color_starts(const int *ccount, int *cstart, int *cursor) {
-  __shared__ int sh[MAX_COLORS];
-  const int t = threadIdx.x;
-  const int v = ccount[t];
-  sh[t] = v;
+// One 256-thread block per colour: offs[block][colour], the colour's bodies in
+// earlier blocks (each thread scans a contiguous run of blocks, then the runs
+// are offset by a block-wide scan), and the colour's total.
+__global__ void k_color_offsets(int nb, const int *hist, int *offs, int *ctotal) {
+  __shared__ int sh[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int per = (nb + 255) / 256, b0 = t * per, b1 = min(nb, b0 + per);
+  int run = 0;
+  for (int b = b0; b < b1; b++) run += hist[size_t(b) * MAX_COLORS + c];
+  sh[t] = run;
   __syncthreads();
-  for (int off = 1; off < MAX_COLORS; off <<= 1) {
+  for (int off = 1; off < 256; off <<= 1) {
     const int u = t >= off ? sh[t - off] : 0;
     __syncthreads();
     sh[t] += u;
     __syncthreads();
   }
-  cstart[t] = sh[t] - v;
-  cursor[t] = sh[t] - v;
-  if (t == MAX_COLORS - 1) cstart[MAX_COLORS] = sh[t];
+  int acc = sh[t] - run;
+  for (int b = b0; b < b1; b++) {
+    const int v = hist[size_t(b) * MAX_COLORS + c];
+    offs[size_t(b) * MAX_COLORS + c] = acc;
+    acc += v;
+  }
+  if (t == 255) ctotal[c] = sh[255];
 }
 
-// Bucket the dynamic bodies by colour: cbodies (the body list of each colour)
-// and cwork, the same list with each body's work-item range and its number of
-// joint items, which is all a primal team needs to find its work.
-// It also marks each contact item whose body comes first in the colour order
-// (the other body is static or of a later colour).
-__global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodies, const int *itemStart,
-                                uint2 *items, int4 *cwork) {
+// Colour starts: exclusive scan of the MAX_COLORS totals (one block).
+__global__ void k_color_starts(const int *ctotal, int *cstart) {
+  __shared__ int sh[MAX_COLORS];
+  const int c = threadIdx.x;
+  const int v = ctotal[c];
+  sh[c] = v;
+  __syncthreads();
+  for (int off = 1; off < MAX_COLORS; off <<= 1) {
+    const int u = c >= off ? sh[c - off] : 0;
+    __syncthreads();
+    sh[c] += u;
+    __syncthreads();
+  }
+  cstart[c] = sh[c] - v;
+  if (c == MAX_COLORS - 1) cstart[MAX_COLORS] = sh[c];
+}
+
+// Place the bodies: cbodies (the body list of each colour) and cwork, the
+// same list with each body's work-item range and its number of joint items,
+// which is all a primal team needs to find its work.
+__global__ void k_color_scatter(int n, const int *color, const int *offs, const int *cstart, int *cbodies,
+                                const int *itemStart, const int *itemJoints, int4 *cwork) {
+  __shared__ int cur[MAX_COLORS];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = threadIdx.x; k < MAX_COLORS; k += blockDim.x)
+    cur[k] = cstart[k] + offs[size_t(blockIdx.x) * MAX_COLORS + k];
+  __syncthreads();
   const int c = i < n ? color[i] : -1;
-  const int pos = waveBucketAdd(c, cursor);
   if (c < 0) return;
-  const int b0 = itemStart[i], b1 = itemStart[i + 1];
-  int nj = 0;
-  for (int t = b0; t < b1; t++) {
+  const int pos = atomicAdd(&cur[c], 1);
+  cbodies[pos] = i;
+  cwork[pos] = make_int4(i, itemStart[i], itemStart[i + 1], itemJoints[i]);
+}
+
+// Mark each contact item whose body comes first in the colour order (the other
+// body is static or of a later colour): that body's sweep applies the
+// contact's dual update. A team of 8 lanes per body.
+__global__ void k_item_flags(int n, const int *color, const int *itemStart, uint2 *items) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / 8, lane = gt % 8;
+  if (i >= n) return;
+  const int c = color[i];
+  if (c < 0) return;
+  for (int t = itemStart[i] + lane; t < itemStart[i + 1]; t += 8) {
     const uint2 w = items[t];
-    if (w.x & 0x80000000u) { nj++; continue; }
+    if (w.x & 0x80000000u) continue;
     const int co = color[w.y & ITEM_BODY];
     items[t].y = (w.y & ~ITEM_FIRST) | ((co < 0 || co > c) ? ITEM_FIRST : 0u);
   }
-  cbodies[pos] = i;
-  cwork[pos] = make_int4(i, b0, b1, nj);
 }
 
 // Per dynamic body: its contact/joint work items in the order the reference
@@ -823,13 +851,21 @@ __global__ void k_color_scatter(int n, const int *color, int *cursor, int *cbodi
 template <int PASS>
 __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart, const int *adjKey, const int *live,
                         const int4 *head, const JointDev *joints, int *itemCnt, const int *itemStart,
-                        uint2 *items, int cap, int *overflow) {
+                        uint2 *items, int cap, int *overflow, int *itemJoints) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (!(props[i].mass > 0)) { if (PASS == 0) itemCnt[i] = 0; return; }
-  int cnt = 0;
+  if (!(props[i].mass > 0)) {
+    if (PASS == 0) itemCnt[i] = 0;
+    if (PASS == 1) itemJoints[i] = 0;
+    return;
+  }
+  int cnt = 0, nj = 0;
   const int base = PASS == 1 ? itemStart[i] : 0;
-  if (PASS == 1 && itemStart[i + 1] > cap) { atomicOr(overflow, 8); return; }
+  if (PASS == 1 && itemStart[i + 1] > cap) {
+    atomicOr(overflow, 8);
+    itemJoints[i] = 0;
+    return;
+  }
   for (int e = adjStart[i + 1] - 1; e >= adjStart[i]; e--) {
     const int key = adjKey[e];
     if (key < J) {
@@ -839,6 +875,7 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
                                        uint32_t(isA ? joints[key].b : joints[key].a));
       }
       cnt++;
+      nj++;
     } else {
       const int m = live[key - J];
       const int4 h = head[m];
@@ -851,6 +888,7 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
     }
   }
   if (PASS == 0) itemCnt[i] = cnt;
+  if (PASS == 1) itemJoints[i] = nj;
 }
 
 // Algorithmic bytes of one primal visit of each dynamic body, summed per

@@ -163,6 +163,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch the solve loop directly, without graph replay")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -180,7 +181,7 @@ def main():
 
     prec = {"mixed": avbd.PREC_MIXED, "f64": avbd.PREC_F64, "exact": avbd.PREC_EXACT}[args.precision]
     scene, params, meta, desc = build_workload(args)
-    gpu = avbd.HipBackend(precision=prec, team=args.team)
+    gpu = avbd.HipBackend(precision=prec, team=args.team, graphs=-1 if args.no_graphs else 0)
     gpu.pack(scene, params)
     iters = params["iterations"]
     t0 = time.perf_counter()
