@@ -664,4 +664,130 @@ __device__ __forceinline__ void stampContact(Sys<A> &s, const R *jl, const R *ja
   }
 }
 
+// ----------------------------------------------- fused contact arithmetic
+// The team modes' contact evaluation and stamp (the exact mode keeps the
+// reference's expression order above): the same quantities with fused
+// multiply-adds, the basis signs folded into the products, and the cone
+// scale from a reciprocal square root.
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float rsqrtR(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrtR(double x) { return 1.0 / __builtin_sqrt(x); }
+
+template <class R> struct FEval {
+  R jA[9], jB[9];  // angular rows: jA_r = rAw x B_r, jB_r = B_r x rBw
+  R C[3], F[3];
+  bool within;
+};
+
+template <class R>
+__device__ __forceinline__ void fcross(R *o, const R *a, const R *b) {
+  o[0] = fmaR(a[1], b[2], -(a[2] * b[1]));
+  o[1] = fmaR(a[2], b[0], -(a[0] * b[2]));
+  o[2] = fmaR(a[0], b[1], -(a[1] * b[0]));
+}
+
+template <class R>
+__device__ __forceinline__ void fevalC(const R *B, const R *hot, const R *dqA, const R *dqB, R alpha, FEval<R> &e) {
+  const R d[3] = {dqA[0] - dqB[0], dqA[1] - dqB[1], dqA[2] - dqB[2]};
+  const R oma = R(1) - alpha;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    fcross(e.jA + r * 3, hot + HRA, B + r * 3);
+    fcross(e.jB + r * 3, B + r * 3, hot + HRB);
+    R c = oma * hot[HC0 + r];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      c = fmaR(B[r * 3 + k], d[k], c);
+      c = fmaR(e.jA[r * 3 + k], dqA[3 + k], c);
+      c = fmaR(e.jB[r * 3 + k], dqB[3 + k], c);
+    }
+    e.C[r] = c;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void fevalF(const R *k, const R *lam, R mu, FEval<R> &e) {
+  const R z0 = fmaR(k[0], e.C[0], lam[0]);
+  const R z1 = fmaR(k[1], e.C[1], lam[1]);
+  const R z2 = fmaR(k[2], e.C[2], lam[2]);
+  const R f0 = z0 < R(0) ? z0 : R(0);
+  const R bounds = -f0 * mu;
+  const R fs2 = fmaR(z1, z1, z2 * z2);
+  e.F[0] = f0;
+  e.within = fs2 <= bounds * bounds;
+  // outside the cone: scale by bounds / |z_t| (zero when the normal force is
+  // zero); the square is kept above the smallest normal number so that the
+  // hardware reciprocal square root never sees a flushed denormal
+  const R tiny = sizeof(R) == 4 ? R(1.1754944e-38f) : R(2.2250738585072014e-308);
+  const R scale = e.within ? R(1) : (bounds > R(0) ? bounds * rsqrtR(fs2 > tiny ? fs2 : tiny) : R(0));
+  e.F[1] = z1 * scale;
+  e.F[2] = z2 * scale;
+}
+
+// Adds one contact to the system of the side whose linear rows are s * B
+// (s = +1 for body A, -1 for body B) and angular rows ja.
+template <class R, class A>
+__device__ __forceinline__ void fstamp(Sys<A> &sy, const R *B, R s, const R *ja, const R *k, const R *F) {
+  R kB[9], u[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      kB[r * 3 + i] = k[r] * B[r * 3 + i];
+      u[r * 3 + i] = k[r] * ja[r * 3 + i];
+    }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int i = LT_R[q], j = LT_C[q];
+    R l = sy.lin[q], a = sy.ang[q];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      l = fmaR(kB[r * 3 + i], B[r * 3 + j], l);
+      a = fmaR(u[r * 3 + i], ja[r * 3 + j], a);
+    }
+    sy.lin[q] = l;
+    sy.ang[q] = a;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      R c = R(0);
+#pragma unroll
+      for (int r = 0; r < 3; r++) c = fmaR(u[r * 3 + i], B[r * 3 + j], c);
+      sy.cross[i * 3 + j] = fmaR(s, c, sy.cross[i * 3 + j]);
+    }
+  const R sF[3] = {s * F[0], s * F[1], s * F[2]};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    R l = sy.rl[i], a = sy.ra[i];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      l = fmaR(sF[r], B[r * 3 + i], l);
+      a = fmaR(F[r], ja[r * 3 + i], a);
+    }
+    sy.rl[i] = l;
+    sy.ra[i] = a;
+  }
+}
+
+// The dual update of a contact (dualUpdate) for the fused evaluation.
+template <class R>
+__device__ __forceinline__ int fdualUpdate(const R *k, const FEval<R> &e, R beta, R pmax, R stickT, bool hasSphere,
+                                           R *kl) {
+  kl[0] = k[0]; kl[1] = k[1]; kl[2] = k[2];
+  kl[3] = e.F[0]; kl[4] = e.F[1]; kl[5] = e.F[2];
+  if (e.F[0] < R(0)) kl[0] = jmin(fmaR(beta, xabs(e.C[0]), k[0]), pmax);
+  int stick = -1;
+  if (e.within) {
+    kl[1] = jmin(fmaR(beta, xabs(e.C[1]), k[1]), pmax);
+    kl[2] = jmin(fmaR(beta, xabs(e.C[2]), k[2]), pmax);
+    stick = fmaR(e.C[1], e.C[1], e.C[2] * e.C[2]) < stickT * stickT ? 1 : 0;
+  } else if (hasSphere) {
+    stick = 0;
+  }
+  return stick;
+}
+
 }  // namespace avbd

@@ -1052,6 +1052,26 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
         vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
         vload<HOT>(hot, mb.hot + si * HOT);
         vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
+        if constexpr (!EXACT) {
+          const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+          FEval<R> e;
+          if (dual && (iw.y & ITEM_FIRST)) {
+            fevalC(bas, hot, dA, dB, rdalpha, e);
+            fevalF(hot + HK, hot + HL, bas[9], e);
+            R kl[6];
+            const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+            vstoreKL(mb.hot + si * HOT, kl);
+            if (st >= 0) mb.stick[si] = st;
+            if (rdalpha != ralpha) fevalC(bas, hot, dA, dB, ralpha, e);
+            fevalF(kl, kl + 3, bas[9], e);
+            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), isA ? e.jA : e.jB, kl, e.F);
+          } else {
+            fevalC(bas, hot, dA, dB, ralpha, e);
+            fevalF(hot + HK, hot + HL, bas[9], e);
+            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), isA ? e.jA : e.jB, hot + HK, e.F);
+          }
+          continue;
+        }
         R negb[9];
         for (int e = 0; e < 9; e++) negb[e] = -bas[e];
         const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
