@@ -269,7 +269,7 @@ __global__ void k_adj_velocity(int n, const BodyProp *props, AdjStep T, AdjBufs 
 // The dual update of iteration l (manifold.js:447-491), one thread per live
 // manifold: the contact adjoints LK turn from those of k, lambda after the
 // update into those before it.
-__global__ void k_adj_dual(AdjStep T, AdjBufs A, int l, StepParams P) {
+__global__ void __launch_bounds__(128) k_adj_dual(AdjStep T, AdjBufs A, int l, StepParams P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T.nLive) return;
   const int m = T.live[t];

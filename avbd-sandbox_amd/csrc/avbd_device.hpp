@@ -119,40 +119,75 @@ __device__ inline void supportPt(double *o, const Obb &b, const double *dir) {
   for (int k = 0; k < 3; k++) o[k] += b.ax[2][k] * (b.h[2] * sz);
 }
 
+// Axis i and half extent i of a box by selects, not by indexing, so that a
+// box stays in registers.
+__device__ __forceinline__ void axOf(const Obb &b, int i, double *o) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) o[k] = i == 0 ? b.ax[0][k] : (i == 1 ? b.ax[1][k] : b.ax[2][k]);
+}
+__device__ __forceinline__ double hOf(const Obb &b, int i) { return i == 0 ? b.h[0] : (i == 1 ? b.h[1] : b.h[2]); }
+
 __device__ inline void faceAx(const Obb &b, int ai, double *u, double *v, double &eu, double &ev) {
   const int iu = ai == 0 ? 1 : 0, iv = ai == 2 ? 1 : 2;
-  for (int k = 0; k < 3; k++) { u[k] = b.ax[iu][k]; v[k] = b.ax[iv][k]; }
-  eu = b.h[iu]; ev = b.h[iv];
+  axOf(b, iu, u);
+  axOf(b, iv, v);
+  eu = hOf(b, iu);
+  ev = hOf(b, iv);
+}
+
+__device__ __forceinline__ void pickObb(Obb &o, bool first, const Obb &a, const Obb &b) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    o.c[k] = first ? a.c[k] : b.c[k];
+    o.h[k] = first ? a.h[k] : b.h[k];
+#pragma unroll
+    for (int j = 0; j < 3; j++) o.ax[j][k] = first ? a.ax[j][k] : b.ax[j][k];
+  }
 }
 
 // Sutherland-Hodgman against the half-space n.x <= off (collide.js:202-247).
-__device__ inline int clipPoly(const double *in, int n, const double *pn, double off, double *out) {
+// Vertex i coordinate k lives at buf[(3 i + k) * STRIDE]. CAP vertices fit;
+// with SIGNAL set, needing more returns -1 (the caller redoes the manifold
+// with MAXPOLY room), otherwise extra vertices are dropped as the reference
+// drops them beyond MAXPOLY.
+template <int STRIDE, int CAP, bool SIGNAL>
+__device__ inline int clipPolyT(const double *in, int n, const double *pn, double off, double *out) {
   if (n <= 0) return 0;
   int cnt = 0;
-  double ax = in[(n - 1) * 3], ay = in[(n - 1) * 3 + 1], az = in[(n - 1) * 3 + 2];
+  double ax = in[((n - 1) * 3) * STRIDE], ay = in[((n - 1) * 3 + 1) * STRIDE], az = in[((n - 1) * 3 + 2) * STRIDE];
   double da = pn[0] * ax + pn[1] * ay + pn[2] * az - off;
   for (int i = 0; i < n; i++) {
-    const double bx = in[i * 3], by = in[i * 3 + 1], bz = in[i * 3 + 2];
+    const double bx = in[(i * 3) * STRIDE], by = in[(i * 3 + 1) * STRIDE], bz = in[(i * 3 + 2) * STRIDE];
     const double db = pn[0] * bx + pn[1] * by + pn[2] * bz - off;
     const bool aIn = da <= PLANE_EPS, bIn = db <= PLANE_EPS;
     if (aIn != bIn) {
       double t = 0;
       const double den = da - db;
       if (xabs(den) > SAT_EPS) t = clampj(da / den, 0.0, 1.0);
-      if (cnt < MAXPOLY) {
-        out[cnt * 3] = ax + (bx - ax) * t;
-        out[cnt * 3 + 1] = ay + (by - ay) * t;
-        out[cnt * 3 + 2] = az + (bz - az) * t;
+      if (cnt < CAP) {
+        out[(cnt * 3) * STRIDE] = ax + (bx - ax) * t;
+        out[(cnt * 3 + 1) * STRIDE] = ay + (by - ay) * t;
+        out[(cnt * 3 + 2) * STRIDE] = az + (bz - az) * t;
         cnt++;
+      } else if (SIGNAL) {
+        return -1;
       }
     }
-    if (bIn && cnt < MAXPOLY) {
-      out[cnt * 3] = bx; out[cnt * 3 + 1] = by; out[cnt * 3 + 2] = bz;
-      cnt++;
+    if (bIn) {
+      if (cnt < CAP) {
+        out[(cnt * 3) * STRIDE] = bx; out[(cnt * 3 + 1) * STRIDE] = by; out[(cnt * 3 + 2) * STRIDE] = bz;
+        cnt++;
+      } else if (SIGNAL) {
+        return -1;
+      }
     }
     ax = bx; ay = by; az = bz; da = db;
   }
   return cnt;
+}
+
+__device__ inline int clipPoly(const double *in, int n, const double *pn, double off, double *out) {
+  return clipPolyT<1, MAXPOLY, false>(in, n, pn, off, out);
 }
 
 struct Sink {
@@ -208,13 +243,18 @@ __device__ inline bool satAxis(const Obb &A, const Obb &B, const double *delta, 
 // collide.js:327-343
 __device__ inline void supportEdge(const Obb &b, int ai, const double *dir, double *e0, double *e1) {
   const int a1 = (ai + 1) % 3, a2 = (ai + 2) % 3;
-  const double s1 = dot3(dir, b.ax[a1]) >= 0 ? 1.0 : -1.0;
-  const double s2 = dot3(dir, b.ax[a2]) >= 0 ? 1.0 : -1.0;
+  double x0[3], x1[3], x2[3];
+  axOf(b, ai, x0);
+  axOf(b, a1, x1);
+  axOf(b, a2, x2);
+  const double h0 = hOf(b, ai), h1 = hOf(b, a1), h2 = hOf(b, a2);
+  const double s1 = dot3(dir, x1) >= 0 ? 1.0 : -1.0;
+  const double s2 = dot3(dir, x2) >= 0 ? 1.0 : -1.0;
   double t[3] = {b.c[0], b.c[1], b.c[2]};
-  for (int k = 0; k < 3; k++) t[k] += b.ax[a1][k] * (b.h[a1] * s1);
-  for (int k = 0; k < 3; k++) t[k] += b.ax[a2][k] * (b.h[a2] * s2);
-  for (int k = 0; k < 3; k++) { e0[k] = t[k]; e0[k] += b.ax[ai][k] * -b.h[ai]; }
-  for (int k = 0; k < 3; k++) { e1[k] = t[k]; e1[k] += b.ax[ai][k] * b.h[ai]; }
+  for (int k = 0; k < 3; k++) t[k] += x1[k] * (h1 * s1);
+  for (int k = 0; k < 3; k++) t[k] += x2[k] * (h2 * s2);
+  for (int k = 0; k < 3; k++) { e0[k] = t[k]; e0[k] += x0[k] * -h0; }
+  for (int k = 0; k < 3; k++) { e1[k] = t[k]; e1[k] += x0[k] * h0; }
 }
 
 // collide.js:345-393
@@ -250,51 +290,60 @@ __device__ inline void closestSeg(const double *p0, const double *p1, const doub
   c1[0] = q0[0] + d2x * t; c1[1] = q0[1] + d2y * t; c1[2] = q0[2] + d2z * t;
 }
 
-// collide.js:403-472
-__device__ inline int faceManifold(const Obb &oA, const Obb &oB, Sink &sink, bool refIsA, int refAxis,
-                                   const double *nAB) {
-  const Obb &ref = refIsA ? oA : oB;
-  const Obb &inc = refIsA ? oB : oA;
+// collide.js:403-472. The clip polygons c0, c1 hold vertex i coordinate k at
+// [(3 i + k) * STRIDE]; returns -1 if CAP vertices do not suffice (SIGNAL).
+template <int STRIDE, int CAP, bool SIGNAL>
+__device__ inline int faceManifoldT(const Obb &oA, const Obb &oB, Sink &sink, bool refIsA, int refAxis,
+                                    const double *nAB, double *c0, double *c1) {
+  Obb ref, inc;
+  pickObb(ref, refIsA, oA, oB);
+  pickObb(inc, !refIsA, oA, oB);
   double on[3];
   for (int k = 0; k < 3; k++) on[k] = refIsA ? nAB[k] : -nAB[k];
-  double fn[3], fc[3], fu[3], fv[3], eu, ev;
-  const double s = dot3(on, ref.ax[refAxis]) >= 0 ? 1.0 : -1.0;
-  for (int k = 0; k < 3; k++) fn[k] = ref.ax[refAxis][k] * s;
-  for (int k = 0; k < 3; k++) { fc[k] = ref.c[k]; fc[k] += fn[k] * ref.h[refAxis]; }
+  double fn[3], fc[3], fu[3], fv[3], eu, ev, rax[3];
+  axOf(ref, refAxis, rax);
+  const double s = dot3(on, rax) >= 0 ? 1.0 : -1.0;
+  for (int k = 0; k < 3; k++) fn[k] = rax[k] * s;
+  const double rh = hOf(ref, refAxis);
+  for (int k = 0; k < 3; k++) { fc[k] = ref.c[k]; fc[k] += fn[k] * rh; }
   faceAx(ref, refAxis, fu, fv, eu, ev);
   int incAxis = 0;
   double best = -__builtin_inf();
+#pragma unroll
   for (int i = 0; i < 3; i++) {
     const double d = xabs(dot3(inc.ax[i], fn));
     if (d > best) { best = d; incAxis = i; }
   }
-  double c0[MAXPOLY * 3], c1[MAXPOLY * 3];
   {
-    const double si = dot3(inc.ax[incAxis], fn) > 0 ? -1.0 : 1.0;
+    double iax[3];
+    axOf(inc, incAxis, iax);
+    const double si = dot3(iax, fn) > 0 ? -1.0 : 1.0;
     double nrm[3], ctr[3], u[3], v[3], iu, iv;
-    for (int k = 0; k < 3; k++) nrm[k] = inc.ax[incAxis][k] * si;
-    for (int k = 0; k < 3; k++) { ctr[k] = inc.c[k]; ctr[k] += nrm[k] * inc.h[incAxis]; }
+    for (int k = 0; k < 3; k++) nrm[k] = iax[k] * si;
+    const double ih = hOf(inc, incAxis);
+    for (int k = 0; k < 3; k++) { ctr[k] = inc.c[k]; ctr[k] += nrm[k] * ih; }
     faceAx(inc, incAxis, u, v, iu, iv);
+#pragma unroll
     for (int i = 0; i < 4; i++) {
       const double su = (i == 0 || i == 3) ? 1.0 : -1.0;
       const double sv = (i == 0 || i == 1) ? 1.0 : -1.0;
-      for (int k = 0; k < 3; k++) c0[i * 3 + k] = ctr[k] + u[k] * (iu * su) + v[k] * (iv * sv);
+      for (int k = 0; k < 3; k++) c0[(i * 3 + k) * STRIDE] = ctr[k] + u[k] * (iu * su) + v[k] * (iv * sv);
     }
   }
   int count = 4;
   double pn[3];
   for (int k = 0; k < 3; k++) pn[k] = fu[k];
-  count = clipPoly(c0, count, pn, dot3(pn, fc) + eu, c1);
-  if (!count) return 0;
+  count = clipPolyT<STRIDE, CAP, SIGNAL>(c0, count, pn, dot3(pn, fc) + eu, c1);
+  if (count <= 0) return count;
   for (int k = 0; k < 3; k++) pn[k] = -fu[k];
-  count = clipPoly(c1, count, pn, dot3(pn, fc) + eu, c0);
-  if (!count) return 0;
+  count = clipPolyT<STRIDE, CAP, SIGNAL>(c1, count, pn, dot3(pn, fc) + eu, c0);
+  if (count <= 0) return count;
   for (int k = 0; k < 3; k++) pn[k] = fv[k];
-  count = clipPoly(c0, count, pn, dot3(pn, fc) + ev, c1);
-  if (!count) return 0;
+  count = clipPolyT<STRIDE, CAP, SIGNAL>(c0, count, pn, dot3(pn, fc) + ev, c1);
+  if (count <= 0) return count;
   for (int k = 0; k < 3; k++) pn[k] = -fv[k];
-  count = clipPoly(c1, count, pn, dot3(pn, fc) + ev, c0);
-  if (!count) return 0;
+  count = clipPolyT<STRIDE, CAP, SIGNAL>(c1, count, pn, dot3(pn, fc) + ev, c0);
+  if (count <= 0) return count;
 
   sink.count = 0;
   uint32_t prefix = uint32_t(refIsA ? 0 : 1) << 24;
@@ -302,7 +351,7 @@ __device__ inline int faceManifold(const Obb &oA, const Obb &oB, Sink &sink, boo
   prefix |= uint32_t(incAxis & 0xff) << 8;
   for (int i = 0; i < count && sink.count < MAXC; i++) {
     double p[3], t[3];
-    for (int k = 0; k < 3; k++) p[k] = c0[i * 3 + k];
+    for (int k = 0; k < 3; k++) p[k] = c0[(i * 3 + k) * STRIDE];
     for (int k = 0; k < 3; k++) t[k] = p[k] - fc[k];
     const double dist = dot3(t, fn);
     if (dist > PLANE_EPS) continue;
@@ -317,6 +366,21 @@ __device__ inline int faceManifold(const Obb &oA, const Obb &oB, Sink &sink, boo
     sink.add(xA, xB, prefix);
   }
   return sink.count;
+}
+
+// Face manifold with the clip polygons in the caller's LDS column (POLY_LDS
+// vertices, lane-interleaved); a manifold that needs more is redone with
+// MAXPOLY room in private memory.
+constexpr int POLY_LDS = 8, LDS_LANES = 64;
+__device__ inline int faceManifold(const Obb &oA, const Obb &oB, Sink &sink, bool refIsA, int refAxis,
+                                   const double *nAB, double *lds) {
+  if (lds) {
+    const int r = faceManifoldT<LDS_LANES, POLY_LDS, true>(oA, oB, sink, refIsA, refAxis, nAB, lds,
+                                                           lds + POLY_LDS * 3 * LDS_LANES);
+    if (r >= 0) return r;
+  }
+  double c0[MAXPOLY * 3], c1[MAXPOLY * 3];
+  return faceManifoldT<1, MAXPOLY, false>(oA, oB, sink, refIsA, refAxis, nAB, c0, c1);
 }
 
 // collide.js:475-493
@@ -414,7 +478,8 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
 }
 
 // collide.js:612-685. Writes the basis only where the reference does.
-__device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *out, double *basis) {
+__device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *out, double *basis,
+                              double *lds = nullptr) {
   Sink sink;
   sink.A = &A; sink.B = &B; sink.out = out; sink.count = 0;
   const bool sA = A.shape == 1, sB = B.shape == 1;
@@ -426,11 +491,15 @@ __device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *
   double delta[3] = {oB.c[0] - oA.c[0], oB.c[1] - oA.c[1], oB.c[2] - oA.c[2]};
   Sat face{0, -1, -1, -__builtin_inf(), {0, 0, 0}, false};
   Sat edge = face;
+#pragma unroll
   for (int i = 0; i < 3; i++)
     if (!satAxis(oA, oB, delta, oA.ax[i], 0, i, -1, face)) return 0;
+#pragma unroll
   for (int i = 0; i < 3; i++)
     if (!satAxis(oA, oB, delta, oB.ax[i], 1, -1, i, face)) return 0;
+#pragma unroll
   for (int i = 0; i < 3; i++)
+#pragma unroll
     for (int j = 0; j < 3; j++) {
       double ax[3];
       cross3(ax, oA.ax[i], oB.ax[j]);
@@ -444,8 +513,8 @@ __device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *
   double t[3] = {-best.n[0], -best.n[1], -best.n[2]};
   orthonormal(basis, t);
   if (best.type == 2) return edgeContact(oA, oB, sink, best.ia, best.ib, best.n);
-  if (best.type == 0) return faceManifold(oA, oB, sink, true, best.ia, best.n);
-  return faceManifold(oA, oB, sink, false, best.ib, best.n);
+  if (best.type == 0) return faceManifold(oA, oB, sink, true, best.ia, best.n, lds);
+  return faceManifold(oA, oB, sink, false, best.ib, best.n, lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -340,9 +340,11 @@ __global__ void k_bp_large(BpArgs a, const int *largeList) {
 // Manifold.initialize (manifold.js:214-318) for every old manifold (t < nOld)
 // and every new pair, written to slot t of the other buffer.
 template <class R>
-__global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
+__global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
                          const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal,
                          int *srcSlot) {
+  // clip polygons in LDS, one lane-interleaved column per thread (64 threads)
+  __shared__ double poly[2 * POLY_LDS * 3 * LDS_LANES];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nOld + nNew) return;
   int a, b, m = -1, ncOld = 0;
@@ -370,7 +372,7 @@ __global__ void k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, M
   const double mu = xsqrt(pa.friction * pb.friction);
 
   NewContact nc[MAXC];
-  const int n = collide(A, B, nc, basis);
+  const int n = collide(A, B, nc, basis, poly + threadIdx.x);
   const bool compatible =
       !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
 
@@ -1153,7 +1155,7 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
 // Dual update of every live manifold (manifold.js:447-491); the live count
 // comes from device memory, threads stride over it.
 template <class R, bool EXACT>
-__global__ void k_dual(const int *nLivePtr, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
+__global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
                        int writeC) {
   const int nLive = *nLivePtr;
   const R ralpha = R(alpha), beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
