@@ -23,7 +23,8 @@ SYMBOLS = (
     "avbd_set_params", "avbd_pack", "avbd_step", "avbd_sync", "avbd_read_state", "avbd_write_state",
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
-    "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward",
+    "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
+    "avbd_upload_coloring",
 )
 K_PRIMAL, K_DUAL, K_NARROW = 0, 1, 2
 
@@ -34,7 +35,7 @@ pd, pi, pu = P(d), P(i32), P(u32)
 
 class Config(C.Structure):
     _fields_ = [("device", i32), ("precision", i32), ("team", i32), ("manifold_cap", i32),
-                ("pair_cap", i32), ("graphs", i32), ("reserved", i32 * 2)]
+                ("pair_cap", i32), ("graphs", i32), ("adjoint_small", i32), ("reserved", i32)]
 
 
 class Params(C.Structure):
@@ -44,7 +45,7 @@ class Params(C.Structure):
                 ("cached_contact_jacobians", i32), ("color_order", i32), ("reserved", i32)]
 
 
-ORDER_INDEX, ORDER_PRIORITY = 0, 1
+ORDER_INDEX, ORDER_PRIORITY, ORDER_CUSTOM = 0, 1, 2
 
 
 class Scene(C.Structure):
@@ -96,6 +97,8 @@ def lib(path=None):
         "avbd_autotune": (C.c_int, [ctx, pi, P(C.c_float)]),
         "avbd_adjoint_forward": (C.c_int, [ctx, i32, i32]),
         "avbd_adjoint_backward": (C.c_int, [ctx, pd, pd, pd]),
+        "avbd_set_body_order": (C.c_int, [ctx, pi, i32]),
+        "avbd_upload_coloring": (C.c_int, [ctx, pi, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -77,11 +77,12 @@ def scene_arrays(scene):
 class HipBackend:
     """One device context (stands in for GpuBackend; backend.js:243-335)."""
 
-    def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0, graphs=0):
+    def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0, graphs=0,
+                 adjoint_small=0):
         self.L = L.lib()
         self.ctx = C.c_void_p()
         cfg = L.Config(device=device, precision=precision, team=team, manifold_cap=manifold_cap,
-                       pair_cap=pair_cap, graphs=graphs)
+                       pair_cap=pair_cap, graphs=graphs, adjoint_small=adjoint_small)
         rc = self.L.avbd_create(C.byref(self.ctx), C.byref(cfg))
         if rc != 0:
             raise AvbdError(rc, "avbd_create failed (no GPU, or bad configuration)")
@@ -186,6 +187,24 @@ class HipBackend:
         nc = C.c_int32()
         self._ck(self.L.avbd_read_manifold(self.ctx, int(index), _dp(bm), _dp(con), C.byref(nc)))
         return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], contacts=con[:nc.value])
+
+    def upload_coloring(self, color_of):
+        """Use this colouring (colour per body) on the following steps, or the
+        greedy one again with None (GpuBackend.uploadColoring)."""
+        if color_of is None:
+            self._ck(self.L.avbd_upload_coloring(self.ctx, None, 0))
+            return
+        c = np.ascontiguousarray(color_of, dtype=np.int32)
+        self._ck(self.L.avbd_upload_coloring(self.ctx, _ip(c), len(c)))
+
+    def set_body_order(self, rank):
+        """Claim order of the greedy colouring (rank per body), or the
+        parameters' order again with None (Solver.bodyOrder's counterpart)."""
+        if rank is None:
+            self._ck(self.L.avbd_set_body_order(self.ctx, None, 0))
+            return
+        r = np.ascontiguousarray(rank, dtype=np.int32)
+        self._ck(self.L.avbd_set_body_order(self.ctx, _ip(r), len(r)))
 
     # --- metrics / profiling (backend.js:309-324, 1836-1891)
     def counters(self):

@@ -239,9 +239,8 @@ constexpr int SRC_KEPT = 0x40000000;
 
 // ---------------------------------------------------------- reverse kernels
 // BDF1 velocities (solver.js:476-488) at the step's end.
-__global__ void k_adj_velocity(int n, const BodyProp *props, AdjStep T, AdjBufs A, int iters, double dt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ inline void adjVelocity(int i, const BodyProp *props, const AdjStep &T, const AdjBufs &A, int iters,
+                                   double dt) {
   double *vb = A.Vb + size_t(i) * VS, *vs = A.VbS + size_t(i) * VS;
   for (int k = 0; k < 12; k++) vs[k] = 0;
   if (!(props[i].mass > 0)) {
@@ -269,9 +268,7 @@ __global__ void k_adj_velocity(int n, const BodyProp *props, AdjStep T, AdjBufs 
 // The dual update of iteration l (manifold.js:447-491), one thread per live
 // manifold: the contact adjoints LK turn from those of k, lambda after the
 // update into those before it.
-__global__ void __launch_bounds__(128) k_adj_dual(AdjStep T, AdjBufs A, int l, StepParams P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T.nLive) return;
+__device__ inline void adjDual(int t, const AdjStep &T, const AdjBufs &A, int l, const StepParams &P) {
   const int m = T.live[t];
   const int4 h = T.head[m];
   const double *B = T.basis + size_t(m) * BAS;
@@ -322,15 +319,13 @@ __global__ void __launch_bounds__(128) k_adj_dual(AdjStep T, AdjBufs A, int l, S
 // its update into that before it; the neighbours, the contacts and the step's
 // start and inertial target receive their shares.
 template <int TEAM>
-__global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A, const BodyProp *props, int l,
-                                                    StepParams P) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
-  const int c0 = T.cstart[c], cnt = T.cstart[c + 1] - c0;
+__device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T, const AdjBufs &A,
+                                     const BodyProp *props, int l, const StepParams &P) {
+  const int c0 = T.cstart[c];
   const double dt2 = P.dt * P.dt;
   const double *Xbef = T.X + T.xStride * l, *Xaft = T.X + T.xStride * (l + 1);
   const double *hotL = T.hot + T.hotStride * l;
-  for (int team = gt / TEAM; team < cnt; team += teams) {
+  {
     const int4 w4 = T.cwork[c0 + team];
     const int i = w4.x, b0 = w4.y, b1 = w4.z;
     const BodyProp &bp = props[i];
@@ -443,7 +438,7 @@ __global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A,
     }
     if constexpr (TEAM > 1)
       for (int k = 0; k < 6; k++) dqSelf[k] = teamSum<TEAM>(dqSelf[k]);
-    if (lane != 0) continue;
+    if (lane != 0) return;
     // 4. the body's own terms: inertia, its delta, the update
     double pb[3];
     double *x0b = A.X0b + size_t(i) * XS, *yb = A.Yb + size_t(i) * XS;
@@ -476,9 +471,8 @@ __global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A,
 
 // Force initialisation (manifold.js:214-318): C0 from the step-start poses,
 // the world anchors, and the warm start from the previous step's contacts.
-__global__ void k_adj_narrow(AdjStep T, AdjBufs A, const BodyProp *props, StepParams P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T.nLive) return;
+__device__ inline void adjNarrow(int t, const AdjStep &T, const AdjBufs &A, const BodyProp *props,
+                                 const StepParams &P) {
   const int m = T.live[t];
   const int4 h = T.head[m];
   const double *B = T.basis + size_t(m) * BAS;
@@ -532,9 +526,8 @@ __global__ void k_adj_narrow(AdjStep T, AdjBufs A, const BodyProp *props, StepPa
 
 // Body initialisation (solver.js:366-401): the adjoint of the step-start
 // state, written as the pose and velocity adjoints of the previous step's end.
-__global__ void k_adj_init(int n, const BodyProp *props, AdjStep T, AdjBufs A, StepParams P) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ inline void adjInit(int i, const BodyProp *props, const AdjStep &T, const AdjBufs &A,
+                               const StepParams &P) {
   const double dt = P.dt;
   const double *x0 = T.X0 + size_t(i) * XS, *v = T.V + size_t(i) * VS;
   double *xb = A.Xb + size_t(i) * XS, *x0b = A.X0b + size_t(i) * XS, *yb = A.Yb + size_t(i) * XS;
@@ -567,6 +560,75 @@ __global__ void k_adj_init(int n, const BodyProp *props, AdjStep T, AdjBufs A, S
   double *vo = A.Vb + size_t(i) * VS;
   for (int k = 0; k < 3; k++) { vo[k] = vb[k]; vo[3 + k] = wb[k]; vo[6 + k] = prevb[k]; }
   for (int k = 0; k < 8; k++) { x0b[k] = 0; yb[k] = 0; }
+}
+
+// ------------------------------------------------------------- the kernels
+__global__ void k_adj_velocity(int n, const BodyProp *props, AdjStep T, AdjBufs A, int iters, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) adjVelocity(i, props, T, A, iters, dt);
+}
+__global__ void __launch_bounds__(128) k_adj_dual(AdjStep T, AdjBufs A, int l, StepParams P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T.nLive) adjDual(t, T, A, l, P);
+}
+template <int TEAM>
+__global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A, const BodyProp *props, int l,
+                                                    StepParams P) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
+  const int cnt = T.cstart[c + 1] - T.cstart[c];
+  for (int team = gt / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
+}
+__global__ void k_adj_narrow(AdjStep T, AdjBufs A, const BodyProp *props, StepParams P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T.nLive) adjNarrow(t, T, A, props, P);
+}
+__global__ void k_adj_init(int n, const BodyProp *props, AdjStep T, AdjBufs A, StepParams P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) adjInit(i, props, T, A, P);
+}
+
+// The whole reverse of one step in a single 512-thread workgroup, for small
+// scenes where the per-kernel launches would dominate: the same phases in the
+// same order, separated by workgroup barriers (one CU, so global writes of one
+// phase are visible to the next), then the per-step contact adjoints cleared.
+template <int TEAM>
+__global__ void __launch_bounds__(512) k_adj_step_small(int n, int nTot, const BodyProp *props, AdjStep T,
+                                                         AdjBufs A, int iters, StepParams P) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < n; i += nt) adjVelocity(i, props, T, A, iters, P.dt);
+  __syncthreads();
+  const int lane = tid % TEAM, teams = nt / TEAM;
+  for (int l = iters - 1; l >= 0; l--) {
+    for (int t = tid; t < T.nLive; t += nt) adjDual(t, T, A, l, P);
+    __syncthreads();
+    for (int c = T.numColors - 1; c >= 0; c--) {
+      const int cnt = T.cstart[c + 1] - T.cstart[c];
+      for (int team = tid / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
+      __syncthreads();
+    }
+  }
+  for (int t = tid; t < T.nLive; t += nt) adjNarrow(t, T, A, props, P);
+  __syncthreads();
+  for (int i = tid; i < n; i += nt) adjInit(i, props, T, A, P);
+  const size_t slots = size_t(nTot) * MAXC;
+  for (size_t k = tid; k < slots * 6; k += nt) { A.LK[k] = 0; A.RWb[k] = 0; }
+  for (size_t k = tid; k < slots * 3; k += nt) A.C0b[k] = 0;
+  for (int k = tid; k < nTot; k += nt) A.MUb[k] = 0;
+}
+
+// Two device copies in one launch (a recording snapshot of poses and hot
+// records).
+__global__ void k_snapshot(const double *a, size_t na, double *da, const double *b, size_t nb, double *db) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < na; k += stride) da[k] = a[k];
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nb; k += stride) db[k] = b[k];
+}
+
+// The recorded work items, as many as the step has (read on the device).
+__global__ void k_copy_items(const int *count, const uint2 *src, uint2 *dst, int cap) {
+  const int m = min(*count, cap);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) dst[k] = src[k];
 }
 
 }  // namespace avbd

@@ -140,6 +140,10 @@ struct avbd_ctx {
   DBuf<int4> cwork;
   std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
+  // verification hooks: a caller's claim order (rank per body) and a caller's
+  // colouring for the following steps
+  bool useRank = false, useUpload = false;
+  DBuf<int> claimRank, uploadColor;
   int nDynamic = 0;
 
   // solve-loop graphs, keyed by colour count and manifold buffer parity
@@ -187,12 +191,11 @@ struct avbd_ctx {
   };
   std::vector<TapeStep> tape;
   TapeStep *rec = nullptr;  // the step being recorded, if any
-  bool fuseDual = true;     // fold the dual update into the next sweep
   DBuf<int> srcSlot;        // per new contact: matched old slot | kept-anchor bit, or -1
   int lastNTot = 0;
   std::vector<Checkpoint> ckpts;
   Checkpoint endState;
-  int adjSteps = 0, adjEvery = 0;
+  int adjSteps = 0, adjEvery = 0, adjSmallMax = 4096;
   DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
@@ -292,6 +295,7 @@ struct avbd_ctx {
     color[0].release(); color[1].release(); cstart.release(); cbodies.release(); cwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
+    claimRank.release(); uploadColor.release();
     for (auto &t : tape) t.release();
     tape.clear();
     for (auto &c : ckpts) c.release();
@@ -390,13 +394,23 @@ struct avbd_ctx {
   void colorize() {
     HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
     const int maxRounds = N + 2;
+    const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
+    bool uploaded = useUpload;
     for (int launched = 0;;) {
-      for (int r = 0; r < colorBatch; r++) {
-        hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
-                           adjOther.p, color[0].p, cmeta.p, prm.color_order, sceneStart.p, scal.p + 5);
-        hipLaunchKernelGGL(k_color_check, dim3(1), dim3(64), 0, stream, cmeta.p);
+      if (uploaded) {
+        // the uploaded colouring, if it is proper for this step's forces
+        hipLaunchKernelGGL(k_color_install, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, uploadColor.p,
+                           color[0].p, cmeta.p);
+        hipLaunchKernelGGL(k_color_validate, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
+                           adjOther.p, color[0].p, cmeta.p + 5, scal.p + 5);
+      } else {
+        for (int r = 0; r < colorBatch; r++) {
+          hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
+                             adjOther.p, color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
+          hipLaunchKernelGGL(k_color_check, dim3(1), dim3(64), 0, stream, cmeta.p);
+        }
+        launched += colorBatch;
       }
-      launched += colorBatch;
       HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
       const int nb = blocks(N);
       hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
@@ -408,11 +422,17 @@ struct avbd_ctx {
                          items.p);
       HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
+      if (uploaded && hmeta[5]) {
+        // not proper for this step (reported in the overflow bits): colour greedily
+        uploaded = false;
+        HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 4, stream));
+        continue;
+      }
       if (hmeta[1]) break;
       if (launched > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
     }
     colorRounds = hmeta[3];
-    colorBatch = std::min(64, std::max(4, colorRounds + 2));
+    if (!uploaded) colorBatch = std::min(64, std::max(4, colorRounds + 2));
     numColors = hmeta[4];
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
@@ -457,7 +477,7 @@ struct avbd_ctx {
     if (withInit)
       hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
                          (R *)DQ.p, P);
-    if (rec) recordSnapshot(0);
+    if (rec) recordSnapshot(0, 0);
     // The contact dual update of iteration it runs inside the primal sweep of
     // iteration it + 1 (first visitors apply it); after the last iteration it
     // runs as its own pass, which also records C for the next step.
@@ -465,18 +485,22 @@ struct avbd_ctx {
     auto alphaOf = [&](int it) { return prm.post_stabilize ? (it < prm.iterations ? 1.0 : 0.0) : prm.alpha; };
     for (int it = 0; it < total; it++) {
       const double alpha = alphaOf(it);
-      const int fused = fuseDual && it >= 1 && it < prm.iterations ? 1 : 0;
+      const int fused = it >= 1 && it < prm.iterations ? 1 : 0;
       const double dualAlpha = fused ? alphaOf(it - 1) : alpha;
       for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team, fused, dualAlpha);
+      // recording: the poses after this sweep, and the contacts' k, lambda as
+      // this sweep used them (the folded update of the previous iteration has
+      // reached every contact once this sweep is done)
+      if (rec && it < prm.iterations) recordSnapshot(it + 1, fused ? it : -1);
       if (it < prm.iterations) {
         const bool last = it == prm.iterations - 1;
-        if ((last || !fuseDual) && (graphCapturing || nLive > 0))
+        if (last && (graphCapturing || nLive > 0))
           timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
                       (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha,
                       last ? 1 : 0);
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
-      if (rec && it < prm.iterations) recordSnapshot(it + 1);
+      if (rec && it == prm.iterations - 1) recordSnapshot(-1, it + 1);
       if (it == prm.iterations - 1)
         hipLaunchKernelGGL(k_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, V.p, prm.dt);
     }
@@ -695,31 +719,34 @@ struct avbd_ctx {
     T.src.copyFrom(srcSlot.p, size_t(nTot) * MAXC, stream);
     T.live.copyFrom(live[cur].p, size_t(nLive), stream);
   }
-  int tapeItems = 0;
   void recordTopology() {
     TapeStep &T = *rec;
     T.numColors = numColors;
     T.color.copyFrom(color[0].p, size_t(N), stream);
     T.cstart.copyFrom(cmeta.p + 8, size_t(numColors + 1), stream);
     T.cwork.copyFrom(cwork.p, size_t(N), stream);
-    HIPCHECK(hipMemcpyAsync(&tapeItems, itemStart.p + N, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    T.items.copyFrom(items.p, size_t(tapeItems), stream);
+    // as many work items as the step has, counted on the device (no readback)
+    T.items.reserve(items.n);
+    hipLaunchKernelGGL(k_copy_items, dim3(std::max(1, std::min(blocks(long(items.n)), 1024))), dim3(256), 0, stream,
+                       (const int *)(itemStart.p + N), (const uint2 *)items.p, T.items.p, int(items.n));
     T.V.copyFrom(V.p, size_t(N) * VS, stream);
     T.X.reserve(size_t(prm.iterations + 1) * N * XS);
     T.hot.reserve(size_t(prm.iterations + 1) * std::max(T.nTot, 1) * MAXC * HOT);
   }
-  void recordSnapshot(int l) {
+  // Snapshot l of the poses (X^l: before sweep l, l = 0 after body
+  // initialisation) and snapshot h of the hot records (the k, lambda sweep h
+  // uses; h = I after the last dual update); -1 skips one.
+  void recordSnapshot(int l, int h) {
     TapeStep &T = *rec;
     if (l == 0) {
       T.X0.copyFrom(X0.p, size_t(N) * XS, stream);
       T.Y.copyFrom(Y.p, size_t(N) * XS, stream);
     }
-    HIPCHECK(hipMemcpyAsync(T.X.p + size_t(l) * N * XS, X.p, sizeof(double) * N * XS, hipMemcpyDeviceToDevice,
-                            stream));
-    if (T.nTot)
-      HIPCHECK(hipMemcpyAsync(T.hot.p + size_t(l) * T.nTot * MAXC * HOT, store[cur].hot.p,
-                              sizeof(double) * T.nTot * MAXC * HOT, hipMemcpyDeviceToDevice, stream));
+    const size_t nx = size_t(N) * XS, nh = size_t(T.nTot) * MAXC * HOT;
+    hipLaunchKernelGGL(k_snapshot, dim3(std::max(1, std::min(blocks(long(std::max(nx, nh))), 1024))), dim3(256), 0,
+                       stream, (const double *)X.p, l >= 0 ? nx : size_t(0), l >= 0 ? T.X.p + size_t(l) * nx : nullptr,
+                       (const double *)store[cur].hot.p, h >= 0 ? nh : size_t(0),
+                       h >= 0 ? T.hot.p + size_t(h) * nh : nullptr);
   }
 
   // Everything one step hands to the next: poses, velocities, the current
@@ -820,14 +847,26 @@ struct avbd_ctx {
 
   // The reverse of one recorded step; lk selects which contact-adjoint buffer
   // holds this step's slots.
-  void reverseStep(const TapeStep &T, int nItems, int &lk) {
+  void reverseStep(const TapeStep &T, int &lk) {
     const StepParams P = sp();
     const int I = prm.iterations;
     const AdjStep S = adjView(T);
     const AdjBufs A = adjBufs(lk);
+    const bool wide = manifoldsPerBody() > 3;  // many contacts per body: wider teams
+    if (nDynamic <= adjSmallMax) {
+      // a small scene: the whole reverse step in one workgroup
+      if (wide)
+        hipLaunchKernelGGL(k_adj_step_small<16>, dim3(1), dim3(512), 0, stream, N, T.nTot,
+                           (const BodyProp *)props.p, S, A, I, P);
+      else
+        hipLaunchKernelGGL(k_adj_step_small<4>, dim3(1), dim3(512), 0, stream, N, T.nTot,
+                           (const BodyProp *)props.p, S, A, I, P);
+      HIPCHECK(hipGetLastError());
+      lk = 1 - lk;
+      return;
+    }
     hipLaunchKernelGGL(k_adj_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, I,
                        prm.dt);
-    const bool wide = nItems > 12 * std::max(nDynamic, 1);
     const int gb = std::max(1, std::min(blocks(long(std::max(nDynamic, 1)) * (wide ? 16 : 4)), 2048));
     for (int l = I - 1; l >= 0; l--) {
       if (T.nLive > 0)
@@ -856,6 +895,8 @@ struct avbd_ctx {
     lk = 1 - lk;
   }
 
+  double manifoldsPerBody() const { return double(std::max(lastNTot, 0)) / double(std::max(nDynamic, 1)); }
+
   void adjointBackward(const double *dLdPos, double *dLdMu, double *dLdV0) {
     if (adjSteps <= 0) throw ApiError(AVBD_E_STATE, "no adjoint forward recorded");
     const size_t slots = size_t(manCap) * MAXC;
@@ -872,11 +913,9 @@ struct avbd_ctx {
     }
     struct Reset {
       avbd_ctx *c;
-      ~Reset() { c->rec = nullptr; c->fuseDual = true; }
+      ~Reset() { c->rec = nullptr; }
     } reset{this};
-    fuseDual = false;
     if (tape.size() < size_t(adjEvery)) tape.resize(size_t(adjEvery));
-    std::vector<int> nItems(static_cast<size_t>(adjEvery));
     int lk = 0;
     const int nseg = int(ckpts.size());
     for (int seg = nseg - 1; seg >= 0; seg--) {
@@ -885,10 +924,9 @@ struct avbd_ctx {
       for (int k = 0; k < len; k++) {
         rec = &tape[size_t(k)];
         step();
-        nItems[size_t(k)] = tapeItems;
       }
       rec = nullptr;
-      for (int k = len - 1; k >= 0; k--) reverseStep(tape[size_t(k)], nItems[size_t(k)], lk);
+      for (int k = len - 1; k >= 0; k--) reverseStep(tape[size_t(k)], lk);
     }
     restoreCheckpoint(endState);
     std::vector<double> mu(static_cast<size_t>(N)), vb(static_cast<size_t>(N) * VS);
@@ -1097,6 +1135,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.itemJoints.reserve(N);
   c.chist.reserve(size_t(blocks(N)) * MAX_COLORS); c.coffs.reserve(size_t(blocks(N)) * MAX_COLORS);
   c.ctotal.reserve(MAX_COLORS);
+  c.claimRank.reserve(N); c.uploadColor.reserve(N);
+  c.useRank = c.useUpload = false;
   c.items.reserve(2 * (size_t(MAXC) * c.manCap + c.J) + 1);
   c.scal.reserve(16);
   c.cbytes.reserve(2 * MAX_COLORS);
@@ -1148,6 +1188,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->team = cfg->team > 0 ? cfg->team : 16;
     c->manCapCfg = cfg->manifold_cap;
     c->graphsCfg = cfg->graphs;
+    c->adjSmallMax = cfg->adjoint_small == 0 ? 4096 : cfg->adjoint_small;
     c->pairCapCfg = cfg->pair_cap;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {
@@ -1427,6 +1468,39 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
     ctx->step();
     if (team) *team = ctx->team;
     if (ms) for (int k = 0; k < 6; k++) ms[k] = ctx->tuneMs[k];
+  });
+}
+
+int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (!rank) {
+      ctx->useRank = false;
+      return;
+    }
+    if (n != ctx->N) throw ApiError(AVBD_E_INVALID, "rank must hold one entry per body");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipMemcpyAsync(ctx->claimRank.p, rank, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->useRank = true;
+  });
+}
+
+int avbd_upload_coloring(avbd_ctx *ctx, const int32_t *color_of, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (!color_of) {
+      ctx->useUpload = false;
+      return;
+    }
+    if (n != ctx->N) throw ApiError(AVBD_E_INVALID, "color_of must hold one entry per body");
+    for (int i = 0; i < n; i++)
+      if (ctx->hprops[size_t(i)].mass > 0 && (color_of[i] < 0 || color_of[i] >= MAX_COLORS))
+        throw ApiError(AVBD_E_INVALID, "a dynamic body's colour is outside [0, 256)");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipMemcpyAsync(ctx->uploadColor.p, color_of, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->useUpload = true;
   });
 }
 

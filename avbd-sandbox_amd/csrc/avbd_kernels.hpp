@@ -696,8 +696,9 @@ __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
 
 // a and b belong to one scene whose first body is `base`; the priority hashes
 // the index within the scene, so a scene colours the same alone or batched.
-__device__ __forceinline__ bool claimsBefore(int a, int b, int order, int base) {
+__device__ __forceinline__ bool claimsBefore(int a, int b, int order, int base, const int *rank) {
   if (order == 0) return a < b;
+  if (order == 2) return rank[a] < rank[b] || (rank[a] == rank[b] && a < b);
   const uint32_t pa = lowbias32(uint32_t(a - base)), pb = lowbias32(uint32_t(b - base));
   return pa < pb || (pa == pb && a < b);
 }
@@ -709,7 +710,7 @@ __device__ __forceinline__ bool claimsBefore(int a, int b, int order, int base) 
 // set once a round changed nothing (later rounds return at once), ctl[3]
 // receives the number of rounds.
 __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart, const int *adjOther, int *color,
-                              int *ctl, int order, const int *sceneStart, int *overflow) {
+                              int *ctl, int order, const int *sceneStart, int *overflow, const int *rank) {
   if (ctl[1]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !(props[i].mass > 0)) return;
@@ -718,7 +719,7 @@ __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart,
   unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
   for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
     const int j = adjOther[e];
-    if (j < 0 || !claimsBefore(j, i, order, base)) continue;
+    if (j < 0 || !claimsBefore(j, i, order, base, rank)) continue;
     const int c = vc[j];
     if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
   }
@@ -733,6 +734,34 @@ __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart,
 }
 
 // After each round: stop once a round changed nothing.
+// An uploaded colouring, checked each step: every dynamic body has a colour
+// in range (else flag 1) and no force joins two bodies of one colour (else
+// flag 2), which the colour-parallel sweep requires.
+__global__ void k_color_validate(int n, const BodyProp *props, const int *adjStart, const int *adjOther,
+                                 const int *color, int *flag, int *overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(props[i].mass > 0)) return;
+  const int c = color[i];
+  int bad = (c < 0 || c >= MAX_COLORS) ? 1 : 0;
+  for (int e = adjStart[i]; !bad && e < adjStart[i + 1]; e++) {
+    const int j = adjOther[e];
+    if (j >= 0 && props[j].mass > 0 && color[j] == c) bad = 2;
+  }
+  if (bad) {
+    atomicOr(flag, bad);
+    atomicOr(overflow, 16);  // the step falls back to the greedy colouring
+  }
+}
+
+// An uploaded colouring in place of the relaxation: the colours of the
+// dynamic bodies, -1 for the static ones; marks the colouring converged.
+__global__ void k_color_install(int n, const BodyProp *props, const int *src, int *color, int *ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) ctl[1] = 1;
+  if (i >= n) return;
+  color[i] = props[i].mass > 0 ? src[i] : -1;
+}
+
 __global__ void k_color_check(int *ctl) {
   if (threadIdx.x || blockIdx.x || ctl[1]) return;
   ctl[3] += 1;

@@ -271,6 +271,38 @@ static napi_value js_read_coloring(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* uploadColoring(ctx, colorOf Int32Array | null): the caller's colouring for
+ * the following steps, or none (GpuBackend.uploadColoring, backend.js:1915-1931) */
+static napi_value js_upload_coloring(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  napi_valuetype t = napi_null;
+  if (argc > 1) CHECK(env, napi_typeof(env, argv[1], &t));
+  size_t n = 0;
+  int32_t *col = (argc > 1 && t != napi_null && t != napi_undefined) ? (int32_t *)typed_data(env, argv[1], &n) : NULL;
+  int rc = avbd_upload_coloring(ctx, col, (int32_t)n);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
+/* setBodyOrder(ctx, rank Int32Array | null): the colouring's claim order
+ * (the counterpart of Solver.bodyOrder, solver.js:225-233) */
+static napi_value js_set_body_order(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  napi_valuetype t = napi_null;
+  if (argc > 1) CHECK(env, napi_typeof(env, argv[1], &t));
+  size_t n = 0;
+  int32_t *rank = (argc > 1 && t != napi_null && t != napi_undefined) ? (int32_t *)typed_data(env, argv[1], &n) : NULL;
+  int rc = avbd_set_body_order(ctx, rank, (int32_t)n);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 /* readPairs(ctx, Int32Array|null) -> count */
 static napi_value js_read_pairs(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -362,6 +394,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"writeState", NULL, js_write_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readColoring", NULL, js_read_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"uploadColoring", NULL, js_upload_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setBodyOrder", NULL, js_set_body_order, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readPairs", NULL, js_read_pairs, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readCounters", NULL, js_read_counters, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setProfiling", NULL, js_set_profiling, NULL, NULL, NULL, napi_enumerable, NULL},

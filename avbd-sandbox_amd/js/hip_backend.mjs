@@ -154,6 +154,27 @@ export class HipBackend {
     return { colorOf, numColors, contactPairs };
   }
 
+  // Verification hook (backend.js:1915-1931): install a colouring given as
+  // body indices grouped by colour (entries, offsets[c]..offsets[c+1]) for
+  // the following steps; clearColoring() returns to the greedy colouring.
+  uploadColoring(entries, offsets, counts) {
+    const colorOf = new Int32Array(this.numBodies).fill(-1);
+    for (let c = 0; c + 1 < offsets.length; c++)
+      for (let k = offsets[c]; k < offsets[c] + (counts ? counts[c] : offsets[c + 1] - offsets[c]); k++)
+        colorOf[entries[k]] = c;
+    addon.uploadColoring(this.ctx, colorOf);
+  }
+
+  clearColoring() {
+    addon.uploadColoring(this.ctx, null);
+  }
+
+  // The colouring's claim order (the device counterpart of Solver.bodyOrder,
+  // solver.js:225-233): rank[i] is body i's position; null for the default.
+  setBodyOrder(rank) {
+    addon.setBodyOrder(this.ctx, rank ? Int32Array.from(rank) : null);
+  }
+
   // --- metrics and profiling (backend.js:309-324, 1836-1891)
   get lastStats() {
     return this.ctx ? addon.readCounters(this.ctx) : null;

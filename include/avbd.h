@@ -65,7 +65,10 @@ typedef struct avbd_config {
   int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body)        */
   int32_t graphs;          /* solve loop as a replayed hipGraph: 0 auto (on
                               unless profiling), -1 never                   */
-  int32_t reserved[2];
+  int32_t adjoint_small;   /* largest dynamic-body count whose adjoint step
+                              is reversed in one workgroup (0: 4096; -1:
+                              never, always one kernel per phase)           */
+  int32_t reserved;
 } avbd_config;
 
 /* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform
@@ -97,6 +100,7 @@ typedef struct avbd_params {
 
 #define AVBD_ORDER_INDEX 0
 #define AVBD_ORDER_PRIORITY 1
+#define AVBD_ORDER_CUSTOM   2   /* a caller's order (avbd_set_body_order) */
 
 /* Fills *p with the reference defaults (solver.js:92-236). */
 void avbd_default_params(avbd_params *p);
@@ -138,7 +142,9 @@ typedef struct avbd_counters {
   int32_t joints;          /* live joints                                   */
   int32_t colors;          /* colours of the last step                      */
   int32_t color_rounds;    /* fixed-point rounds the colouring took         */
-  int32_t overflow;        /* bit 1 pairs, 2 manifolds (layout.js:174)      */
+  int32_t overflow;        /* bit 1 pairs, 2 manifolds (layout.js:174), 4
+                              colours, 8 work items, 16 an uploaded
+                              colouring was not proper for a step           */
 } avbd_counters;
 
 /* --- lifecycle: GpuBackend.init / initDevice / initPipelines (backend.js:337-469) */
@@ -216,6 +222,24 @@ int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int6
  * sweep times in that order. Not available in AVBD_PREC_EXACT, whose
  * assembly order is fixed. */
 int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
+
+/* Verification hook: the claim order of the greedy colouring, and so the
+ * sweep's schedule (the reference CPU engine's Solver.bodyOrder,
+ * solver.js:225-233 and 419-422, reproduces a colour schedule from the
+ * flattened colouring; here the colouring is built in the given order).
+ * rank[i] is body i's position (lower claims first, ties to the lower index);
+ * NULL returns to avbd_params.color_order. Kept until changed or repacked. */
+int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n);
+
+/* Verification hook (GpuBackend.uploadColoring / overrideColoring,
+ * backend.js:1915-1931, 299, 1508-1516): use the caller's colouring (colour of
+ * each body; ignored for static bodies) instead of the greedy one on the
+ * following steps, until called with NULL or repacked. Each step checks it
+ * against that step's forces; a step where two bodies that share a force have
+ * one colour colours greedily instead and sets bit 16 of
+ * avbd_counters.overflow. AVBD_E_INVALID for a dynamic body's colour outside
+ * [0, 256). */
+int avbd_upload_coloring(avbd_ctx *ctx, const int32_t *color_of, int32_t n);
 
 /* Adjoint of the step (no counterpart in the reference; the build plan's
  * adjoint boundary). avbd_adjoint_forward steps the scene nsteps times, as
