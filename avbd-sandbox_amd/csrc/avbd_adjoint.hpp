@@ -28,7 +28,8 @@ namespace avbd {
 __device__ inline void qsubVjp(const double *a, const double *b, const double *ob, double *ab, double *bb) {
   const double lsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + b[3] * b[3];
   const double c[4] = {-b[0], -b[1], -b[2], b[3]};
-  const double ix = c[0] / lsq, iy = c[1] / lsq, iz = c[2] / lsq, iw = c[3] / lsq;
+  const double il = frcp(lsq);  // the adjoint needs no division-exact rounding
+  const double ix = c[0] * il, iy = c[1] * il, iz = c[2] * il, iw = c[3] * il;
   const double o0 = 2 * ob[0], o1 = 2 * ob[1], o2 = 2 * ob[2];
   ab[0] += o0 * iw - o1 * iz + o2 * iy;
   ab[1] += o0 * iz + o1 * iw - o2 * ix;
@@ -39,7 +40,8 @@ __device__ inline void qsubVjp(const double *a, const double *b, const double *o
                         o0 * ax + o1 * ay + o2 * az};
   const double s[4] = {-1, -1, -1, 1};
   const double ci = c[0] * ib[0] + c[1] * ib[1] + c[2] * ib[2] + c[3] * ib[3];
-  for (int j = 0; j < 4; j++) bb[j] += s[j] * ib[j] / lsq - 2 * b[j] * ci / (lsq * lsq);
+  const double il2 = il * il;
+  for (int j = 0; j < 4; j++) bb[j] += s[j] * ib[j] * il - 2 * b[j] * ci * il2;
 }
 
 // qint (Eq. 21): o = r / |r|, r = q + 1/2 (w, 0) (x) q.
@@ -49,10 +51,11 @@ __device__ inline void qintVjp(const double *q, const double *w, const double *o
   const double r[4] = {qx + 0.5 * (wx * qw + wy * qz - wz * qy), qy + 0.5 * (-wx * qz + wy * qw + wz * qx),
                        qz + 0.5 * (wx * qy - wy * qx + wz * qw), qw + 0.5 * (-wx * qx - wy * qy - wz * qz)};
   const double len = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
-  const double o[4] = {r[0] / len, r[1] / len, r[2] / len, r[3] / len};
+  const double il = frcp(len);
+  const double o[4] = {r[0] * il, r[1] * il, r[2] * il, r[3] * il};
   const double od = o[0] * ob[0] + o[1] * ob[1] + o[2] * ob[2] + o[3] * ob[3];
   double rb[4];
-  for (int k = 0; k < 4; k++) rb[k] = (ob[k] - o[k] * od) / len;
+  for (int k = 0; k < 4; k++) rb[k] = (ob[k] - o[k] * od) * il;
   wb[0] += 0.5 * (rb[0] * qw - rb[1] * qz + rb[2] * qy - rb[3] * qx);
   wb[1] += 0.5 * (rb[0] * qz + rb[1] * qw - rb[2] * qx - rb[3] * qy);
   wb[2] += 0.5 * (-rb[0] * qy + rb[1] * qx + rb[2] * qw - rb[3] * qz);
@@ -118,11 +121,10 @@ __device__ inline void bodyDelta(const double *x, const double *x0, double *dq) 
   qsub(dq + 3, x + 3, x0 + 3);
 }
 
-__device__ inline void cEval(const double *B, double mu, const double *hot, const double *k, const double *lam,
-                             const double *xA, const double *x0A, const double *xB, const double *x0B, double alpha,
-                             CEval &e) {
-  bodyDelta(xA, x0A, e.dqA);
-  bodyDelta(xB, x0B, e.dqB);
+// With the body deltas given (dqA, dqB: 6 each).
+__device__ inline void cEvalDq(const double *B, double mu, const double *hot, const double *k, const double *lam,
+                               const double *dqA, const double *dqB, double alpha, CEval &e) {
+  for (int j = 0; j < 6; j++) { e.dqA[j] = dqA[j]; e.dqB[j] = dqB[j]; }
   for (int r = 0; r < 3; r++) {
     const double nb[3] = {-B[r * 3], -B[r * 3 + 1], -B[r * 3 + 2]};
     cross3(e.jA + r * 3, hot + HRA, B + r * 3);
@@ -140,20 +142,31 @@ __device__ inline void cEval(const double *B, double mu, const double *hot, cons
   e.bounds = xabs(e.F[0]) * mu;
   e.fs = sqrt(e.z[1] * e.z[1] + e.z[2] * e.z[2]);
   e.clamped = e.fs > e.bounds && e.fs > 0;
-  e.F[1] = e.clamped ? e.z[1] * (e.bounds / e.fs) : e.z[1];
-  e.F[2] = e.clamped ? e.z[2] * (e.bounds / e.fs) : e.z[2];
+  const double sc = e.bounds * frcp(e.fs);  // used only where clamped (fs > 0)
+  e.F[1] = e.clamped ? e.z[1] * sc : e.z[1];
+  e.F[2] = e.clamped ? e.z[2] * sc : e.z[2];
   e.within = e.fs <= e.bounds;
+}
+
+// The deltas of a contact's bodies A and B from those of body i and of the
+// other body o (element selects: no pointer into either array is chosen).
+__device__ inline void orderDeltas(bool isA, const double *dqI, const double *dqO, double *dqA, double *dqB) {
+  for (int j = 0; j < 6; j++) {
+    dqA[j] = isA ? dqI[j] : dqO[j];
+    dqB[j] = isA ? dqO[j] : dqI[j];
+  }
 }
 
 // Cone projection adjoint: Fb -> zb, mub (added).
 __device__ inline void coneVjp(const CEval &e, double mu, const double *Fb, double *zb, double &mub) {
   double bb = 0;
   if (e.clamped) {
-    const double s = e.bounds / e.fs;
+    const double ifs = frcp(e.fs);
+    const double s = e.bounds * ifs, s3 = s * ifs * ifs;
     const double fz = Fb[1] * e.z[1] + Fb[2] * e.z[2];
-    zb[1] = s * Fb[1] - e.bounds / (e.fs * e.fs * e.fs) * e.z[1] * fz;
-    zb[2] = s * Fb[2] - e.bounds / (e.fs * e.fs * e.fs) * e.z[2] * fz;
-    bb = fz / e.fs;
+    zb[1] = s * Fb[1] - s3 * e.z[1] * fz;
+    zb[2] = s * Fb[2] - s3 * e.z[2] * fz;
+    bb = fz * ifs;
   } else {
     zb[1] = Fb[1];
     zb[2] = Fb[2];
@@ -265,49 +278,64 @@ __device__ inline void adjVelocity(int i, const BodyProp *props, const AdjStep &
   for (int k = 0; k < 3; k++) vs[k] = vb[6 + k];  // prevVelocity <- the step-start velocity
 }
 
-// The dual update of iteration l (manifold.js:447-491), one thread per live
-// manifold: the contact adjoints LK turn from those of k, lambda after the
-// update into those before it.
+// The dual update of iteration l (manifold.js:447-491) for one contact slot si
+// at the poses after the iteration's sweep: the contact adjoints LK turn from
+// those of k, lambda after the update into those before it; the body deltas'
+// adjoints dqAb, dqBb and mub are added to.
+__device__ inline void adjDualContact(const double *B, size_t si, const double *hot, const AdjBufs &A,
+                                      const double *dqA, const double *dqB, const StepParams &P, double *dqAb,
+                                      double *dqBb, double &mub) {
+  // the slot's adjoints are read first and written last: no load waits on a
+  // store to memory it might alias
+  double *lk = A.LK + si * 6, *rwb = A.RWb + si * 6, *c0b = A.C0b + si * 3;
+  double lkv[6], rwv[6], c0v[3];
+  for (int k = 0; k < 6; k++) { lkv[k] = lk[k]; rwv[k] = rwb[k]; }
+  for (int k = 0; k < 3; k++) c0v[k] = c0b[k];
+  CEval e;
+  cEvalDq(B, B[9], hot, hot + HK, hot + HL, dqA, dqB, P.alpha, e);
+  double kb[3] = {lkv[0], lkv[1], lkv[2]}, Cb[3] = {0, 0, 0};
+  const double Fb[3] = {lkv[3], lkv[4], lkv[5]};
+  // k' = min(k + beta |C|, kmax) where the update applies (normal: pushing;
+  // tangents: inside the cone)
+  for (int r = 0; r < 3; r++) {
+    const bool applies = r == 0 ? e.F[0] < 0 : e.within;
+    if (!applies) continue;
+    const double raw = hot[HK + r] + P.betaLin * xabs(e.C[r]);
+    if (raw < P.pmax) {
+      Cb[r] += kb[r] * P.betaLin * sgn(e.C[r]);
+    } else {
+      kb[r] = 0;
+    }
+  }
+  double zb[3];
+  coneVjp(e, B[9], Fb, zb, mub);
+  for (int r = 0; r < 3; r++) {
+    Cb[r] += hot[HK + r] * zb[r];
+    kb[r] += zb[r] * e.C[r];
+  }
+  cVjp(B, e, Cb, P.alpha, dqAb, dqBb, c0v, rwv, rwv + 3);
+  lk[0] = kb[0]; lk[1] = kb[1]; lk[2] = kb[2];
+  lk[3] = zb[0]; lk[4] = zb[1]; lk[5] = zb[2];
+  for (int k = 0; k < 6; k++) rwb[k] = rwv[k];
+  for (int k = 0; k < 3; k++) c0b[k] = c0v[k];
+}
+
+// The dual update of iteration l, one thread per live manifold.
 __device__ inline void adjDual(int t, const AdjStep &T, const AdjBufs &A, int l, const StepParams &P) {
   const int m = T.live[t];
   const int4 h = T.head[m];
   const double *B = T.basis + size_t(m) * BAS;
-  const double mu = B[9];
   const double *X = T.X + T.xStride * (l + 1);
   const double *xA = X + size_t(h.x) * XS, *xB = X + size_t(h.y) * XS;
   const double *x0A = T.X0 + size_t(h.x) * XS, *x0B = T.X0 + size_t(h.y) * XS;
   double dqAb[6] = {0, 0, 0, 0, 0, 0}, dqBb[6] = {0, 0, 0, 0, 0, 0};
   double mub = 0;
+  double dqA[6], dqB[6];
+  bodyDelta(xA, x0A, dqA);
+  bodyDelta(xB, x0B, dqB);
   for (int s = 0; s < h.z; s++) {
     const size_t si = size_t(m) * MAXC + s;
-    const double *hot = T.hot + T.hotStride * l + si * HOT;
-    CEval e;
-    cEval(B, mu, hot, hot + HK, hot + HL, xA, x0A, xB, x0B, P.alpha, e);
-    double *lk = A.LK + si * 6;
-    double kb[3] = {lk[0], lk[1], lk[2]}, Cb[3] = {0, 0, 0};
-    const double Fb[3] = {lk[3], lk[4], lk[5]};
-    // k' = min(k + beta |C|, kmax) where the update applies (normal: pushing;
-    // tangents: inside the cone)
-    for (int r = 0; r < 3; r++) {
-      const bool applies = r == 0 ? e.F[0] < 0 : e.within;
-      if (!applies) continue;
-      const double raw = hot[HK + r] + P.betaLin * xabs(e.C[r]);
-      if (raw < P.pmax) {
-        Cb[r] += kb[r] * P.betaLin * sgn(e.C[r]);
-      } else {
-        kb[r] = 0;
-      }
-    }
-    double zb[3];
-    coneVjp(e, mu, Fb, zb, mub);
-    for (int r = 0; r < 3; r++) {
-      Cb[r] += hot[HK + r] * zb[r];
-      kb[r] += zb[r] * e.C[r];
-    }
-    double *c0b = A.C0b + si * 3, *rwb = A.RWb + si * 6;
-    cVjp(B, e, Cb, P.alpha, dqAb, dqBb, c0b, rwb, rwb + 3);
-    lk[0] = kb[0]; lk[1] = kb[1]; lk[2] = kb[2];
-    lk[3] = zb[0]; lk[4] = zb[1]; lk[5] = zb[2];
+    adjDualContact(B, si, T.hot + T.hotStride * l + si * HOT, A, dqA, dqB, P, dqAb, dqBb, mub);
   }
   A.MUb[m] += mub;
   deltaVjpAtomic(xA, x0A, dqAb, A.Xb + size_t(h.x) * XS, A.X0b + size_t(h.x) * XS);
@@ -318,11 +346,20 @@ __device__ inline void adjDual(int t, const AdjStep &T, const AdjBufs &A, int l,
 // of TEAM lanes per body: the pose adjoint of each body turns from that after
 // its update into that before it; the neighbours, the contacts and the step's
 // start and inertial target receive their shares.
-template <int TEAM>
+//
+// FOLD: the team also applies, first, the dual adjoint of iteration l for the
+// contacts of which its body is the reverse sweep's first visitor (the other
+// body has a lower colour or is static). The forward's dual of iteration l
+// follows the whole sweep, so its adjoint must precede the primal adjoints of
+// both bodies of iteration l, and follow those of iteration l + 1: the first
+// visitor's team is such a place. Each contact is handled by the lane that
+// holds its item, in both parts, so that its LK slot is read and written in
+// program order; the body's own share stays in registers.
+template <int TEAM, bool FOLD>
 __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T, const AdjBufs &A,
                                      const BodyProp *props, int l, const StepParams &P) {
   const int c0 = T.cstart[c];
-  const double dt2 = P.dt * P.dt;
+  const double dt2 = P.dt * P.dt, idt2 = 1.0 / dt2;
   const double *Xbef = T.X + T.xStride * l, *Xaft = T.X + T.xStride * (l + 1);
   const double *hotL = T.hot + T.hotStride * l;
   {
@@ -336,6 +373,37 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
       const int cj = T.color[j];
       return (cj >= 0 && cj < c ? Xaft : Xbef) + size_t(j) * XS;
     };
+    // 0. (FOLD) the dual update's adjoint at the poses after the sweep
+    double dqDual[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (FOLD) {
+      const double *xa = Xaft + size_t(i) * XS;
+      for (int t = b0 + lane; t < b1; t += TEAM) {
+        const uint2 iw = T.items[t];
+        const int o = int(iw.y & ITEM_BODY);
+        if (T.color[o] > c) continue;  // the other body's team came first
+        const int m = int(iw.x >> 4);
+        const size_t si = size_t(m) * MAXC + ((iw.x >> 1) & 7u);
+        const bool isA = iw.x & 1u;
+        const bool oDyn = props[o].mass > 0;
+        double xo[7], x0o[7];
+        for (int k = 0; k < 7; k++) { xo[k] = Xaft[size_t(o) * XS + k]; x0o[k] = T.X0[size_t(o) * XS + k]; }
+        double dqI[6], dqO[6], dqA[6], dqB[6];
+        bodyDelta(xa, x0, dqI);
+        bodyDelta(xo, x0o, dqO);
+        orderDeltas(isA, dqI, dqO, dqA, dqB);
+        double dqAb[6] = {0, 0, 0, 0, 0, 0}, dqBb[6] = {0, 0, 0, 0, 0, 0}, mub = 0;
+        adjDualContact(T.basis + size_t(m) * BAS, si, hotL + si * HOT, A, dqA, dqB, P, dqAb, dqBb, mub);
+        double dqo[6];
+        for (int k = 0; k < 6; k++) {
+          dqDual[k] += isA ? dqAb[k] : dqBb[k];
+          dqo[k] = isA ? dqBb[k] : dqAb[k];
+        }
+        atomicAdd(A.MUb + m, mub);
+        if (oDyn) deltaVjpAtomic(xo, x0o, dqo, A.Xb + size_t(o) * XS, A.X0b + size_t(o) * XS);
+      }
+      if constexpr (TEAM > 1)
+        for (int k = 0; k < 6; k++) dqDual[k] = teamSum<TEAM>(dqDual[k]);
+    }
     // 1. the system H, g as the forward built it
     Sys<double> s;
     s.zero();
@@ -347,12 +415,18 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
       const int o = int(iw.y & ITEM_BODY);
       const double *B = T.basis + size_t(m) * BAS;
       const double *hot = hotL + si * HOT;
+      double dqI[6], dqO[6], dqA[6], dqB[6];
+      bodyDelta(x, x0, dqI);
+      bodyDelta(poseOf(o), T.X0 + size_t(o) * XS, dqO);
+      orderDeltas(isA, dqI, dqO, dqA, dqB);
       CEval e;
-      cEval(B, B[9], hot, hot + HK, hot + HL, isA ? x : poseOf(o), isA ? x0 : T.X0 + size_t(o) * XS,
-            isA ? poseOf(o) : x, isA ? T.X0 + size_t(o) * XS : x0, P.alpha, e);
-      double nb[9];
-      for (int k = 0; k < 9; k++) nb[k] = -B[k];
-      stampContact<double, double>(s, isA ? B : nb, isA ? e.jA : e.jB, hot + HK, e.F);
+      cEvalDq(B, B[9], hot, hot + HK, hot + HL, dqA, dqB, P.alpha, e);
+      double jl[9], ja[9];
+      for (int k = 0; k < 9; k++) {
+        jl[k] = isA ? B[k] : -B[k];
+        ja[k] = isA ? e.jA[k] : e.jB[k];
+      }
+      stampContact<double, double>(s, jl, ja, hot + HK, e.F);
     }
     if constexpr (TEAM > 1) {
       for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
@@ -361,35 +435,46 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
     }
     double M[9], ei[3];
     worldMoment(M, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-    const double ml = bp.mass / dt2;
+    const double ml = bp.mass * idt2;
     qsub(ei, x + 3, y + 3);
     s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
-    s.ang[0] += M[0] / dt2; s.ang[1] += M[3] / dt2; s.ang[2] += M[4] / dt2;
-    s.ang[3] += M[6] / dt2; s.ang[4] += M[7] / dt2; s.ang[5] += M[8] / dt2;
+    s.ang[0] += M[0] * idt2; s.ang[1] += M[3] * idt2; s.ang[2] += M[4] * idt2;
+    s.ang[3] += M[6] * idt2; s.ang[4] += M[7] * idt2; s.ang[5] += M[8] * idt2;
     for (int k = 0; k < 3; k++) {
       s.rl[k] += ml * (x[k] - y[k]);
-      s.ra[k] += (M[k * 3] * ei[0] + M[k * 3 + 1] * ei[1] + M[k * 3 + 2] * ei[2]) / dt2;
+      s.ra[k] += (M[k * 3] * ei[0] + M[k * 3 + 1] * ei[1] + M[k * 3 + 2] * ei[2]) * idt2;
     }
     const double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
     const double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
     double Dl[3], Da[3];
+    Ldlt6 H;  // factored once for the update and its transpose solve
+    ldlt6(aLin, aAng, s.cross, H);
     {
       const double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
-      solve6(aLin, aAng, s.cross, bl, ba, Dl, Da);
+      ldlt6Solve(H, bl, ba, Dl, Da);
     }
     const double Dv[6] = {Dl[0], Dl[1], Dl[2], Da[0], Da[1], Da[2]};
     // 2. the update x' = (p + D_lin, qint(q, D_ang)) and the solve
     const double *xbv = A.Xb + size_t(i) * XS;
-    const double pb1[3] = {xbv[0], xbv[1], xbv[2]}, qb1[4] = {xbv[3], xbv[4], xbv[5], xbv[6]};
+    double pb1[3] = {xbv[0], xbv[1], xbv[2]}, qb1[4] = {xbv[3], xbv[4], xbv[5], xbv[6]};
+    double q0Dual[4] = {0, 0, 0, 0};
+    if constexpr (FOLD) {
+      // the body's own share of its dual updates, as deltaVjpAtomic would add it
+      double qbD[4] = {0, 0, 0, 0};
+      qsubVjp(Xaft + size_t(i) * XS + 3, x0 + 3, dqDual + 3, qbD, q0Dual);
+      for (int k = 0; k < 3; k++) pb1[k] += dqDual[k];
+      for (int k = 0; k < 4; k++) qb1[k] += qbD[k];
+    }
     double qb[4] = {0, 0, 0, 0}, Dab[3] = {0, 0, 0};
     qintVjp(x + 3, Da, qb1, qb, Dab);
     double wl[3], wa[3];
     {
       const double bl[3] = {pb1[0], pb1[1], pb1[2]};
-      solve6(aLin, aAng, s.cross, bl, Dab, wl, wa);
+      ldlt6Solve(H, bl, Dab, wl, wa);
     }
     const double w[6] = {wl[0], wl[1], wl[2], wa[0], wa[1], wa[2]};  // H^-1 Db; gb = -w, Hb = -w D^T
-    // 3. the contacts' shares
+    // 3. the contacts' shares (each item's inputs read first, its outputs
+    // written last)
     double dqSelf[6] = {0, 0, 0, 0, 0, 0};
     for (int t = b0 + lane; t < b1; t += TEAM) {
       const uint2 iw = T.items[t];
@@ -397,13 +482,22 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
       const size_t si = size_t(m) * MAXC + ((iw.x >> 1) & 7u);
       const bool isA = iw.x & 1u;
       const int o = int(iw.y & ITEM_BODY);
+      const bool oDyn = props[o].mass > 0;
       const double *B = T.basis + size_t(m) * BAS;
       const double *hot = hotL + si * HOT;
-      const double *xo = poseOf(o), *x0o = T.X0 + size_t(o) * XS;
+      const double *xoP = poseOf(o), *x0oP = T.X0 + size_t(o) * XS;
+      double xo[7], x0o[7];
+      for (int k = 0; k < 7; k++) { xo[k] = xoP[k]; x0o[k] = x0oP[k]; }
+      double *lk = A.LK + si * 6, *rwb = A.RWb + si * 6, *c0b = A.C0b + si * 3;
+      double lkv[6], rwv[6], c0v[3];
+      for (int k = 0; k < 6; k++) { lkv[k] = lk[k]; rwv[k] = rwb[k]; }
+      for (int k = 0; k < 3; k++) c0v[k] = c0b[k];
+      double dqI[6], dqO[6], dqA[6], dqB[6];
+      bodyDelta(x, x0, dqI);
+      bodyDelta(xo, x0o, dqO);
+      orderDeltas(isA, dqI, dqO, dqA, dqB);
       CEval e;
-      cEval(B, B[9], hot, hot + HK, hot + HL, isA ? x : xo, isA ? x0 : x0o, isA ? xo : x, isA ? x0o : x0, P.alpha,
-            e);
-      double *lk = A.LK + si * 6, *rwb = A.RWb + si * 6;
+      cEvalDq(B, B[9], hot, hot + HK, hot + HL, dqA, dqB, P.alpha, e);
       double Fb[3], Cb[3], zb[3], mub = 0;
       for (int r = 0; r < 3; r++) {
         double jl[3], ja[3];
@@ -415,55 +509,69 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
         const double JD = jl[0] * Dv[0] + jl[1] * Dv[1] + jl[2] * Dv[2] + ja[0] * Dv[3] + ja[1] * Dv[4] +
                           ja[2] * Dv[5];
         Fb[r] = -Jw;
-        lk[r] += -Jw * JD;  // k from H = J^T K J
+        lkv[r] += -Jw * JD;  // k from H = J^T K J
         // angular row adjoint: F_r gb + k_r J_r (Hb + Hb^T)
         double jab[3];
         for (int k = 0; k < 3; k++) jab[k] = -e.F[r] * w[3 + k] - hot[HK + r] * (Jw * Dv[3 + k] + JD * w[3 + k]);
         double o3[3];
         cross3(o3, jl, jab);  // Ja_r = rW x Jl_r
-        for (int k = 0; k < 3; k++) rwb[(isA ? 0 : 3) + k] += o3[k];
+        for (int k = 0; k < 3; k++) {
+          rwv[k] += isA ? o3[k] : 0.0;
+          rwv[3 + k] += isA ? 0.0 : o3[k];
+        }
       }
       coneVjp(e, B[9], Fb, zb, mub);
       for (int r = 0; r < 3; r++) {
         Cb[r] = hot[HK + r] * zb[r];
-        lk[r] += zb[r] * e.C[r];
-        lk[3 + r] += zb[r];
+        lkv[r] += zb[r] * e.C[r];
+        lkv[3 + r] += zb[r];
       }
-      atomicAdd(A.MUb + m, mub);
       double dqAb[6] = {0, 0, 0, 0, 0, 0}, dqBb[6] = {0, 0, 0, 0, 0, 0};
-      cVjp(B, e, Cb, P.alpha, dqAb, dqBb, A.C0b + si * 3, rwb, rwb + 3);
-      const double *dqo = isA ? dqBb : dqAb, *dqs = isA ? dqAb : dqBb;
-      for (int k = 0; k < 6; k++) dqSelf[k] += dqs[k];
-      if (props[o].mass > 0) deltaVjpAtomic(xo, x0o, dqo, A.Xb + size_t(o) * XS, A.X0b + size_t(o) * XS);
+      cVjp(B, e, Cb, P.alpha, dqAb, dqBb, c0v, rwv, rwv + 3);
+      double dqo[6];
+      for (int k = 0; k < 6; k++) {
+        dqSelf[k] += isA ? dqAb[k] : dqBb[k];
+        dqo[k] = isA ? dqBb[k] : dqAb[k];
+      }
+      for (int k = 0; k < 6; k++) { lk[k] = lkv[k]; rwb[k] = rwv[k]; }
+      for (int k = 0; k < 3; k++) c0b[k] = c0v[k];
+      atomicAdd(A.MUb + m, mub);
+      if (oDyn) deltaVjpAtomic(xo, x0o, dqo, A.Xb + size_t(o) * XS, A.X0b + size_t(o) * XS);
     }
     if constexpr (TEAM > 1)
       for (int k = 0; k < 6; k++) dqSelf[k] = teamSum<TEAM>(dqSelf[k]);
     if (lane != 0) return;
-    // 4. the body's own terms: inertia, its delta, the update
+    // 4. the body's own terms: inertia, its delta, the update (inputs read
+    // first, outputs written last)
+    double *x0b = A.X0b + size_t(i) * XS, *yb = A.Yb + size_t(i) * XS, *xb = A.Xb + size_t(i) * XS;
+    double x0v[7], yv[7], xq[4], x0q[4], yq[4];
+    for (int k = 0; k < 7; k++) { x0v[k] = x0b[k]; yv[k] = yb[k]; }
+    for (int k = 0; k < 4; k++) { xq[k] = x[3 + k]; x0q[k] = x0[3 + k]; yq[k] = y[3 + k]; }
     double pb[3];
-    double *x0b = A.X0b + size_t(i) * XS, *yb = A.Yb + size_t(i) * XS;
     for (int k = 0; k < 3; k++) {
       pb[k] = pb1[k] - ml * w[k] + dqSelf[k];  // gb_lin = -w_lin
-      x0b[k] -= dqSelf[k];
-      yb[k] += ml * w[k];
+      x0v[k] -= dqSelf[k] + dqDual[k];
+      yv[k] += ml * w[k];
     }
+    if constexpr (FOLD)
+      for (int k = 0; k < 4; k++) x0v[3 + k] += q0Dual[k];
     {
       double q0b[4] = {0, 0, 0, 0};
-      qsubVjp(x + 3, x0 + 3, dqSelf + 3, qb, q0b);
-      for (int k = 0; k < 4; k++) x0b[3 + k] += q0b[k];
+      qsubVjp(xq, x0q, dqSelf + 3, qb, q0b);
+      for (int k = 0; k < 4; k++) x0v[3 + k] += q0b[k];
     }
     // g_ang = M e / dt^2, H_ang = M / dt^2
     double eb[3], Mb[9];
-    for (int k = 0; k < 3; k++) eb[k] = -(M[k] * w[3] + M[3 + k] * w[4] + M[6 + k] * w[5]) / dt2;
+    for (int k = 0; k < 3; k++) eb[k] = -(M[k] * w[3] + M[3 + k] * w[4] + M[6 + k] * w[5]) * idt2;
     for (int r = 0; r < 3; r++)
-      for (int k = 0; k < 3; k++) Mb[r * 3 + k] = (-w[3 + r] * ei[k] - w[3 + r] * Dv[3 + k]) / dt2;
+      for (int k = 0; k < 3; k++) Mb[r * 3 + k] = (-w[3 + r] * ei[k] - w[3 + r] * Dv[3 + k]) * idt2;
     {
       double yqb[4] = {0, 0, 0, 0};
-      qsubVjp(x + 3, y + 3, eb, qb, yqb);
-      for (int k = 0; k < 4; k++) yb[3 + k] += yqb[k];
+      qsubVjp(xq, yq, eb, qb, yqb);
+      for (int k = 0; k < 4; k++) yv[3 + k] += yqb[k];
     }
-    if (P.rotated && bp.shape != 1) worldMomentVjp(x + 3, bp.m0, bp.m1, bp.m2, Mb, qb);
-    double *xb = A.Xb + size_t(i) * XS;
+    if (P.rotated && bp.shape != 1) worldMomentVjp(xq, bp.m0, bp.m1, bp.m2, Mb, qb);
+    for (int k = 0; k < 7; k++) { x0b[k] = x0v[k]; yb[k] = yv[k]; }
     for (int k = 0; k < 3; k++) xb[k] = pb[k];
     for (int k = 0; k < 4; k++) xb[3 + k] = qb[k];
   }
@@ -536,10 +644,13 @@ __device__ inline void adjInit(int i, const BodyProp *props, const AdjStep &T, c
   double prevb[3] = {vs[6], vs[7], vs[8]};
   for (int k = 0; k < 3; k++) p0b[k] = x0b[k] + xb[k] + yb[k];
   for (int k = 0; k < 4; k++) q0b[k] = x0b[3 + k];
-  for (int k = 0; k < 3; k++) vb[k] += (xb[k] + yb[k]) * dt;
+  // y = x + v dt for every body; x^0 = x + v dt + ... for dynamic bodies only
+  // (a static body never moves: its pose adjoint is a sink)
+  const bool dyn = props[i].mass > 0;
+  for (int k = 0; k < 3; k++) vb[k] += ((dyn ? xb[k] : 0.0) + yb[k]) * dt;
   const double wdt[3] = {v[3] * dt, v[4] * dt, v[5] * dt};
   double wdtb[3] = {0, 0, 0};
-  if (props[i].mass > 0) {
+  if (dyn) {
     // x^0 = p + v dt + g wgt dt^2, wgt = clamp(a_z sgn(g) / |g|, 0, 1)
     const double accelExt = (v[2] - v[8]) / dt * sgn(P.gravity);
     const double raw = accelExt / xabs(P.gravity);
@@ -577,7 +688,7 @@ __global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A,
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
   const int cnt = T.cstart[c + 1] - T.cstart[c];
-  for (int team = gt / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
+  for (int team = gt / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM, false>(c, team, lane, T, A, props, l, P);
 }
 __global__ void k_adj_narrow(AdjStep T, AdjBufs A, const BodyProp *props, StepParams P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -588,23 +699,30 @@ __global__ void k_adj_init(int n, const BodyProp *props, AdjStep T, AdjBufs A, S
   if (i < n) adjInit(i, props, T, A, P);
 }
 
-// The whole reverse of one step in a single 512-thread workgroup, for small
-// scenes where the per-kernel launches would dominate: the same phases in the
-// same order, separated by workgroup barriers (one CU, so global writes of one
-// phase are visible to the next), then the per-step contact adjoints cleared.
+// The whole reverse of one step in a single workgroup, for small scenes where
+// the per-kernel launches would dominate: the phases of the per-kernel path,
+// with each dual adjoint folded into the colour phases (one barrier per colour
+// and iteration instead of one more per iteration), separated by workgroup
+// barriers (one CU, so global writes of one phase are visible to the next),
+// then the per-step contact adjoints cleared. One wave per SIMD: the phases are
+// chains of dependent f64 work with little parallelism, and a wave of its own
+// per SIMD has the whole register file (accumulation registers included)
+// where two would share it and spill to memory.
+constexpr int ADJ_SMALL_THREADS = 256;
 template <int TEAM>
-__global__ void __launch_bounds__(512) k_adj_step_small(int n, int nTot, const BodyProp *props, AdjStep T,
-                                                         AdjBufs A, int iters, StepParams P) {
+__global__ void __launch_bounds__(ADJ_SMALL_THREADS) k_adj_step_small(int n, int nTot, const BodyProp *props,
+                                                                      AdjStep T, AdjBufs A, int iters,
+                                                                      StepParams P) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int i = tid; i < n; i += nt) adjVelocity(i, props, T, A, iters, P.dt);
   __syncthreads();
   const int lane = tid % TEAM, teams = nt / TEAM;
+  // each iteration's dual adjoint rides with its first colour in reverse
   for (int l = iters - 1; l >= 0; l--) {
-    for (int t = tid; t < T.nLive; t += nt) adjDual(t, T, A, l, P);
-    __syncthreads();
     for (int c = T.numColors - 1; c >= 0; c--) {
       const int cnt = T.cstart[c + 1] - T.cstart[c];
-      for (int team = tid / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
+      for (int team = tid / TEAM; team < cnt; team += teams)
+        adjPrimalTeam<TEAM, true>(c, team, lane, T, A, props, l, P);
       __syncthreads();
     }
   }

@@ -856,10 +856,10 @@ struct avbd_ctx {
     if (nDynamic <= adjSmallMax) {
       // a small scene: the whole reverse step in one workgroup
       if (wide)
-        hipLaunchKernelGGL(k_adj_step_small<16>, dim3(1), dim3(512), 0, stream, N, T.nTot,
+        hipLaunchKernelGGL(k_adj_step_small<16>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, stream, N, T.nTot,
                            (const BodyProp *)props.p, S, A, I, P);
       else
-        hipLaunchKernelGGL(k_adj_step_small<4>, dim3(1), dim3(512), 0, stream, N, T.nTot,
+        hipLaunchKernelGGL(k_adj_step_small<4>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, stream, N, T.nTot,
                            (const BodyProp *)props.p, S, A, I, P);
       HIPCHECK(hipGetLastError());
       lk = 1 - lk;

@@ -616,10 +616,15 @@ __device__ __forceinline__ double frcp(double d) {
   return __builtin_fma(r, e, r);
 }
 
-// solve6 with one reciprocal per pivot instead of a division per entry (the
-// f32/f64 team modes; the exact mode keeps the reference's divisions).
-__device__ __forceinline__ void solve6f(const double *aLin, const double *aAng, const double *aCross,
-                                        const double *bLin, const double *bAng, double *xLin, double *xAng) {
+// LDL^T of the 6x6 system (solve6's factorisation) with one reciprocal per
+// pivot instead of a division per entry, kept for callers that solve twice
+// with one matrix (the adjoint: the update, then its transpose solve).
+struct Ldlt6 {
+  double L21, L31, L41, L51, L61, L32, L42, L52, L62, L43, L53, L63, L54, L64, L65;
+  double i1, i2, i3, i4, i5, i6;
+};
+
+__device__ __forceinline__ void ldlt6(const double *aLin, const double *aAng, const double *aCross, Ldlt6 &f) {
   const double A11 = aLin[0];
   const double A21 = aLin[3], A22 = aLin[4];
   const double A31 = aLin[6], A32 = aLin[7], A33 = aLin[8];
@@ -643,19 +648,33 @@ __device__ __forceinline__ void solve6f(const double *aLin, const double *aAng, 
   const double D5 = A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), i5 = frcp(D5);
   const double L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) * i5;
   const double D6 = A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5);
-  const double i6 = frcp(D6);
+  f = Ldlt6{L21, L31, L41, L51, L61, L32, L42, L52, L62, L43, L53, L63, L54, L64, L65,
+            i1, i2, i3, i4, i5, frcp(D6)};
+}
+
+__device__ __forceinline__ void ldlt6Solve(const Ldlt6 &f, const double *bLin, const double *bAng, double *xLin,
+                                           double *xAng) {
   const double y1 = bLin[0];
-  const double y2 = bLin[1] - L21 * y1;
-  const double y3 = bLin[2] - L31 * y1 - L32 * y2;
-  const double y4 = bAng[0] - L41 * y1 - L42 * y2 - L43 * y3;
-  const double y5 = bAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
-  const double y6 = bAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
-  xAng[2] = y6 * i6;
-  xAng[1] = y5 * i5 - L65 * xAng[2];
-  xAng[0] = y4 * i4 - L54 * xAng[1] - L64 * xAng[2];
-  xLin[2] = y3 * i3 - L43 * xAng[0] - L53 * xAng[1] - L63 * xAng[2];
-  xLin[1] = y2 * i2 - L32 * xLin[2] - L42 * xAng[0] - L52 * xAng[1] - L62 * xAng[2];
-  xLin[0] = y1 * i1 - L21 * xLin[1] - L31 * xLin[2] - L41 * xAng[0] - L51 * xAng[1] - L61 * xAng[2];
+  const double y2 = bLin[1] - f.L21 * y1;
+  const double y3 = bLin[2] - f.L31 * y1 - f.L32 * y2;
+  const double y4 = bAng[0] - f.L41 * y1 - f.L42 * y2 - f.L43 * y3;
+  const double y5 = bAng[1] - f.L51 * y1 - f.L52 * y2 - f.L53 * y3 - f.L54 * y4;
+  const double y6 = bAng[2] - f.L61 * y1 - f.L62 * y2 - f.L63 * y3 - f.L64 * y4 - f.L65 * y5;
+  xAng[2] = y6 * f.i6;
+  xAng[1] = y5 * f.i5 - f.L65 * xAng[2];
+  xAng[0] = y4 * f.i4 - f.L54 * xAng[1] - f.L64 * xAng[2];
+  xLin[2] = y3 * f.i3 - f.L43 * xAng[0] - f.L53 * xAng[1] - f.L63 * xAng[2];
+  xLin[1] = y2 * f.i2 - f.L32 * xLin[2] - f.L42 * xAng[0] - f.L52 * xAng[1] - f.L62 * xAng[2];
+  xLin[0] = y1 * f.i1 - f.L21 * xLin[1] - f.L31 * xLin[2] - f.L41 * xAng[0] - f.L51 * xAng[1] - f.L61 * xAng[2];
+}
+
+// solve6 through that factorisation (the f32/f64 team modes; the exact mode
+// keeps the reference's divisions).
+__device__ __forceinline__ void solve6f(const double *aLin, const double *aAng, const double *aCross,
+                                        const double *bLin, const double *bAng, double *xLin, double *xAng) {
+  Ldlt6 f;
+  ldlt6(aLin, aAng, aCross, f);
+  ldlt6Solve(f, bLin, bAng, xLin, xAng);
 }
 
 // Eq. 21 quaternion update with the normalisation as one reciprocal.
