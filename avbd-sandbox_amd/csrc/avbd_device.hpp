@@ -713,6 +713,22 @@ template <int TEAM> __device__ __forceinline__ double teamSum(double v) {
   return v;
 }
 
+// The same butterfly in f32: each step is one add with a DPP source operand
+// (the compiler folds the lane move into the add), a third of the f64 form's
+// instructions.
+template <int CTRL> __device__ __forceinline__ float dppMoveF(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int TEAM> __device__ __forceinline__ float teamSumF(float v) {
+  if constexpr (TEAM >= 2) v += dppMoveF<0xB1>(v);
+  if constexpr (TEAM >= 4) v += dppMoveF<0x4E>(v);
+  if constexpr (TEAM >= 8) v += dppMoveF<0x141>(v);
+  if constexpr (TEAM >= 16) v += dppMoveF<0x140>(v);
+  if constexpr (TEAM >= 32) v += __shfl_xor(v, 16, TEAM);
+  if constexpr (TEAM >= 64) v += __shfl_xor(v, 32, TEAM);
+  return v;
+}
+
 // Accumulated 6x6 system: lower triangles of the Lin and Ang blocks, the full
 // Cross block, and the right-hand side (the 27 numbers solve6 reads).
 // lin: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) 4:(2,1) 5:(2,2)  (row, col)

@@ -1045,6 +1045,14 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
     const double *x0 = S.X0 + size_t(i) * XS;
     R dqi[8];
     vload<8>(dqi, DQ + size_t(i) * 8);
+    // own pose and the step's start and inertial target, for lane 0's solve
+    // (requested with the body's other records, so that they arrive during
+    // the contact gathers; the exact mode, short of registers, asks after
+    // them)
+    double xs[7], ys[7], x0s[7];
+    if constexpr (!EXACT)
+      if (lane == 0)
+        for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
     Sys<Acc> sl;
     sl.zero();
     if constexpr (EXACT) {
@@ -1123,16 +1131,24 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
         stampContact<R, Acc>(sl, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
       }
     }
-    // own pose and the step's start and inertial target, for lane 0's solve
-    // (requested before the reduction so that they arrive meanwhile)
-    double xs[7], ys[7], x0s[7];
-    if (lane == 0)
-      for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
+    if constexpr (EXACT)
+      if (lane == 0)
+        for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
     Sys<double> s;
+    if constexpr (TEAM > 1 && std::is_same<Acc, float>::value) {
+      // f32 lane sums reduce in f32: the tree adds four roundings to the
+      // lanes' own sums, at a third of the instructions of an f64 tree
+#pragma unroll
+      for (int k = 0; k < 6; k++) { sl.lin[k] = teamSumF<TEAM>(sl.lin[k]); sl.ang[k] = teamSumF<TEAM>(sl.ang[k]); }
+#pragma unroll
+      for (int k = 0; k < 9; k++) sl.cross[k] = teamSumF<TEAM>(sl.cross[k]);
+#pragma unroll
+      for (int k = 0; k < 3; k++) { sl.rl[k] = teamSumF<TEAM>(sl.rl[k]); sl.ra[k] = teamSumF<TEAM>(sl.ra[k]); }
+    }
     for (int k = 0; k < 6; k++) { s.lin[k] = double(sl.lin[k]); s.ang[k] = double(sl.ang[k]); }
     for (int k = 0; k < 9; k++) s.cross[k] = double(sl.cross[k]);
     for (int k = 0; k < 3; k++) { s.rl[k] = double(sl.rl[k]); s.ra[k] = double(sl.ra[k]); }
-    if constexpr (TEAM > 1) {
+    if constexpr (TEAM > 1 && !std::is_same<Acc, float>::value) {
 #pragma unroll
       for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
 #pragma unroll

@@ -176,21 +176,45 @@ def test_launch_through_write_state_matches_oracle():
     assert g["pos"][L["body"], 2] < L["pos"][2] - 1.0  # the sphere came down onto the apex
 
 
-def test_autotune_keeps_results_and_picks_a_team():
+def test_autotune_picks_the_fastest_team():
     scene, params, _ = avbd.scenes.pyramid(layers=10)
     a = avbd.HipBackend(precision=avbd.PREC_MIXED)
     a.pack(scene, params)
     a.step(3)
     team, ms = a.autotune()
+    print(f"autotune: team {team}, ms per sweep {ms}")
     assert team in (1, 4, 8, 16, 32, 64) and all(v > 0 for v in ms.values())
     assert ms[team] == min(ms.values())
     a.step(5)
+    st = a.read_state()
+    assert np.isfinite(st["pos"]).all() and np.isfinite(st["quat"]).all()
+    e = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    e.pack(scene, params)
+    with pytest.raises(AvbdError) as err:
+        e.autotune()
+    assert err.value.code == -4  # the exact mode has one team shape
+
+
+def test_autotune_trials_restore_the_state():
+    # spheres resting on the ground apart from each other: one contact item
+    # per body, so every team shape sums the same terms exactly and a run
+    # equals any other bit for bit; the trials must leave no trace
+    bodies = [avbd.scenes.ground()]
+    for i in range(24):
+        bodies.append(avbd.scenes.sphere(1.0, 1.0, 0.5, [3.0 * (i % 6), 3.0 * (i // 6), 0.5 + 0.02 * (i % 5)],
+                                         vel=[0.3 * (i % 3), 0.0, 0.0]))
+    scene = dict(bodies=bodies, joints=[])
+    a = avbd.HipBackend(precision=avbd.PREC_MIXED)
+    a.pack(scene, dict(iterations=10))
+    a.step(3)
+    team, _ = a.autotune()
+    a.step(5)
     b = avbd.HipBackend(precision=avbd.PREC_MIXED, team=team)
-    b.pack(scene, params)
+    b.pack(scene, dict(iterations=10))
     b.step(9)
     sa, sb = a.read_state(), b.read_state()
-    # the trials restore the state they touch: same result as a run with that team
     assert np.array_equal(sa["pos"], sb["pos"]) and np.array_equal(sa["quat"], sb["quat"])
+    assert a.counters()["contacts"] == 24
 
 
 @pytest.mark.parametrize("graphs", [0, -1], ids=["graph", "direct"])
