@@ -779,7 +779,7 @@ __device__ __forceinline__ float rsqrtR(float x) { return __builtin_amdgcn_rsqf(
 __device__ __forceinline__ double rsqrtR(double x) { return 1.0 / __builtin_sqrt(x); }
 
 template <class R> struct FEval {
-  R jA[9], jB[9];  // angular rows: jA_r = rAw x B_r, jB_r = B_r x rBw
+  R v[3];  // dA - dB + wA x rAw + rBw x wB: C_r = (1 - alpha) C0_r + B_r . v
   R C[3], F[3];
   bool within;
 };
@@ -791,23 +791,26 @@ __device__ __forceinline__ void fcross(R *o, const R *a, const R *b) {
   o[2] = fmaR(a[0], b[1], -(a[1] * b[0]));
 }
 
+// C = (1 - alpha) C0 + J (x - x_t) with the cached angular rows folded into
+// one vector: (rAw x B_r) . wA = B_r . (wA x rAw) and (B_r x rBw) . wB =
+// B_r . (rBw x wB), so every row is B_r . v.
 template <class R>
 __device__ __forceinline__ void fevalC(const R *B, const R *hot, const R *dqA, const R *dqB, R alpha, FEval<R> &e) {
-  const R d[3] = {dqA[0] - dqB[0], dqA[1] - dqB[1], dqA[2] - dqB[2]};
+  R wa[3], wb[3];
+  fcross(wa, dqA + 3, hot + HRA);
+  fcross(wb, hot + HRB, dqB + 3);
+#pragma unroll
+  for (int k = 0; k < 3; k++) e.v[k] = (dqA[k] - dqB[k]) + (wa[k] + wb[k]);
+  fevalCAlpha(B, hot, alpha, e);
+}
+
+// C from the stored v under another alpha.
+template <class R>
+__device__ __forceinline__ void fevalCAlpha(const R *B, const R *hot, R alpha, FEval<R> &e) {
   const R oma = R(1) - alpha;
 #pragma unroll
-  for (int r = 0; r < 3; r++) {
-    fcross(e.jA + r * 3, hot + HRA, B + r * 3);
-    fcross(e.jB + r * 3, B + r * 3, hot + HRB);
-    R c = oma * hot[HC0 + r];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      c = fmaR(B[r * 3 + k], d[k], c);
-      c = fmaR(e.jA[r * 3 + k], dqA[3 + k], c);
-      c = fmaR(e.jB[r * 3 + k], dqB[3 + k], c);
-    }
-    e.C[r] = c;
-  }
+  for (int r = 0; r < 3; r++)
+    e.C[r] = fmaR(B[r * 3], e.v[0], fmaR(B[r * 3 + 1], e.v[1], fmaR(B[r * 3 + 2], e.v[2], oma * hot[HC0 + r])));
 }
 
 template <class R>
@@ -829,50 +832,54 @@ __device__ __forceinline__ void fevalF(const R *k, const R *lam, R mu, FEval<R> 
   e.F[2] = z2 * scale;
 }
 
-// Adds one contact to the system of the side whose linear rows are s * B
-// (s = +1 for body A, -1 for body B) and angular rows ja.
+// Adds one contact to the system of one of its bodies: linear rows s B_r
+// (s = +1 for body A, -1 for body B) and angular rows s (rW x B_r) with rW
+// that body's world anchor. With L = sum_r k_r B_r B_r^T and g = B^T F the
+// blocks are
+//   Lin += L,  Cross += [rW]x L,  Ang += [rW]x L [rW]x^T,
+//   rl += s g,  ra += s (rW x g),
+// the same sums as J^T K J and J^T F, with fewer products.
 template <class R, class A>
-__device__ __forceinline__ void fstamp(Sys<A> &sy, const R *B, R s, const R *ja, const R *k, const R *F) {
-  R kB[9], u[9];
+__device__ __forceinline__ void fstamp(Sys<A> &sy, const R *B, R s, const R *rW, const R *k, const R *F) {
+  R kB[9];
 #pragma unroll
   for (int r = 0; r < 3; r++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      kB[r * 3 + i] = k[r] * B[r * 3 + i];
-      u[r * 3 + i] = k[r] * ja[r * 3 + i];
-    }
+    for (int i = 0; i < 3; i++) kB[r * 3 + i] = k[r] * B[r * 3 + i];
+  R L[9];  // full symmetric
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int i = LT_R[q], j = LT_C[q];
-    R l = sy.lin[q], a = sy.ang[q];
+    const R l = fmaR(kB[i], B[j], fmaR(kB[3 + i], B[3 + j], kB[6 + i] * B[6 + j]));
+    L[i * 3 + j] = l;
+    L[j * 3 + i] = l;
+    sy.lin[q] = sy.lin[q] + A(l);
+  }
+  // M = [rW]x L: column j is rW x L_j
+  R M[9];
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-      l = fmaR(kB[r * 3 + i], B[r * 3 + j], l);
-      a = fmaR(u[r * 3 + i], ja[r * 3 + j], a);
-    }
-    sy.lin[q] = l;
-    sy.ang[q] = a;
+  for (int j = 0; j < 3; j++) {
+    M[0 * 3 + j] = fmaR(rW[1], L[2 * 3 + j], -(rW[2] * L[1 * 3 + j]));
+    M[1 * 3 + j] = fmaR(rW[2], L[0 * 3 + j], -(rW[0] * L[2 * 3 + j]));
+    M[2 * 3 + j] = fmaR(rW[0], L[1 * 3 + j], -(rW[1] * L[0 * 3 + j]));
   }
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+  for (int q = 0; q < 9; q++) sy.cross[q] = sy.cross[q] + A(M[q]);
+  // Ang_ij = (rW x M_row_i)_j
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      R c = R(0);
+  for (int q = 0; q < 6; q++) {
+    const int i = LT_R[q], j = LT_C[q];
+    const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    sy.ang[q] = sy.ang[q] + A(fmaR(rW[j1], M[i * 3 + j2], -(rW[j2] * M[i * 3 + j1])));
+  }
+  R g[3], rg[3];
 #pragma unroll
-      for (int r = 0; r < 3; r++) c = fmaR(u[r * 3 + i], B[r * 3 + j], c);
-      sy.cross[i * 3 + j] = fmaR(s, c, sy.cross[i * 3 + j]);
-    }
-  const R sF[3] = {s * F[0], s * F[1], s * F[2]};
+  for (int i = 0; i < 3; i++) g[i] = fmaR(F[0], B[i], fmaR(F[1], B[3 + i], F[2] * B[6 + i]));
+  fcross(rg, rW, g);
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    R l = sy.rl[i], a = sy.ra[i];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-      l = fmaR(sF[r], B[r * 3 + i], l);
-      a = fmaR(F[r], ja[r * 3 + i], a);
-    }
-    sy.rl[i] = l;
-    sy.ra[i] = a;
+    sy.rl[i] = fmaR(s, g[i], sy.rl[i]);
+    sy.ra[i] = fmaR(s, rg[i], sy.ra[i]);
   }
 }
 

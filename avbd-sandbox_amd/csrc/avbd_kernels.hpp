@@ -1093,6 +1093,9 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
         vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
         if constexpr (!EXACT) {
           const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+          R rW[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) rW[q] = isA ? hot[HRA + q] : hot[HRB + q];
           FEval<R> e;
           if (dual && (iw.y & ITEM_FIRST)) {
             fevalC(bas, hot, dA, dB, rdalpha, e);
@@ -1101,13 +1104,13 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
             const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
             vstoreKL(mb.hot + si * HOT, kl);
             if (st >= 0) mb.stick[si] = st;
-            if (rdalpha != ralpha) fevalC(bas, hot, dA, dB, ralpha, e);
+            if (rdalpha != ralpha) fevalCAlpha(bas, hot, ralpha, e);
             fevalF(kl, kl + 3, bas[9], e);
-            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), isA ? e.jA : e.jB, kl, e.F);
+            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, kl, e.F);
           } else {
             fevalC(bas, hot, dA, dB, ralpha, e);
             fevalF(hot + HK, hot + HL, bas[9], e);
-            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), isA ? e.jA : e.jB, hot + HK, e.F);
+            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, hot + HK, e.F);
           }
           continue;
         }
