@@ -440,9 +440,15 @@ struct avbd_ctx {
   // Launch shapes of the solve loop depend only on the body count, the colour
   // count and the team: the kernels take their counts from device memory and
   // stride, so one captured graph serves every step with that colour count.
-  int primalBlocks(int tm) const {
-    const long teams = long(1.5 * double(std::max(nDynamic, 1)) / std::max(numColors, 1)) + 1;
-    return std::max(1, std::min(blocks(teams * tm), 4096));
+  // A team per body of colour c, so that no team sweeps two bodies one after
+  // the other (colours differ in size: the pyramid's first colours hold 1.7x
+  // the average). A captured graph is replayed on later steps whose colours
+  // may have grown: it gets a quarter more teams (and teams stride over any
+  // excess, so a larger colour is still swept whole).
+  int primalBlocks(int c, int tm) const {
+    const long cnt = std::max(hcstart[c + 1] - hcstart[c], 1);
+    const long teams = graphCapturing ? cnt + cnt / 4 + 64 : cnt;
+    return std::max(1, std::min(blocks(teams * tm), 8192));
   }
   int dualBlocks() const { return std::max(1, std::min(blocks(manCap, 128), 2048)); }
 
@@ -451,7 +457,7 @@ struct avbd_ctx {
                     double dualAlpha) {
     if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
     const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
-    timedLaunch(AVBD_K_PRIMAL, bytes, k_primal<R, EXACT, TEAM>, dim3(primalBlocks(TEAM)), dim3(256), c,
+    timedLaunch(AVBD_K_PRIMAL, bytes, k_primal<R, EXACT, TEAM>, dim3(primalBlocks(c, TEAM)), dim3(256), c,
                 (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
   }
 

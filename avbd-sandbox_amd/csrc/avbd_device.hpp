@@ -119,13 +119,28 @@ __device__ inline void supportPt(double *o, const Obb &b, const double *dir) {
   for (int k = 0; k < 3; k++) o[k] += b.ax[2][k] * (b.h[2] * sz);
 }
 
+// A value the optimiser cannot trace back to the memory it was read from: a
+// select between two such values stays a select of registers. (A select
+// between two loads of one structure is otherwise turned into a load at a
+// computed offset, which keeps the whole structure in private memory.)
+__device__ __forceinline__ double opaque(double x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
 // Axis i and half extent i of a box by selects, not by indexing, so that a
 // box stays in registers.
 __device__ __forceinline__ void axOf(const Obb &b, int i, double *o) {
 #pragma unroll
-  for (int k = 0; k < 3; k++) o[k] = i == 0 ? b.ax[0][k] : (i == 1 ? b.ax[1][k] : b.ax[2][k]);
+  for (int k = 0; k < 3; k++) {
+    const double a0 = opaque(b.ax[0][k]), a1 = opaque(b.ax[1][k]), a2 = opaque(b.ax[2][k]);
+    o[k] = i == 0 ? a0 : (i == 1 ? a1 : a2);
+  }
 }
-__device__ __forceinline__ double hOf(const Obb &b, int i) { return i == 0 ? b.h[0] : (i == 1 ? b.h[1] : b.h[2]); }
+__device__ __forceinline__ double hOf(const Obb &b, int i) {
+  const double h0 = opaque(b.h[0]), h1 = opaque(b.h[1]), h2 = opaque(b.h[2]);
+  return i == 0 ? h0 : (i == 1 ? h1 : h2);
+}
 
 __device__ inline void faceAx(const Obb &b, int ai, double *u, double *v, double &eu, double &ev) {
   const int iu = ai == 0 ? 1 : 0, iv = ai == 2 ? 1 : 2;
@@ -138,10 +153,10 @@ __device__ inline void faceAx(const Obb &b, int ai, double *u, double *v, double
 __device__ __forceinline__ void pickObb(Obb &o, bool first, const Obb &a, const Obb &b) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    o.c[k] = first ? a.c[k] : b.c[k];
-    o.h[k] = first ? a.h[k] : b.h[k];
+    o.c[k] = first ? opaque(a.c[k]) : opaque(b.c[k]);
+    o.h[k] = first ? opaque(a.h[k]) : opaque(b.h[k]);
 #pragma unroll
-    for (int j = 0; j < 3; j++) o.ax[j][k] = first ? a.ax[j][k] : b.ax[j][k];
+    for (int j = 0; j < 3; j++) o.ax[j][k] = first ? opaque(a.ax[j][k]) : opaque(b.ax[j][k]);
   }
 }
 
@@ -190,11 +205,15 @@ __device__ inline int clipPoly(const double *in, int n, const double *pn, double
   return clipPolyT<1, MAXPOLY, false>(in, n, pn, off, out);
 }
 
+// The contacts of one manifold as they are generated. It holds the bodies'
+// pose pointers rather than the views themselves and the merge midpoints in a
+// separate array, so that only that array (indexed by the contact count) and
+// the output need private memory.
 struct Sink {
-  const BodyView *A, *B;
+  const double *pA, *qA, *pB, *qB;
   NewContact *out;
+  double *mid;  // MAXC * 3
   int count;
-  double mid[MAXC * 3];
   // collide.js:254-285
   __device__ bool add(const double *xA, const double *xB, uint32_t key) {
     const double mx = (xA[0] + xB[0]) * 0.5, my = (xA[1] + xB[1]) * 0.5, mz = (xA[2] + xB[2]) * 0.5;
@@ -206,11 +225,11 @@ struct Sink {
     NewContact &c = out[count];
     c.feature = key;
     double inv[4], t[3];
-    inv[0] = -A->q[0]; inv[1] = -A->q[1]; inv[2] = -A->q[2]; inv[3] = A->q[3];
-    t[0] = xA[0] - A->p[0]; t[1] = xA[1] - A->p[1]; t[2] = xA[2] - A->p[2];
+    inv[0] = -qA[0]; inv[1] = -qA[1]; inv[2] = -qA[2]; inv[3] = qA[3];
+    t[0] = xA[0] - pA[0]; t[1] = xA[1] - pA[1]; t[2] = xA[2] - pA[2];
     qrot(c.rA, inv, t);
-    inv[0] = -B->q[0]; inv[1] = -B->q[1]; inv[2] = -B->q[2]; inv[3] = B->q[3];
-    t[0] = xB[0] - B->p[0]; t[1] = xB[1] - B->p[1]; t[2] = xB[2] - B->p[2];
+    inv[0] = -qB[0]; inv[1] = -qB[1]; inv[2] = -qB[2]; inv[3] = qB[3];
+    t[0] = xB[0] - pB[0]; t[1] = xB[1] - pB[1]; t[2] = xB[2] - pB[2];
     qrot(c.rB, inv, t);
     mid[count * 3] = mx; mid[count * 3 + 1] = my; mid[count * 3 + 2] = mz;
     count++;
@@ -423,8 +442,14 @@ __device__ inline int sphereSphere(const BodyView &A, const BodyView &B, Sink &s
 
 // collide.js:526-600
 __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink, double *basis, bool sphereIsA) {
-  const BodyView &sph = sphereIsA ? A : B;
-  const BodyView &bx = sphereIsA ? B : A;
+  // the two roles by field selects (a select of references would keep both
+  // views in private memory)
+  const BodyView sph{sphereIsA ? A.p : B.p, sphereIsA ? A.q : B.q, sphereIsA ? A.hx : B.hx,
+                     sphereIsA ? A.hy : B.hy, sphereIsA ? A.hz : B.hz, sphereIsA ? A.radius : B.radius,
+                     sphereIsA ? A.shape : B.shape};
+  const BodyView bx{sphereIsA ? B.p : A.p, sphereIsA ? B.q : A.q, sphereIsA ? B.hx : A.hx,
+                    sphereIsA ? B.hy : A.hy, sphereIsA ? B.hz : A.hz, sphereIsA ? B.radius : A.radius,
+                    sphereIsA ? B.shape : A.shape};
   Obb o;
   makeObb(o, bx);
   double d[3], cp[3], sv[3], n[3], xA[3], xB[3];
@@ -450,7 +475,9 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
     fa = gx < gy ? (gx < gz ? 0 : 2) : (gy < gz ? 1 : 2);
     const double local = fa == 0 ? lx : fa == 1 ? ly : lz;
     const double s = local >= 0 ? 1.0 : -1.0;
-    for (int k = 0; k < 3; k++) n[k] = o.ax[fa][k] * s;
+    double af[3];
+    axOf(o, fa, af);
+    for (int k = 0; k < 3; k++) n[k] = af[k] * s;
     const double gap = fa == 0 ? gx : fa == 1 ? gy : gz;
     for (int k = 0; k < 3; k++) { cp[k] = sph.p[k]; cp[k] += n[k] * gap; }
   }
@@ -480,8 +507,8 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
 // collide.js:612-685. Writes the basis only where the reference does.
 __device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *out, double *basis,
                               double *lds = nullptr) {
-  Sink sink;
-  sink.A = &A; sink.B = &B; sink.out = out; sink.count = 0;
+  double mid[MAXC * 3];
+  Sink sink{A.p, A.q, B.p, B.q, out, mid, 0};
   const bool sA = A.shape == 1, sB = B.shape == 1;
   if (sA && sB) return sphereSphere(A, B, sink, basis);
   if (sA || sB) return sphereBox(A, B, sink, basis, sA);
@@ -513,8 +540,9 @@ __device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *
   double t[3] = {-best.n[0], -best.n[1], -best.n[2]};
   orthonormal(basis, t);
   if (best.type == 2) return edgeContact(oA, oB, sink, best.ia, best.ib, best.n);
-  if (best.type == 0) return faceManifold(oA, oB, sink, true, best.ia, best.n, lds);
-  return faceManifold(oA, oB, sink, false, best.ib, best.n, lds);
+  // one face routine for both reference sides (one inlined copy)
+  const bool refIsA = best.type == 0;
+  return faceManifold(oA, oB, sink, refIsA, refIsA ? best.ia : best.ib, best.n, lds);
 }
 
 // ---------------------------------------------------------------------------
