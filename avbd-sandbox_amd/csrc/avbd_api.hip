@@ -326,23 +326,17 @@ struct avbd_ctx {
 
   // ---------------------------------------------------------------- scan
   // Exclusive scan of n ints; *total (device) receives the sum.
-  void scan(const int *in, int *out, int n, int *total, int level = 0) {
+  // Exclusive scan of n ints in two launches; *total receives the sum.
+  void scan(const int *in, int *out, int n, int *total) {
     if (n <= 0) {
       hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, total, 1, 0);
       return;
     }
     const int nb = (n + 1023) / 1024;
-    if (level * 2 + 1 >= int(scanTmp.size())) throw ApiError(AVBD_E_CAPACITY, "scan too deep");
-    DBuf<int> &sums = scanTmp[level * 2];
-    DBuf<int> &offs = scanTmp[level * 2 + 1];
+    DBuf<int> &sums = scanTmp[0];
     sums.reserve(nb);
-    offs.reserve(nb + 1);
     hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, stream, in, out, n, sums.p);
-    if (nb > 1) {
-      scan(sums.p, offs.p, nb, offs.p + nb, level + 1);
-      hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, stream, out, n, offs.p);
-    }
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, stream, in, out, n, total);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nb), dim3(256), 0, stream, out, n, (const int *)sums.p, nb, total);
   }
 
   void readScal(int first, int count) {
@@ -407,7 +401,6 @@ struct avbd_ctx {
         for (int r = 0; r < colorBatch; r++) {
           hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
                              adjOther.p, color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
-          hipLaunchKernelGGL(k_color_check, dim3(1), dim3(64), 0, stream, cmeta.p);
         }
         launched += colorBatch;
       }

@@ -96,7 +96,6 @@ constexpr double PLANE_EPS = 1.0e-5;
 constexpr double MERGE_DIST_SQ = 1.0e-6;
 
 struct Obb { double c[3], h[3], ax[3][3]; };
-struct NewContact { uint32_t feature; double rA[3], rB[3]; };
 
 struct BodyView { const double *p, *q; double hx, hy, hz, radius; int shape; };
 
@@ -205,33 +204,44 @@ __device__ inline int clipPoly(const double *in, int n, const double *pn, double
   return clipPolyT<1, MAXPOLY, false>(in, n, pn, off, out);
 }
 
-// The contacts of one manifold as they are generated. It holds the bodies'
-// pose pointers rather than the views themselves and the merge midpoints in a
-// separate array, so that only that array (indexed by the contact count) and
-// the output need private memory.
+// A double that may share memory with other types (the merge midpoints below
+// live for a while in space that later holds a slot's R-typed hot record).
+typedef double __attribute__((may_alias)) AliasDouble;
+
+// Where the contacts of one manifold go as they are generated: the bodies'
+// pose pointers, the manifold's output slots (body-frame anchors rA, rB: 6
+// doubles per contact; feature keys) and, for the merge test, the contact
+// midpoints (contact i at mid + i * midStride: the slots' hot-record space,
+// free until the records are written). Nothing is kept in private memory.
 struct Sink {
   const double *pA, *qA, *pB, *qB;
-  NewContact *out;
-  double *mid;  // MAXC * 3
+  double *anchor;
+  uint32_t *feat;
+  AliasDouble *mid;
+  int midStride;
   int count;
   // collide.js:254-285
   __device__ bool add(const double *xA, const double *xB, uint32_t key) {
     const double mx = (xA[0] + xB[0]) * 0.5, my = (xA[1] + xB[1]) * 0.5, mz = (xA[2] + xB[2]) * 0.5;
     for (int i = 0; i < count; i++) {
-      const double dx = mx - mid[i * 3], dy = my - mid[i * 3 + 1], dz = mz - mid[i * 3 + 2];
+      const AliasDouble *m = mid + i * midStride;
+      const double dx = mx - m[0], dy = my - m[1], dz = mz - m[2];
       if (dx * dx + dy * dy + dz * dz < MERGE_DIST_SQ) return false;
     }
     if (count >= MAXC) return false;
-    NewContact &c = out[count];
-    c.feature = key;
-    double inv[4], t[3];
+    feat[count] = key;
+    double inv[4], t[3], r[3];
+    double *an = anchor + count * 6;
     inv[0] = -qA[0]; inv[1] = -qA[1]; inv[2] = -qA[2]; inv[3] = qA[3];
     t[0] = xA[0] - pA[0]; t[1] = xA[1] - pA[1]; t[2] = xA[2] - pA[2];
-    qrot(c.rA, inv, t);
+    qrot(r, inv, t);
+    an[0] = r[0]; an[1] = r[1]; an[2] = r[2];
     inv[0] = -qB[0]; inv[1] = -qB[1]; inv[2] = -qB[2]; inv[3] = qB[3];
     t[0] = xB[0] - pB[0]; t[1] = xB[1] - pB[1]; t[2] = xB[2] - pB[2];
-    qrot(c.rB, inv, t);
-    mid[count * 3] = mx; mid[count * 3 + 1] = my; mid[count * 3 + 2] = mz;
+    qrot(r, inv, t);
+    an[3] = r[0]; an[4] = r[1]; an[5] = r[2];
+    AliasDouble *m = mid + count * midStride;
+    m[0] = mx; m[1] = my; m[2] = mz;
     count++;
     return true;
   }
@@ -505,10 +515,9 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
 }
 
 // collide.js:612-685. Writes the basis only where the reference does.
-__device__ inline int collide(const BodyView &A, const BodyView &B, NewContact *out, double *basis,
-                              double *lds = nullptr) {
-  double mid[MAXC * 3];
-  Sink sink{A.p, A.q, B.p, B.q, out, mid, 0};
+__device__ inline int collide(const BodyView &A, const BodyView &B, double *anchor, uint32_t *feat, AliasDouble *mid,
+                              int midStride, double *basis, double *lds = nullptr) {
+  Sink sink{A.p, A.q, B.p, B.q, anchor, feat, mid, midStride, 0};
   const bool sA = A.shape == 1, sB = B.shape == 1;
   if (sA && sB) return sphereSphere(A, B, sink, basis);
   if (sA || sB) return sphereBox(A, B, sink, basis, sA);

@@ -52,16 +52,27 @@ __global__ void k_scan_block(const int *in, int *out, int n, int *blockSums) {
   if (threadIdx.x == 255) blockSums[blockIdx.x] = sh[255];
 }
 
-__global__ void k_scan_add(int *out, int n, const int *blockOffs) {
-  const int i = blockIdx.x * 1024 + threadIdx.x * 4;
-  const int add = blockOffs[blockIdx.x];
+// Second and last pass: each 1024-element block adds the sum of the blocks
+// before it, which its 256 threads add up themselves from the block sums (a
+// few hundred at most for the sizes here, so no third pass is needed); the
+// last block also writes the total.
+__global__ void k_scan_finish(int *out, int n, const int *blockSums, int nb, int *total) {
+  __shared__ int red[256];
+  const int b = blockIdx.x;
+  int s = 0;
+  for (int j = threadIdx.x; j < b; j += 256) s += blockSums[j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const int add = red[0];
+  if (b == nb - 1 && threadIdx.x == 0) *total = add + blockSums[b];
+  if (add == 0) return;
+  const int i = b * 1024 + threadIdx.x * 4;
   for (int k = 0; k < 4; k++)
     if (i + k < n) out[i + k] += add;
-}
-
-// total = out[n-1] + in[n-1]
-__global__ void k_scan_total(const int *in, const int *out, int n, int *total) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? out[n - 1] + in[n - 1] : 0;
 }
 
 // ------------------------------------------------------------- broad phase
@@ -371,21 +382,25 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     for (int k = 0; k < 9; k++) oldBasis[k] = basis[k];
   const double mu = xsqrt(pa.friction * pb.friction);
 
-  NewContact nc[MAXC];
-  const int n = collide(A, B, nc, basis, poly + threadIdx.x);
+  // the new contacts go straight to this manifold's output slots
+  const size_t nbase = size_t(t) * MAXC;
+  const int n = collide(A, B, nb.anchor + nbase * 6, nb.feat + nbase,
+                        reinterpret_cast<AliasDouble *>(nb.hot + nbase * HOT), int(HOT * sizeof(R) / sizeof(double)),
+                        basis, poly + threadIdx.x);
   const bool compatible =
       !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
 
   // merged per-contact state: anchors, k, lambda, C, stick
-  const size_t nbase = size_t(t) * MAXC;
   for (int i = 0; i < n; i++) {
-    double rA[3] = {nc[i].rA[0], nc[i].rA[1], nc[i].rA[2]};
-    double rB[3] = {nc[i].rB[0], nc[i].rB[1], nc[i].rB[2]};
+    const double *nca = nb.anchor + (nbase + i) * 6;
+    double rA[3] = {nca[0], nca[1], nca[2]};
+    double rB[3] = {nca[3], nca[4], nca[5]};
+    const uint32_t feature = nb.feat[nbase + i];
     double kk[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
     int stick = 0, src = -1;
     for (int j = 0; j < ncOld; j++) {
       const size_t os = size_t(m) * MAXC + j;
-      if (compatible && ob.feat[os] == nc[i].feature) {
+      if (compatible && ob.feat[os] == feature) {
         src = int(os);
         const R *oh = ob.hot + os * HOT;
         for (int k = 0; k < 3; k++) {
@@ -448,7 +463,6 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     h[15] = R(0);
     double *an = nb.anchor + s * 6;
     for (int k = 0; k < 3; k++) { an[k] = rA[k]; an[3 + k] = rB[k]; }
-    nb.feat[s] = nc[i].feature;
     nb.stick[s] = stick;
     srcSlot[s] = src;
     for (int k = 0; k < 3; k++) nb.Cd[s * 4 + k] = R(Cc[k]);
@@ -711,29 +725,46 @@ __device__ __forceinline__ bool claimsBefore(int a, int b, int order, int base, 
 // receives the number of rounds.
 __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart, const int *adjOther, int *color,
                               int *ctl, int order, const int *sceneStart, int *overflow, const int *rank) {
-  if (ctl[1]) return;
+  if (ctl[1]) return;  // converged in an earlier launch
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !(props[i].mass > 0)) return;
-  volatile int *vc = color;
-  const int base = sceneStart[props[i].scene];
-  unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
-  for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
-    const int j = adjOther[e];
-    if (j < 0 || !claimsBefore(j, i, order, base, rank)) continue;
-    const int c = vc[j];
-    if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
+  if (i < n && props[i].mass > 0) {
+    volatile int *vc = color;
+    const int base = sceneStart[props[i].scene];
+    unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
+    for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
+      const int j = adjOther[e];
+      if (j < 0 || !claimsBefore(j, i, order, base, rank)) continue;
+      const int c = vc[j];
+      if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
+    }
+    int c = MAX_COLORS;
+    for (int w = 0; w < MAX_COLORS / 64; w++)
+      if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
+    if (c >= MAX_COLORS) { atomicOr(overflow, 4); c = MAX_COLORS - 1; }
+    if (c != vc[i]) {
+      vc[i] = c;
+      atomicAdd(&ctl[0], 1);
+    }
   }
-  int c = MAX_COLORS;
-  for (int w = 0; w < MAX_COLORS / 64; w++)
-    if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
-  if (c >= MAX_COLORS) { atomicOr(overflow, 4); c = MAX_COLORS - 1; }
-  if (c != vc[i]) {
-    vc[i] = c;
-    atomicAdd(&ctl[0], 1);
+  // the last block to finish closes the round: it counts it, marks the
+  // colouring converged if nothing changed, and resets the counters
+  // (ctl[2] counts the finished blocks)
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&ctl[2], 1) == int(gridDim.x) - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    ctl[3] += 1;
+    if (atomicAdd(&ctl[0], 0) == 0) ctl[1] = 1;
+    ctl[0] = 0;
+    ctl[2] = 0;
   }
 }
 
-// After each round: stop once a round changed nothing.
 // An uploaded colouring, checked each step: every dynamic body has a colour
 // in range (else flag 1) and no force joins two bodies of one colour (else
 // flag 2), which the colour-parallel sweep requires.
@@ -762,12 +793,6 @@ __global__ void k_color_install(int n, const BodyProp *props, const int *src, in
   color[i] = props[i].mass > 0 ? src[i] : -1;
 }
 
-__global__ void k_color_check(int *ctl) {
-  if (threadIdx.x || blockIdx.x || ctl[1]) return;
-  ctl[3] += 1;
-  if (ctl[0] == 0) ctl[1] = 1;
-  ctl[0] = 0;
-}
 
 // Flags in the y word of a contact work item (see k_items).
 constexpr uint32_t ITEM_FIRST = 0x80000000u, ITEM_SPHERE = 0x40000000u, ITEM_BODY = 0x3fffffffu;
