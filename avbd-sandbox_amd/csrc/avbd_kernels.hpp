@@ -728,14 +728,28 @@ __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart,
   if (ctl[1]) return;  // converged in an earlier launch
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && props[i].mass > 0) {
-    volatile int *vc = color;
+    // plain loads: a colour read before another thread's change in this
+    // round is a past value, and that change makes the round count (only a
+    // round with no change ends the relaxation), so no ordering is needed
+    int *vc = color;
     const int base = sceneStart[props[i].scene];
     unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
-    for (int e = adjStart[i]; e < adjStart[i + 1]; e++) {
-      const int j = adjOther[e];
-      if (j < 0 || !claimsBefore(j, i, order, base, rank)) continue;
-      const int c = vc[j];
-      if (c >= 0 && c < MAX_COLORS) used[c >> 6] |= 1ull << (c & 63);
+    // neighbours in chunks of 8: the chunk's indices are requested together,
+    // then its colours, instead of one dependent pair of loads per neighbour
+    constexpr int CH = 8;
+    const int e1 = adjStart[i + 1];
+    for (int e0 = adjStart[i]; e0 < e1; e0 += CH) {
+      int j[CH], c[CH];
+#pragma unroll
+      for (int k = 0; k < CH; k++) j[k] = e0 + k < e1 ? adjOther[e0 + k] : -1;
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        if (j[k] >= 0 && !claimsBefore(j[k], i, order, base, rank)) j[k] = -1;
+        c[k] = j[k] >= 0 ? vc[j[k]] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; k++)
+        if (c[k] >= 0 && c[k] < MAX_COLORS) used[c[k] >> 6] |= 1ull << (c[k] & 63);
     }
     int c = MAX_COLORS;
     for (int w = 0; w < MAX_COLORS / 64; w++)
@@ -1037,7 +1051,7 @@ __device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
 // shape does not depend on the step's colouring.
 // Work-item words a lane fetches at once (one memory round trip for up to
 // PRIMAL_BATCH of its items).
-constexpr int PRIMAL_BATCH = 4;
+constexpr int PRIMAL_BATCH = 8;
 
 // One colour of the primal sweep (solver.js:419-458): a team of TEAM lanes per
 // body gathers its contacts and joints, reduces the 6x6 system and solves it.
