@@ -115,15 +115,28 @@ __global__ void k_bp_scatter(int n, const int4 *cellOf, uint32_t mask, int *curs
 
 // Is there a force between x and y? kind: 1 live joint, 2 manifold, 0 none.
 // Looks at x's force list (the CSR of the last step).
+// The list is read in chunks of 8 entries (one round trip for the chunk's
+// others and keys, not one per entry); matches are taken in list order.
 __device__ __forceinline__ int linkKind(int x, int y, const int *adjStart, const int *adjKey, const int *adjOther,
                                         int J, const JointDev *joints) {
-  for (int e = adjStart[x]; e < adjStart[x + 1]; e++) {
-    if (adjOther[e] != y) continue;
-    const int key = adjKey[e];
-    if (key < J) {
-      if (!joints[key].broken) return 1;
-    } else {
-      return 2;
+  constexpr int CH = 8;
+  const int e1 = adjStart[x + 1];
+  for (int e0 = adjStart[x]; e0 < e1; e0 += CH) {
+    int o[CH], key[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      const bool in = e0 + k < e1;
+      o[k] = in ? adjOther[e0 + k] : -1;
+      key[k] = in ? adjKey[e0 + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      if (o[k] != y) continue;
+      if (key[k] < J) {
+        if (!joints[key[k]].broken) return 1;
+      } else {
+        return 2;
+      }
     }
   }
   return 0;
@@ -390,6 +403,10 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
   const bool compatible =
       !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
 
+  // the old contacts' feature keys, read once for every new contact's match
+  uint32_t ofeat[MAXC];
+#pragma unroll
+  for (int j = 0; j < MAXC; j++) ofeat[j] = j < ncOld ? ob.feat[size_t(m) * MAXC + j] : 0u;
   // merged per-contact state: anchors, k, lambda, C, stick
   for (int i = 0; i < n; i++) {
     const double *nca = nb.anchor + (nbase + i) * 6;
@@ -398,9 +415,14 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     const uint32_t feature = nb.feat[nbase + i];
     double kk[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
     int stick = 0, src = -1;
-    for (int j = 0; j < ncOld; j++) {
-      const size_t os = size_t(m) * MAXC + j;
-      if (compatible && ob.feat[os] == feature) {
+    // the first old contact with this feature key (manifold.js:241-286)
+    int match = -1;
+#pragma unroll
+    for (int j = MAXC - 1; j >= 0; j--)
+      if (j < ncOld && ofeat[j] == feature) match = j;
+    if (compatible && match >= 0) {
+      const size_t os = size_t(m) * MAXC + match;
+      {
         src = int(os);
         const R *oh = ob.hot + os * HOT;
         for (int k = 0; k < 3; k++) {
@@ -434,7 +456,6 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
           for (int k = 0; k < 3; k++) { rA[k] = an[k]; rB[k] = an[3 + k]; }
           src |= 0x40000000;  // anchors kept
         }
-        break;
       }
     }
     // C0 at x_t and the warm start (manifold.js:290-315)
