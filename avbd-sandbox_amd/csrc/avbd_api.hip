@@ -96,6 +96,7 @@ struct Phase {
 struct avbd_ctx {
   std::string err;
   int device = 0, prec = AVBD_PREC_MIXED, team = 16;
+  int simds = 1024;  // SIMDs of the device (4 per CU), read at creation
   int manCapCfg = 0, pairCapCfg = 0;
   hipStream_t stream = nullptr;
   avbd_params prm{};
@@ -454,6 +455,18 @@ struct avbd_ctx {
                 (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
   }
 
+  // Team shape for colour c: a colour too small to give every SIMD two waves
+  // at the tuned shape gets wider teams (fewer contacts per lane, so a shorter
+  // chain of gathers per body); the smallest colours, one launch each like
+  // the large ones, are where the sweep waits longest per body.
+  int teamFor(int c) const {
+    int tm = team;
+    if (tm < 4) return tm;
+    const long cnt = hcstart[c + 1] - hcstart[c];
+    while (tm < 64 && cnt * long(tm) * 2 <= 2L * simds * 64) tm *= 2;
+    return tm;
+  }
+
   // One colour of the sweep; dual != 0 folds in the previous iteration's
   // contact dual update (evaluated with dualAlpha).
   template <class R, bool EXACT>
@@ -486,7 +499,7 @@ struct avbd_ctx {
       const double alpha = alphaOf(it);
       const int fused = it >= 1 && it < prm.iterations ? 1 : 0;
       const double dualAlpha = fused ? alphaOf(it - 1) : alpha;
-      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, team, fused, dualAlpha);
+      for (int c = 0; c < numColors; c++) primal<R, EXACT>(c, sa, mb, P, alpha, teamFor(c), fused, dualAlpha);
       // recording: the poses after this sweep, and the contacts' k, lambda as
       // this sweep used them (the folded update of the previous iteration has
       // reached every contact once this sweep is done)
@@ -1199,6 +1212,9 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     delete c;
     return AVBD_E_HIP;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+    c->simds = 4 * cus;
   *out = c;
   return AVBD_OK;
 }
