@@ -1113,6 +1113,25 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
     if constexpr (!EXACT)
       if (lane == 0)
         for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
+    // lane 0's inertial terms (f64, team modes): M/dt^2 (rotated inertia) and
+    // M/dt^2 (x - y). Computed while the first contact records are in flight
+    // (the wave would otherwise only wait for them), or after the gathers for
+    // a body without contacts.
+    double inMl = 0, inAng[6], inRl[3], inRa[3];
+    bool inDone = false;
+    auto inertia = [&]() {
+      double MAng[9], Ma9[9], dq[3];
+      worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+      const double idt2 = frcp(dt2);
+      inMl = bp.mass * idt2;
+      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] * idt2;
+      qsub(dq, xs + 3, ys + 3);
+      mulv(inRa, Ma9, dq);
+      inAng[0] = Ma9[0]; inAng[1] = Ma9[3]; inAng[2] = Ma9[4];
+      inAng[3] = Ma9[6]; inAng[4] = Ma9[7]; inAng[5] = Ma9[8];
+      for (int k = 0; k < 3; k++) inRl[k] = inMl * (xs[k] - ys[k]);
+      inDone = true;
+    };
     Sys<Acc> sl;
     sl.zero();
     if constexpr (EXACT) {
@@ -1151,6 +1170,8 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
         vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
         vload<HOT>(hot, mb.hot + si * HOT);
         vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
+        if constexpr (!EXACT && sizeof(R) == 4)  // f64 records leave no registers for it
+          if (k == 0 && lane == 0 && !inDone) inertia();
         if constexpr (!EXACT) {
           const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
           R rW[3];
@@ -1221,17 +1242,10 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
     }
     if (lane != 0) continue;
     if constexpr (!EXACT) {
-      double MAng[9], Ma9[9], t[3] = {xs[0] - ys[0], xs[1] - ys[1], xs[2] - ys[2]}, dq[3], ra[3];
-      worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-      const double idt2 = frcp(dt2);
-      const double ml = bp.mass * idt2;
-      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] * idt2;
-      qsub(dq, xs + 3, ys + 3);
-      mulv(ra, Ma9, dq);
-      s.lin[0] += ml; s.lin[2] += ml; s.lin[5] += ml;
-      s.ang[0] += Ma9[0]; s.ang[1] += Ma9[3]; s.ang[2] += Ma9[4];
-      s.ang[3] += Ma9[6]; s.ang[4] += Ma9[7]; s.ang[5] += Ma9[8];
-      for (int k = 0; k < 3; k++) { s.rl[k] += ml * t[k]; s.ra[k] += ra[k]; }
+      if (!inDone) inertia();
+      s.lin[0] += inMl; s.lin[2] += inMl; s.lin[5] += inMl;
+      for (int k = 0; k < 6; k++) s.ang[k] += inAng[k];
+      for (int k = 0; k < 3; k++) { s.rl[k] += inRl[k]; s.ra[k] += inRa[k]; }
       if (w.w > 0)
         for (int t = b0; t < b1; t++) {
           const uint2 itw = S.items[t];
