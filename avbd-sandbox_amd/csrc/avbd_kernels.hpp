@@ -395,6 +395,11 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     for (int k = 0; k < 9; k++) oldBasis[k] = basis[k];
   const double mu = xsqrt(pa.friction * pb.friction);
 
+  // the old contacts' feature keys, read once for every new contact's match
+  // (requested before the collision test, which hides their latency)
+  uint32_t ofeat[MAXC];
+#pragma unroll
+  for (int j = 0; j < MAXC; j++) ofeat[j] = j < ncOld ? ob.feat[size_t(m) * MAXC + j] : 0u;
   // the new contacts go straight to this manifold's output slots
   const size_t nbase = size_t(t) * MAXC;
   const int n = collide(A, B, nb.anchor + nbase * 6, nb.feat + nbase,
@@ -403,10 +408,6 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
   const bool compatible =
       !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
 
-  // the old contacts' feature keys, read once for every new contact's match
-  uint32_t ofeat[MAXC];
-#pragma unroll
-  for (int j = 0; j < MAXC; j++) ofeat[j] = j < ncOld ? ob.feat[size_t(m) * MAXC + j] : 0u;
   // merged per-contact state: anchors, k, lambda, C, stick
   for (int i = 0; i < n; i++) {
     const double *nca = nb.anchor + (nbase + i) * 6;
