@@ -431,9 +431,9 @@ struct avbd_ctx {
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
 
-  // Launch shapes of the solve loop depend only on the body count, the colour
-  // count and the team: the kernels take their counts from device memory and
-  // stride, so one captured graph serves every step with that colour count.
+  // The solve loop's kernels take their counts from device memory and stride,
+  // so one captured graph serves every step with that colour count; the
+  // launch shapes are chosen from the colour sizes at capture.
   // A team per body of colour c, so that no team sweeps two bodies one after
   // the other (colours differ in size: the pyramid's first colours hold 1.7x
   // the average). A captured graph is replayed on later steps whose colours

@@ -1069,8 +1069,9 @@ __device__ __forceinline__ void loadRow(R *o, const R *p, int n) {
 // 6x6 system in f64 and apply Eq. 21. TEAM lanes share a body: each stamps a
 // strided share of its items and the team tree-reduces; TEAM = 1 with EXACT
 // stamps in the reference's order onto the inertial terms. The colour's body
-// range comes from device memory and teams stride over it, so the launch
-// shape does not depend on the step's colouring.
+// range comes from device memory and teams stride over it, so any launch
+// shape sweeps the whole colour (a graph captured at one step serves later
+// steps whose colours have grown).
 // Work-item words a lane fetches at once (one memory round trip for up to
 // PRIMAL_BATCH of its items).
 constexpr int PRIMAL_BATCH = 8;
