@@ -135,6 +135,7 @@ struct avbd_ctx {
 
   // topology
   DBuf<int> deg, adjStart, adjCursor, adjKey, adjOther;
+  DBuf<int> adjKeyFill, adjOtherFill, adjOwner;  // lists as filled, before the key order
   DBuf<int> color[2];
   int colorCur = 0;
   DBuf<int> cstart, cbodies, cmeta;
@@ -293,6 +294,7 @@ struct avbd_ctx {
     countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
+    adjKeyFill.release(); adjOtherFill.release(); adjOwner.release();
     color[0].release(); color[1].release(); cstart.release(); cbodies.release(); cwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
@@ -359,14 +361,17 @@ struct avbd_ctx {
     const int4 *head = store[cur].head.p;
     if (nf > 0)
       hipLaunchKernelGGL(k_adj<0>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
-                         deg.p, adjCursor.p, adjKey.p, adjOther.p);
+                         deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
     scan(deg.p, adjStart.p, N, adjStart.p + N);
     HIPCHECK(hipMemcpyAsync(adjCursor.p, adjStart.p, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
-    if (nf > 0)
+    if (nf > 0) {
       hipLaunchKernelGGL(k_adj<1>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
-                         deg.p, adjCursor.p, adjKey.p, adjOther.p);
-    hipLaunchKernelGGL(k_adj_sort, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjKey.p,
-                       adjOther.p);
+                         deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
+      // at most two entries per force
+      hipLaunchKernelGGL(k_adj_rank, dim3(blocks(2L * nf)), dim3(256), 0, stream, 2 * nf, N, props.p, adjStart.p,
+                         (const int *)adjOwner.p, (const int *)adjKeyFill.p, (const int *)adjOtherFill.p, adjKey.p,
+                         adjOther.p);
+    }
   }
 
   void buildItems() {
@@ -1135,6 +1140,9 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.deg.reserve(N); c.adjStart.reserve(N + 1); c.adjCursor.reserve(N);
   c.adjKey.reserve(2 * (size_t(c.manCap) + c.J) + 1);
   c.adjOther.reserve(2 * (size_t(c.manCap) + c.J) + 1);
+  c.adjKeyFill.reserve(2 * (size_t(c.manCap) + c.J) + 1);
+  c.adjOtherFill.reserve(2 * (size_t(c.manCap) + c.J) + 1);
+  c.adjOwner.reserve(2 * (size_t(c.manCap) + c.J) + 1);
   c.color[0].reserve(N); c.color[1].reserve(N);
   std::vector<int> col(N);
   for (int i = 0; i < N; i++) col[i] = P[i].mass > 0 ? 0 : -1;

@@ -670,7 +670,7 @@ __global__ void k_live_compact(int nTot, const int *keep, const int *pos, int *l
 // ascending keys are the force list's creation order.
 template <int PASS>
 __global__ void k_adj(int nLive, int J, const int *live, const int4 *head, const JointDev *joints, int *deg,
-                      int *cursor, int *adjKey, int *adjOther) {
+                      int *cursor, int *adjKey, int *adjOther, int *owner) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   int a, b, key;
   if (t < J) {
@@ -688,29 +688,38 @@ __global__ void k_adj(int nLive, int J, const int *live, const int4 *head, const
   } else {
     if (a >= 0) {
       const int p = atomicAdd(&cursor[a], 1);
-      adjKey[p] = key; adjOther[p] = b;
+      adjKey[p] = key; adjOther[p] = b; owner[p] = a;
     }
     const int p = atomicAdd(&cursor[b], 1);
-    adjKey[p] = key; adjOther[p] = a;
+    adjKey[p] = key; adjOther[p] = a; owner[p] = b;
   }
 }
 
-// Sort each dynamic body's list by key (insertion sort; lists are short).
-__global__ void k_adj_sort(int n, const BodyProp *props, const int *adjStart, int *adjKey, int *adjOther) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !(props[i].mass > 0)) return;
-  const int s0 = adjStart[i], s1 = adjStart[i + 1];
-  for (int e = s0 + 1; e < s1; e++) {
-    const int k = adjKey[e], o = adjOther[e];
-    int f = e;
-    while (f > s0 && adjKey[f - 1] > k) {
-      adjKey[f] = adjKey[f - 1];
-      adjOther[f] = adjOther[f - 1];
-      f--;
+// Each dynamic body's list in key order, one thread per entry: an entry's
+// place is the number of smaller keys in its list (keys are distinct within a
+// list), read eight at a time. Static bodies' lists keep their fill order
+// (the ground's can hold thousands of entries).
+__global__ void k_adj_rank(int maxEntries, int n, const BodyProp *props, const int *adjStart, const int *owner,
+                           const int *keyIn, const int *otherIn, int *keyOut, int *otherOut) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= maxEntries || p >= adjStart[n]) return;
+  const int b = owner[p];
+  const int s0 = adjStart[b], s1 = adjStart[b + 1];
+  const int key = keyIn[p];
+  int rank = p - s0;
+  if (props[b].mass > 0) {
+    rank = 0;
+    constexpr int CH = 8;
+    for (int f0 = s0; f0 < s1; f0 += CH) {
+      int kk[CH];
+#pragma unroll
+      for (int k = 0; k < CH; k++) kk[k] = f0 + k < s1 ? keyIn[f0 + k] : key;
+#pragma unroll
+      for (int k = 0; k < CH; k++) rank += kk[k] < key;
     }
-    adjKey[f] = k;
-    adjOther[f] = o;
   }
+  keyOut[s0 + rank] = key;
+  otherOut[s0 + rank] = otherIn[p];
 }
 
 // Greedy colouring (coloring.js:75-92): colour(i) is the smallest colour not
