@@ -967,25 +967,38 @@ __global__ void k_items(int n, int J, const BodyProp *props, const int *adjStart
     itemJoints[i] = 0;
     return;
   }
-  for (int e = adjStart[i + 1] - 1; e >= adjStart[i]; e--) {
-    const int key = adjKey[e];
-    if (key < J) {
-      if (PASS == 1) {
-        const bool isA = joints[key].a == i;
-        items[base + cnt] = make_uint2(0x80000000u | (uint32_t(key) << 1) | (isA ? 1u : 0u),
-                                       uint32_t(isA ? joints[key].b : joints[key].a));
+  // forces newest first, in chunks of 8: the chunk's keys, then their
+  // manifolds and heads, each in one round trip
+  constexpr int CH = 8;
+  const int e0 = adjStart[i];
+  for (int eTop = adjStart[i + 1] - 1; eTop >= e0; eTop -= CH) {
+    int key[CH], m[CH];
+    int4 h[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) key[k] = eTop - k >= e0 ? adjKey[eTop - k] : -1;
+#pragma unroll
+    for (int k = 0; k < CH; k++) m[k] = key[k] >= J ? live[key[k] - J] : -1;
+#pragma unroll
+    for (int k = 0; k < CH; k++) h[k] = m[k] >= 0 ? head[m[k]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      if (key[k] < 0) break;
+      if (key[k] < J) {
+        if (PASS == 1) {
+          const bool isA = joints[key[k]].a == i;
+          items[base + cnt] = make_uint2(0x80000000u | (uint32_t(key[k]) << 1) | (isA ? 1u : 0u),
+                                         uint32_t(isA ? joints[key[k]].b : joints[key[k]].a));
+        }
+        cnt++;
+        nj++;
+      } else {
+        if (PASS == 1)
+          for (int s = 0; s < h[k].z; s++)
+            items[base + cnt + s] =
+                make_uint2((uint32_t(m[k]) << 4) | (uint32_t(s) << 1) | (h[k].x == i ? 1u : 0u),
+                           uint32_t(h[k].x == i ? h[k].y : h[k].x) | ((h[k].w & MF_SPHERE) ? ITEM_SPHERE : 0u));
+        cnt += h[k].z;
       }
-      cnt++;
-      nj++;
-    } else {
-      const int m = live[key - J];
-      const int4 h = head[m];
-      if (PASS == 1)
-        for (int s = 0; s < h.z; s++)
-          items[base + cnt + s] =
-              make_uint2((uint32_t(m) << 4) | (uint32_t(s) << 1) | (h.x == i ? 1u : 0u),
-                         uint32_t(h.x == i ? h.y : h.x) | ((h.w & MF_SPHERE) ? ITEM_SPHERE : 0u));
-      cnt += h.z;
     }
   }
   if (PASS == 0) itemCnt[i] = cnt;
