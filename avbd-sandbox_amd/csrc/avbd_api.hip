@@ -197,7 +197,7 @@ struct avbd_ctx {
   int lastNTot = 0;
   std::vector<Checkpoint> ckpts;
   Checkpoint endState;
-  int adjSteps = 0, adjEvery = 0, adjSmallMax = 4096;
+  int adjSteps = 0, adjEvery = 0, adjSmallMax = 128;
   DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
@@ -1208,7 +1208,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->team = cfg->team > 0 ? cfg->team : 16;
     c->manCapCfg = cfg->manifold_cap;
     c->graphsCfg = cfg->graphs;
-    c->adjSmallMax = cfg->adjoint_small == 0 ? 4096 : cfg->adjoint_small;
+    c->adjSmallMax = cfg->adjoint_small == 0 ? 128 : cfg->adjoint_small;
     c->pairCapCfg = cfg->pair_cap;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {
