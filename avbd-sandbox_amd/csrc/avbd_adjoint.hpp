@@ -320,34 +320,12 @@ __device__ inline void adjDualContact(const double *B, size_t si, const double *
   for (int k = 0; k < 3; k++) c0b[k] = c0v[k];
 }
 
-// The dual update of iteration l, one thread per live manifold.
-__device__ inline void adjDual(int t, const AdjStep &T, const AdjBufs &A, int l, const StepParams &P) {
-  const int m = T.live[t];
-  const int4 h = T.head[m];
-  const double *B = T.basis + size_t(m) * BAS;
-  const double *X = T.X + T.xStride * (l + 1);
-  const double *xA = X + size_t(h.x) * XS, *xB = X + size_t(h.y) * XS;
-  const double *x0A = T.X0 + size_t(h.x) * XS, *x0B = T.X0 + size_t(h.y) * XS;
-  double dqAb[6] = {0, 0, 0, 0, 0, 0}, dqBb[6] = {0, 0, 0, 0, 0, 0};
-  double mub = 0;
-  double dqA[6], dqB[6];
-  bodyDelta(xA, x0A, dqA);
-  bodyDelta(xB, x0B, dqB);
-  for (int s = 0; s < h.z; s++) {
-    const size_t si = size_t(m) * MAXC + s;
-    adjDualContact(B, si, T.hot + T.hotStride * l + si * HOT, A, dqA, dqB, P, dqAb, dqBb, mub);
-  }
-  A.MUb[m] += mub;
-  deltaVjpAtomic(xA, x0A, dqAb, A.Xb + size_t(h.x) * XS, A.X0b + size_t(h.x) * XS);
-  deltaVjpAtomic(xB, x0B, dqBb, A.Xb + size_t(h.y) * XS, A.X0b + size_t(h.y) * XS);
-}
-
 // The primal update of one colour in iteration l (solver.js:419-458), a team
 // of TEAM lanes per body: the pose adjoint of each body turns from that after
 // its update into that before it; the neighbours, the contacts and the step's
 // start and inertial target receive their shares.
 //
-// FOLD: the team also applies, first, the dual adjoint of iteration l for the
+// The team also applies, first, the dual adjoint of iteration l for the
 // contacts of which its body is the reverse sweep's first visitor (the other
 // body has a lower colour or is static). The forward's dual of iteration l
 // follows the whole sweep, so its adjoint must precede the primal adjoints of
@@ -355,7 +333,7 @@ __device__ inline void adjDual(int t, const AdjStep &T, const AdjBufs &A, int l,
 // visitor's team is such a place. Each contact is handled by the lane that
 // holds its item, in both parts, so that its LK slot is read and written in
 // program order; the body's own share stays in registers.
-template <int TEAM, bool FOLD>
+template <int TEAM>
 __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T, const AdjBufs &A,
                                      const BodyProp *props, int l, const StepParams &P) {
   const int c0 = T.cstart[c];
@@ -373,9 +351,9 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
       const int cj = T.color[j];
       return (cj >= 0 && cj < c ? Xaft : Xbef) + size_t(j) * XS;
     };
-    // 0. (FOLD) the dual update's adjoint at the poses after the sweep
+    // 0. the dual update's adjoint at the poses after the sweep
     double dqDual[6] = {0, 0, 0, 0, 0, 0};
-    if constexpr (FOLD) {
+    {
       const double *xa = Xaft + size_t(i) * XS;
       for (int t = b0 + lane; t < b1; t += TEAM) {
         const uint2 iw = T.items[t];
@@ -458,7 +436,7 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
     const double *xbv = A.Xb + size_t(i) * XS;
     double pb1[3] = {xbv[0], xbv[1], xbv[2]}, qb1[4] = {xbv[3], xbv[4], xbv[5], xbv[6]};
     double q0Dual[4] = {0, 0, 0, 0};
-    if constexpr (FOLD) {
+    {
       // the body's own share of its dual updates, as deltaVjpAtomic would add it
       double qbD[4] = {0, 0, 0, 0};
       qsubVjp(Xaft + size_t(i) * XS + 3, x0 + 3, dqDual + 3, qbD, q0Dual);
@@ -553,8 +531,7 @@ __device__ inline void adjPrimalTeam(int c, int team, int lane, const AdjStep &T
       x0v[k] -= dqSelf[k] + dqDual[k];
       yv[k] += ml * w[k];
     }
-    if constexpr (FOLD)
-      for (int k = 0; k < 4; k++) x0v[3 + k] += q0Dual[k];
+    for (int k = 0; k < 4; k++) x0v[3 + k] += q0Dual[k];
     {
       double q0b[4] = {0, 0, 0, 0};
       qsubVjp(xq, x0q, dqSelf + 3, qb, q0b);
@@ -678,17 +655,13 @@ __global__ void k_adj_velocity(int n, const BodyProp *props, AdjStep T, AdjBufs 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) adjVelocity(i, props, T, A, iters, dt);
 }
-__global__ void __launch_bounds__(128) k_adj_dual(AdjStep T, AdjBufs A, int l, StepParams P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < T.nLive) adjDual(t, T, A, l, P);
-}
 template <int TEAM>
 __global__ void __launch_bounds__(256) k_adj_primal(int c, AdjStep T, AdjBufs A, const BodyProp *props, int l,
                                                     StepParams P) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
   const int cnt = T.cstart[c + 1] - T.cstart[c];
-  for (int team = gt / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM, false>(c, team, lane, T, A, props, l, P);
+  for (int team = gt / TEAM; team < cnt; team += teams) adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
 }
 __global__ void k_adj_narrow(AdjStep T, AdjBufs A, const BodyProp *props, StepParams P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -700,11 +673,10 @@ __global__ void k_adj_init(int n, const BodyProp *props, AdjStep T, AdjBufs A, S
 }
 
 // The whole reverse of one step in a single workgroup, for small scenes where
-// the per-kernel launches would dominate: the phases of the per-kernel path,
-// with each dual adjoint folded into the colour phases (one barrier per colour
-// and iteration instead of one more per iteration), separated by workgroup
-// barriers (one CU, so global writes of one phase are visible to the next),
-// then the per-step contact adjoints cleared. One wave per SIMD: the phases are
+// the per-kernel launches would dominate: the phases of the per-kernel path
+// (one per colour and iteration, the dual adjoints folded in), separated by
+// workgroup barriers (one CU, so global writes of one phase are visible to
+// the next), then the per-step contact adjoints cleared. One wave per SIMD: the phases are
 // chains of dependent f64 work with little parallelism, and a wave of its own
 // per SIMD has the whole register file (accumulation registers included)
 // where two would share it and spill to memory.
@@ -722,7 +694,7 @@ __global__ void __launch_bounds__(ADJ_SMALL_THREADS) k_adj_step_small(int n, int
     for (int c = T.numColors - 1; c >= 0; c--) {
       const int cnt = T.cstart[c + 1] - T.cstart[c];
       for (int team = tid / TEAM; team < cnt; team += teams)
-        adjPrimalTeam<TEAM, true>(c, team, lane, T, A, props, l, P);
+        adjPrimalTeam<TEAM>(c, team, lane, T, A, props, l, P);
       __syncthreads();
     }
   }

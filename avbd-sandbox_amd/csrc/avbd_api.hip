@@ -197,7 +197,7 @@ struct avbd_ctx {
   int lastNTot = 0;
   std::vector<Checkpoint> ckpts;
   Checkpoint endState;
-  int adjSteps = 0, adjEvery = 0, adjSmallMax = 128;
+  int adjSteps = 0, adjEvery = 0, adjSmallMax = 64;
   DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
@@ -885,9 +885,8 @@ struct avbd_ctx {
     hipLaunchKernelGGL(k_adj_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, I,
                        prm.dt);
     const int gb = std::max(1, std::min(blocks(long(std::max(nDynamic, 1)) * (wide ? 16 : 4)), 2048));
+    // each iteration's dual adjoint rides with its first colour in reverse
     for (int l = I - 1; l >= 0; l--) {
-      if (T.nLive > 0)
-        hipLaunchKernelGGL(k_adj_dual, dim3(blocks(T.nLive, 128)), dim3(128), 0, stream, S, A, l, P);
       for (int c = T.numColors - 1; c >= 0; c--) {
         if (wide)
           hipLaunchKernelGGL(k_adj_primal<16>, dim3(gb), dim3(256), 0, stream, c, S, A, (const BodyProp *)props.p, l,
@@ -1208,7 +1207,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->team = cfg->team > 0 ? cfg->team : 16;
     c->manCapCfg = cfg->manifold_cap;
     c->graphsCfg = cfg->graphs;
-    c->adjSmallMax = cfg->adjoint_small == 0 ? 128 : cfg->adjoint_small;
+    c->adjSmallMax = cfg->adjoint_small == 0 ? 64 : cfg->adjoint_small;
     c->pairCapCfg = cfg->pair_cap;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {

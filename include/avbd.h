@@ -66,7 +66,7 @@ typedef struct avbd_config {
   int32_t graphs;          /* solve loop as a replayed hipGraph: 0 auto (on
                               unless profiling), -1 never                   */
   int32_t adjoint_small;   /* largest dynamic-body count whose adjoint step
-                              is reversed in one workgroup (0: 128; -1:
+                              is reversed in one workgroup (0: 64; -1:
                               never, always one kernel per phase)           */
   int32_t reserved;
 } avbd_config;

@@ -98,8 +98,8 @@ def test_adjoint_forward_equals_stepping_and_rejects_unsupported():
 
 def test_adjoint_paths_agree_on_colliding_boxes():
     # the full ramp benchmark (boxes run into each other): both reverse paths
-    # give the same gradients up to the order of atomic additions (the
-    # single-workgroup one folds each dual adjoint into the colour phases)
+    # give the same gradients up to the order of atomic additions (one
+    # workgroup with barriers between phases, or a kernel per phase)
     scene, params, meta = avbd.scenes.ramp(16)
     grads = []
     for small in (0, -1):
