@@ -36,17 +36,20 @@ HBM_PEAK_GBS = 8000.0
 def measured_traffic(precision, team):
     """HBM bytes per k_primal launch (read + written) from the committed
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the same workload,
-    scaled by the copy-probe calibration; None when no summary matches."""
+    scaled by the copy-probe calibration: the dispatch-weighted mean over the
+    team shapes the sweep used (small colours get wider teams); None when no
+    summary matches."""
     path = os.path.join(ROOT, "profiles", f"k_primal_pyramid51k_{precision}_traffic.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         doc = json.load(f)
-    k = doc["kernels"].get(f"k_primal<{'float' if precision == 'mixed' else 'double'}, "
-                           f"{'true' if precision == 'exact' else 'false'}, {team}>")
-    if doc.get("team") != team or k is None:
+    prefix = f"k_primal<{'float' if precision == 'mixed' else 'double'}, {'true' if precision == 'exact' else 'false'}, "
+    ks = [k for name, k in doc["kernels"].items() if name.startswith(prefix)]
+    n = sum(k["dispatches"] for k in ks)
+    if doc.get("team") != team or n == 0:
         return None, None
-    return k["read_bytes"] + k["write_bytes"], os.path.relpath(path, ROOT)
+    return sum((k["read_bytes"] + k["write_bytes"]) * k["dispatches"] for k in ks) / n, os.path.relpath(path, ROOT)
 
 
 def copy_probe(mib):
