@@ -463,12 +463,15 @@ struct avbd_ctx {
   // Team shape for colour c: a colour too small to give every SIMD two waves
   // at the tuned shape gets wider teams (fewer contacts per lane, so a shorter
   // chain of gathers per body); the smallest colours, one launch each like
-  // the large ones, are where the sweep waits longest per body.
+  // the large ones, are where the sweep waits longest per body. Only while
+  // each lane keeps two work items on average: below that a wider team only
+  // idles its lanes and lengthens its reduction.
   int teamFor(int c) const {
     int tm = team;
     if (tm < 4) return tm;
     const long cnt = hcstart[c + 1] - hcstart[c];
-    while (tm < 64 && cnt * long(tm) * 2 <= 2L * simds * 64) tm *= 2;
+    const double perBody = double(2L * contactsTotal + 2L * J) / double(std::max(nDynamic, 1));
+    while (tm < 64 && cnt * long(tm) * 2 <= 2L * simds * 64 && tm * 2 <= perBody) tm *= 2;
     return tm;
   }
 
