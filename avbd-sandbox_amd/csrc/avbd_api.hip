@@ -464,8 +464,9 @@ struct avbd_ctx {
   // at the tuned shape gets wider teams (fewer contacts per lane, so a shorter
   // chain of gathers per body); the smallest colours, one launch each like
   // the large ones, are where the sweep waits longest per body. Only while
-  // each lane keeps two work items on average: below that a wider team only
-  // idles its lanes and lengthens its reduction.
+  // each lane of the wider team keeps at least one work item on average (a
+  // contact or joint is one item for each dynamic body it touches): beyond
+  // that a wider team only idles its lanes and lengthens its reduction.
   int teamFor(int c) const {
     int tm = team;
     if (tm < 4) return tm;
