@@ -2,3 +2,4 @@
 from ._lib import (PREC_EXACT, PREC_F64, PREC_MIXED, SYMBOLS, LIB_PATH, lib)  # noqa: F401
 from .backend import AvbdError, HipBackend, scene_arrays  # noqa: F401
 from . import scenes  # noqa: F401
+from . import sceneio  # noqa: F401

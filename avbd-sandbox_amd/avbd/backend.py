@@ -151,6 +151,31 @@ class HipBackend:
         self._ck(self.L.avbd_read_state(self.ctx, _dp(pos), _dp(quat), _dp(vel), _dp(angvel), _dp(prev), n))
         return dict(pos=pos, quat=quat, vel=vel, angvel=angvel, prev_vel=prev)
 
+    def export_poses(self, alpha=1.0, out=None):
+        """Per body {p, 0, q} as float32 (n, 8) at a fraction alpha of the
+        last step (renderer.js:1327-1383). `out` may be a host float32 array
+        of that shape, or a CUDA tensor of it (written on the device, in the
+        context's stream order: call idle() before reading it elsewhere)."""
+        n = self.n
+        if out is not None and hasattr(out, "data_ptr") and getattr(out, "is_cuda", False):
+            if tuple(out.shape) != (n, 8) or str(out.dtype) != "torch.float32" or not out.is_contiguous():
+                raise AvbdError(L.AVBD_E_INVALID, "out must be a contiguous float32 (n, 8) tensor")
+            self._ck(self.L.avbd_export_poses(self.ctx, float(alpha), C.c_void_p(out.data_ptr()), n, 1))
+            return out
+        if out is None:
+            out = np.zeros((n, 8), np.float32)
+        if out.dtype != np.float32 or out.shape != (n, 8) or not out.flags["C_CONTIGUOUS"]:
+            raise AvbdError(L.AVBD_E_INVALID, "out must be a contiguous float32 (n, 8) array")
+        self._ck(self.L.avbd_export_poses(self.ctx, float(alpha), out.ctypes.data_as(C.c_void_p), n, 0))
+        return out
+
+    def pose_source(self):
+        """(pose, start_pose, stride_bytes): device addresses of the current
+        and step-start f64 pose records {p[3], q[4], pad}."""
+        a, b, st = C.c_void_p(), C.c_void_p(), C.c_int32()
+        self._ck(self.L.avbd_pose_source(self.ctx, C.byref(a), C.byref(b), C.byref(st)))
+        return a.value, b.value, st.value
+
     def write_state(self, pos=None, quat=None, vel=None, angvel=None, prev_vel=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in
                 (pos, quat, vel, angvel, prev_vel)]

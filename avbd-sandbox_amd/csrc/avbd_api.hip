@@ -113,6 +113,7 @@ struct avbd_ctx {
   DBuf<BodyProp> props;
   DBuf<double> X, X0, Y, V;
   DBuf<char> DQ;
+  DBuf<float4> poseStage;  // avbd_export_poses to host memory
   std::vector<BodyProp> hprops;
 
   // broad phase
@@ -282,7 +283,7 @@ struct avbd_ctx {
   void releaseAll() {
     dropGraphs();
     Xbk.release(); DQbk.release();
-    props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release();
+    props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release(); poseStage.release();
     cbytes.release();
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     pending.clear();
@@ -1292,6 +1293,51 @@ int avbd_read_state(avbd_ctx *ctx, double *pos, double *quat, double *vel, doubl
       }
       if (quat) for (int k = 0; k < 4; k++) quat[i * 4 + k] = x[3 + k];
     }
+  });
+}
+
+int avbd_export_poses(avbd_ctx *ctx, double alpha, float *dst, int32_t n, int32_t dst_on_device) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    if (!dst) throw ApiError(AVBD_E_INVALID, "dst is NULL");
+    // Math.min(1, Math.max(0, alpha)), and a NaN (which fails a < 1) draws
+    // the current pose, as the renderer does (renderer.js:1329-1330)
+    const double a = std::isnan(alpha) ? 1.0 : std::min(1.0, std::max(0.0, alpha));
+    if (n == 0) return;
+    float4 *out;
+    if (dst_on_device) {
+      hipPointerAttribute_t attr;
+      if (hipPointerGetAttributes(&attr, dst) != hipSuccess) {
+        (void)hipGetLastError();
+        throw ApiError(AVBD_E_INVALID, "dst is not device memory");
+      }
+      if (attr.type != hipMemoryTypeDevice || attr.device != ctx->device)
+        throw ApiError(AVBD_E_INVALID, "dst is not device memory on the context's device");
+      if (reinterpret_cast<uintptr_t>(dst) % 16 != 0) throw ApiError(AVBD_E_INVALID, "dst must be 16-byte aligned");
+      out = reinterpret_cast<float4 *>(dst);
+    } else {
+      ctx->poseStage.reserve(size_t(n) * 2);
+      out = ctx->poseStage.p;
+    }
+    // algorithmic bytes: p and q of one or two poses read (56 B each), 32 B written
+    const int lerp = a < 1.0 ? 1 : 0;
+    ctx->timedLaunch(AVBD_K_POSES, double(n) * (lerp ? 144 : 88), k_export_poses, dim3(blocks(n)), dim3(256), n, a,
+                     lerp, reinterpret_cast<const double2 *>(ctx->X.p),
+                     reinterpret_cast<const double2 *>(ctx->X0.p), out);
+    HIPCHECK(hipGetLastError());
+    if (!dst_on_device) {
+      HIPCHECK(hipMemcpyAsync(dst, out, sizeof(float4) * 2 * size_t(n), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
+int avbd_pose_source(avbd_ctx *ctx, const double **pose, const double **start_pose, int32_t *stride_bytes) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (pose) *pose = ctx->X.p;
+    if (start_pose) *start_pose = ctx->X0.p;
+    if (stride_bytes) *stride_bytes = int32_t(sizeof(double) * XS);
   });
 }
 

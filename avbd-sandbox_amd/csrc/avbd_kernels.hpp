@@ -1348,4 +1348,34 @@ __global__ void k_velocity(int n, const BodyProp *props, const double *X, const 
   }
 }
 
+// Poses for drawing (renderer.js:1327-1383): each body at a fraction a of the
+// last step, blended from the pose x_t at the step's start: the position
+// lerped, the orientation nlerped along the shorter arc and renormalised, in
+// f64, then rounded to f32. With lerp off (a >= 1) the current pose is copied.
+// Out: per body two float4, {p, 0} and {q}. One thread per body; the 64 B pose
+// records are read as four 16 B loads each, the output as two 16 B stores.
+__global__ void k_export_poses(int n, double a, int lerp, const double2 *__restrict__ X,
+                               const double2 *__restrict__ X0, float4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int R2 = XS / 2;  // double2 per record
+  const double2 p01 = X[size_t(i) * R2], p2q0 = X[size_t(i) * R2 + 1], q12 = X[size_t(i) * R2 + 2],
+                q3 = X[size_t(i) * R2 + 3];
+  double p[3] = {p01.x, p01.y, p2q0.x}, q[4] = {p2q0.y, q12.x, q12.y, q3.x};
+  if (lerp) {
+    const double2 o01 = X0[size_t(i) * R2], o2q0 = X0[size_t(i) * R2 + 1], oq12 = X0[size_t(i) * R2 + 2],
+                  oq3 = X0[size_t(i) * R2 + 3];
+    const double p0[3] = {o01.x, o01.y, o2q0.x}, q0[4] = {o2q0.y, oq12.x, oq12.y, oq3.x};
+    for (int k = 0; k < 3; k++) p[k] = p0[k] + (p[k] - p0[k]) * a;
+    const double d = q0[0] * q[0] + q0[1] * q[1] + q0[2] * q[2] + q0[3] * q[3];
+    const double sg = d < 0 ? -1.0 : 1.0;
+    for (int k = 0; k < 4; k++) q[k] = q0[k] * sg + (q[k] - q0[k] * sg) * a;
+    const double len = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const double inv = 1.0 / (len != 0.0 ? len : 1.0);
+    for (int k = 0; k < 4; k++) q[k] *= inv;
+  }
+  out[2 * size_t(i)] = make_float4(float(p[0]), float(p[1]), float(p[2]), 0.0f);
+  out[2 * size_t(i) + 1] = make_float4(float(q[0]), float(q[1]), float(q[2]), float(q[3]));
+}
+
 }  // namespace avbd

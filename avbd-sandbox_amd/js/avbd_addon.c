@@ -255,6 +255,22 @@ static napi_value js_write_state(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* exportPoses(ctx, alpha, out Float32Array of 8 per body): the poses a
+ * renderer draws at a fraction alpha of the last step (renderer.js:1327-1383) */
+static napi_value js_export_poses(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  double alpha = 1.0;
+  if (argc > 1) CHECK(env, napi_get_value_double(env, argv[1], &alpha));
+  size_t n = 0;
+  float *out = argc > 2 ? (float *)typed_data(env, argv[2], &n) : NULL;
+  int rc = avbd_export_poses(ctx, alpha, out, (int32_t)(n / 8), 0);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 /* readColoring(ctx, colorOf Int32Array) -> numColors (GpuBackend.readColoring, backend.js:2053-2151) */
 static napi_value js_read_coloring(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -393,6 +409,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"sync", NULL, js_sync, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"writeState", NULL, js_write_state, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"exportPoses", NULL, js_export_poses, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readColoring", NULL, js_read_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"uploadColoring", NULL, js_upload_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setBodyOrder", NULL, js_set_body_order, NULL, NULL, NULL, napi_enumerable, NULL},

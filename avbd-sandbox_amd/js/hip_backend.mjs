@@ -143,6 +143,15 @@ export class HipBackend {
     });
   }
 
+  // --- poses for drawing (renderer.js:1327-1383; boot.js:2223-2231): per
+  // body {p, 0, q} as f32 at a fraction alpha of the last step
+  async exportPoses(alpha = 1, out = null) {
+    await this.idle();
+    const poses = out || new Float32Array(this.numBodies * 8);
+    addon.exportPoses(this.ctx, alpha, poses);
+    return poses;
+  }
+
   // --- verification (backend.js:2053-2151)
   async readColoring() {
     await this.idle();

@@ -33,6 +33,9 @@ METRIC = "body-iterations/s on 51k-body pyramid vs host f64 CPU; adjoint/forward
 HBM_PEAK_GBS = 8000.0
 
 
+POSE_LAUNCHES = 100
+
+
 def measured_traffic(precision, team):
     """HBM bytes per k_primal launch (read + written) from the committed
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the same workload,
@@ -217,6 +220,12 @@ def main():
     pms, plaunch, pbytes = gpu.kernel_stats(L.K_PRIMAL)
     dms, dlaunch, dbytes = gpu.kernel_stats(L.K_DUAL)
     nms, nlaunch, _ = gpu.kernel_stats(L.K_NARROW)
+    # the renderer's pose export (blend at alpha 0.5), timed by the same
+    # dispatch-stamped events
+    poses = None
+    for _ in range(POSE_LAUNCHES):
+        poses = gpu.export_poses(0.5, out=poses)
+    xms, xlaunch, xbytes = gpu.kernel_stats(L.K_POSES)
     c = gpu.counters()
     gpu.set_profiling(False)
 
@@ -256,6 +265,8 @@ def main():
             "primal_ms_per_step": pms / args.steps, "dual_ms_per_step": dms / args.steps,
             "dual_GBs": (dbytes / dms * 1e-6) if dms > 0 else None,
             "narrow_ms_per_step": nms / args.steps,
+            "export_poses_us": xms / max(xlaunch, 1) * 1e3,
+            "export_poses_GBs": (xbytes / xms * 1e-6) if xms > 0 else None,
         },
         "phases_ms": phases,
         "profiled_ms_per_step": psecs / args.steps * 1e3,

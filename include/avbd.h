@@ -168,6 +168,25 @@ int avbd_sync(avbd_ctx *ctx);
  * Any pointer may be NULL. n must equal the packed body count. */
 int avbd_read_state(avbd_ctx *ctx, double *pos, double *quat, double *vel, double *angvel,
                     double *prev_vel, int32_t n);
+/* --- poses for drawing. A renderer draws each body at a fraction alpha of
+ * the last step, blended from the pose x_t at the step's start: positions
+ * lerped, orientations nlerped along the shorter arc and renormalised, in f64
+ * and stored as f32 (renderer.js:1327-1383; the WebGPU renderer reads x_t from
+ * the body arena's bInitP/bInitQ, boot.js:2223-2231). avbd_export_poses writes
+ * that blend, per body 8 floats {p[3], 0, q[4]}; alpha is clamped to [0, 1]
+ * and from 1 on (or NaN) the current pose is copied. dst is host memory, or,
+ * with dst_on_device nonzero, 16-byte aligned device memory on the context's
+ * device, written in stream order (avbd_sync waits for it). n must equal the
+ * packed body count. */
+int avbd_export_poses(avbd_ctx *ctx, double alpha, float *dst, int32_t n, int32_t dst_on_device);
+/* The body arena for a renderer on the same device that reads poses in place
+ * (setPoseSourceExternal(buffer, posBase, quatBase, prevPosBase, prevQuatBase),
+ * renderer_webgpu.js:1627): the device addresses of the current pose and of
+ * the step-start pose x_t, each a record of f64 {p[3], q[4], pad} every
+ * *stride_bytes. Valid until the next avbd_pack or avbd_destroy; read them
+ * after avbd_sync. Any output pointer may be NULL. */
+int avbd_pose_source(avbd_ctx *ctx, const double **pose, const double **start_pose, int32_t *stride_bytes);
+
 /* Overwrites body state (the device-authoritative half of syncToCPU's
  * inverse, backend.js:662-736). */
 int avbd_write_state(avbd_ctx *ctx, const double *pos, const double *quat, const double *vel,
@@ -210,7 +229,8 @@ int avbd_read_profile(avbd_ctx *ctx, float *ms, int32_t n, int32_t *steps);
 #define AVBD_K_PRIMAL 0
 #define AVBD_K_DUAL 1
 #define AVBD_K_NARROW 2
-#define AVBD_NUM_KERNELS 3
+#define AVBD_K_POSES 3   /* avbd_export_poses */
+#define AVBD_NUM_KERNELS 4
 int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int64_t *launches,
                            double *bytes_total);
 

@@ -4,9 +4,14 @@
 // CPU-baseline leg. Scene construction mirrors the reference's
 // test/parity/reference_main.cpp:45-130 (body / joint records) and
 // src/physics/rigid.js (box and sphere constructors).
+#include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <string>
+#include <vector>
 
 #include "avbd_oracle.hpp"
 
@@ -92,6 +97,81 @@ int orc_num_dynamic(void *w) {
   int n = 0;
   for (const Body &b : static_cast<World *>(w)->bodies) n += b.mass > 0;
   return n;
+}
+
+// Reads a `.scene` file into the world (test/parity/reference_main.cpp:45-124,
+// the same records as test/parity/scene_loader.mjs:20-84): '#' starts a
+// comment, tokens split on blanks, tabs, CR and LF, "inf" and "-inf" are the
+// infinities and every other token goes through strtod.
+//   body   sx sy sz density friction px py pz vx vy vz qx qy qz qw wx wy wz
+//   joint  idxA idxB rax ray raz rbx rby rbz stiffLin stiffAng fracture
+// Returns the number of bodies read; -1 when the file cannot be opened, -2
+// for a short record or a joint naming a body not yet read, -3 for a spring
+// record (springs are not modelled) and -4 for an unknown record.
+static double sceneValue(const char *t) {
+  if (std::strcmp(t, "inf") == 0) return std::numeric_limits<double>::infinity();
+  if (std::strcmp(t, "-inf") == 0) return -std::numeric_limits<double>::infinity();
+  return std::strtod(t, nullptr);
+}
+
+int orc_load_scene(void *w, const char *path) {
+  World &W = *static_cast<World *>(w);
+  FILE *f = std::fopen(path, "r");
+  if (!f) return -1;
+  std::vector<int> ids;
+  char line[8192];
+  int rc = 0;
+  while (rc == 0 && std::fgets(line, sizeof(line), f)) {
+    if (char *h = std::strchr(line, '#')) *h = '\0';
+    std::vector<const char *> tok;
+    for (char *t = std::strtok(line, " \t\r\n"); t; t = std::strtok(nullptr, " \t\r\n")) tok.push_back(t);
+    if (tok.empty()) continue;
+    if (std::strcmp(tok[0], "body") == 0) {
+      if (tok.size() < 19) { rc = -2; break; }
+      double v[18];
+      for (int i = 0; i < 18; i++) v[i] = sceneValue(tok[1 + i]);
+      const int id = W.addBox(v, v[3], v[4], v + 5, v + 8);
+      setPose(W.bodies[id], v + 11, v + 15);
+      ids.push_back(id);
+    } else if (std::strcmp(tok[0], "joint") == 0) {
+      if (tok.size() < 12) { rc = -2; break; }
+      const int ia = int(sceneValue(tok[1])), ib = int(sceneValue(tok[2]));
+      if (ib < 0 || ib >= int(ids.size()) || ia >= int(ids.size())) { rc = -2; break; }
+      double v[9];
+      for (int i = 0; i < 9; i++) v[i] = sceneValue(tok[3 + i]);
+      W.addJoint(ia < 0 ? -1 : ids[ia], ids[ib], v, v + 3, v[6], v[7], v[8]);
+    } else if (std::strcmp(tok[0], "spring") == 0) {
+      rc = -3;
+    } else {
+      rc = -4;
+    }
+  }
+  std::fclose(f);
+  return rc == 0 ? int(ids.size()) : rc;
+}
+
+// One frame of the trajectory dump of test/parity/reference_main.cpp:126-140:
+// a line per body, "step index p[3] q[4] v[3] w[3]", numbers as printf's
+// %.17g. Writes up to cap bytes (NUL-terminated when cap > 0) and returns the
+// length of the whole frame.
+long orc_format_state(void *w, int step, char *buf, long cap) {
+  const World &W = *static_cast<World *>(w);
+  std::string out;
+  char line[512];
+  for (size_t i = 0; i < W.bodies.size(); i++) {
+    const Body &b = W.bodies[i];
+    const int len = std::snprintf(
+        line, sizeof(line),
+        "%d %zu %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", step, i,
+        b.p[0], b.p[1], b.p[2], b.q[0], b.q[1], b.q[2], b.q[3], b.v[0], b.v[1], b.v[2], b.w[0], b.w[1], b.w[2]);
+    out.append(line, size_t(len));
+  }
+  if (cap > 0) {
+    const size_t n = std::min(out.size(), size_t(cap - 1));
+    std::memcpy(buf, out.data(), n);
+    buf[n] = '\0';
+  }
+  return long(out.size());
 }
 
 // Per body, 16 doubles: p[3] q[4] v[3] w[3] vprev[3].

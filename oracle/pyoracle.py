@@ -55,6 +55,8 @@ def lib():
             "orc_num_pairs": (i, [v]), "orc_get_pairs": (None, [v, pi]),
             "orc_collide": (i, [v, i, i, pd, pd]),
             "orc_solve6": (None, [pd, pd, pd, pd, pd, pd, pd]),
+            "orc_load_scene": (i, [v, C.c_char_p]),
+            "orc_format_state": (C.c_long, [v, i, C.c_char_p, C.c_long]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -103,6 +105,13 @@ class Oracle:
         iv = np.array([p["iterations"], int(p["postStabilize"]), int(p["rotatedInertia"]),
                        int(p["cachedContactJacobians"])], dtype=np.int32)
         self.L.orc_set_params(self.h, _dp(dv), _ip(iv))
+        if isinstance(scene, (str, os.PathLike)):
+            # a .scene file, read by the oracle's C restatement of the
+            # reference loader (test/parity/reference_main.cpp:45-124)
+            r = self.L.orc_load_scene(self.h, os.fsencode(scene))
+            if r < 0:
+                raise ValueError(f"orc_load_scene({scene}) failed: {r}")
+            scene = {"bodies": [], "joints": []}
         for b in scene["bodies"]:
             q = _vec(b.get("quat", (0, 0, 0, 1)), 4)
             w = _vec(b.get("angvel", (0, 0, 0)), 3)
@@ -158,6 +167,14 @@ class Oracle:
         out = np.zeros((self.n, 16), dtype=np.float64)
         self.L.orc_get_state(self.h, _dp(out))
         return out
+
+    def format_state(self, step):
+        """The frame of reference_main.cpp's trajectory dump (lines 126-140)
+        for the current state, formatted by C's printf %.17g."""
+        n = self.L.orc_format_state(self.h, int(step), None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.L.orc_format_state(self.h, int(step), buf, n + 1)
+        return buf.value.decode()
 
     def mass(self):
         out = np.zeros((self.n, 8), dtype=np.float64)
@@ -223,3 +240,29 @@ def solve6(aLin, aAng, aCross, bLin, bAng):
     x1, x2 = np.zeros(3), np.zeros(3)
     L.orc_solve6(_dp(a1), _dp(a2), _dp(a3), _dp(b1), _dp(b2), _dp(x1), _dp(x2))
     return x1, x2
+
+
+def render_poses(pos, quat, pos0, quat0, alpha):
+    """Per body {p, 0, q} as float32 (n, 8): the pose a renderer draws at a
+    fraction alpha of the last step, restating renderer.js:1327-1383 (alpha
+    clamped to [0, 1]; below 1 the position is lerped from the step-start
+    pose and the orientation nlerped along the shorter arc, in f64, then
+    stored as f32). The renormalisation divides by sqrt(x^2+y^2+z^2+w^2)
+    summed left to right where the renderer calls Math.hypot, which may
+    differ from it in the last f64 bit."""
+    a = 1.0 if math.isnan(alpha) else min(1.0, max(0.0, float(alpha)))
+    p = np.array(pos, dtype=np.float64)
+    q = np.array(quat, dtype=np.float64)
+    if a < 1.0:
+        p0 = np.asarray(pos0, dtype=np.float64)
+        q0 = np.asarray(quat0, dtype=np.float64)
+        p = p0 + (p - p0) * a
+        d = q0[:, 0] * q[:, 0] + q0[:, 1] * q[:, 1] + q0[:, 2] * q[:, 2] + q0[:, 3] * q[:, 3]
+        s = np.where(d < 0, -1.0, 1.0)[:, None]
+        q = q0 * s + (q - q0 * s) * a
+        ln = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+        q = q * (1.0 / np.where(ln != 0.0, ln, 1.0))[:, None]
+    out = np.zeros((len(p), 8), np.float32)
+    out[:, 0:3] = p.astype(np.float32)
+    out[:, 4:8] = q.astype(np.float32)
+    return out

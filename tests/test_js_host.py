@@ -18,7 +18,7 @@ NODE = shutil.which("node")
 
 needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
-EXPORTS = ["abiVersion", "create", "setParams", "pack", "step", "sync", "readState", "writeState",
+EXPORTS = ["abiVersion", "create", "setParams", "pack", "step", "sync", "readState", "writeState", "exportPoses",
            "readColoring", "readPairs", "readCounters", "setProfiling", "readProfile"]
 
 
