@@ -120,6 +120,26 @@ class HipBackend:
             self.set_params(**{k: v for k, v in params.items() if k in PARAM_NAMES})
         else:
             self._ck(self.L.avbd_set_params(self.ctx, C.byref(self.params)))
+        s = self._scene_struct(scene)
+        self._ck(self.L.avbd_pack(self.ctx, C.byref(s)))
+        self.n = s.num_bodies
+
+    def repack(self, scene, old_index, params=None):
+        """A topology edit that keeps state (GpuBackend.pack's identity-
+        preserving repack, backend.js:662-1301): `scene` is the scene after
+        bodies were erased and/or appended, `old_index[i]` the index body i had
+        before (-1 for a new body). Survivors keep their device state,
+        manifolds with warm start, joints and colours."""
+        if params:
+            self.set_params(**{k: v for k, v in params.items() if k in PARAM_NAMES})
+        s = self._scene_struct(scene)
+        oi = np.ascontiguousarray(np.asarray(old_index, dtype=np.int32))
+        if oi.shape != (s.num_bodies,):
+            raise AvbdError(L.AVBD_E_INVALID, "old_index needs one entry per body")
+        self._ck(self.L.avbd_repack(self.ctx, C.byref(s), _ip(oi)))
+        self.n = s.num_bodies
+
+    def _scene_struct(self, scene):
         A = scene_arrays(scene)
         self._arrays = A  # keep alive for the call
         s = L.Scene()
@@ -133,8 +153,7 @@ class HipBackend:
             s.joint_ra, s.joint_rb = _dp(A["joint_ra"]), _dp(A["joint_rb"])
             s.joint_stiff_lin, s.joint_stiff_ang = _dp(A["joint_stiff_lin"]), _dp(A["joint_stiff_ang"])
             s.joint_fracture = _dp(A["joint_fracture"])
-        self._ck(self.L.avbd_pack(self.ctx, C.byref(s)))
-        self.n = s.num_bodies
+        return s
 
     # --- stepping (backend.js:1393-1620, 1804-1818)
     def step(self, n=1):

@@ -1176,6 +1176,37 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
 
 }  // namespace
 
+// State carried over a repack: the previous scene's device buffers, taken out
+// of the context before it packs the new scene, freed when this goes.
+struct RepackCarry {
+  DBuf<double> X, X0, V;
+  ManStore store;
+  DBuf<int> live, color;
+  int nLive = 0;
+  ~RepackCarry() {
+    X.release(); X0.release(); V.release(); store.release(); live.release(); color.release();
+  }
+};
+
+template <class R>
+int carryManifolds(avbd_ctx &c, RepackCarry &old, const int *remap) {
+  if (old.nLive == 0) return 0;
+  c.keep.reserve(size_t(old.nLive));
+  c.keepPos.reserve(size_t(old.nLive) + 1);
+  hipLaunchKernelGGL(k_repack_keep, dim3(blocks(old.nLive)), dim3(256), 0, c.stream, old.nLive, old.live.p,
+                     (const int4 *)old.store.head.p, remap, c.keep.p);
+  c.scan(c.keep.p, c.keepPos.p, old.nLive, c.scal.p + 2);
+  c.readScal(2, 1);
+  const int kept = c.hscal[2];
+  if (kept > c.manCap)
+    throw ApiError(AVBD_E_CAPACITY, "manifolds " + std::to_string(kept) + " exceed capacity " + std::to_string(c.manCap));
+  hipLaunchKernelGGL((k_repack_manifolds<R>), dim3(old.nLive), dim3(64), 0, c.stream, (const int *)old.live.p,
+                     (const int *)c.keep.p, (const int *)c.keepPos.p, remap, old.store.view<R>(),
+                     c.store[c.cur].view<R>(), c.live[c.cur].p);
+  HIPCHECK(hipGetLastError());
+  return kept;
+}
+
 extern "C" {
 
 int avbd_abi_version(void) { return AVBD_ABI_VERSION; }
@@ -1258,6 +1289,81 @@ int avbd_pack(avbd_ctx *ctx, const avbd_scene *scene) {
     if (!scene) throw ApiError(AVBD_E_INVALID, "null scene");
     ctx->packed = false;
     packScene(*ctx, *scene);
+  });
+}
+
+int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed (use avbd_pack)");
+    if (!scene || !old_index) throw ApiError(AVBD_E_INVALID, "null scene or old_index");
+    avbd_ctx &c = *ctx;
+    const int oldN = c.N, N = scene->num_bodies;
+    if (N <= 0) throw ApiError(AVBD_E_INVALID, "num_bodies must be positive");
+    std::vector<int> remap(oldN, -1);
+    int last = -1;
+    for (int i = 0; i < N; i++) {
+      const int o = old_index[i];
+      if (o < -1 || o >= oldN) throw ApiError(AVBD_E_INVALID, "old_index out of range");
+      if (o < 0) continue;
+      if (o <= last) throw ApiError(AVBD_E_INVALID, "surviving bodies must keep their order");
+      last = o;
+      remap[o] = i;
+    }
+    // the joints: exactly the survivors of the previous scene's, in order
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    std::vector<JointDev> oldJ(c.J);
+    if (c.J) HIPCHECK(hipMemcpy(oldJ.data(), c.joints.p, sizeof(JointDev) * c.J, hipMemcpyDeviceToHost));
+    std::vector<JointDev> keptJ;
+    for (const JointDev &j : oldJ) {
+      const int na = j.a < 0 ? -1 : remap[j.a], nb = remap[j.b];
+      if (nb < 0 || (j.a >= 0 && na < 0)) continue;
+      JointDev k = j;
+      k.a = na;
+      k.b = nb;
+      keptJ.push_back(k);
+    }
+    if (scene->num_joints != int(keptJ.size()))
+      throw ApiError(AVBD_E_INVALID, "a repack keeps the surviving joints and adds none (" +
+                                         std::to_string(keptJ.size()) + " survive, scene has " +
+                                         std::to_string(scene->num_joints) + "); add joints with avbd_pack");
+    for (int j = 0; j < scene->num_joints; j++)
+      if (scene->joint_a[j] != keptJ[j].a || scene->joint_b[j] != keptJ[j].b)
+        throw ApiError(AVBD_E_INVALID, "joint " + std::to_string(j) + " is not the surviving joint in that place");
+
+    RepackCarry old;
+    std::swap(old.X, c.X);
+    std::swap(old.X0, c.X0);
+    std::swap(old.V, c.V);
+    std::swap(old.live, c.live[c.cur]);
+    std::swap(old.color, c.color[0]);
+    std::swap(old.store.head, c.store[c.cur].head);
+    std::swap(old.store.basis, c.store[c.cur].basis);
+    std::swap(old.store.hot, c.store[c.cur].hot);
+    std::swap(old.store.Cd, c.store[c.cur].Cd);
+    std::swap(old.store.anchor, c.store[c.cur].anchor);
+    std::swap(old.store.feat, c.store[c.cur].feat);
+    std::swap(old.store.stick, c.store[c.cur].stick);
+    old.nLive = c.nLive;
+
+    c.packed = false;
+    packScene(c, *scene);  // fresh buffers, the description's state everywhere
+    DBuf<int> dOld, dRemap;
+    dOld.reserve(N);
+    dRemap.reserve(oldN);
+    HIPCHECK(hipMemcpy(dOld.p, old_index, sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dRemap.p, remap.data(), sizeof(int) * oldN, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_repack_bodies, dim3(blocks(N)), dim3(256), 0, c.stream, N, (const int *)dOld.p,
+                       (const BodyProp *)c.props.p, (const double *)old.X.p, (const double *)old.X0.p,
+                       (const double *)old.V.p, (const int *)old.color.p, c.X.p, c.X0.p, c.V.p, c.color[0].p);
+    HIPCHECK(hipGetLastError());
+    c.nLive = c.prec == AVBD_PREC_MIXED ? carryManifolds<float>(c, old, dRemap.p)
+                                        : carryManifolds<double>(c, old, dRemap.p);
+    if (c.J) HIPCHECK(hipMemcpy(c.joints.p, keptJ.data(), sizeof(JointDev) * c.J, hipMemcpyHostToDevice));
+    // the force lists the next broad phase tells existing pairs by
+    c.buildAdjacency();
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    dOld.release();
+    dRemap.release();
   });
 }
 

@@ -1348,6 +1348,60 @@ __global__ void k_velocity(int n, const BodyProp *props, const double *X, const 
   }
 }
 
+// ------------------------------------------------------------ repack
+// A topology edit that keeps the survivors' state (the identity-preserving
+// repack of backend.js:662-1301 over Solver.removeBodies, solver.js:261-300):
+// body i of the new scene takes the device state of old body oldIndex[i], when
+// it has one: pose, step-start pose, velocities and, for a body still
+// dynamic, its last colour (the colouring's warm start).
+__global__ void k_repack_bodies(int n, const int *oldIndex, const BodyProp *props, const double *oX,
+                                const double *oX0, const double *oV, const int *oColor, double *X, double *X0,
+                                double *V, int *color) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = oldIndex[i];
+  if (o < 0) return;
+  for (int k = 0; k < XS; k++) {
+    X[size_t(i) * XS + k] = oX[size_t(o) * XS + k];
+    X0[size_t(i) * XS + k] = oX0[size_t(o) * XS + k];
+  }
+  for (int k = 0; k < VS; k++) V[size_t(i) * VS + k] = oV[size_t(o) * VS + k];
+  if (props[i].mass > 0 && oColor[o] >= 0) color[i] = oColor[o];
+}
+
+// Whether each live manifold survives: both its bodies do.
+__global__ void k_repack_keep(int nLive, const int *live, const int4 *head, const int *remap, int *keep) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nLive) return;
+  const int4 h = head[live[t]];
+  keep[t] = remap[h.x] >= 0 && remap[h.y] >= 0 ? 1 : 0;
+}
+
+// One wave per live manifold: a survivor's whole record (head with its bodies
+// renumbered, basis, contact slots with their warm-start state) goes to slot
+// keepPos[t] of the fresh store, in force-list order.
+template <class R>
+__global__ void __launch_bounds__(64) k_repack_manifolds(const int *live, const int *keep, const int *keepPos,
+                                                         const int *remap, ManBuf<R> src, ManBuf<R> dst,
+                                                         int *dstLive) {
+  const int t = blockIdx.x, lane = threadIdx.x;
+  if (!keep[t]) return;
+  const size_t m = size_t(live[t]), p = size_t(keepPos[t]);
+  if (lane == 0) {
+    const int4 h = src.head[m];
+    dst.head[p] = make_int4(remap[h.x], remap[h.y], h.z, h.w);
+    dstLive[p] = int(p);
+  }
+  for (int k = lane; k < BAS; k += 64) dst.basis[p * BAS + k] = src.basis[m * BAS + k];
+  for (int k = lane; k < MAXC * HOT; k += 64) dst.hot[p * MAXC * HOT + k] = src.hot[m * MAXC * HOT + k];
+  for (int k = lane; k < MAXC * 4; k += 64) dst.Cd[p * MAXC * 4 + k] = src.Cd[m * MAXC * 4 + k];
+  for (int k = lane; k < MAXC * 6; k += 64) dst.anchor[p * MAXC * 6 + k] = src.anchor[m * MAXC * 6 + k];
+  for (int k = lane; k < MAXC; k += 64) {
+    dst.feat[p * MAXC + k] = src.feat[m * MAXC + k];
+    dst.stick[p * MAXC + k] = src.stick[m * MAXC + k];
+  }
+}
+
 // Poses for drawing (renderer.js:1327-1383): each body at a fraction a of the
 // last step, blended from the pose x_t at the step's start: the position
 // lerped, the orientation nlerped along the shorter arc and renormalised, in

@@ -126,8 +126,8 @@ static napi_value js_set_params(napi_env env, napi_callback_info info) {
 
 /* pack(ctx, arenas) with typed arrays named as avbd_scene's fields (GpuBackend.pack, backend.js:662-1301) */
 static napi_value js_pack(napi_env env, napi_callback_info info) {
-  size_t argc = 2;
-  napi_value argv[2];
+  size_t argc = 3;
+  napi_value argv[3];
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   avbd_ctx *ctx = get_ctx(env, argv[0]);
   napi_value o = argv[1];
@@ -157,7 +157,14 @@ static napi_value js_pack(napi_env env, napi_callback_info info) {
 #undef TA
   s.num_bodies = (int32_t)n;
   s.num_joints = s.joint_a ? (int32_t)nj : 0;
-  int rc = avbd_pack(ctx, &s);
+  /* pack(ctx, scene, oldIndex Int32Array): avbd_repack, keeping state */
+  size_t no = 0;
+  const int32_t *old_index = argc > 2 ? (const int32_t *)typed_data(env, argv[2], &no) : NULL;
+  if (old_index && no != n) {
+    napi_throw_range_error(env, NULL, "oldIndex needs one entry per body");
+    return NULL;
+  }
+  int rc = old_index ? avbd_repack(ctx, &s, old_index) : avbd_pack(ctx, &s);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }

@@ -40,9 +40,17 @@ export class HipBackend {
   }
 
   // --- topology (backend.js:662-1301, 1898-1915)
+  // A topology edit keeps state where it can (backend.js:662-1301): bodies
+  // found by identity in the last pack keep their device state, and, when the
+  // survivors kept their order and the joints are exactly the surviving ones,
+  // their contacts (warm start), joints (fracture included) and colours too
+  // (avbd_repack). Otherwise the scene is packed afresh from the survivors'
+  // device state, as the reference does when it cannot keep contact state.
   pack(solver) {
     const bodies = solver.bodies;
     const n = bodies.length;
+    const prev = this.bodyIndex;
+    const prevJoints = this.packedJoints || [];
     const A = {
       shape: new Int32Array(n), size: new Float64Array(n * 3), density: new Float64Array(n),
       friction: new Float64Array(n), pos: new Float64Array(n * 3), quat: new Float64Array(n * 4),
@@ -62,6 +70,45 @@ export class HipBackend {
     });
     const joints = solver.forces.filter((f) => f.rA && f.rB && 'stiffnessLin' in f);
     const m = joints.length;
+    let oldIndex = null;
+    if (prev && this.numBodies > 0) {
+      oldIndex = new Int32Array(n);
+      let survivors = 0;
+      let last = -1;
+      let ordered = true;
+      bodies.forEach((b, i) => {
+        const o = prev.get(b);
+        oldIndex[i] = o === undefined ? -1 : o;
+        if (o !== undefined) {
+          survivors++;
+          if (o <= last) ordered = false;
+          last = o;
+        }
+      });
+      const alive = (b) => b === null || this.bodyIndex.has(b);
+      const kept = prevJoints.filter((j) => alive(j.bodyA || null) && alive(j.bodyB));
+      const sameJoints = kept.length === joints.length && kept.every((j, k) => j === joints[k]);
+      if (survivors === 0) {
+        oldIndex = null;
+      } else if (!ordered || !sameJoints) {
+        // fresh pack: carry the survivors' device state into the solver first
+        const N0 = this.numBodies;
+        const d = {
+          pos: new Float64Array(N0 * 3), quat: new Float64Array(N0 * 4), vel: new Float64Array(N0 * 3),
+          angvel: new Float64Array(N0 * 3),
+        };
+        addon.readState(this.ctx, d.pos, d.quat, d.vel, d.angvel, null);
+        bodies.forEach((b, i) => {
+          const o = oldIndex[i];
+          if (o < 0) return;
+          A.pos.set(d.pos.subarray(o * 3, o * 3 + 3), i * 3);
+          A.quat.set(d.quat.subarray(o * 4, o * 4 + 4), i * 4);
+          A.vel.set(d.vel.subarray(o * 3, o * 3 + 3), i * 3);
+          A.angvel.set(d.angvel.subarray(o * 3, o * 3 + 3), i * 3);
+        });
+        oldIndex = null;
+      }
+    }
     if (m) {
       A.jointA = new Int32Array(m);
       A.jointB = new Int32Array(m);
@@ -88,7 +135,10 @@ export class HipBackend {
       rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
       colorOrder: solver.colorOrder === 'priority' || solver.colorOrder === 1 ? 1 : 0,
     });
-    addon.pack(this.ctx, A);
+    if (oldIndex) addon.pack(this.ctx, A, oldIndex);
+    else addon.pack(this.ctx, A);
+    this.lastPackPreservedContacts = oldIndex !== null;
+    this.packedJoints = joints;
     this.numBodies = n;
     this.topologyDirty = false;
   }

@@ -27,6 +27,28 @@ export class Solver {
     this.rotatedInertia = true;
     this.cachedContactJacobians = true;
   }
+
+  /**
+   * Remove every body `predicate` accepts and every force touching one,
+   * keeping the order of the surviving bodies and forces (solver.js:261-300).
+   * A backend told of the change (prepareTopologyChange) keeps the
+   * survivors' state on its next pack.
+   */
+  removeBodies(predicate) {
+    const doomed = new Set(this.bodies.filter((b) => predicate(b)));
+    if (doomed.size === 0) return 0;
+    const retained = (f) => !doomed.has(f.bodyA) && !doomed.has(f.bodyB);
+    this.forces = this.forces.filter(retained);
+    this.bodies = this.bodies.filter((b) => {
+      if (doomed.has(b)) {
+        b.forces.length = 0;
+        return false;
+      }
+      b.forces = b.forces.filter(retained);
+      return true;
+    });
+    return doomed.size;
+  }
 }
 
 export class Rigid {

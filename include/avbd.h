@@ -159,6 +159,24 @@ int avbd_abi_version(void);
 int avbd_set_params(avbd_ctx *ctx, const avbd_params *p);
 int avbd_pack(avbd_ctx *ctx, const avbd_scene *scene);
 
+/* Topology edit that keeps state (GpuBackend.pack's identity-preserving
+ * repack, backend.js:662-1301): the scene after bodies were erased
+ * (Solver.removeBodies / Rigid.destroy, solver.js:261-300, rigid.js:270-274)
+ * and new ones appended (new Rigid(...)). old_index[i] is the index body i had
+ * in the previous scene, or -1 for a new body; survivors keep their relative
+ * order (AVBD_E_INVALID otherwise). Survivors keep their device state (pose,
+ * step-start pose, velocities; the description's pos/quat/vel/angvel for them
+ * are not read), their contact manifolds with every warm-start field (feature
+ * keys, anchors, penalties, multipliers, stick flags) and their colours;
+ * manifolds and joints touching an erased body are dropped, keeping the order
+ * of the rest. The description's joints must be exactly the surviving joints
+ * in order (their state, a fracture included, is kept): a joint added to a
+ * running scene follows its contacts in the reference's force list, which the
+ * packed layout (joints first) cannot express, so joints are added with
+ * avbd_pack. Properties (size, density, friction, shape) come from the
+ * description. On an error after validation the context holds no scene. */
+int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index);
+
 /* --- stepping: GpuBackend.step (backend.js:1393-1620); avbd_sync is idle()
  * (backend.js:1804-1818). */
 int avbd_step(avbd_ctx *ctx, int32_t nsteps);

@@ -1193,6 +1193,45 @@ void World::buildColoring(Coloring &out) const {
 }
 
 // solver.js:307-493 — Algorithm 1
+// Solver.removeBodies (solver.js:261-300): drops every flagged body and every
+// force touching one (a world-anchored joint touches only its body B), keeps
+// the order of the surviving bodies, of the force list and of each body's
+// force list, and renumbers the bodies the surviving forces name. Returns the
+// number of bodies removed.
+int World::removeBodies(const std::vector<char> &doomed) {
+  const int n = int(bodies.size());
+  std::vector<int> remap(n, -1);
+  int kept = 0;
+  for (int i = 0; i < n; i++)
+    if (!doomed[i]) remap[i] = kept++;
+  if (kept == n) return 0;
+  auto retained = [&](const Force &f) { return (f.a < 0 || !doomed[f.a]) && !doomed[f.b]; };
+  int write = 0;
+  for (size_t k = 0; k < forces.size(); k++)
+    if (retained(pool[forces[k]])) forces[write++] = forces[k];
+  forces.resize(write);
+  std::vector<Body> survivors;
+  survivors.reserve(kept);
+  for (int i = 0; i < n; i++) {
+    if (doomed[i]) continue;
+    Body b = bodies[i];
+    int fw = 0;
+    for (size_t k = 0; k < b.forces.size(); k++)
+      if (retained(pool[b.forces[k]])) b.forces[fw++] = b.forces[k];
+    b.forces.resize(fw);
+    survivors.push_back(b);
+  }
+  bodies.swap(survivors);
+  for (int id : forces) {
+    Force &f = pool[id];
+    if (f.a >= 0) f.a = remap[f.a];
+    f.b = remap[f.b];
+  }
+  bodyOrder.clear();
+  lastPairs.clear();
+  return n - kept;
+}
+
 void World::step() {
   const Params &P = prm;
   const double dt = P.dt;

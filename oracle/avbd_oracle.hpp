@@ -115,6 +115,7 @@ struct World {
                 const double pos[3], const double vel[3]);
   int addJoint(int a, int b, const double rA[3], const double rB[3],
                double stiffLin, double stiffAng, double fracture);
+  int removeBodies(const std::vector<char> &doomed);
   void step();
   void buildColoring(Coloring &out) const;
 };

@@ -82,6 +82,13 @@ void orc_set_body(void *w, int i, const double *pos, const double *quat, const d
   }
   if (quat) for (int k = 0; k < 4; k++) b.q[k] = quat[k];
 }
+// Removes the bodies whose doomed[i] is nonzero (Solver.removeBodies).
+int orc_remove_bodies(void *w, const int *doomed, int n) {
+  World &W = *static_cast<World *>(w);
+  if (n != int(W.bodies.size())) return -1;
+  std::vector<char> d(doomed, doomed + n);
+  return W.removeBodies(d);
+}
 void orc_record_pairs(void *w, int on) { static_cast<World *>(w)->recordPairs = on != 0; }
 
 // Runs n steps; returns the wall time in seconds.

@@ -56,6 +56,7 @@ def lib():
             "orc_collide": (i, [v, i, i, pd, pd]),
             "orc_solve6": (None, [pd, pd, pd, pd, pd, pd, pd]),
             "orc_load_scene": (i, [v, C.c_char_p]),
+            "orc_remove_bodies": (i, [v, pi, i]),
             "orc_format_state": (C.c_long, [v, i, C.c_char_p, C.c_long]),
         }
         for name, (res, args) in sig.items():
@@ -146,6 +147,31 @@ class Oracle:
     def set_body(self, i, pos=None, quat=None, vel=None, angvel=None):
         a = [None if x is None else _vec(x, n) for x, n in ((pos, 3), (quat, 4), (vel, 3), (angvel, 3))]
         self.L.orc_set_body(self.h, int(i), *[None if x is None else _dp(x) for x in a])
+
+    def remove_bodies(self, doomed):
+        """Solver.removeBodies (solver.js:261-300) for a boolean mask."""
+        d = np.ascontiguousarray(np.asarray(doomed, dtype=np.int32))
+        r = self.L.orc_remove_bodies(self.h, _ip(d), int(d.size))
+        if r < 0:
+            raise ValueError("mask length differs from the body count")
+        return r
+
+    def add_bodies(self, bodies):
+        """Appends bodies (new Rigid(...) on a running solver); returns their ids."""
+        ids = []
+        for b in bodies:
+            q = _vec(b.get("quat", (0, 0, 0, 1)), 4)
+            w = _vec(b.get("angvel", (0, 0, 0)), 3)
+            vel = _vec(b.get("vel", (0, 0, 0)), 3)
+            pos = _vec(b["pos"], 3)
+            if b["shape"] == "sphere":
+                ids.append(self.L.orc_add_sphere(self.h, float(b["size"][0]), float(b["density"]),
+                                                 float(b["friction"]), _dp(pos), _dp(vel), _dp(q), _dp(w)))
+            else:
+                size = _vec(b["size"], 3)
+                ids.append(self.L.orc_add_box(self.h, _dp(size), float(b["density"]), float(b["friction"]),
+                                              _dp(pos), _dp(vel), _dp(q), _dp(w)))
+        return ids
 
     def record_pairs(self, on=True):
         self.L.orc_record_pairs(self.h, int(on))

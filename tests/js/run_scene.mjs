@@ -3,11 +3,13 @@
 //   node tests/js/run_scene.mjs dump0 <file.scene>      the frame of step 0
 //   node tests/js/run_scene.mjs run <file.scene> <steps> <precision>
 //                                                        steps on the GPU, a frame per step
+//   node tests/js/run_scene.mjs repack <file.scene> <steps> <precision>
+//                                                        steps, erases and appends, steps (JSON)
 //   node tests/js/run_scene.mjs poses <file.scene> <steps> <precision>
 //                                                        the poses drawn at alpha 0.5 (JSON)
 import { readFileSync } from 'fs';
 import { dumpState, formatG17, loadScene } from '../../avbd-sandbox_amd/js/scene_io.mjs';
-import { Solver } from '../../avbd-sandbox_amd/js/scene_graph.mjs';
+import { Rigid, Solver } from '../../avbd-sandbox_amd/js/scene_graph.mjs';
 
 async function main() {
   const [mode, path, steps, precision] = process.argv.slice(2);
@@ -36,6 +38,20 @@ async function main() {
     await backend.syncToCPU(solver);
     const poses = await backend.exportPoses(0.5);
     process.stdout.write(JSON.stringify({ s0, s1: pose(bodies), poses: Array.from(poses) }));
+    return;
+  }
+  if (mode === 'repack') {
+    // steps, erases body 8 and appends a cube (a topology edit that keeps
+    // state), steps again; prints the final frame and whether the pack kept
+    // the contact state
+    for (let s = 0; s < Number(steps); s++) backend.step(solver);
+    solver.removeBodies((b) => b === bodies[8]);
+    new Rigid(solver, [1, 1, 1], 1, 0.5, [3, 0, -15]);
+    backend.prepareTopologyChange();
+    for (let s = 0; s < Number(steps); s++) backend.step(solver);
+    await backend.syncToCPU(solver);
+    process.stdout.write(JSON.stringify({ kept: backend.lastPackPreservedContacts,
+      frame: dumpState(solver.bodies, 2 * Number(steps)) }));
     return;
   }
   let out = '';
