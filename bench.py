@@ -271,6 +271,18 @@ def main():
         "phases_ms": phases,
         "profiled_ms_per_step": psecs / args.steps * 1e3,
     }
+    # a topology edit that keeps state: erase one cube, append one
+    if args.workload == "pyramid":
+        gpu.idle()
+        n = len(scene["bodies"])
+        old_index = [i for i in range(n) if i != 1] + [-1]
+        scene2 = dict(bodies=[b for i, b in enumerate(scene["bodies"]) if i != 1] +
+                      [dict(scene["bodies"][1], pos=[0.0, 0.0, params.get("layers", 53) + 5.0])], joints=[])
+        t2 = time.perf_counter()
+        gpu.repack(scene2, old_index)
+        line["kernels"]["repack_ms"] = (time.perf_counter() - t2) * 1e3
+        c2 = gpu.counters()
+        line["kernels"]["repack_kept_manifolds"] = c2["manifolds"]
     if not args.no_adjoint and args.workload == "pyramid":
         line["adjoint"] = adjoint_leg()
     if not args.no_cpu:
