@@ -956,12 +956,148 @@ void jointDual(World &W, Force &f, double alpha) {
   }
 }
 
+// ------------------------------------------------------------------ springs
+// spring.js:107-118 (_evaluate): d = pA - pB and n = d / |d|; NaN when the
+// anchors (nearly) coincide.
+double springEvaluate(const World &W, const Force &f, double *d, double *n) {
+  const Body &A = W.bodies[f.a], &B = W.bodies[f.b];
+  double pA[3], pB[3];
+  transformPoint(pA, A.p, A.q, f.jrA);
+  transformPoint(pB, B.p, B.q, f.jrB);
+  v_sub(d, pA, pB);
+  const double dLen = std::sqrt(v_lenSq(d));
+  if (dLen <= 1.0e-6) return std::numeric_limits<double>::quiet_NaN();
+  v_div(n, d, dLen);
+  return dLen - f.rest;
+}
+
+// spring.js:121-131 (_updateTearing)
+bool springTear(Force &f, double C) {
+  if (f.rest <= 1.0e-6) return false;
+  const double strain = jmax(0.0, C / f.rest);
+  f.peakStrain = jmax(f.peakStrain, strain);
+  if (strain <= f.tearStrain) return false;
+  f.broken = true;
+  f.penalty = 0;
+  f.springC = 0;
+  return true;
+}
+
+// spring.js:133-166 with the stiffness ramp off (solver.js:187, the default):
+// the penalty is the material stiffness. A torn spring stays in the force
+// list as an inactive sentinel.
+bool springInitialize(World &W, Force &f) {
+  if (f.broken) {
+    f.penalty = 0;
+    f.springC = 0;
+    return true;
+  }
+  if (std::isfinite(f.tearStrain)) {
+    double d[3], n[3];
+    const double C = springEvaluate(W, f, d, n);
+    if (!std::isnan(C)) {
+      f.springC = C;
+      if (springTear(f, C)) return true;
+    }
+  }
+  f.penalty = f.stiffness;
+  return f.stiffness > 0;
+}
+
+inline void m_outer(double *o, const double *a, const double *b) {
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r] * b[c];
+}
+
+// spring.js:168-212 (updatePrimal) and 214-269 (the geometric stiffness of
+// Equation 17, on by default: solver.js:190-193)
+void springPrimal(World &W, Force &f, int body, double *lhsLin, double *lhsAng, double *lhsCross, double *rhsLin,
+                  double *rhsAng) {
+  if (f.broken) return;
+  double d[3], n[3];
+  const double C = springEvaluate(W, f, d, n);
+  if (std::isnan(C)) return;
+  const bool isA = body == f.a;
+  const double k = f.penalty;
+  const double fo = k * C;
+  double rW[3], jl[3], ja[3], m0[9];
+  if (isA) {
+    q_rotate(rW, W.bodies[f.a].q, f.jrA);
+    v_copy(jl, n);
+    v_cross(ja, rW, n);
+  } else {
+    q_rotate(rW, W.bodies[f.b].q, f.jrB);
+    v_neg(jl, n);
+    v_cross(ja, rW, n);
+    v_neg(ja, ja);
+  }
+  m_outer(m0, jl, jl);
+  m_scale(m0, m0, k);
+  m_addInto(lhsLin, m0);
+  m_outer(m0, ja, ja);
+  m_scale(m0, m0, k);
+  m_addInto(lhsAng, m0);
+  m_outer(m0, ja, jl);
+  m_scale(m0, m0, k);
+  m_addInto(lhsCross, m0);
+  v_addScaled(rhsLin, jl, fo);
+  v_addScaled(rhsAng, ja, fo);
+
+  const double dLen = std::sqrt(v_lenSq(d));
+  const double s = isA ? 1.0 : -1.0;
+  double P[9], S[9], Gll[9], Gla[9], Gal[9], Gaa[9];
+  m_outer(P, n, n);
+  m_neg(P, P);
+  P[0] += 1;
+  P[4] += 1;
+  P[8] += 1;
+  m_div(P, P, dLen);
+  m_skew(S, rW);
+  m_copy(Gll, P);
+  m_mul(Gla, P, S);
+  m_neg(Gla, Gla);
+  m_mul(Gal, S, P);
+  m_mul(Gaa, S, P);
+  m_mul(Gaa, Gaa, S);
+  m_neg(Gaa, Gaa);
+  const double ndotr = v_dot(n, rW);
+  m_outer(m0, rW, n);
+  m0[0] -= ndotr;
+  m0[4] -= ndotr;
+  m0[8] -= ndotr;
+  m_scale(m0, m0, s);
+  m_addInto(Gaa, m0);
+  const double af = std::fabs(fo);
+  for (int c = 0; c < 3; c++) {
+    const double a0 = Gll[c], a1 = Gll[3 + c], a2 = Gll[6 + c];
+    const double a3 = Gal[c], a4 = Gal[3 + c], a5 = Gal[6 + c];
+    lhsLin[c * 3 + c] += std::sqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4 + a5 * a5) * af;
+  }
+  for (int c = 0; c < 3; c++) {
+    const double a0 = Gla[c], a1 = Gla[3 + c], a2 = Gla[6 + c];
+    const double a3 = Gaa[c], a4 = Gaa[3 + c], a5 = Gaa[6 + c];
+    lhsAng[c * 3 + c] += std::sqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4 + a5 * a5) * af;
+  }
+  f.springC = C;
+}
+
+// spring.js:271-288 (updateDual): the constraint value and tearing only,
+// with the ramp off
+void springDual(World &W, Force &f) {
+  if (f.broken) return;
+  double d[3], n[3];
+  const double C = springEvaluate(W, f, d, n);
+  if (std::isnan(C)) return;
+  f.springC = C;
+  if (std::isfinite(f.tearStrain)) springTear(f, C);
+}
+
 // rigid.js:250-259
 bool constrainedTo(const World &W, int a, int b) {
   const std::vector<int> &fl = W.bodies[a].forces;
   for (int i = int(fl.size()) - 1; i >= 0; i--) {
     const Force &f = W.pool[fl[i]];
-    if (f.kind == JOINT && f.broken) continue;
+    if (f.kind != MANIFOLD && f.broken) continue;
     if ((f.a == a && f.b == b) || (f.a == b && f.b == a)) return true;
   }
   return false;
@@ -971,7 +1107,7 @@ bool jointConnected(const World &W, int a, int b) {
   const std::vector<int> &fl = W.bodies[a].forces;
   for (int i = int(fl.size()) - 1; i >= 0; i--) {
     const Force &f = W.pool[fl[i]];
-    if (f.kind != JOINT || f.broken) continue;
+    if (f.kind == MANIFOLD || f.broken) continue;
     if ((f.a == a && f.b == b) || (f.a == b && f.b == a)) return true;
   }
   return false;
@@ -1123,6 +1259,27 @@ int World::addJoint(int a, int b, const double rA[3], const double rB[3], double
   return id;
 }
 
+int World::addSpring(int a, int b, const double rA[3], const double rB[3], double stiffness, double rest,
+                     double tearStrain) {
+  const int id = newForce(*this, SPRING, a, b);
+  Force &f = pool[id];
+  v_copy(f.jrA, rA);
+  v_copy(f.jrB, rB);
+  f.stiffness = stiffness;
+  f.tearStrain = tearStrain;
+  if (rest < 0) {
+    // spring.js:78-84: the anchors' current distance
+    double pA[3], pB[3], d[3];
+    transformPoint(pA, bodies[a].p, bodies[a].q, f.jrA);
+    transformPoint(pB, bodies[b].p, bodies[b].q, f.jrB);
+    v_sub(d, pA, pB);
+    f.rest = std::sqrt(v_lenSq(d));
+  } else {
+    f.rest = rest;
+  }
+  return id;
+}
+
 // layout.js:50-57 — lowbias32 integer hash
 uint32_t colorPriority(uint32_t x) {
   x ^= x >> 16;
@@ -1260,7 +1417,9 @@ void World::step() {
   for (size_t k = 0; k < forces.size(); k++) {
     const int id = forces[k];
     Force &f = pool[id];
-    const bool keep = f.kind == MANIFOLD ? manifoldInitialize(*this, f) : jointInitialize(*this, f);
+    const bool keep = f.kind == MANIFOLD ? manifoldInitialize(*this, f)
+                      : f.kind == JOINT  ? jointInitialize(*this, f)
+                                         : springInitialize(*this, f);
     if (keep) {
       forces[write++] = id;
     } else {
@@ -1321,7 +1480,8 @@ void World::step() {
       for (int k = int(b.forces.size()) - 1; k >= 0; k--) {
         Force &f = pool[b.forces[k]];
         if (f.kind == MANIFOLD) manifoldPrimal(*this, f, bi, alpha, lhsLin, lhsAng, lhsCross, rhsLin, rhsAng);
-        else jointPrimal(*this, f, bi, alpha, lhsLin, lhsAng, lhsCross, rhsLin, rhsAng);
+        else if (f.kind == JOINT) jointPrimal(*this, f, bi, alpha, lhsLin, lhsAng, lhsCross, rhsLin, rhsAng);
+        else springPrimal(*this, f, bi, lhsLin, lhsAng, lhsCross, rhsLin, rhsAng);
       }
       v_neg(rhsLin, rhsLin);
       v_neg(rhsAng, rhsAng);
@@ -1334,7 +1494,8 @@ void World::step() {
       for (int k = int(forces.size()) - 1; k >= 0; k--) {
         Force &f = pool[forces[k]];
         if (f.kind == MANIFOLD) manifoldDual(*this, f, alpha);
-        else jointDual(*this, f, alpha);
+        else if (f.kind == JOINT) jointDual(*this, f, alpha);
+        else springDual(*this, f);
       }
     }
     if (it == P.iterations - 1) {

@@ -59,7 +59,7 @@ struct Contact {                 // manifold.js:34-70
   }
 };
 
-enum ForceKind : int { MANIFOLD = 0, JOINT = 1 };
+enum ForceKind : int { MANIFOLD = 0, JOINT = 1, SPRING = 2 };
 
 struct Force {
   int kind = MANIFOLD;
@@ -78,6 +78,8 @@ struct Force {
   bool broken = false;
   double C0Lin[3] = {0}, C0Ang[3] = {0}, kLin[3] = {0}, kAng[3] = {0};
   double lamLin[3] = {0}, lamAng[3] = {0}, CLin[3] = {0}, CAng[3] = {0};
+  // --- spring (spring.js:63-105); anchors in jrA / jrB, torn flag in broken
+  double stiffness = 0, rest = 0, tearStrain = 0, penalty = 0, springC = 0, peakStrain = 0;
 };
 
 // Primal sweep schedule.
@@ -115,6 +117,9 @@ struct World {
                 const double pos[3], const double vel[3]);
   int addJoint(int a, int b, const double rA[3], const double rB[3],
                double stiffLin, double stiffAng, double fracture);
+  // A distance spring (spring.js:63-105); rest < 0 takes the current length.
+  int addSpring(int a, int b, const double rA[3], const double rB[3], double stiffness, double rest,
+                double tearStrain);
   int removeBodies(const std::vector<char> &doomed);
   void step();
   void buildColoring(Coloring &out) const;

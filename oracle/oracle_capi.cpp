@@ -62,6 +62,28 @@ int orc_add_joint(void *w, int a, int b, const double *rA, const double *rB, dou
   return W.addJoint(a, b, rA, rB, stiffLin, stiffAng, fracture);
 }
 
+int orc_add_spring(void *w, int a, int b, const double *rA, const double *rB, double stiffness, double rest,
+                   double tearStrain) {
+  World &W = *static_cast<World *>(w);
+  const int n = int(W.bodies.size());
+  if (a < 0 || b < 0 || a >= n || b >= n) return -1;
+  return W.addSpring(a, b, rA, rB, stiffness, rest, tearStrain);
+}
+
+// Spring force idx of the force list: penalty, C, torn, peak strain, rest.
+int orc_get_spring(void *w, int idx, double *out) {
+  const World &W = *static_cast<World *>(w);
+  if (idx < 0 || idx >= int(W.forces.size())) return -1;
+  const Force &f = W.pool[W.forces[idx]];
+  if (f.kind != SPRING) return -1;
+  out[0] = f.penalty;
+  out[1] = f.springC;
+  out[2] = f.broken ? 1.0 : 0.0;
+  out[3] = f.peakStrain;
+  out[4] = f.rest;
+  return 0;
+}
+
 void orc_set_schedule(void *w, int mode, const int *order, int n) {
   World &W = *static_cast<World *>(w);
   W.schedule = Schedule(mode);
@@ -112,9 +134,10 @@ int orc_num_dynamic(void *w) {
 // infinities and every other token goes through strtod.
 //   body   sx sy sz density friction px py pz vx vy vz qx qy qz qw wx wy wz
 //   joint  idxA idxB rax ray raz rbx rby rbz stiffLin stiffAng fracture
+//   spring idxA idxB rax ray raz rbx rby rbz stiffness rest
 // Returns the number of bodies read; -1 when the file cannot be opened, -2
-// for a short record or a joint naming a body not yet read, -3 for a spring
-// record (springs are not modelled) and -4 for an unknown record.
+// for a short record or a joint or spring naming a body not yet read, and -4
+// for an unknown record.
 static double sceneValue(const char *t) {
   if (std::strcmp(t, "inf") == 0) return std::numeric_limits<double>::infinity();
   if (std::strcmp(t, "-inf") == 0) return -std::numeric_limits<double>::infinity();
@@ -148,7 +171,13 @@ int orc_load_scene(void *w, const char *path) {
       for (int i = 0; i < 9; i++) v[i] = sceneValue(tok[3 + i]);
       W.addJoint(ia < 0 ? -1 : ids[ia], ids[ib], v, v + 3, v[6], v[7], v[8]);
     } else if (std::strcmp(tok[0], "spring") == 0) {
-      rc = -3;
+      // spring idxA idxB rax ray raz rbx rby rbz stiffness rest
+      if (tok.size() < 11) { rc = -2; break; }
+      const int ia = int(sceneValue(tok[1])), ib = int(sceneValue(tok[2]));
+      if (ia < 0 || ib < 0 || ib >= int(ids.size()) || ia >= int(ids.size())) { rc = -2; break; }
+      double v[8];
+      for (int i = 0; i < 8; i++) v[i] = sceneValue(tok[3 + i]);
+      W.addSpring(ids[ia], ids[ib], v, v + 3, v[6], v[7], std::numeric_limits<double>::infinity());
     } else {
       rc = -4;
     }

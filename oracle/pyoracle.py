@@ -57,6 +57,8 @@ def lib():
             "orc_solve6": (None, [pd, pd, pd, pd, pd, pd, pd]),
             "orc_load_scene": (i, [v, C.c_char_p]),
             "orc_remove_bodies": (i, [v, pi, i]),
+            "orc_add_spring": (i, [v, i, i, pd, pd, d, d, d]),
+            "orc_get_spring": (i, [v, i, pd]),
             "orc_format_state": (C.c_long, [v, i, C.c_char_p, C.c_long]),
         }
         for name, (res, args) in sig.items():
@@ -127,6 +129,13 @@ class Oracle:
                                    _dp(pos), _dp(vel), _dp(q), _dp(w))
         for j in scene.get("joints", []):
             rA, rB = _vec(j["rA"], 3), _vec(j["rB"], 3)
+            if j.get("type") == "spring":
+                r = self.L.orc_add_spring(self.h, int(j["a"]), int(j["b"]), _dp(rA), _dp(rB),
+                                          _num(j["stiffness"]), _num(j.get("rest", -1.0)),
+                                          _num(j.get("tearStrain", "inf")))
+                if r < 0:
+                    raise ValueError(f"bad spring {j}")
+                continue
             r = self.L.orc_add_joint(self.h, int(j["a"]), int(j["b"]), _dp(rA), _dp(rB), _num(j["kLin"]),
                                      _num(j["kAng"]), _num(j["fracture"]))
             if r < 0:
@@ -224,6 +233,13 @@ class Oracle:
         if nc < 0:
             raise IndexError(idx)
         return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], features=feat[:nc], contacts=con[:nc])
+
+    def spring(self, idx):
+        """Spring force idx: penalty, C, torn, peak strain, rest."""
+        out = np.zeros(5, dtype=np.float64)
+        if self.L.orc_get_spring(self.h, int(idx), _dp(out)) < 0:
+            raise IndexError(idx)
+        return out
 
     def joint(self, idx):
         out = np.zeros(25, dtype=np.float64)

@@ -21,7 +21,7 @@ import { pathToFileURL } from 'url';
 const REF = process.argv[2] || '/root/reference';
 const OUT = process.argv[3] || 'tests/golden';
 const imp = (p) => import(pathToFileURL(join(REF, p)).href);
-let Solver, Rigid, Joint, Manifold, Coloring, colorOrder;
+let Solver, Rigid, Joint, Spring, Manifold, Coloring, colorOrder;
 
 // Seeded LCG (Numerical Recipes constants), uniform in [0, 1).
 function lcg(seed) {
@@ -156,6 +156,33 @@ function scenePyramid(layers, broadphase, steps) {
   return { bodies, joints: [], params: { gravity: -9.81, iterations: 20, broadphase }, steps };
 }
 
+function sceneSprings() {
+  // The parity harness's two spring scenes side by side (gen_scenes.mjs:213-237),
+  // a spring that tears, and a spring of current rest length between two
+  // touching boxes (the spring keeps them from making a contact).
+  const bodies = [box([100, 100, 1], 0, 0.5, [0, 0, 0])];
+  const joints = [];
+  const spring = (a, b, rA, rB, stiffness, rest, extra = {}) =>
+    joints.push({ type: 'spring', a, b, rA, rB, stiffness, rest, tearStrain: Infinity, ...extra });
+  bodies.push(box([1, 1, 1], 0, 0.5, [0, 0, 14]));
+  bodies.push(box([2, 2, 2], 1, 0.5, [0, 0, 8]));
+  spring(1, 2, [0, 0, 0], [0, 0, 0], 100, 4);
+  let prev = -1;
+  for (let i = 0; i < 8; i++) {
+    bodies.push(box([1, 0.75, 0.75], i === 0 || i === 7 ? 0 : 1, 0.5, [(i - 3.5) * 3.0, 10, 12]));
+    const cur = bodies.length - 1;
+    if (prev >= 0) spring(prev, cur, [0.5, 0, 0], [-0.5, 0, 0], i % 2 === 0 ? 10 : 10000, 3);
+    prev = cur;
+  }
+  bodies.push(box([1, 1, 1], 0, 0.5, [0, -10, 14]));
+  bodies.push(box([1.5, 1.5, 1.5], 5, 0.5, [0, -10, 9]));
+  spring(bodies.length - 2, bodies.length - 1, [0, 0, -0.5], [0, 0, 0.75], 2000, 3.5, { tearStrain: 0.05 });
+  bodies.push(box([1, 1, 1], 1, 0.5, [8, 0, 0.5]));
+  bodies.push(box([1, 1, 1], 1, 0.5, [9.002, 0, 0.5]));
+  spring(bodies.length - 2, bodies.length - 1, [0.5, 0, 0], [-0.5, 0, 0], 500, -1);
+  return { bodies, joints, params: { gravity: -10, iterations: 10 }, steps: 120 };
+}
+
 // ---------------------------------------------------------------- runner
 function build(def) {
   const solver = new Solver();
@@ -174,6 +201,11 @@ function build(def) {
     return b;
   });
   for (const j of def.joints) {
+    if (j.type === 'spring') {
+      const sp = new Spring(solver, bodies[j.a], bodies[j.b], j.rA, j.rB, j.stiffness, j.rest);
+      sp.tearStrain = j.tearStrain;
+      continue;
+    }
     new Joint(solver, j.a < 0 ? null : bodies[j.a], bodies[j.b], j.rA, j.rB, j.kLin, j.kAng, j.fracture);
   }
   return solver;
@@ -284,7 +316,9 @@ function run(name, def, schedule) {
   const enc = (x) => (x === Infinity ? 'inf' : x);
   const scene = {
     bodies: def.bodies,
-    joints: def.joints.map((j) => ({ ...j, kLin: enc(j.kLin), kAng: enc(j.kAng), fracture: enc(j.fracture) })),
+    joints: def.joints.map((j) => (j.type === 'spring'
+      ? { ...j, stiffness: enc(j.stiffness), tearStrain: enc(j.tearStrain) }
+      : { ...j, kLin: enc(j.kLin), kAng: enc(j.kAng), fracture: enc(j.fracture) })),
   };
   const doc = { name, schedule, params: { dt: 1 / 60, ...def.params }, steps: def.steps, scene, perStep: steps, samples };
   writeFileSync(join(OUT, `${name}.json`), JSON.stringify(doc));
@@ -295,6 +329,7 @@ async function main() {
   ({ Solver } = await imp('src/physics/solver.js'));
   ({ Rigid } = await imp('src/physics/rigid.js'));
   ({ Joint } = await imp('src/physics/joint.js'));
+  ({ Spring } = await imp('src/physics/spring.js'));
   ({ Manifold } = await imp('src/physics/manifold.js'));
   ({ Coloring } = await imp('src/physics/coloring.js'));
   ({ colorOrder } = await imp('src/physics/gpu/layout.js'));
@@ -314,6 +349,8 @@ run('pyramid5_prio', scenePyramid(5, 'bruteforce', 60), 'prio');
 run('rain_prio', sceneRain(), 'prio');
 run('pendulum_prio', scenePendulum(), 'prio');
 run('chain_prio', sceneChain(), 'prio');
+run('springs_color', sceneSprings(), 'color');
+run('springs_reverse', sceneSprings(), 'reverse');
 }
 
 main().catch((e) => { console.error(e); process.exit(1); });

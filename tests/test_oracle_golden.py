@@ -34,7 +34,8 @@ def _force_words(o):
     kab, _ = o.forces()
     words = []
     for i, (kind, a, b, nc) in enumerate(kab):
-        words += [int(kind), int(a) & 0xFFFFFFFF, int(b), int(nc)]
+        # the fixtures write 0 for a manifold and 1 for any other force
+        words += [min(int(kind), 1), int(a) & 0xFFFFFFFF, int(b), int(nc)]
         if kind == 0:
             words += [int(f) for f in o.manifold(i)["features"]]
     return np.array(words, dtype=np.uint32)
