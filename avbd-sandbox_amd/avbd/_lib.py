@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("AVBD_LIB") or os.path.join(PKG_ROOT, "libavbd_hip.so"
 
 AVBD_OK, AVBD_E_INVALID, AVBD_E_HIP, AVBD_E_CAPACITY, AVBD_E_STATE = 0, -1, -2, -3, -4
 PREC_EXACT, PREC_F64, PREC_MIXED = 0, 1, 2
+JOINT_BALL, JOINT_SPRING = 0, 1
 SHAPE_BOX, SHAPE_SPHERE = 0, 1
 NUM_PHASES = 6
 PHASES = ("broad", "narrow", "topology", "colour", "solve", "total")
@@ -52,7 +53,7 @@ class Scene(C.Structure):
     _fields_ = [("num_bodies", i32), ("shape", pi), ("size", pd), ("density", pd), ("friction", pd),
                 ("pos", pd), ("quat", pd), ("vel", pd), ("angvel", pd), ("scene_of", pi),
                 ("num_joints", i32), ("joint_a", pi), ("joint_b", pi), ("joint_ra", pd), ("joint_rb", pd),
-                ("joint_stiff_lin", pd), ("joint_stiff_ang", pd), ("joint_fracture", pd)]
+                ("joint_stiff_lin", pd), ("joint_stiff_ang", pd), ("joint_fracture", pd), ("joint_kind", pi)]
 
 
 class Counters(C.Structure):

@@ -60,16 +60,24 @@ def scene_arrays(scene):
         "angvel": np.array([b.get("angvel", (0, 0, 0)) for b in bodies], np.float64).reshape(n, 3),
         "scene_of": np.array([b.get("scene", 0) for b in bodies], np.int32),
     }
+    # joints and springs, in creation order; a spring ({"type": "spring", a,
+    # b, rA, rB, stiffness, rest (negative or absent: current length),
+    # tearStrain (absent: never)}) rides the joint arrays (include/avbd.h)
     js = scene.get("joints", [])
     m = len(js)
+    spring = [j.get("type") == "spring" for j in js]
     A.update({
         "joint_a": np.array([j["a"] for j in js], np.int32),
         "joint_b": np.array([j["b"] for j in js], np.int32),
         "joint_ra": np.array([j["rA"] for j in js], np.float64).reshape(m, 3),
         "joint_rb": np.array([j["rB"] for j in js], np.float64).reshape(m, 3),
-        "joint_stiff_lin": np.array([_num(j["kLin"]) for j in js], np.float64),
-        "joint_stiff_ang": np.array([_num(j["kAng"]) for j in js], np.float64),
-        "joint_fracture": np.array([_num(j["fracture"]) for j in js], np.float64),
+        "joint_stiff_lin": np.array([_num(j["stiffness"] if sp else j["kLin"]) for j, sp in zip(js, spring)],
+                                    np.float64),
+        "joint_stiff_ang": np.array([_num(j.get("rest", -1.0) if sp else j["kAng"]) for j, sp in zip(js, spring)],
+                                    np.float64),
+        "joint_fracture": np.array([_num(j.get("tearStrain", "inf") if sp else j["fracture"])
+                                    for j, sp in zip(js, spring)], np.float64),
+        "joint_kind": np.array([L.JOINT_SPRING if sp else L.JOINT_BALL for sp in spring], np.int32),
     })
     return {k: np.ascontiguousarray(v) for k, v in A.items()}
 
@@ -153,6 +161,7 @@ class HipBackend:
             s.joint_ra, s.joint_rb = _dp(A["joint_ra"]), _dp(A["joint_rb"])
             s.joint_stiff_lin, s.joint_stiff_ang = _dp(A["joint_stiff_lin"]), _dp(A["joint_stiff_ang"])
             s.joint_fracture = _dp(A["joint_fracture"])
+            s.joint_kind = _ip(A["joint_kind"])
         return s
 
     # --- stepping (backend.js:1393-1620, 1804-1818)

@@ -10,8 +10,9 @@ test/parity/scene_loader.mjs:20-84 and test/parity/reference_main.cpp:45-124):
 
 '#' starts a comment; tokens are separated by blanks, tabs and CR; "inf" and
 "-inf" are the infinities. Bodies are boxes, numbered in creation order; a
-joint's idxA of -1 anchors it to the world (rA is then a world point). Spring
-records are rejected: springs are not part of this step.
+joint's idxA of -1 anchors it to the world (rA is then a world point).
+Joints and springs are forces in file order; a spring's negative rest length
+means the anchors' current distance.
 
 The trajectory dump is reference_main.cpp's output (lines 126-140): after each
 step, one line per body, "step index px py pz qx qy qz qw vx vy vz wx wy wz",
@@ -26,6 +27,7 @@ from .backend import AvbdError, HipBackend
 
 BODY_FIELDS = 18
 JOINT_FIELDS = 11  # idxA idxB + 9 numbers
+SPRING_FIELDS = 10  # idxA idxB + 8 numbers
 
 
 def _value(tok):
@@ -65,8 +67,14 @@ def parse_scene(text):
             joints.append(dict(a=ia if ia >= 0 else -1, b=ib, rA=v[0:3], rB=v[3:6], kLin=v[6], kAng=v[7],
                                fracture=v[8]))
         elif kind == "spring":
-            raise AvbdError(L.AVBD_E_INVALID, f"line {lineno}: spring records are not supported (no springs "
-                                              "on this step)")
+            if len(tok) < 1 + SPRING_FIELDS:
+                raise AvbdError(L.AVBD_E_INVALID, f"line {lineno}: spring record needs {SPRING_FIELDS} fields")
+            ia, ib = int(_value(tok[1])), int(_value(tok[2]))
+            if not (0 <= ia < len(bodies)) or not (0 <= ib < len(bodies)):
+                raise AvbdError(L.AVBD_E_INVALID, f"line {lineno}: spring names a body not yet defined")
+            v = [_value(t) for t in tok[3:11]]
+            joints.append(dict(type="spring", a=ia, b=ib, rA=v[0:3], rB=v[3:6], stiffness=v[6], rest=v[7],
+                               tearStrain=math.inf))
         else:
             raise AvbdError(L.AVBD_E_INVALID, f"line {lineno}: unknown record {kind!r}")
     return dict(bodies=bodies, joints=joints)
@@ -113,6 +121,13 @@ def format_scene(scene, name=None):
                 list(b.get("angvel", (0, 0, 0))))
         lines.append(" ".join(["body"] + [format_number(x) for x in vals]))
     for j in scene.get("joints", []):
+        if j.get("type") == "spring":
+            if not math.isinf(_num(j.get("tearStrain", "inf"))):
+                raise AvbdError(L.AVBD_E_INVALID, "the .scene format has no tearing strain")
+            vals = list(j["rA"]) + list(j["rB"]) + [_num(j["stiffness"]), _num(j.get("rest", -1.0))]
+            lines.append(" ".join(["spring", str(int(j["a"])), str(int(j["b"]))] +
+                                  [format_number(x) for x in vals]))
+            continue
         vals = list(j["rA"]) + list(j["rB"]) + [_num(j["kLin"]), _num(j["kAng"]), _num(j["fracture"])]
         lines.append(" ".join(["joint", str(int(j["a"])), str(int(j["b"]))] + [format_number(x) for x in vals]))
     return "\n".join(lines) + "\n"

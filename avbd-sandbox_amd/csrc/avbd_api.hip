@@ -1133,6 +1133,20 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
     d.stiffLin = s.joint_stiff_lin[j];
     d.stiffAng = s.joint_stiff_ang[j];
     d.fracture = s.joint_fracture[j];
+    d.kind = s.joint_kind ? s.joint_kind[j] : AVBD_JOINT_BALL;
+    if (d.kind != AVBD_JOINT_BALL && d.kind != AVBD_JOINT_SPRING)
+      throw ApiError(AVBD_E_INVALID, "joint " + std::to_string(j) + " has an unknown kind");
+    if (d.kind == AVBD_JOINT_SPRING) {
+      if (d.a < 0) throw ApiError(AVBD_E_INVALID, "spring " + std::to_string(j) + " needs two bodies");
+      if (d.stiffAng < 0) {
+        // spring.js:78-84: rest length = the anchors' current distance
+        double pa[3], pb[3], dd[3];
+        xform(pa, &X[size_t(d.a) * XS], &X[size_t(d.a) * XS + 3], d.rA);
+        xform(pb, &X[size_t(d.b) * XS], &X[size_t(d.b) * XS + 3], d.rB);
+        for (int k = 0; k < 3; k++) dd[k] = pa[k] - pb[k];
+        d.stiffAng = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+      }
+    }
     const double ax = d.a >= 0 ? s.size[d.a * 3] : 0, ay = d.a >= 0 ? s.size[d.a * 3 + 1] : 0,
                  az = d.a >= 0 ? s.size[d.a * 3 + 2] : 0;
     const double sx = ax + s.size[d.b * 3], sy = ay + s.size[d.b * 3 + 1], sz = az + s.size[d.b * 3 + 2];
@@ -1327,7 +1341,8 @@ int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index
                                          std::to_string(keptJ.size()) + " survive, scene has " +
                                          std::to_string(scene->num_joints) + "); add joints with avbd_pack");
     for (int j = 0; j < scene->num_joints; j++)
-      if (scene->joint_a[j] != keptJ[j].a || scene->joint_b[j] != keptJ[j].b)
+      if (scene->joint_a[j] != keptJ[j].a || scene->joint_b[j] != keptJ[j].b ||
+          (scene->joint_kind ? scene->joint_kind[j] : AVBD_JOINT_BALL) != keptJ[j].kind)
         throw ApiError(AVBD_E_INVALID, "joint " + std::to_string(j) + " is not the surviving joint in that place");
 
     RepackCarry old;
@@ -1534,7 +1549,7 @@ int avbd_read_forces(avbd_ctx *ctx, int32_t *kabn, uint32_t *features, int32_t c
       k++;
     };
     for (int j = 0; j < ctx->J; j++)
-      if (J[j].broken != 2) put(1, J[j].a, J[j].b, 0, nullptr);
+      if (J[j].broken != 2) put(J[j].kind == AVBD_JOINT_SPRING ? 2 : 1, J[j].a, J[j].b, 0, nullptr);
     for (int t = 0; t < ctx->nLive; t++) {
       const int4 h = allHead[live[t]];
       put(0, h.x, h.y, h.z, &allFeat[size_t(live[t]) * MAXC]);

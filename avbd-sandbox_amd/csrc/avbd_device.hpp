@@ -26,8 +26,13 @@ constexpr int XS = 8;
 constexpr int VS = 12;
 
 // Joint, always double (joint.js:78-129).
+// kind AVBD_JOINT_SPRING: a distance spring (spring.js:63-105) in the same
+// record: rA/rB its anchors, stiffLin the stiffness, stiffAng the rest
+// length, fracture the tearing strain, kLin[0] the penalty, CLin[0] the last
+// constraint value, lamLin[0] the peak strain; broken 1 = torn (an inactive
+// entry that stays in the force list), 2 = left the force list.
 struct JointDev {
-  int32_t a, b, broken, pad;
+  int32_t a, b, broken, kind;
   double rA[3], rB[3];
   double stiffLin, stiffAng, fracture, torqueArm;
   double C0Lin[3], C0Ang[3], kLin[3], kAng[3], lamLin[3], lamAng[3], CLin[3], CAng[3];

@@ -515,11 +515,53 @@ __device__ __forceinline__ void jointAng(const JointDev &j, const double *X, dou
   o[0] = o[0] * j.torqueArm; o[1] = o[1] * j.torqueArm; o[2] = o[2] * j.torqueArm;
 }
 
-// joint.js:131-160; a broken joint leaves the force list.
+// spring.js:107-118 (_evaluate): d = pA - pB, n = d / |d|, C = |d| - rest;
+// false when the anchors (nearly) coincide.
+__device__ __forceinline__ bool springEval(const JointDev &j, const double *X, double *d, double *n, double &C) {
+  double pa[3], pb[3];
+  xform(pa, X + size_t(j.a) * XS, X + size_t(j.a) * XS + 3, j.rA);
+  xform(pb, X + size_t(j.b) * XS, X + size_t(j.b) * XS + 3, j.rB);
+  d[0] = pa[0] - pb[0]; d[1] = pa[1] - pb[1]; d[2] = pa[2] - pb[2];
+  const double len = xsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (len <= 1.0e-6) return false;
+  n[0] = d[0] / len; n[1] = d[1] / len; n[2] = d[2] / len;
+  C = len - j.stiffAng;
+  return true;
+}
+// spring.js:121-131 (_updateTearing): true when it tears now
+__device__ __forceinline__ bool springTear(JointDev &j, double C) {
+  if (j.stiffAng <= 1.0e-6) return false;
+  const double strain = jmax(0.0, C / j.stiffAng);
+  j.lamLin[0] = jmax(j.lamLin[0], strain);
+  if (strain <= j.fracture) return false;
+  j.broken = 1;
+  j.kLin[0] = 0;
+  j.CLin[0] = 0;
+  return true;
+}
+
+// joint.js:131-160; a broken joint leaves the force list. A spring
+// (spring.js:133-166, stiffness ramp off as solver.js:187 ships it) takes its
+// material stiffness as the penalty; a torn one stays, inactive, and one of
+// stiffness 0 leaves.
 __global__ void k_joint_init(int J, JointDev *joints, const double *X, StepParams P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= J) return;
   JointDev &j = joints[i];
+  if (j.kind == AVBD_JOINT_SPRING) {
+    if (j.broken == 2) return;
+    if (j.broken) { j.kLin[0] = 0; j.CLin[0] = 0; return; }
+    if (__builtin_isfinite(j.fracture)) {
+      double d[3], n[3], C;
+      if (springEval(j, X, d, n, C)) {
+        j.CLin[0] = C;
+        if (springTear(j, C)) return;
+      }
+    }
+    j.kLin[0] = j.stiffLin;
+    if (!(j.stiffLin > 0)) j.broken = 2;
+    return;
+  }
   if (j.broken) { j.broken = 2; return; }  // 2: removed
   jointLin(j, X, j.C0Lin);
   jointAng(j, X, j.C0Ang);
@@ -550,8 +592,79 @@ __device__ __forceinline__ void addLower(double *lt, const double *m) {
   lt[0] += m[0]; lt[1] += m[3]; lt[2] += m[4]; lt[3] += m[6]; lt[4] += m[7]; lt[5] += m[8];
 }
 
+__device__ __forceinline__ void outer3(double *o, const double *a, const double *b) {
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r] * b[c];
+}
+
+// Spring.updatePrimal (spring.js:168-212) with the geometric stiffness of
+// Equation 17 lumped by full 6-vector column norms (spring.js:214-269, on as
+// shipped: solver.js:190-193).
+__device__ void springStamp(Sys<double> &s, const JointDev &j, bool isA, const double *X) {
+  if (j.broken) return;
+  double d[3], n[3], C;
+  if (!springEval(j, X, d, n, C)) return;
+  const double k = j.kLin[0];
+  const double fo = k * C;
+  double rW[3], jl[3], ja[3], m0[9];
+  if (isA) {
+    qrot(rW, X + size_t(j.a) * XS + 3, j.rA);
+    jl[0] = n[0]; jl[1] = n[1]; jl[2] = n[2];
+    cross3(ja, rW, n);
+  } else {
+    qrot(rW, X + size_t(j.b) * XS + 3, j.rB);
+    jl[0] = -n[0]; jl[1] = -n[1]; jl[2] = -n[2];
+    cross3(ja, rW, n);
+    ja[0] = -ja[0]; ja[1] = -ja[1]; ja[2] = -ja[2];
+  }
+  outer3(m0, jl, jl);
+  for (int e = 0; e < 9; e++) m0[e] = m0[e] * k;
+  addLower(s.lin, m0);
+  outer3(m0, ja, ja);
+  for (int e = 0; e < 9; e++) m0[e] = m0[e] * k;
+  addLower(s.ang, m0);
+  outer3(m0, ja, jl);
+  for (int e = 0; e < 9; e++) s.cross[e] += m0[e] * k;
+  for (int e = 0; e < 3; e++) s.rl[e] += jl[e] * fo;
+  for (int e = 0; e < 3; e++) s.ra[e] += ja[e] * fo;
+
+  const double len = xsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double sg = isA ? 1.0 : -1.0;
+  double Pm[9], Sk[9], Gla[9], Gal[9], Gaa[9];
+  outer3(Pm, n, n);
+  for (int e = 0; e < 9; e++) Pm[e] = -Pm[e];
+  Pm[0] += 1; Pm[4] += 1; Pm[8] += 1;
+  for (int e = 0; e < 9; e++) Pm[e] = Pm[e] / len;
+  Sk[0] = 0; Sk[1] = -rW[2]; Sk[2] = rW[1]; Sk[3] = rW[2]; Sk[4] = 0; Sk[5] = -rW[0]; Sk[6] = -rW[1]; Sk[7] = rW[0];
+  Sk[8] = 0;
+  mul33(Gla, Pm, Sk);
+  for (int e = 0; e < 9; e++) Gla[e] = -Gla[e];
+  mul33(Gal, Sk, Pm);
+  mul33(Gaa, Sk, Pm);
+  mul33(Gaa, Gaa, Sk);
+  for (int e = 0; e < 9; e++) Gaa[e] = -Gaa[e];
+  const double ndr = n[0] * rW[0] + n[1] * rW[1] + n[2] * rW[2];
+  outer3(m0, rW, n);
+  m0[0] -= ndr; m0[4] -= ndr; m0[8] -= ndr;
+  for (int e = 0; e < 9; e++) Gaa[e] += m0[e] * sg;
+  const double af = xabs(fo);
+  const int diag[3] = {0, 2, 5};
+  for (int c = 0; c < 3; c++) {
+    const double a0 = Pm[c], a1 = Pm[3 + c], a2 = Pm[6 + c], a3 = Gal[c], a4 = Gal[3 + c], a5 = Gal[6 + c];
+    s.lin[diag[c]] += xsqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4 + a5 * a5) * af;
+  }
+  for (int c = 0; c < 3; c++) {
+    const double a0 = Gla[c], a1 = Gla[3 + c], a2 = Gla[6 + c], a3 = Gaa[c], a4 = Gaa[3 + c], a5 = Gaa[6 + c];
+    s.ang[diag[c]] += xsqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4 + a5 * a5) * af;
+  }
+}
+
 // Joint.updatePrimal (joint.js:162-253) into the body's system.
 __device__ void jointStamp(Sys<double> &s, const JointDev &j, bool isA, const double *X, double alpha) {
+  if (j.kind == AVBD_JOINT_SPRING) {
+    springStamp(s, j, isA, X);
+    return;
+  }
   const double INF = __builtin_inf();
   if (j.kLin[0] * j.kLin[0] + j.kLin[1] * j.kLin[1] + j.kLin[2] * j.kLin[2] > 0) {
     double K[9] = {j.kLin[0], 0, 0, 0, j.kLin[1], 0, 0, 0, j.kLin[2]};
@@ -628,6 +741,16 @@ __global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParam
   if (i >= J) return;
   JointDev &j = joints[i];
   if (j.broken == 2) return;
+  if (j.kind == AVBD_JOINT_SPRING) {
+    // Spring.updateDual (spring.js:271-288): the constraint value and
+    // tearing; with the ramp off there is no penalty update
+    if (j.broken) return;
+    double d[3], n[3], C;
+    if (!springEval(j, X, d, n, C)) return;
+    j.CLin[0] = C;
+    if (__builtin_isfinite(j.fracture)) springTear(j, C);
+    return;
+  }
   const double INF = __builtin_inf();
   if (j.kLin[0] * j.kLin[0] + j.kLin[1] * j.kLin[1] + j.kLin[2] * j.kLin[2] > 0) {
     double K[9] = {j.kLin[0], 0, 0, 0, j.kLin[1], 0, 0, 0, j.kLin[2]}, C[3], F[3];

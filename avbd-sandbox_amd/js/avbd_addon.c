@@ -154,6 +154,7 @@ static napi_value js_pack(napi_env env, napi_callback_info info) {
   TA(joint_stiff_lin, "jointStiffLin", NULL);
   TA(joint_stiff_ang, "jointStiffAng", NULL);
   TA(joint_fracture, "jointFracture", NULL);
+  TA(joint_kind, "jointKind", NULL);
 #undef TA
   s.num_bodies = (int32_t)n;
   s.num_joints = s.joint_a ? (int32_t)nj : 0;

@@ -68,7 +68,10 @@ export class HipBackend {
       A.vel.set(b.velocityLin, i * 3);
       A.angvel.set(b.velocityAng, i * 3);
     });
-    const joints = solver.forces.filter((f) => f.rA && f.rB && 'stiffnessLin' in f);
+    // joints and springs (spring.js: stiffness, rest, tearStrain), in the
+    // force list's creation order
+    const isSpring = (f) => 'stiffness' in f && 'rest' in f && !('stiffnessLin' in f);
+    const joints = solver.forces.filter((f) => f.rA && f.rB && ('stiffnessLin' in f || isSpring(f)));
     const m = joints.length;
     let oldIndex = null;
     if (prev && this.numBodies > 0) {
@@ -117,14 +120,22 @@ export class HipBackend {
       A.jointStiffLin = new Float64Array(m);
       A.jointStiffAng = new Float64Array(m);
       A.jointFracture = new Float64Array(m);
+      A.jointKind = new Int32Array(m);
       joints.forEach((j, k) => {
         A.jointA[k] = j.bodyA ? this.bodyIndex.get(j.bodyA) : -1;
         A.jointB[k] = this.bodyIndex.get(j.bodyB);
         A.jointRA.set(j.rA, k * 3);
         A.jointRB.set(j.rB, k * 3);
-        A.jointStiffLin[k] = j.stiffnessLin;
-        A.jointStiffAng[k] = j.stiffnessAng;
-        A.jointFracture[k] = j.fracture;
+        if (isSpring(j)) {
+          A.jointKind[k] = 1;
+          A.jointStiffLin[k] = j.stiffness;
+          A.jointStiffAng[k] = j.rest;
+          A.jointFracture[k] = j.tearStrain === undefined ? Infinity : j.tearStrain;
+        } else {
+          A.jointStiffLin[k] = j.stiffnessLin;
+          A.jointStiffAng[k] = j.stiffnessAng;
+          A.jointFracture[k] = j.fracture;
+        }
       });
     }
     addon.setParams(this.ctx, {

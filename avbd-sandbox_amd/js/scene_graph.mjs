@@ -101,6 +101,43 @@ export class Joint {
   }
 }
 
+/** Distance spring (spring.js:63-105): rest < 0 takes the anchors' current
+ * distance; tearStrain (tensile strain at which it tears) starts at Infinity. */
+export class Spring {
+  constructor(solver, bodyA, bodyB, rA, rB, stiffness, rest = -1) {
+    this.solver = solver;
+    this.bodyA = bodyA;
+    this.bodyB = bodyB;
+    this.rA = Float64Array.from(rA);
+    this.rB = Float64Array.from(rB);
+    this.stiffness = stiffness;
+    if (rest < 0) {
+      const pA = transformPoint(bodyA.positionLin, bodyA.positionAng, this.rA);
+      const pB = transformPoint(bodyB.positionLin, bodyB.positionAng, this.rB);
+      const d = [pA[0] - pB[0], pA[1] - pB[1], pA[2] - pB[2]];
+      this.rest = Math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    } else {
+      this.rest = rest;
+    }
+    this.tearStrain = Infinity;
+    this.broken = false;
+    solver.forces.push(this);
+    bodyA.forces.push(this);
+    bodyB.forces.push(this);
+  }
+}
+
+// p + q v q* (maths.js:604-627)
+function transformPoint(p, q, v) {
+  const [ux, uy, uz, w] = q;
+  const [vx, vy, vz] = v;
+  const tx = (uy * vz - uz * vy) * 2;
+  const ty = (uz * vx - ux * vz) * 2;
+  const tz = (ux * vy - uy * vx) * 2;
+  return [vx + tx * w + (uy * tz - uz * ty) + p[0], vy + ty * w + (uz * tx - ux * tz) + p[1],
+    vz + tz * w + (ux * ty - uy * tx) + p[2]];
+}
+
 /** Build a scene graph from the JSON scene description the Python side uses. */
 export function fromDescription(desc, params = {}) {
   const solver = new Solver();
@@ -115,6 +152,11 @@ export function fromDescription(desc, params = {}) {
     return b;
   });
   for (const j of desc.joints || []) {
+    if (j.type === 'spring') {
+      const sp = new Spring(solver, bodies[j.a], bodies[j.b], j.rA, j.rB, num(j.stiffness), j.rest === undefined ? -1 : j.rest);
+      sp.tearStrain = j.tearStrain === undefined ? Infinity : num(j.tearStrain);
+      continue;
+    }
     new Joint(solver, j.a < 0 ? null : bodies[j.a], bodies[j.b], j.rA, j.rB, num(j.kLin), num(j.kAng),
       num(j.fracture));
   }

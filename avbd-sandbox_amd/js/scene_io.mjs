@@ -4,13 +4,12 @@
 // loadScene(solver, path) builds Rigid and Joint objects from a `.scene` file
 // record for record, in creation order (the reader of
 // test/parity/scene_loader.mjs:20-84 and reference_main.cpp:45-124; format
-// gen_scenes.mjs:8-14). Spring records are rejected: springs are not part of
-// this step. dumpState(bodies, step) returns the lines reference_main.cpp
+// gen_scenes.mjs:8-14). dumpState(bodies, step) returns the lines reference_main.cpp
 // prints after each step (lines 126-140), numbers formatted exactly as C's
 // printf %.17g would, so dumps written here and by a C program compare as
 // text.
 import { readFileSync } from 'node:fs';
-import { Joint, Rigid } from './scene_graph.mjs';
+import { Joint, Rigid, Spring } from './scene_graph.mjs';
 
 function value(tok) {
   if (tok === 'inf') return Infinity;
@@ -44,7 +43,14 @@ export function parseScene(solver, text) {
       const v = tok.slice(3, 12).map(value);
       new Joint(solver, ia < 0 ? null : bodies[ia], bodies[ib], v.slice(0, 3), v.slice(3, 6), v[6], v[7], v[8]);
     } else if (tok[0] === 'spring') {
-      throw new Error(`line ${line}: spring records are not supported (no springs on this step)`);
+      if (tok.length < 11) throw new Error(`line ${line}: spring record needs 10 fields`);
+      const ia = Math.trunc(value(tok[1]));
+      const ib = Math.trunc(value(tok[2]));
+      if (!(ia >= 0 && ia < bodies.length) || !(ib >= 0 && ib < bodies.length)) {
+        throw new Error(`line ${line}: spring names a body not yet defined`);
+      }
+      const v = tok.slice(3, 11).map(value);
+      new Spring(solver, bodies[ia], bodies[ib], v.slice(0, 3), v.slice(3, 6), v[6], v[7]);
     } else {
       throw new Error(`line ${line}: unknown record: ${tok[0]}`);
     }

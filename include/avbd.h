@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 1
+#define AVBD_ABI_VERSION 2
 
 /* Error codes. */
 #define AVBD_OK 0
@@ -129,7 +129,18 @@ typedef struct avbd_scene {
   const double *joint_stiff_lin; /* [J] INFINITY = hard                     */
   const double *joint_stiff_ang; /* [J] 0 = no angular rows                 */
   const double *joint_fracture;  /* [J] INFINITY = unbreakable              */
+  /* [J] or NULL (all ball joints): AVBD_JOINT_BALL, or AVBD_JOINT_SPRING for a
+   * distance spring (spring.js:63-105), whose entries read joint_a/joint_b
+   * (both bodies), joint_ra/joint_rb (body-frame anchors), joint_stiff_lin as
+   * the stiffness, joint_stiff_ang as the rest length (negative: the anchors'
+   * current distance) and joint_fracture as the tensile strain at which it
+   * tears (INFINITY: never). Joints and springs are forces in the order given,
+   * which is their creation order. */
+  const int32_t *joint_kind;
 } avbd_scene;
+
+#define AVBD_JOINT_BALL 0
+#define AVBD_JOINT_SPRING 1
 
 /* Step counters (layout.js:171-178 CTR_*; backend.js:1622-1792 lastStats). */
 typedef struct avbd_counters {
@@ -218,7 +229,7 @@ int avbd_read_pairs(avbd_ctx *ctx, int32_t *pairs, int32_t cap, int32_t *count);
  * colour count (coloring.js:43-118). */
 int avbd_read_coloring(avbd_ctx *ctx, int32_t *color_of, int32_t n, int32_t *num_colors);
 /* Live force list in the order the step holds it (solver.js forces[]):
- * per force kind (0 manifold, 1 joint), body A (-1 world), body B and contact
+ * per force kind (0 manifold, 1 joint, 2 spring), body A (-1 world), body B and contact
  * count; `features` receives 8 words per force (contact feature keys,
  * collide.js:254-285). Either output may be NULL. */
 int avbd_read_forces(avbd_ctx *ctx, int32_t *kind_a_b_nc, uint32_t *features, int32_t cap,

@@ -176,7 +176,7 @@ function sceneSprings() {
   }
   bodies.push(box([1, 1, 1], 0, 0.5, [0, -10, 14]));
   bodies.push(box([1.5, 1.5, 1.5], 5, 0.5, [0, -10, 9]));
-  spring(bodies.length - 2, bodies.length - 1, [0, 0, -0.5], [0, 0, 0.75], 2000, 3.5, { tearStrain: 0.05 });
+  spring(bodies.length - 2, bodies.length - 1, [0, 0, -0.5], [0, 0, 0.75], 2000, 3.75, { tearStrain: 0.03 });
   bodies.push(box([1, 1, 1], 1, 0.5, [8, 0, 0.5]));
   bodies.push(box([1, 1, 1], 1, 0.5, [9.002, 0, 0.5]));
   spring(bodies.length - 2, bodies.length - 1, [0.5, 0, 0], [-0.5, 0, 0], 500, -1);

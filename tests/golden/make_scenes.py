@@ -1,9 +1,7 @@
 """Writes the `.scene` fixtures under tests/golden/scenes/.
 
 The scenes are the reference parity harness's set (test/parity/gen_scenes.mjs:
-71-261), restated here as data through this package's own writer; the two
-spring scenes (gen_scenes.mjs:213-237) are left out, as springs are not part
-of this step. Run from the repository root:
+71-261), restated here as data through this package's own writer. Run from the repository root:
 
     python tests/golden/make_scenes.py
 """
@@ -111,6 +109,25 @@ def scenes():
                                 quat=[0.3 / nn * sn, 0.8 / nn * sn, 0.5 / nn * sn, math.cos(a * 0.5)],
                                 angvel=[0.5, -0.9, 0.3]))
     S["tumble"] = s
+
+    def spring(a, b, rA, rB, stiffness, rest):
+        return dict(type="spring", a=a, b=b, rA=list(rA), rB=list(rB), stiffness=stiffness, rest=rest,
+                    tearStrain=math.inf)
+    s = ground()
+    s["bodies"].append(body([1, 1, 1], 0, 0.5, [0, 0, 14]))
+    s["bodies"].append(body([2, 2, 2], 1, 0.5, [0, 0, 8]))
+    s["joints"].append(spring(1, 2, [0, 0, 0], [0, 0, 0], 100, 4))
+    S["spring"] = s
+
+    s = dict(bodies=[body([100, 100, 1], 0, 0.5, [0, 0, -10])], joints=[])
+    prev = -1
+    for i in range(8):
+        s["bodies"].append(body([1, 0.75, 0.75], 0 if i in (0, 7) else 1, 0.5, [(i - 3.5) * 3.0, 0, 12]))
+        cur = len(s["bodies"]) - 1
+        if prev >= 0:
+            s["joints"].append(spring(prev, cur, [0.5, 0, 0], [-0.5, 0, 0], 10 if i % 2 == 0 else 10000, 3))
+        prev = cur
+    S["spring_ratio"] = s
     return S
 
 

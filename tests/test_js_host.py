@@ -29,7 +29,7 @@ def test_addon_loads_and_exports():
                           "console.log(JSON.stringify({k:Object.keys(a),v:a.abiVersion()}))"],
                          capture_output=True, text=True, check=True).stdout
     d = json.loads(out)
-    assert d["v"] == 1
+    assert d["v"] == 2
     assert set(EXPORTS) <= set(d["k"])
 
 
@@ -40,7 +40,7 @@ def test_backend_module_imports():
           " typeof b.readColoring, typeof b.setProfiling, m.abiVersion()); })")
     r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, cwd=ROOT)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.split() == ["function"] * 7 + ["1"]
+    assert r.stdout.split() == ["function"] * 7 + ["2"]
 
 
 @needs_node

@@ -27,9 +27,9 @@ NAMES = [os.path.splitext(os.path.basename(p))[0] for p in SCENES]
 
 
 def test_fixture_set_is_the_reference_parity_set():
-    # test/parity/gen_scenes.mjs:71-261 without its two spring scenes
+    # test/parity/gen_scenes.mjs:71-261
     assert NAMES == sorted(["ground", "spinner", "stack", "stack_ratio", "pyramid", "rope", "heavy_rope",
-                            "rope_angular", "static_friction", "tumble"])
+                            "rope_angular", "static_friction", "spring", "spring_ratio", "tumble"])
 
 
 @pytest.mark.parametrize("path", SCENES, ids=NAMES)
@@ -81,11 +81,12 @@ def test_comments_blanks_tabs_and_crlf():
 
 
 @pytest.mark.parametrize("text,c_code", [
-    ("spring 0 1 0 0 0 0 0 0 100 4\n", -3),
+    ("body 1 1 1 1 0.5 0 0 0 0 0 0 0 0 0 1 0 0 0\nspring 0 1 0 0 0 0 0 0 100 4\n", -2),
+    ("body 1 1 1 1 0.5 0 0 0 0 0 0 0 0 0 1 0 0 0\nspring 0 0 0 0 0 0 0 100\n", -2),
     ("box 1 1 1\n", -4),
     ("body 1 1 1 1 0.5 0 0 0\n", -2),
     ("body 1 1 1 1 0.5 0 0 0 0 0 0 0 0 0 1 0 0 0\njoint 0 1 0 0 0 0 0 0 inf 0 inf\n", -2),
-], ids=["spring", "unknown", "short-body", "joint-forward-reference"])
+], ids=["spring-forward-reference", "short-spring", "unknown", "short-body", "joint-forward-reference"])
 def test_rejections_agree_with_the_c_loader(tmp_path, text, c_code):
     with pytest.raises(AvbdError) as e:
         sceneio.parse_scene(text)
