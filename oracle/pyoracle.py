@@ -5,8 +5,9 @@ CPU-baseline leg of bench.py, always as the checker or the timed CPU
 reference, never as the product path.
 
 A scene is given as a dict with ``bodies`` (shape, size, density, friction,
-pos, vel, quat, angvel) and ``joints`` (a, b, rA, rB, kLin, kAng, fracture),
-the same description the HIP backend's ``pack`` takes.
+pos, vel, quat, angvel) and ``joints`` (a, b, rA, rB, kLin, kAng, fracture; or,
+for a spring, type="spring", a, b, rA, rB, stiffness, rest, tearStrain), the
+same description the HIP backend's ``pack`` takes.
 """
 import ctypes as C
 import math

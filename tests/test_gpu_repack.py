@@ -191,3 +191,23 @@ def test_js_host_keeps_state_across_an_erase():
     ref.add_bodies([S.box([1, 1, 1], 1, 0.5, [3, 0, -15])])
     ref.step(20)
     assert d["frame"] == ref.format_state(40)
+
+
+def test_erase_with_springs_exact():
+    # the spring fixture (reference engine goldens): a link of the stiffness-
+    # ratio chain is erased after the tearable spring tore (step 12), so the
+    # springs on either side of it go and the torn one stays, inactive
+    doc = json.load(open(os.path.join(ROOT, "tests", "golden", "springs_color.json")))
+    scene, params = doc["scene"], doc["params"]
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    gpu.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    check_lockstep(gpu, ref, 20, "before")
+    doomed = [6]
+    scene2, old_index = edit(scene, doomed, [S.box([1, 1, 1], 1.0, 0.5, [0.0, 4.0, 6.0])])
+    gpu.repack(scene2, old_index)
+    mask = np.zeros(ref.n, bool)
+    mask[doomed] = True
+    ref.remove_bodies(mask)
+    ref.add_bodies(scene2["bodies"][-1:])
+    check_lockstep(gpu, ref, 30, "after")
