@@ -94,7 +94,7 @@ class HipBackend:
         rc = self.L.avbd_create(C.byref(self.ctx), C.byref(cfg))
         if rc != 0:
             raise AvbdError(rc, "avbd_create failed (no GPU, or bad configuration)")
-        self.n = 0
+        self.n = self.nj = 0
         self.precision = precision
         self.params = L.Params()
         self.L.avbd_default_params(C.byref(self.params))
@@ -130,7 +130,7 @@ class HipBackend:
             self._ck(self.L.avbd_set_params(self.ctx, C.byref(self.params)))
         s = self._scene_struct(scene)
         self._ck(self.L.avbd_pack(self.ctx, C.byref(s)))
-        self.n = s.num_bodies
+        self.n, self.nj = s.num_bodies, s.num_joints
 
     def repack(self, scene, old_index, params=None):
         """A topology edit that keeps state (GpuBackend.pack's identity-
@@ -145,7 +145,7 @@ class HipBackend:
         if oi.shape != (s.num_bodies,):
             raise AvbdError(L.AVBD_E_INVALID, "old_index needs one entry per body")
         self._ck(self.L.avbd_repack(self.ctx, C.byref(s), _ip(oi)))
-        self.n = s.num_bodies
+        self.n, self.nj = s.num_bodies, s.num_joints
 
     def _scene_struct(self, scene):
         A = scene_arrays(scene)
@@ -178,6 +178,20 @@ class HipBackend:
         vel, angvel, prev = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
         self._ck(self.L.avbd_read_state(self.ctx, _dp(pos), _dp(quat), _dp(vel), _dp(angvel), _dp(prev), n))
         return dict(pos=pos, quat=quat, vel=vel, angvel=angvel, prev_vel=prev)
+
+    def read_joints(self):
+        """(num_joints, 16) per joint or spring, in the scene's order:
+        penaltyLin[3] penaltyAng[3] lambdaLin[3] lambdaAng[3] broken, and a
+        spring's penalty, C and peak strain (backend.js:2020-2034)."""
+        out = np.zeros((self.nj, 16))
+        self._ck(self.L.avbd_read_joints(self.ctx, _dp(out), self.nj))
+        return out
+
+    def read_step_start(self):
+        """The pose x_t the last step started from: (pos, quat)."""
+        pos, quat = np.zeros((self.n, 3)), np.zeros((self.n, 4))
+        self._ck(self.L.avbd_read_step_start(self.ctx, _dp(pos), _dp(quat), self.n))
+        return pos, quat
 
     def export_poses(self, alpha=1.0, out=None):
         """Per body {p, 0, q} as float32 (n, 8) at a fraction alpha of the

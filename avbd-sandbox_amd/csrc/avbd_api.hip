@@ -1519,6 +1519,43 @@ static std::vector<JointDev> hostJoints(avbd_ctx *ctx) {
   return J;
 }
 
+int avbd_read_joints(avbd_ctx *ctx, double *out, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (n != ctx->J) throw ApiError(AVBD_E_INVALID, "joint count mismatch");
+    if (!out) throw ApiError(AVBD_E_INVALID, "out is NULL");
+    const std::vector<JointDev> J = hostJoints(ctx);
+    for (int j = 0; j < n; j++) {
+      const JointDev &d = J[j];
+      double *o = out + size_t(j) * 16;
+      for (int k = 0; k < 16; k++) o[k] = 0.0;
+      o[12] = d.broken ? 1.0 : 0.0;
+      if (d.kind == AVBD_JOINT_SPRING) {
+        o[13] = d.kLin[0];
+        o[14] = d.CLin[0];
+        o[15] = d.lamLin[0];
+      } else {
+        for (int k = 0; k < 3; k++) {
+          o[k] = d.kLin[k]; o[3 + k] = d.kAng[k]; o[6 + k] = d.lamLin[k]; o[9 + k] = d.lamAng[k];
+        }
+      }
+    }
+  });
+}
+
+int avbd_read_step_start(avbd_ctx *ctx, double *pos, double *quat, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    std::vector<double> X(size_t(n) * XS);
+    HIPCHECK(hipMemcpyAsync(X.data(), ctx->X0.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; i++) {
+      if (pos) for (int k = 0; k < 3; k++) pos[i * 3 + k] = X[size_t(i) * XS + k];
+      if (quat) for (int k = 0; k < 4; k++) quat[i * 4 + k] = X[size_t(i) * XS + 3 + k];
+    }
+  });
+}
+
 int avbd_read_forces(avbd_ctx *ctx, int32_t *kabn, uint32_t *features, int32_t cap, int32_t *count) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");

@@ -263,6 +263,33 @@ static napi_value js_write_state(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* readJoints(ctx, out Float64Array of 16 per joint or spring) and
+ * readStepStart(ctx, pos, quat): the rest of syncToCPU (backend.js:1984-2039) */
+static napi_value js_read_joints(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  size_t n = 0;
+  double *out = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
+  int rc = avbd_read_joints(ctx, out, (int32_t)(n / 16));
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
+static napi_value js_read_step_start(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  size_t n = 0;
+  double *pos = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
+  double *quat = argc > 2 ? (double *)typed_data(env, argv[2], NULL) : NULL;
+  int rc = avbd_read_step_start(ctx, pos, quat, (int32_t)(n / 3));
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 /* exportPoses(ctx, alpha, out Float32Array of 8 per body): the poses a
  * renderer draws at a fraction alpha of the last step (renderer.js:1327-1383) */
 static napi_value js_export_poses(napi_env env, napi_callback_info info) {
@@ -418,6 +445,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"writeState", NULL, js_write_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"exportPoses", NULL, js_export_poses, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readJoints", NULL, js_read_joints, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readStepStart", NULL, js_read_step_start, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readColoring", NULL, js_read_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"uploadColoring", NULL, js_upload_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setBodyOrder", NULL, js_set_body_order, NULL, NULL, NULL, napi_enumerable, NULL},

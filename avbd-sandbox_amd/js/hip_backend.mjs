@@ -193,15 +193,48 @@ export class HipBackend {
     return { data, numBodies: n, counters: addon.readCounters(this.ctx) };
   }
 
+  // Device state back into the scene graph (backend.js:1984-2039): bodies by
+  // their packed index, including the step-start pose (initialLin /
+  // initialAng), and the packed joints' penalties, multipliers and fracture,
+  // and the springs' penalty, tearing and peak strain.
   async syncToCPU(solver) {
     const { data } = await this.readState();
+    const n = this.numBodies;
+    const p0 = new Float64Array(n * 3);
+    const q0 = new Float64Array(n * 4);
+    addon.readStepStart(this.ctx, p0, q0);
+    const setv = (obj, key, src) => {
+      if (obj[key] && obj[key].set) obj[key].set(src);
+      else obj[key] = Float64Array.from(src);
+    };
     solver.bodies.forEach((b, i) => {
       b.positionLin.set(data.pos.subarray(i * 3, i * 3 + 3));
       b.positionAng.set(data.quat.subarray(i * 4, i * 4 + 4));
       b.velocityLin.set(data.vel.subarray(i * 3, i * 3 + 3));
       b.velocityAng.set(data.angvel.subarray(i * 3, i * 3 + 3));
       b.prevVelocityLin.set(data.prevVel.subarray(i * 3, i * 3 + 3));
+      setv(b, 'initialLin', p0.subarray(i * 3, i * 3 + 3));
+      setv(b, 'initialAng', q0.subarray(i * 4, i * 4 + 4));
     });
+    const joints = this.packedJoints || [];
+    if (joints.length) {
+      const st = new Float64Array(joints.length * 16);
+      addon.readJoints(this.ctx, st);
+      joints.forEach((j, k) => {
+        const o = st.subarray(k * 16, k * 16 + 16);
+        if ('stiffnessLin' in j) {
+          setv(j, 'penaltyLin', o.subarray(0, 3));
+          setv(j, 'penaltyAng', o.subarray(3, 6));
+          setv(j, 'lambdaLin', o.subarray(6, 9));
+          setv(j, 'lambdaAng', o.subarray(9, 12));
+          if (o[12] > 0.5) j.broken = true;
+        } else {
+          j.penalty = o[13];
+          j.broken = o[12] > 0.5;
+          j.peakStrain = o[15];
+        }
+      });
+    }
   }
 
   // --- poses for drawing (renderer.js:1327-1383; boot.js:2223-2231): per

@@ -216,6 +216,16 @@ int avbd_export_poses(avbd_ctx *ctx, double alpha, float *dst, int32_t n, int32_
  * after avbd_sync. Any output pointer may be NULL. */
 int avbd_pose_source(avbd_ctx *ctx, const double **pose, const double **start_pose, int32_t *stride_bytes);
 
+/* The rest of syncToCPU (backend.js:1984-2039): the device-owned constraint
+ * state and the step-start poses, for a host that mirrors them.
+ * avbd_read_joints: per entry of the scene's joint arrays (n = num_joints), 16
+ * doubles: penaltyLin[3] penaltyAng[3] lambdaLin[3] lambdaAng[3] (joints),
+ * broken (1 once fractured or torn), then for a spring its penalty, last C
+ * and peak tensile strain. avbd_read_step_start: the pose x_t the last step
+ * started from (initialLin / initialAng); either pointer may be NULL. */
+int avbd_read_joints(avbd_ctx *ctx, double *out, int32_t n);
+int avbd_read_step_start(avbd_ctx *ctx, double *pos, double *quat, int32_t n);
+
 /* Overwrites body state (the device-authoritative half of syncToCPU's
  * inverse, backend.js:662-736). */
 int avbd_write_state(avbd_ctx *ctx, const double *pos, const double *quat, const double *vel,
