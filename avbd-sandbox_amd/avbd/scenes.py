@@ -202,3 +202,53 @@ def ramp_lanes(n=8, seed=3, angle_deg=30.0):
 
 
 CONFIGS = {"stack10": stack10, "pyramid": pyramid, "rain": rain, "batched": batched, "ramp": ramp}
+
+
+def fabric(count=15000, cell=0.9, density=1.0, friction=0.5, tear_strain=0.75, pins=2, balls=16, seed=9):
+    """A hanging spring-lattice cloth of `count` sphere particles, the
+    sandbox's fabric spawner (scenes.js:1185-1301): cols = ceil(sqrt(1.5
+    count)), 0.55-cell beads, structural links (k = 240 density (cell/0.9)^2,
+    scenes.js:989-990), shear links (0.7 k) and bend links (0.22 k), all of
+    their initial length and tearing at strain `tear_strain`; `pins` top-edge
+    particles static. `balls` heavy spheres (3 cells, 20x the density) are
+    thrown at the curtain at 8 m/s from 5 cells in front of it."""
+    cols = min(count, max(1, math.ceil(math.sqrt(count * 1.5))))
+    rows = max(1, math.ceil(count / cols))
+    pinned = set()
+    for p in range(min(pins, cols)):
+        pinned.add(0 if pins == 1 else round(p * (cols - 1) / (pins - 1)))
+    diameter = cell * 0.55
+    k = 240.0 * max(0.05, density) * (cell / 0.9) ** 2
+    top = diameter + (rows - 1) * cell + 1.0
+    bodies, joints = [ground(friction)], []
+    grid = {}
+    made = 0
+    for r in range(rows):
+        for c in range(cols):
+            if made == count:
+                break
+            u = c / max(1, cols - 1)
+            v = r / max(1, rows - 1)
+            billow = math.sin(math.pi * u) * math.sin(math.pi * v) * cell * 0.18
+            pin = r == 0 and c in pinned
+            bodies.append(sphere(diameter, 0 if pin else density, friction,
+                                 [(c - (cols - 1) * 0.5) * cell, billow, top - r * cell]))
+            me = len(bodies) - 1
+            grid[(r, c)] = me
+            made += 1
+            for (dr, dc, scale) in ((0, -1, 1.0), (-1, 0, 1.0), (-1, -1, 0.7), (-1, 1, 0.7), (0, -2, 0.22),
+                                    (-2, 0, 0.22)):
+                other = grid.get((r + dr, c + dc))
+                if other is not None:
+                    joints.append(dict(type="spring", a=other, b=me, rA=[0, 0, 0], rB=[0, 0, 0],
+                                       stiffness=k * scale, rest=-1.0, tearStrain=tear_strain))
+    r = lcg(seed)
+    width = (cols - 1) * cell
+    height = (rows - 1) * cell
+    for i in range(balls):
+        bodies.append(sphere(3 * cell, 20 * density, friction,
+                             [(r() - 0.5) * width * 0.8, -5.0 * cell - i * 4 * cell, top - (0.2 + 0.6 * r()) * height],
+                             vel=[0.0, 8.0, 0.0]))
+    return (dict(bodies=bodies, joints=joints), dict(gravity=-9.81, iterations=10),
+            dict(steps=300, rows=rows, cols=cols, springs=len(joints)))
+

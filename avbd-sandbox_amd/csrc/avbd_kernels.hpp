@@ -1387,17 +1387,33 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
 #pragma unroll
       for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
     }
+    if constexpr (!EXACT) {
+      // joint and spring rows in f64, spread over the team: each lane stamps
+      // its share (lane 0 on top of the reduced contact terms, the others
+      // from zero) and an f64 reduction brings them to lane 0 (w.w, the
+      // body's joint-item count, is the same on every lane of the team)
+      if (w.w > 0) {
+        if (lane != 0) s.zero();
+        for (int t = b0 + lane; t < b1; t += TEAM) {
+          const uint2 itw = S.items[t];
+          if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
+        }
+        if constexpr (TEAM > 1) {
+#pragma unroll
+          for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
+#pragma unroll
+          for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
+#pragma unroll
+          for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
+        }
+      }
+    }
     if (lane != 0) continue;
     if constexpr (!EXACT) {
       if (!inDone) inertia();
       s.lin[0] += inMl; s.lin[2] += inMl; s.lin[5] += inMl;
       for (int k = 0; k < 6; k++) s.ang[k] += inAng[k];
       for (int k = 0; k < 3; k++) { s.rl[k] += inRl[k]; s.ra[k] += inRa[k]; }
-      if (w.w > 0)
-        for (int t = b0; t < b1; t++) {
-          const uint2 itw = S.items[t];
-          if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
-        }
     }
     double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
     double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};

@@ -69,7 +69,7 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
 
 
-WORKLOADS = ("pyramid", "rain", "batched")
+WORKLOADS = ("pyramid", "rain", "batched", "fabric")
 
 
 def build_workload(args):
@@ -83,6 +83,11 @@ def build_workload(args):
         scene, params, meta = avbd.scenes.rain(n=args.bodies)
         desc = (f"rain of {args.bodies} bodies (70% boxes 0.2-1 m, 30% spheres r 0.1-0.5 m, density log-uniform "
                 f"1-1000, mu 0.2-0.9) into a 40 x 40 m walled column, dt 1/60, {params['iterations']} iterations")
+    elif args.workload == "fabric":
+        scene, params, meta = avbd.scenes.fabric(count=args.bodies if args.bodies != 50000 else 15000)
+        desc = (f"hanging fabric of {meta['rows']} x {meta['cols']} sphere particles and {meta['springs']} springs "
+                f"(structural, shear, bend; tear strain 0.75) with 16 heavy spheres thrown through it, dt 1/60, "
+                f"{params['iterations']} iterations")
     else:
         scene, params, meta = avbd.scenes.batched(num_scenes=args.scenes)
         desc = (f"{args.scenes} independent scenes (chains of 10-200 hard links with a 26,000:1 end mass, stacks of "
@@ -159,7 +164,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="pyramid", choices=WORKLOADS,
-                    help="pyramid (the headline), rain (50k mixed shapes) or batched (256 scenes)")
+                    help="pyramid (the headline), rain (50k mixed shapes), batched (256 scenes) or fabric "
+                         "(15k-particle spring cloth)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f64", "exact"])
     ap.add_argument("--team", type=int, default=0, help="lanes per body; 0 = autotune")
     ap.add_argument("--order", default="priority", choices=["priority", "index"])
