@@ -199,6 +199,8 @@ struct avbd_ctx {
   std::vector<Checkpoint> ckpts;
   Checkpoint endState;
   int adjSteps = 0, adjEvery = 0, adjSmallMax = 64;
+  long adjTapeBudget = 4096L << 20;  // bytes the whole forward tape may take
+  bool tapeResident = false;         // the last adjoint forward kept every step's tape
   DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
 
   // small device scalars: 0 pairs, 1 new, 2 live, 3 changed, 4 numColors,
@@ -831,13 +833,40 @@ struct avbd_ctx {
       if (b.shape == 1) throw ApiError(AVBD_E_STATE, "the adjoint covers box contacts only; the scene has spheres");
   }
 
+  // Device bytes one recorded step holds.
+  static size_t tapeBytes(const TapeStep &T) {
+    return T.head.n * sizeof(int4) + T.cwork.n * sizeof(int4) +
+           (T.basis.n + T.anchor.n + T.hot.n + T.X.n + T.X0.n + T.Y.n + T.V.n) * sizeof(double) +
+           (T.src.n + T.live.n + T.color.n + T.cstart.n) * sizeof(int) + T.items.n * sizeof(uint2);
+  }
+
+  // Checkpoints every `every` steps. When the first step's tape, times the
+  // step count with a factor 2 for contacts that appear later, fits the tape
+  // budget, every step is recorded as it runs and the tape stays resident:
+  // the backward then reverses it directly instead of re-running each
+  // segment from its checkpoint to record it.
   void adjointForward(int nsteps, int every) {
     for (auto &c : ckpts) c.release();
     ckpts.clear();
     ckpts.resize(size_t((nsteps + every - 1) / every));
+    for (auto &t : tape) t.release();
+    tape.clear();
+    tapeResident = adjTapeBudget > 0 && nsteps > 0;
+    struct Reset {
+      avbd_ctx *c;
+      ~Reset() { c->rec = nullptr; }
+    } reset{this};
+    if (tapeResident) tape.resize(size_t(nsteps));
     for (int s = 0; s < nsteps; s++) {
       if (s % every == 0) saveCheckpoint(ckpts[size_t(s / every)]);
+      rec = tapeResident ? &tape[size_t(s)] : nullptr;
       step();
+      rec = nullptr;
+      if (s == 0 && tapeResident && double(tapeBytes(tape[0])) * 2.0 * nsteps > double(adjTapeBudget)) {
+        tape[0].release();
+        tape.clear();
+        tapeResident = false;
+      }
     }
     saveCheckpoint(endState);
     adjSteps = nsteps;
@@ -936,9 +965,12 @@ struct avbd_ctx {
       avbd_ctx *c;
       ~Reset() { c->rec = nullptr; }
     } reset{this};
-    if (tape.size() < size_t(adjEvery)) tape.resize(size_t(adjEvery));
     int lk = 0;
-    const int nseg = int(ckpts.size());
+    if (tapeResident) {
+      for (int k = adjSteps - 1; k >= 0; k--) reverseStep(tape[size_t(k)], lk);
+    }
+    if (!tapeResident && tape.size() < size_t(adjEvery)) tape.resize(size_t(adjEvery));
+    const int nseg = tapeResident ? 0 : int(ckpts.size());
     for (int seg = nseg - 1; seg >= 0; seg--) {
       restoreCheckpoint(ckpts[size_t(seg)]);
       const int len = std::min(adjEvery, adjSteps - seg * adjEvery);
@@ -1258,6 +1290,9 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->manCapCfg = cfg->manifold_cap;
     c->graphsCfg = cfg->graphs;
     c->adjSmallMax = cfg->adjoint_small == 0 ? 64 : cfg->adjoint_small;
+    c->adjTapeBudget = cfg->adjoint_tape_mb == 0 ? (4096L << 20)
+                       : cfg->adjoint_tape_mb < 0 ? -1L
+                                                  : long(cfg->adjoint_tape_mb) << 20;
     c->pairCapCfg = cfg->pair_cap;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {

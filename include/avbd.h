@@ -68,7 +68,11 @@ typedef struct avbd_config {
   int32_t adjoint_small;   /* largest dynamic-body count whose adjoint step
                               is reversed in one workgroup (0: 64; -1:
                               never, always one kernel per phase)           */
-  int32_t reserved;
+  int32_t adjoint_tape_mb; /* device memory in MB the adjoint forward may
+                              fill with the whole trajectory's tape, so that
+                              the backward needs no segment re-runs (0:
+                              4096; -1: never, re-run every checkpoint
+                              segment)                                      */
 } avbd_config;
 
 /* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform

@@ -116,3 +116,23 @@ def test_adjoint_paths_agree_on_colliding_boxes():
     assert np.abs(v0 - v1).max() <= 1e-9 * max(1.0, np.abs(v0).max())
     # static bodies never move (solver.js:396-400): their velocity is inert
     assert not v0[:2].any() and not v1[:2].any()
+
+
+@pytest.mark.parametrize("small", [0, -1], ids=["one-workgroup", "per-phase"])
+def test_resident_tape_equals_segment_reruns(small):
+    # the same forward recorded as it runs (tape kept in device memory) or
+    # re-run segment by segment from the checkpoints in the backward: the
+    # reverse reads the same tape, so the gradients are the same bits
+    scene, params, meta = avbd.scenes.ramp(16)
+    grads = []
+    for tape_mb in (0, -1):
+        g = avbd.HipBackend(precision=avbd.PREC_F64, adjoint_small=small, adjoint_tape_mb=tape_mb)
+        g.pack(scene, params)
+        g.adjoint_forward(60, 10)
+        x = g.read_state()["pos"]
+        dL = 2 * x
+        dL[:2] = 0
+        grads.append(g.adjoint_backward(dL))
+        assert np.array_equal(g.read_state()["pos"], x)
+    (m0, v0), (m1, v1) = grads
+    assert np.array_equal(m0, m1) and np.array_equal(v0, v1)
