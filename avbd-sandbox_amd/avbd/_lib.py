@@ -26,7 +26,7 @@ SYMBOLS = (
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
     "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
     "avbd_upload_coloring", "avbd_export_poses", "avbd_pose_source", "avbd_repack", "avbd_read_joints",
-    "avbd_read_step_start",
+    "avbd_read_step_start", "avbd_set_joint_anchor",
 )
 K_PRIMAL, K_DUAL, K_NARROW, K_POSES = 0, 1, 2, 3
 
@@ -105,6 +105,7 @@ def lib(path=None):
         "avbd_repack": (C.c_int, [ctx, P(Scene), pi]),
         "avbd_read_joints": (C.c_int, [ctx, pd, i32]),
         "avbd_read_step_start": (C.c_int, [ctx, pd, pd, i32]),
+        "avbd_set_joint_anchor": (C.c_int, [ctx, i32, pd]),
         "avbd_pose_source": (C.c_int, [ctx, P(vp), P(vp), pi]),
     }
     for name, (res, args) in sig.items():

@@ -188,6 +188,12 @@ class HipBackend:
         self._ck(self.L.avbd_read_joints(self.ctx, _dp(out), self.nj))
         return out
 
+    def set_joint_anchor(self, joint, rA):
+        """New anchor rA of joint `joint` (a world point for a world-anchored
+        joint) without a repack (GpuBackend.updateJointAnchor)."""
+        a = np.ascontiguousarray(np.asarray(rA, dtype=np.float64).reshape(3))
+        self._ck(self.L.avbd_set_joint_anchor(self.ctx, int(joint), _dp(a)))
+
     def read_step_start(self):
         """The pose x_t the last step started from: (pos, quat)."""
         pos, quat = np.zeros((self.n, 3)), np.zeros((self.n, 4))

@@ -8,6 +8,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -114,6 +115,7 @@ struct avbd_ctx {
   DBuf<double> X, X0, Y, V;
   DBuf<char> DQ;
   DBuf<float4> poseStage;  // avbd_export_poses to host memory
+  std::vector<double> anchorStage;  // avbd_set_joint_anchor's host copies
   std::vector<BodyProp> hprops;
 
   // broad phase
@@ -1575,6 +1577,21 @@ int avbd_read_joints(avbd_ctx *ctx, double *out, int32_t n) {
         }
       }
     }
+  });
+}
+
+int avbd_set_joint_anchor(avbd_ctx *ctx, int32_t joint, const double *rA) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (joint < 0 || joint >= ctx->J) throw ApiError(AVBD_E_INVALID, "joint index out of range");
+    if (!rA) throw ApiError(AVBD_E_INVALID, "rA is NULL");
+    // stream-ordered: the next step reads the new anchor
+    ctx->anchorStage.resize(size_t(ctx->J) * 3);
+    double *h = &ctx->anchorStage[size_t(joint) * 3];
+    for (int k = 0; k < 3; k++) h[k] = rA[k];
+    HIPCHECK(hipMemcpyAsync(reinterpret_cast<char *>(ctx->joints.p + joint) + offsetof(JointDev, rA), h,
+                            sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
   });
 }
 

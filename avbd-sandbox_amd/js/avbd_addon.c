@@ -277,6 +277,25 @@ static napi_value js_read_joints(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setJointAnchor(ctx, index, rA Float64Array[3]) (GpuBackend.updateJointAnchor) */
+static napi_value js_set_joint_anchor(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  int32_t idx = -1;
+  if (argc > 1) CHECK(env, napi_get_value_int32(env, argv[1], &idx));
+  size_t n = 0;
+  const double *rA = argc > 2 ? (const double *)typed_data(env, argv[2], &n) : NULL;
+  if (!rA || n < 3) {
+    napi_throw_range_error(env, NULL, "rA needs 3 numbers");
+    return NULL;
+  }
+  int rc = avbd_set_joint_anchor(ctx, idx, rA);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 static napi_value js_read_step_start(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -447,6 +466,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"exportPoses", NULL, js_export_poses, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readJoints", NULL, js_read_joints, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readStepStart", NULL, js_read_step_start, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setJointAnchor", NULL, js_set_joint_anchor, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readColoring", NULL, js_read_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"uploadColoring", NULL, js_upload_coloring, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setBodyOrder", NULL, js_set_body_order, NULL, NULL, NULL, napi_enumerable, NULL},

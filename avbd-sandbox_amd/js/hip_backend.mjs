@@ -25,6 +25,9 @@ export class HipBackend {
     this.gpuErrors = [];
     this.topologyDirty = true;
     this.bodyIndex = new Map();
+    this.jointIndex = new Map();
+    this.springIndex = new Map();
+    this.forceRealloc = false;
     this.numBodies = 0;
     this.lastProfile = null;
     this._profiling = false;
@@ -32,11 +35,25 @@ export class HipBackend {
   }
 
   // --- lifecycle (backend.js:337-469)
-  async init() {
-    this.ctx = addon.create({ device: this.device, precision: PRECISION[this.precision], team: this.team });
-    this.adapterInfo = 'AMD Instinct MI355X (gfx950, HIP)';
+  // initDevice + initPipelines in one (backend.js:337-469); HIP code objects
+  // are loaded with the library, so there is no pipeline build to wait for.
+  async initDevice() {
+    if (!this.ctx) {
+      this.ctx = addon.create({ device: this.device, precision: PRECISION[this.precision], team: this.team });
+      this.adapterInfo = 'AMD Instinct MI355X (gfx950, HIP)';
+    }
+    return this;
+  }
+
+  async initPipelines(onProgress) {
+    await this.initDevice();
+    if (onProgress) onProgress(1);
     this.ready = true;
     return this;
+  }
+
+  async init() {
+    return this.initPipelines();
   }
 
   // --- topology (backend.js:662-1301, 1898-1915)
@@ -74,7 +91,7 @@ export class HipBackend {
     const joints = solver.forces.filter((f) => f.rA && f.rB && ('stiffnessLin' in f || isSpring(f)));
     const m = joints.length;
     let oldIndex = null;
-    if (prev && this.numBodies > 0) {
+    if (prev && this.numBodies > 0 && !this.forceRealloc) {
       oldIndex = new Int32Array(n);
       let survivors = 0;
       let last = -1;
@@ -149,7 +166,11 @@ export class HipBackend {
     if (oldIndex) addon.pack(this.ctx, A, oldIndex);
     else addon.pack(this.ctx, A);
     this.lastPackPreservedContacts = oldIndex !== null;
+    this.forceRealloc = false;
     this.packedJoints = joints;
+    this.jointIndex = new Map();
+    this.springIndex = new Map();
+    joints.forEach((j, k) => (isSpring(j) ? this.springIndex : this.jointIndex).set(j, k));
     this.numBodies = n;
     this.topologyDirty = false;
   }
@@ -158,8 +179,40 @@ export class HipBackend {
     this.topologyDirty = true;
   }
 
+  // A synchronous one-shot flush: true when it packed (backend.js:1911-1915).
   flushTopology(solver) {
-    if (this.topologyDirty) this.pack(solver);
+    if (!this.topologyDirty) return false;
+    this.pack(solver);
+    return true;
+  }
+
+  /** Push a drag joint's world anchor without a repack (backend.js:1969-1977). */
+  updateJointAnchor(joint) {
+    const idx = this.jointIndex.get(joint);
+    if (idx === undefined) return false;
+    addon.setJointAnchor(this.ctx, idx, Float64Array.from(joint.rA));
+    return true;
+  }
+
+  /**
+   * Where a renderer finds constraint break state (backend.js:1939-1966): here
+   * a host snapshot of the packed joints and springs, 16 numbers each, the
+   * broken flag at offset 12, indexed through jointIndex / springIndex.
+   */
+  getConstraintGpuState() {
+    const n = (this.packedJoints || []).length;
+    if (!this.ctx || n === 0) return null;
+    const records = new Float64Array(n * 16);
+    addon.readJoints(this.ctx, records);
+    return {
+      records, stride: 16, jointBrokenOffset: 12, springBrokenOffset: 12,
+      jointIndex: this.jointIndex, springIndex: this.springIndex,
+    };
+  }
+
+  getSpringGpuState() {
+    const s = this.getConstraintGpuState();
+    return s && { records: s.records, springStride: 16, brokenOffset: 12, springIndex: s.springIndex };
   }
 
   // --- stepping (backend.js:1393-1620); returns the host encode time in ms

@@ -228,6 +228,11 @@ int avbd_pose_source(avbd_ctx *ctx, const double **pose, const double **start_po
  * and peak tensile strain. avbd_read_step_start: the pose x_t the last step
  * started from (initialLin / initialAng); either pointer may be NULL. */
 int avbd_read_joints(avbd_ctx *ctx, double *out, int32_t n);
+/* GpuBackend.updateJointAnchor (backend.js:1969-1977): a new anchor rA for
+ * joint `joint` (its index in the scene's joint arrays; the world point of a
+ * world-anchored joint, as a mouse drag moves it) without a repack. Ordered
+ * before the next step. */
+int avbd_set_joint_anchor(avbd_ctx *ctx, int32_t joint, const double *rA);
 int avbd_read_step_start(avbd_ctx *ctx, double *pos, double *quat, int32_t n);
 
 /* Overwrites body state (the device-authoritative half of syncToCPU's

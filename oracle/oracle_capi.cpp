@@ -70,6 +70,22 @@ int orc_add_spring(void *w, int a, int b, const double *rA, const double *rB, do
   return W.addSpring(a, b, rA, rB, stiffness, rest, tearStrain);
 }
 
+// A joint's anchor rA (a world point when it is world-anchored); joint is the
+// joint's creation rank among the joints and springs.
+int orc_set_joint_anchor(void *w, int joint, const double *rA) {
+  World &W = *static_cast<World *>(w);
+  int rank = 0;
+  for (Force &f : W.pool) {
+    if (f.kind == MANIFOLD) continue;
+    if (rank++ == joint) {
+      if (f.kind != JOINT) return -1;
+      for (int k = 0; k < 3; k++) f.jrA[k] = rA[k];
+      return 0;
+    }
+  }
+  return -1;
+}
+
 // Spring force idx of the force list: penalty, C, torn, peak strain, rest.
 int orc_get_spring(void *w, int idx, double *out) {
   const World &W = *static_cast<World *>(w);

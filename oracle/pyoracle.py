@@ -60,6 +60,7 @@ def lib():
             "orc_remove_bodies": (i, [v, pi, i]),
             "orc_add_spring": (i, [v, i, i, pd, pd, d, d, d]),
             "orc_get_spring": (i, [v, i, pd]),
+            "orc_set_joint_anchor": (i, [v, i, pd]),
             "orc_format_state": (C.c_long, [v, i, C.c_char_p, C.c_long]),
         }
         for name, (res, args) in sig.items():
@@ -234,6 +235,12 @@ class Oracle:
         if nc < 0:
             raise IndexError(idx)
         return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], features=feat[:nc], contacts=con[:nc])
+
+    def set_joint_anchor(self, joint, rA):
+        """New rA of the joint created joint-th among joints and springs."""
+        a = _vec(rA, 3)
+        if self.L.orc_set_joint_anchor(self.h, int(joint), _dp(a)) < 0:
+            raise IndexError(joint)
 
     def spring(self, idx):
         """Spring force idx: penalty, C, torn, peak strain, rest."""

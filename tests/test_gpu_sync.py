@@ -102,3 +102,60 @@ def test_js_sync_to_cpu_brings_back_tearing(tmp_path):
     assert [g[1] for g in got] == [s[3] for s in springs]
     assert [g[2] for g in got] == [s[0] for s in springs]
     assert np.array_equal(p0, want_p0)
+
+
+def drag_path(s):
+    # a world anchor dragged about (a mouse drag, boot.js), in steps that are
+    # exact binary fractions so that the JS host computes the same numbers
+    return [(s % 8) * 0.125, 3.0 + (s % 5) * 0.0625, 11.0]
+
+
+def test_joint_anchor_update_exact():
+    doc = json.load(open(os.path.join(GOLDEN, "chain_color.json")))
+    scene, params = doc["scene"], doc["params"]
+    world = [k for k, j in enumerate(scene["joints"]) if j["a"] < 0]
+    assert world == [7]
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    gpu.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    for s in range(1, 41):
+        gpu.set_joint_anchor(7, drag_path(s))
+        ref.set_joint_anchor(7, drag_path(s))
+        gpu.step(1)
+        ref.step(1)
+        g = gpu.read_state()["pos"]
+        assert np.array_equal(g, ref.state()[:, :3]), f"step {s}"
+    with pytest.raises(avbd.AvbdError):
+        gpu.set_joint_anchor(len(scene["joints"]), [0, 0, 0])
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_update_joint_anchor(tmp_path):
+    doc = json.load(open(os.path.join(GOLDEN, "chain_color.json")))
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": doc["scene"], "params": doc["params"]}))
+    url = "file://" + os.path.join(ROOT, "avbd-sandbox_amd", "js")
+    js = ("import { readFileSync } from 'fs';"
+          f"import {{ HipBackend }} from '{url}/hip_backend.mjs';"
+          f"import {{ fromDescription }} from '{url}/scene_graph.mjs';"
+          f"const d = JSON.parse(readFileSync({json.dumps(str(path))}, 'utf8'));"
+          "const solver = fromDescription(d.scene, d.params);"
+          "const b = new HipBackend({ precision: 'exact' });"
+          "const drag = solver.forces.find((f) => f.bodyA === null);"
+          "(async () => { await b.init(); b.flushTopology(solver);"
+          " for (let s = 1; s <= 20; s++) {"
+          "  drag.rA.set([(s % 8) * 0.125, 3.0 + (s % 5) * 0.0625, 11.0]);"
+          "  if (!b.updateJointAnchor(drag)) throw new Error('not a packed joint');"
+          "  b.step(solver); }"
+          " await b.syncToCPU(solver);"
+          " console.log(JSON.stringify(solver.bodies.map((x) => Array.from(x.positionLin))));"
+          "})().catch((e) => { console.error(e); process.exit(1); });")
+    script = tmp_path / "drag.mjs"
+    script.write_text(js)
+    r = subprocess.run([shutil.which("node"), str(script)], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = pyoracle.Oracle(doc["scene"], doc["params"], schedule=pyoracle.SCHED_COLOR)
+    for s in range(1, 21):
+        ref.set_joint_anchor(7, drag_path(s))
+        ref.step(1)
+    assert np.array_equal(np.array(json.loads(r.stdout)), ref.state()[:, :3])

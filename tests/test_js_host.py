@@ -35,12 +35,19 @@ def test_addon_loads_and_exports():
 
 @needs_node
 def test_backend_module_imports():
+    # the GpuBackend surface (backend.js:243-2226) a host calls
+    methods = ["init", "initDevice", "initPipelines", "pack", "flushTopology", "prepareTopologyChange", "step",
+               "idle", "readState", "syncToCPU", "readColoring", "uploadColoring", "setProfiling",
+               "updateJointAnchor", "getConstraintGpuState", "getSpringGpuState", "exportPoses"]
     js = ("import('./avbd-sandbox_amd/js/hip_backend.mjs').then(m => { const b = new m.HipBackend();"
-          " console.log(typeof b.pack, typeof b.step, typeof b.idle, typeof b.readState, typeof b.syncToCPU,"
-          " typeof b.readColoring, typeof b.setProfiling, m.abiVersion()); })")
+          f" const names = {json.dumps(methods)};"
+          " b.topologyDirty = false;"
+          " console.log(names.map((k) => typeof b[k]).join(' '), b.bodyIndex instanceof Map,"
+          " b.jointIndex instanceof Map, b.springIndex instanceof Map, b.flushTopology({}), m.abiVersion()); })")
     r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, cwd=ROOT)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.split() == ["function"] * 7 + ["2"]
+    # a clean flush is a no-op that says so (backend.js:1911-1915)
+    assert r.stdout.split() == ["function"] * len(methods) + ["true", "true", "true", "false", "2"]
 
 
 @needs_node
