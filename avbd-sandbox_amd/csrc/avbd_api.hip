@@ -145,6 +145,9 @@ struct avbd_ctx {
   DBuf<int4> cwork;
   std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
+  // The colour array is the greedy fixed point of the force graph as it stood
+  // at the last colouring, in claim order colorFor (-1: no valid colouring).
+  int colorFor = -1;
   // verification hooks: a caller's claim order (rank per body) and a caller's
   // colouring for the following steps
   bool useRank = false, useUpload = false;
@@ -396,11 +399,17 @@ struct avbd_ctx {
   // at once. The bucket kernels follow the batch, so when it converged a single
   // readback brings the colour count and buckets to the host. The batch is
   // sized from the previous step's round count.
-  void colorize() {
+  // graphSame: the force graph is the one the colour array was computed for
+  // (the step kept every manifold and made none, and there are no joints that
+  // could have broken): one round then confirms the fixed point instead of a
+  // batch re-deriving it. Pairs the broad phase offers that make no contact
+  // do not change the graph.
+  void colorize(bool graphSame) {
     HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
     const int maxRounds = N + 2;
     const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
     bool uploaded = useUpload;
+    const bool confirm = graphSame && !uploaded && colorFor == order;
     for (int launched = 0;;) {
       if (uploaded) {
         // the uploaded colouring, if it is proper for this step's forces
@@ -409,11 +418,12 @@ struct avbd_ctx {
         hipLaunchKernelGGL(k_color_validate, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
                            adjOther.p, color[0].p, cmeta.p + 5, scal.p + 5);
       } else {
-        for (int r = 0; r < colorBatch; r++) {
+        const int batch = confirm && launched == 0 ? 1 : colorBatch;
+        for (int r = 0; r < batch; r++) {
           hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p,
                              adjOther.p, color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
         }
-        launched += colorBatch;
+        launched += batch;
       }
       HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
       const int nb = blocks(N);
@@ -436,7 +446,8 @@ struct avbd_ctx {
       if (launched > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
     }
     colorRounds = hmeta[3];
-    if (!uploaded) colorBatch = std::min(64, std::max(4, colorRounds + 2));
+    if (!uploaded && !confirm) colorBatch = std::min(64, std::max(4, colorRounds + 2));
+    colorFor = uploaded ? -1 : order;
     numColors = hmeta[4];
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
@@ -662,8 +673,12 @@ struct avbd_ctx {
     if (nTot > 0)
       hipLaunchKernelGGL(k_live_compact, dim3(blocks(nTot)), dim3(256), 0, stream, nTot, keep.p, keepPos.p,
                          live[1 - cur].p);
-    readScal(2, 7);
+    // how many of last step's manifolds survived (the exclusive scan at nOld)
+    if (nNew > 0 && nOld > 0)
+      HIPCHECK(hipMemcpyAsync(scal.p + 9, keepPos.p + nOld, sizeof(int), hipMemcpyDeviceToDevice, stream));
+    readScal(2, 8);
     nLive = hscal[2];
+    const int keptOld = nNew > 0 ? (nOld > 0 ? hscal[9] : 0) : nLive;
     contactsTotal = hscal[8];
     cur = 1 - cur;
     lastNTot = nTot;
@@ -674,7 +689,8 @@ struct avbd_ctx {
     buildAdjacency();
     buildItems();
     record(3);
-    colorize();
+    // the same force graph as last step: every old manifold kept, none made
+    colorize(keptOld == nOld && nLive == nOld && J == 0);
     if (profiling && numColors > 0) {
       HIPCHECK(hipMemsetAsync(cbytes.p, 0, sizeof(unsigned long long) * 2 * MAX_COLORS, stream));
       hipLaunchKernelGGL(k_color_bytes, dim3(blocks(N)), dim3(256), 0, stream, N, J, props.p, color[colorCur].p,
@@ -1034,6 +1050,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.cur = 0;
   c.nPairs = c.nNew = 0;
   c.numColors = 0;
+  c.colorFor = -1;
 
   // scenes must be contiguous runs of bodies
   std::vector<int> sceneOf(N, 0);

@@ -229,3 +229,23 @@ def test_graph_replay_equals_direct_launches(graphs):
     st = g.read_state()
     o = ref.state()
     assert np.array_equal(st["pos"], o[:, :3]) and np.array_equal(st["quat"], o[:, 3:7])
+
+
+def test_unchanged_force_graph_confirms_the_colouring_in_one_round():
+    # a settled stack keeps its contact set from step to step: the colouring
+    # is confirmed by one relaxation round and still equals the oracle's
+    scene, params, _ = avbd.scenes.stack10()
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    gpu.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    one_round = 0
+    for s in range(1, 151):
+        gpu.step(1)
+        ref.step(1)
+        gc, gn = gpu.read_coloring()
+        rc, rn = ref.last_coloring()
+        assert gn == rn and np.array_equal(gc, rc), f"step {s}: colourings differ"
+        one_round += gpu.counters()["color_rounds"] == 1
+    assert np.array_equal(gpu.read_state()["pos"], ref.state()[:, :3])
+    print(f"[colour] {one_round} of 150 steps confirmed in one round", flush=True)
+    assert one_round > 0
