@@ -116,6 +116,7 @@ struct avbd_ctx {
   DBuf<char> DQ;
   DBuf<float4> poseStage;  // avbd_export_poses to host memory
   std::vector<double> anchorStage;  // avbd_set_joint_anchor's host copies
+  std::vector<int> hjointKind;      // AVBD_JOINT_* of each packed joint
   std::vector<BodyProp> hprops;
 
   // broad phase
@@ -1043,6 +1044,10 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   HIPCHECK(hipSetDevice(c.device));
   HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   HIPCHECK(hipHostMalloc((void **)&c.hscal, sizeof(int) * 16, hipHostMallocDefault));
+  if (s.num_joints < 0) throw ApiError(AVBD_E_INVALID, "num_joints must not be negative");
+  if (s.num_joints > 0 && (!s.joint_a || !s.joint_b || !s.joint_ra || !s.joint_rb || !s.joint_stiff_lin ||
+                           !s.joint_stiff_ang || !s.joint_fracture))
+    throw ApiError(AVBD_E_INVALID, "joint arrays are required when num_joints > 0");
   c.N = N;
   c.J = s.num_joints;
   c.steps = 0;
@@ -1203,6 +1208,8 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
     const double sx = ax + s.size[d.b * 3], sy = ay + s.size[d.b * 3 + 1], sz = az + s.size[d.b * 3 + 2];
     d.torqueArm = sx * sx + sy * sy + sz * sz;
   }
+  c.hjointKind.resize(size_t(c.J));
+  for (int j = 0; j < c.J; j++) c.hjointKind[size_t(j)] = JD[size_t(j)].kind;
   c.joints.reserve(std::max(c.J, 1));
   if (c.J) HIPCHECK(hipMemcpy(c.joints.p, JD.data(), sizeof(JointDev) * c.J, hipMemcpyHostToDevice));
 
@@ -1390,6 +1397,8 @@ int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index
       k.b = nb;
       keptJ.push_back(k);
     }
+    if (scene->num_joints > 0 && (!scene->joint_a || !scene->joint_b))
+      throw ApiError(AVBD_E_INVALID, "joint arrays are required when num_joints > 0");
     if (scene->num_joints != int(keptJ.size()))
       throw ApiError(AVBD_E_INVALID, "a repack keeps the surviving joints and adds none (" +
                                          std::to_string(keptJ.size()) + " survive, scene has " +
@@ -1577,6 +1586,7 @@ int avbd_read_joints(avbd_ctx *ctx, double *out, int32_t n) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     if (n != ctx->J) throw ApiError(AVBD_E_INVALID, "joint count mismatch");
+    if (n == 0) return;
     if (!out) throw ApiError(AVBD_E_INVALID, "out is NULL");
     const std::vector<JointDev> J = hostJoints(ctx);
     for (int j = 0; j < n; j++) {
@@ -1601,6 +1611,7 @@ int avbd_set_joint_anchor(avbd_ctx *ctx, int32_t joint, const double *rA) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     if (joint < 0 || joint >= ctx->J) throw ApiError(AVBD_E_INVALID, "joint index out of range");
+    if (ctx->hjointKind[size_t(joint)] != AVBD_JOINT_BALL) throw ApiError(AVBD_E_INVALID, "not a ball joint");
     if (!rA) throw ApiError(AVBD_E_INVALID, "rA is NULL");
     // stream-ordered: the next step reads the new anchor
     ctx->anchorStage.resize(size_t(ctx->J) * 3);
