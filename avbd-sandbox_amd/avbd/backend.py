@@ -45,20 +45,32 @@ def _ip(a):
     return a.ctypes.data_as(L.pi) if a is not None else None
 
 
+def _rows(bodies, key, default, width):
+    """(n, width) float64 array of one per-body vector field (flattened
+    through fromiter: a fraction of the time of nested lists)."""
+    n = len(bodies)
+    if default is None:
+        it = (x for b in bodies for x in b[key])
+    else:
+        it = (x for b in bodies for x in b.get(key, default))
+    return np.fromiter(it, np.float64, n * width).reshape(n, width)
+
+
 def scene_arrays(scene):
     """Structure-of-arrays form of a scene dict (numpy, contiguous)."""
     bodies = scene["bodies"]
     n = len(bodies)
     A = {
-        "shape": np.array([L.SHAPE_SPHERE if b["shape"] == "sphere" else L.SHAPE_BOX for b in bodies], np.int32),
-        "size": np.array([b["size"] for b in bodies], np.float64).reshape(n, 3),
-        "density": np.array([b["density"] for b in bodies], np.float64),
-        "friction": np.array([b["friction"] for b in bodies], np.float64),
-        "pos": np.array([b["pos"] for b in bodies], np.float64).reshape(n, 3),
-        "quat": np.array([b.get("quat", (0, 0, 0, 1)) for b in bodies], np.float64).reshape(n, 4),
-        "vel": np.array([b.get("vel", (0, 0, 0)) for b in bodies], np.float64).reshape(n, 3),
-        "angvel": np.array([b.get("angvel", (0, 0, 0)) for b in bodies], np.float64).reshape(n, 3),
-        "scene_of": np.array([b.get("scene", 0) for b in bodies], np.int32),
+        "shape": np.fromiter((L.SHAPE_SPHERE if b["shape"] == "sphere" else L.SHAPE_BOX for b in bodies), np.int32,
+                             n),
+        "size": _rows(bodies, "size", None, 3),
+        "density": np.fromiter((b["density"] for b in bodies), np.float64, n),
+        "friction": np.fromiter((b["friction"] for b in bodies), np.float64, n),
+        "pos": _rows(bodies, "pos", None, 3),
+        "quat": _rows(bodies, "quat", (0, 0, 0, 1), 4),
+        "vel": _rows(bodies, "vel", (0, 0, 0), 3),
+        "angvel": _rows(bodies, "angvel", (0, 0, 0), 3),
+        "scene_of": np.fromiter((b.get("scene", 0) for b in bodies), np.int32, n),
     }
     # joints and springs, in creation order; a spring ({"type": "spring", a,
     # b, rA, rB, stiffness, rest (negative or absent: current length),
