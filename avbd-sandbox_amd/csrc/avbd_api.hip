@@ -99,6 +99,7 @@ struct avbd_ctx {
   int device = 0, prec = AVBD_PREC_MIXED, team = 16;
   int simds = 1024;  // SIMDs of the device (4 per CU), read at creation
   int manCapCfg = 0, pairCapCfg = 0;
+  int needPairs = 0, needMan = 0;  // sizes an overflow asked for, for the next pack
   hipStream_t stream = nullptr;
   avbd_params prm{};
   bool packed = false;
@@ -653,6 +654,7 @@ struct avbd_ctx {
     nNew = hscal[1];
     if (nPairs > pairCap) {
       nPairs = 0;
+      needPairs = std::max(needPairs, hscal[0]);
       throw ApiError(AVBD_E_CAPACITY, "broad-phase pairs " + std::to_string(hscal[0]) + " exceed capacity " +
                                           std::to_string(pairCap));
     }
@@ -660,9 +662,11 @@ struct avbd_ctx {
 
     // 2. narrow phase and force initialisation
     const int nOld = nLive, nTot = nOld + nNew;
-    if (nTot > manCap)
+    if (nTot > manCap) {
+      needMan = std::max(needMan, nTot);
       throw ApiError(AVBD_E_CAPACITY, "manifolds " + std::to_string(nTot) + " exceed capacity " +
                                           std::to_string(manCap));
+    }
     ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
     HIPCHECK(hipMemsetAsync(scal.p + 8, 0, sizeof(int), stream));
     if (nTot > 0)
@@ -1161,8 +1165,11 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   if (!largeList.empty())
     HIPCHECK(hipMemcpy(c.largeList.p, largeList.data(), sizeof(int) * largeList.size(), hipMemcpyHostToDevice));
 
-  c.pairCap = c.pairCapCfg > 0 ? c.pairCapCfg : std::max(32 * N, 4096);
-  c.manCap = c.manCapCfg > 0 ? c.manCapCfg : std::max(32 * N, 4096);
+  // an overflow seen since the last pack regrows the arena (backend.js:
+  // 1683-1739: report, then regrow on the next repack)
+  c.pairCap = std::max(c.pairCapCfg > 0 ? c.pairCapCfg : std::max(32 * N, 4096), c.needPairs + c.needPairs / 4);
+  c.manCap = std::max(c.manCapCfg > 0 ? c.manCapCfg : std::max(32 * N, 4096), c.needMan + c.needMan / 4);
+  c.needPairs = c.needMan = 0;
   c.countRev.reserve(N); c.startRev.reserve(N + 1);
   c.countNewRev.reserve(N); c.startNewRev.reserve(N + 1);
   c.pairs.reserve(c.pairCap);

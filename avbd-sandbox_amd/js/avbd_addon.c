@@ -22,10 +22,12 @@
     }                                                          \
   } while (0)
 
+/* Throws an Error whose .code is the AVBD_E_* value as a string ("-3"). */
 static napi_value throw_avbd(napi_env env, avbd_ctx *ctx, int rc) {
-  char msg[512];
+  char msg[512], code[16];
   snprintf(msg, sizeof msg, "avbd error %d: %s", rc, ctx ? avbd_last_error(ctx) : "no context");
-  napi_throw_error(env, NULL, msg);
+  snprintf(code, sizeof code, "%d", rc);
+  napi_throw_error(env, code, msg);
   return NULL;
 }
 
@@ -86,6 +88,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.device = (int32_t)get_num(env, argv[0], "device", 0);
     cfg.precision = (int32_t)get_num(env, argv[0], "precision", AVBD_PREC_MIXED);
     cfg.team = (int32_t)get_num(env, argv[0], "team", 0);
+    cfg.manifold_cap = (int32_t)get_num(env, argv[0], "manifoldCap", 0);
+    cfg.pair_cap = (int32_t)get_num(env, argv[0], "pairCap", 0);
   }
   avbd_ctx *ctx = NULL;
   int rc = avbd_create(&ctx, &cfg);

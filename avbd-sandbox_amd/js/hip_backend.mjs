@@ -15,10 +15,12 @@ export const PRECISION = { exact: 0, f64: 1, mixed: 2 };
 const PHASE_NAMES = ['broad', 'narrow', 'topology', 'colour', 'solve', 'total'];
 
 export class HipBackend {
-  constructor({ precision = 'mixed', team = 0, device = 0 } = {}) {
+  constructor({ precision = 'mixed', team = 0, device = 0, manifoldCap = 0, pairCap = 0 } = {}) {
     this.precision = precision;
     this.team = team;
     this.device = device;
+    this.manifoldCap = manifoldCap;
+    this.pairCap = pairCap;
     this.ctx = null;
     this.ready = false;
     this.adapterInfo = null;
@@ -39,7 +41,10 @@ export class HipBackend {
   // are loaded with the library, so there is no pipeline build to wait for.
   async initDevice() {
     if (!this.ctx) {
-      this.ctx = addon.create({ device: this.device, precision: PRECISION[this.precision], team: this.team });
+      this.ctx = addon.create({
+        device: this.device, precision: PRECISION[this.precision], team: this.team,
+        manifoldCap: this.manifoldCap, pairCap: this.pairCap,
+      });
       this.adapterInfo = 'AMD Instinct MI355X (gfx950, HIP)';
     }
     return this;
@@ -216,12 +221,20 @@ export class HipBackend {
   }
 
   // --- stepping (backend.js:1393-1620); returns the host encode time in ms
+  // Never throws (backend.js:366-381, 1683-1739): a failed step is recorded
+  // in gpuErrors; a capacity overflow also marks the topology dirty, so that
+  // the next step's state-keeping repack grows the arena it overflowed.
   step(solver) {
     const t0 = Date.now();
-    this.flushTopology(solver);
-    addon.step(this.ctx, 1);
-    this._queued++;
-    if (this._profiling) this._readProfile();
+    try {
+      this.flushTopology(solver);
+      addon.step(this.ctx, 1);
+      this._queued++;
+      if (this._profiling) this._readProfile();
+    } catch (e) {
+      this.gpuErrors.push(String(e && e.message ? e.message : e));
+      if (e && String(e.code) === '-3') this.topologyDirty = true;  // AVBD_E_CAPACITY
+    }
     return Date.now() - t0;
   }
 

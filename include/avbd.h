@@ -61,8 +61,12 @@ typedef struct avbd_config {
   int32_t device;          /* HIP device ordinal                            */
   int32_t precision;       /* AVBD_PREC_*                                   */
   int32_t team;            /* lanes per body in the primal solve (0: auto)  */
-  int32_t manifold_cap;    /* max live + new manifolds (0: 32 per body)     */
-  int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body)        */
+  int32_t manifold_cap;    /* max live + new manifolds (0: 32 per body); a
+                              step that overflows it fails with
+                              AVBD_E_CAPACITY and the next avbd_pack or
+                              avbd_repack grows it to 1.25x what was asked */
+  int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body); grows
+                              the same way                                  */
   int32_t graphs;          /* solve loop as a replayed hipGraph: 0 auto (on
                               unless profiling), -1 never                   */
   int32_t adjoint_small;   /* largest dynamic-body count whose adjoint step

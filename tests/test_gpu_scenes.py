@@ -17,6 +17,8 @@ import glob
 import json
 import math
 import os
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -249,3 +251,50 @@ def test_unchanged_force_graph_confirms_the_colouring_in_one_round():
     assert np.array_equal(gpu.read_state()["pos"], ref.state()[:, :3])
     print(f"[colour] {one_round} of 150 steps confirmed in one round", flush=True)
     assert one_round > 0
+
+
+def test_capacity_regrows_at_the_next_repack():
+    # backend.js:1683-1739: an overflow is reported and the next repack grows
+    # the arena; the step that overflowed left the state untouched, so the
+    # scene continues bit for bit with the oracle
+    scene, params, _ = avbd.scenes.pyramid(layers=4, speed=0.0)
+    gpu = avbd.HipBackend(precision=avbd.PREC_EXACT, pair_cap=24)
+    gpu.pack(scene, params)
+    with pytest.raises(AvbdError) as e:
+        gpu.step(1)
+    assert e.value.code == -3
+    gpu.repack(scene, list(range(len(scene["bodies"]))))
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    for s in range(1, 21):
+        gpu.step(1)
+        ref.step(1)
+        assert np.array_equal(gpu.read_state()["pos"], ref.state()[:, :3]), f"step {s}"
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_step_collects_errors_and_regrows(tmp_path):
+    scene, params, _ = avbd.scenes.pyramid(layers=4, speed=0.0)
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": scene, "params": params}))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    url = "file://" + os.path.join(root, "avbd-sandbox_amd", "js")
+    js = ("import { readFileSync } from 'fs';"
+          f"import {{ HipBackend }} from '{url}/hip_backend.mjs';"
+          f"import {{ fromDescription }} from '{url}/scene_graph.mjs';"
+          f"const d = JSON.parse(readFileSync({json.dumps(str(path))}, 'utf8'));"
+          "const solver = fromDescription(d.scene, d.params);"
+          "const b = new HipBackend({ precision: 'exact', pairCap: 24 });"
+          "(async () => { await b.init();"
+          " for (let s = 0; s < 21; s++) b.step(solver);"
+          " await b.syncToCPU(solver);"
+          " console.log(JSON.stringify({ errors: b.gpuErrors, pos: solver.bodies.map((x) => Array.from(x.positionLin)) }));"
+          "})().catch((e) => { console.error(e); process.exit(1); });")
+    script = tmp_path / "grow.mjs"
+    script.write_text(js)
+    r = subprocess.run([shutil.which("node"), str(script)], capture_output=True, text=True, cwd=root, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert len(out["errors"]) == 1 and "capacity" in out["errors"][0]
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    ref.step(20)  # 21 calls, one of which overflowed
+    assert np.array_equal(np.array(out["pos"]), ref.state()[:, :3])
