@@ -287,7 +287,11 @@ struct avbd_ctx {
     return double(nLive) * (4 + 16 + 10 * rs + 12 * rs) + double(contactsTotal) * (15 * rs + 6 * rs + 4);
   }
 
-  ~avbd_ctx() { releaseAll(); }
+  ~avbd_ctx() {
+    releaseAll();
+    for (auto &t : tape) t.release();
+    tape.clear();
+  }
 
   void releaseAll() {
     dropGraphs();
@@ -309,8 +313,11 @@ struct avbd_ctx {
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
     claimRank.release(); uploadColor.release();
-    for (auto &t : tape) t.release();
-    tape.clear();
+    // the tape's buffers are kept for the next adjoint forward (an
+    // optimisation loop packs and records the same scene again); only the
+    // record of which steps they hold is dropped
+    tapeResident = false;
+    adjSteps = 0;
     for (auto &c : ckpts) c.release();
     ckpts.clear();
     endState.release();
@@ -872,21 +879,22 @@ struct avbd_ctx {
     for (auto &c : ckpts) c.release();
     ckpts.clear();
     ckpts.resize(size_t((nsteps + every - 1) / every));
-    for (auto &t : tape) t.release();
-    tape.clear();
     tapeResident = adjTapeBudget > 0 && nsteps > 0;
     struct Reset {
       avbd_ctx *c;
       ~Reset() { c->rec = nullptr; }
     } reset{this};
-    if (tapeResident) tape.resize(size_t(nsteps));
+    if (tapeResident) {
+      for (size_t k = size_t(nsteps); k < tape.size(); k++) tape[k].release();
+      tape.resize(size_t(nsteps));
+    }
     for (int s = 0; s < nsteps; s++) {
       if (s % every == 0) saveCheckpoint(ckpts[size_t(s / every)]);
       rec = tapeResident ? &tape[size_t(s)] : nullptr;
       step();
       rec = nullptr;
       if (s == 0 && tapeResident && double(tapeBytes(tape[0])) * 2.0 * nsteps > double(adjTapeBudget)) {
-        tape[0].release();
+        for (auto &t : tape) t.release();
         tape.clear();
         tapeResident = false;
       }

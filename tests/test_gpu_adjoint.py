@@ -136,3 +136,24 @@ def test_resident_tape_equals_segment_reruns(small):
         assert np.array_equal(g.read_state()["pos"], x)
     (m0, v0), (m1, v1) = grads
     assert np.array_equal(m0, m1) and np.array_equal(v0, v1)
+
+
+def test_tape_buffers_reused_across_packs():
+    # a context keeps its tape's buffers across packs (an optimisation loop
+    # re-packs and records again): gradients equal those of a fresh context,
+    # also when the second recording is shorter than the first
+    scene, params, meta = avbd.scenes.ramp(16)
+
+    def grads(g, steps):
+        g.pack(scene, params)
+        g.adjoint_forward(steps, 10)
+        x = g.read_state()["pos"]
+        dL = 2 * x
+        dL[:2] = 0
+        return g.adjoint_backward(dL)
+
+    reused = avbd.HipBackend(precision=avbd.PREC_F64)
+    grads(reused, 60)
+    m1, v1 = grads(reused, 40)
+    m0, v0 = grads(avbd.HipBackend(precision=avbd.PREC_F64), 40)
+    assert np.array_equal(m0, m1) and np.array_equal(v0, v1)
