@@ -179,6 +179,30 @@ __device__ __forceinline__ int pairTest(const BpArgs &a, int A, int B) {
   return kind == 2 ? 1 : 2;  // 1: recorded, existing manifold; 2: recorded, new
 }
 
+// pairTest with A's force list already in LDS (entries: other body in the low
+// 30 bits, kind in the top two: 1 live joint, 2 manifold, 0 a broken joint,
+// which links nothing). The first linking entry in list order decides, as
+// linkKind does; A's list holds every force between A and B, so the answer
+// is linkKind's whichever side it reads.
+__device__ __forceinline__ int pairTestList(const BpArgs &a, int A, int B, const uint32_t *list, int len) {
+  const BodyProp &pa = a.props[A], &pb = a.props[B];
+  const double *xa = a.X + size_t(A) * XS, *xb = a.X + size_t(B) * XS;
+  const double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
+  const double r = pa.radius + pb.radius;
+  if (!(dx * dx + dy * dy + dz * dz <= r * r)) return 0;
+  if (!(pa.mass > 0 || pb.mass > 0)) return 0;
+  uint32_t kind = 0;
+  for (int k = 0; k < len; k++) {
+    const uint32_t v = list[k];
+    if ((v & 0x3fffffffu) == uint32_t(B) && (v >> 30) != 0u) {
+      kind = v >> 30;
+      break;
+    }
+  }
+  if (kind == 1u) return 0;
+  return kind == 2u ? 1 : 2;
+}
+
 // Insert (x, y) into the run list[base .. base+cnt) kept sorted by y descending.
 __device__ __forceinline__ void insertDesc(int2 *list, int base, int cnt, int cap, int x, int y) {
   int k = base + cnt;
@@ -203,14 +227,30 @@ __device__ __forceinline__ void insertDesc(int2 *list, int base, int cnt, int ca
 constexpr int BP_TEAM = 32, BP_CAP = 64, BP_NEW = 0x40000000;
 
 __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
-  constexpr int TEAMS = 256 / BP_TEAM;
+  constexpr int TEAMS = 256 / BP_TEAM, ADJ_LDS = 64;
   __shared__ int sy[TEAMS][BP_CAP];
   __shared__ int scount[TEAMS], snew[TEAMS];
+  __shared__ uint32_t sadj[TEAMS][ADJ_LDS];
   const int tm = threadIdx.x / BP_TEAM, lane = threadIdx.x % BP_TEAM;
   const int x = blockIdx.x * TEAMS + tm;
   if (lane == 0) { scount[tm] = 0; snew[tm] = 0; }
+  // x's force list, with each entry's kind, into LDS: every pair test of
+  // this body then searches it there instead of in memory
+  int alen = ADJ_LDS + 1;
+  if (x < a.n) {
+    const int e0 = a.adjStart[x];
+    alen = a.adjStart[x + 1] - e0;
+    if (alen <= ADJ_LDS)
+      for (int k = lane; k < alen; k += BP_TEAM) {
+        const int key = a.adjKey[e0 + k];
+        const uint32_t kind = key < a.J ? (a.joints[key].broken ? 0u : 1u) : 2u;
+        sadj[tm][k] = (uint32_t(a.adjOther[e0 + k]) & 0x3fffffffu) | (kind << 30);
+      }
+  }
   __syncthreads();
   const bool active = x < a.n && !a.props[x].large;
+  const bool inLds = alen <= ADJ_LDS;
+  auto test = [&](int y) { return inLds ? pairTestList(a, x, y, sadj[tm], alen) : pairTest(a, x, y); };
   if (active) {
     auto accept = [&](int y, int t) {
       const int k = atomicAdd(&scount[tm], 1);
@@ -227,7 +267,7 @@ __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
         if (y >= x) continue;
         const int4 cy = a.entryCell[e];
         if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
-        const int t = pairTest(a, x, y);
+        const int t = test(y);
         if (t) accept(y, t);
       }
     } else {
@@ -236,7 +276,7 @@ __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
       for (int e = l0 + lane - 27; e < l1; e += BP_TEAM - 27) {
         const int y = a.sceneLarges[e];
         if (y >= x) continue;
-        const int t = pairTest(a, x, y);
+        const int t = test(y);
         if (t) accept(y, t);
       }
     }
