@@ -144,3 +144,33 @@ def test_js_reader_and_dump_match_the_c_loader(path):
                        cwd=ROOT, timeout=60)
     assert r.returncode == 0, r.stderr
     assert r.stdout == pyoracle.Oracle(path).format_state(0)
+
+
+# --- against the reference's own reader (tests/golden/make_scene_vectors.mjs ran
+# test/parity/scene_loader.mjs on every fixture and kept what it built)
+VECTORS = json.load(open(os.path.join(ROOT, "tests", "golden", "scene_loader_vectors.json")))
+
+
+@pytest.mark.parametrize("path", SCENES, ids=NAMES)
+def test_readers_build_what_the_reference_reader_builds(path):
+    want = VECTORS[NAMES[SCENES.index(path)]]
+    wb = np.array(want["bodies"], dtype=np.float64)
+    # the package's reader
+    scene = sceneio.read_scene(path)
+    got = np.array([b["size"] + [b["density"], b["friction"]] + b["pos"] + b["quat"] + b["vel"] + b["angvel"]
+                    for b in scene["bodies"]])
+    assert np.array_equal(got, np.delete(wb, 5, axis=1))  # column 5 is the mass
+    forces = []
+    for j in scene["joints"]:
+        if j.get("type") == "spring":
+            forces.append(["spring", j["a"], j["b"]] + j["rA"] + j["rB"] + [j["stiffness"], j["rest"]])
+        else:
+            enc = lambda x: "inf" if x == math.inf else ("-inf" if x == -math.inf else x)  # noqa: E731
+            forces.append(["joint", j["a"], j["b"]] + j["rA"] + j["rB"] + [enc(j["kLin"]), enc(j["kAng"]),
+                                                                        enc(j["fracture"])])
+    assert forces == want["forces"]
+    # the oracle's C reader: mass, pose and velocities of every body
+    o = pyoracle.Oracle(path)
+    st = o.state()
+    assert np.array_equal(o.mass()[:, 0], wb[:, 5])
+    assert np.array_equal(st[:, :13], wb[:, 6:19])
