@@ -1,4 +1,4 @@
-// Writes tests/golden/scene_loader_vectors.json: what the reference's own
+// Writes tests/golden/scenes/reference_loader.json: what the reference's own
 // `.scene` reader (test/parity/scene_loader.mjs) builds from each fixture under
 // tests/golden/scenes/, as data (per body: size, density, friction, mass,
 // position, quaternion, velocity, angular velocity; per joint and spring:
@@ -36,7 +36,7 @@ async function main() {
           enc(c.stiffnessLin), enc(c.stiffnessAng), enc(c.fracture)])),
     };
   }
-  writeFileSync(join(OUT, 'scene_loader_vectors.json'), JSON.stringify(doc));
+  writeFileSync(join(dir, 'reference_loader.json'), JSON.stringify(doc));
   console.log(`${Object.keys(doc).length} scenes`);
 }
 

@@ -148,7 +148,7 @@ def test_js_reader_and_dump_match_the_c_loader(path):
 
 # --- against the reference's own reader (tests/golden/make_scene_vectors.mjs ran
 # test/parity/scene_loader.mjs on every fixture and kept what it built)
-VECTORS = json.load(open(os.path.join(ROOT, "tests", "golden", "scene_loader_vectors.json")))
+VECTORS = json.load(open(os.path.join(ROOT, "tests", "golden", "scenes", "reference_loader.json")))
 
 
 @pytest.mark.parametrize("path", SCENES, ids=NAMES)
