@@ -3,8 +3,7 @@ them (test/parity/reference_main.cpp:142-183: load, step, print %.17g state
 after every step), against the same dump from the oracle, which reads the
 file with its C restatement of the reference loader. Exact precision: the two
 dumps must be the same text for all 120 steps of every fixture (the
-reference's parity scene set, test/parity/gen_scenes.mjs:71-261, springs
-aside)."""
+reference's parity scene set, test/parity/gen_scenes.mjs:71-261)."""
 import glob
 import os
 import shutil
