@@ -8,6 +8,7 @@
  * carrying avbd_last_error().
  */
 #include <math.h>
+#include <stdio.h>
 #include <node_api.h>
 #include <stdlib.h>
 #include <string.h>
@@ -31,23 +32,84 @@ static napi_value throw_avbd(napi_env env, avbd_ctx *ctx, int rc) {
   return NULL;
 }
 
-static avbd_ctx *get_ctx(napi_env env, napi_value v) {
+/* The context external: the library context and the number of sync() jobs
+ * in flight on the thread pool. While one runs, every other call on the
+ * context throws (the library's calls are not re-entrant, and a pack would
+ * destroy the stream the job is waiting on). */
+typedef struct {
+  avbd_ctx *ctx;
+  int syncs;
+} Handle;
+
+static Handle *get_handle(napi_env env, napi_value v) {
   void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
-  return (avbd_ctx *)p;
+  if (!v || napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  return (Handle *)p;
 }
 
-static void *typed_data(napi_env env, napi_value v, size_t *len) {
+/* The context of argument v for a call that is not sync(); NULL after
+ * throwing when v is not a context or a sync is in flight. */
+static avbd_ctx *get_ctx(napi_env env, napi_value v) {
+  Handle *h = get_handle(env, v);
+  if (!h || !h->ctx) {
+    napi_throw_type_error(env, NULL, "the first argument must be a context from create()");
+    return NULL;
+  }
+  if (h->syncs > 0) {
+    napi_throw_error(env, NULL, "a sync() of this context is in flight: await it before any other call");
+    return NULL;
+  }
+  return h->ctx;
+}
+
+static const char *type_name(napi_typedarray_type t) {
+  switch (t) {
+    case napi_int32_array: return "Int32Array";
+    case napi_uint32_array: return "Uint32Array";
+    case napi_float32_array: return "Float32Array";
+    case napi_float64_array: return "Float64Array";
+    default: return "another typed array";
+  }
+}
+
+/* Argument v as a typed array of element type `type` holding exactly `want`
+ * elements (want < 0: any count, returned in *count). An optional argument may
+ * be absent, null or undefined (*data = NULL). Returns 1, or 0 after throwing
+ * a TypeError (not a typed array of that type) or a RangeError (wrong length),
+ * so that no array reaches the library with fewer elements than it reads or
+ * writes. */
+static int arr_arg(napi_env env, napi_value v, napi_typedarray_type type, const char *name, long long want,
+                   int optional, void **data, size_t *count) {
+  char msg[160];
+  *data = NULL;
+  if (count) *count = 0;
+  napi_valuetype vt = napi_undefined;
+  if (v && napi_typeof(env, v, &vt) != napi_ok) vt = napi_undefined;
+  if (!v || vt == napi_undefined || vt == napi_null) {
+    if (optional) return 1;
+    snprintf(msg, sizeof msg, "%s is required (a %s)", name, type_name(type));
+    napi_throw_type_error(env, NULL, msg);
+    return 0;
+  }
   bool is = false;
-  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return NULL;
   napi_typedarray_type t;
-  size_t n;
-  void *data;
+  size_t n = 0, off = 0;
+  void *p = NULL;
   napi_value ab;
-  size_t off;
-  if (napi_get_typedarray_info(env, v, &t, &n, &data, &ab, &off) != napi_ok) return NULL;
-  if (len) *len = n;
-  return data;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is ||
+      napi_get_typedarray_info(env, v, &t, &n, &p, &ab, &off) != napi_ok || t != type) {
+    snprintf(msg, sizeof msg, "%s must be a %s", name, type_name(type));
+    napi_throw_type_error(env, NULL, msg);
+    return 0;
+  }
+  if (want >= 0 && (long long)n != want) {
+    snprintf(msg, sizeof msg, "%s holds %zu elements, %lld expected", name, n, want);
+    napi_throw_range_error(env, NULL, msg);
+    return 0;
+  }
+  *data = p;
+  if (count) *count = n;
+  return 1;
 }
 
 static double get_num(napi_env env, napi_value obj, const char *key, double dflt) {
@@ -73,7 +135,11 @@ static napi_value get_prop(napi_env env, napi_value obj, const char *key) {
 static void finalize_ctx(napi_env env, void *data, void *hint) {
   (void)env;
   (void)hint;
-  if (data) avbd_destroy((avbd_ctx *)data);
+  Handle *h = (Handle *)data;
+  if (h) {
+    if (h->ctx) avbd_destroy(h->ctx);
+    free(h);
+  }
 }
 
 /* create({device, precision, team}) -> external (GpuBackend.init, backend.js:337-469) */
@@ -94,17 +160,33 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   avbd_ctx *ctx = NULL;
   int rc = avbd_create(&ctx, &cfg);
   if (rc) return throw_avbd(env, NULL, rc);
+  Handle *h = (Handle *)calloc(1, sizeof(Handle));
+  if (!h) {
+    avbd_destroy(ctx);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  h->ctx = ctx;
   napi_value ext;
-  CHECK(env, napi_create_external(env, ctx, finalize_ctx, NULL, &ext));
+  if (napi_create_external(env, h, finalize_ctx, NULL, &ext) != napi_ok) {
+    avbd_destroy(ctx);
+    free(h);
+    napi_throw_error(env, NULL, "N-API call failed: napi_create_external");
+    return NULL;
+  }
   return ext;
 }
 
 /* setParams(ctx, {dt, gravity, iterations, ...}) — Solver.defaultParams names (solver.js:92-236) */
 static napi_value js_set_params(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  napi_valuetype t = napi_undefined;
+  if (argc < 2 || napi_typeof(env, argv[1], &t) != napi_ok || t != napi_object) {
+    napi_throw_type_error(env, NULL, "params must be an object");
+    return NULL;
+  }
   avbd_params p;
   avbd_default_params(&p);
   napi_value o = argv[1];
@@ -123,52 +205,67 @@ static napi_value js_set_params(napi_env env, napi_callback_info info) {
   p.rotated_inertia = (int32_t)get_num(env, o, "rotatedInertia", p.rotated_inertia);
   p.cached_contact_jacobians = (int32_t)get_num(env, o, "cachedContactJacobians", p.cached_contact_jacobians);
   p.color_order = (int32_t)get_num(env, o, "colorOrder", p.color_order);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   int rc = avbd_set_params(ctx, &p);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
 
-/* pack(ctx, arenas) with typed arrays named as avbd_scene's fields (GpuBackend.pack, backend.js:662-1301) */
+/* pack(ctx, arenas[, oldIndex]) with typed arrays named as avbd_scene's
+ * fields (GpuBackend.pack, backend.js:662-1301). The body count is the length
+ * of `shape`; every other body array must hold exactly that many records, and
+ * every joint array as many as `jointA`. */
 static napi_value js_pack(napi_env env, napi_callback_info info) {
   size_t argc = 3;
-  napi_value argv[3];
+  napi_value argv[3] = {NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  napi_valuetype t = napi_undefined;
+  if (argc < 2 || napi_typeof(env, argv[1], &t) != napi_ok || t != napi_object) {
+    napi_throw_type_error(env, NULL, "the scene must be an object of typed arrays");
+    return NULL;
+  }
   napi_value o = argv[1];
   avbd_scene s;
   memset(&s, 0, sizeof s);
   size_t n = 0, nj = 0;
-  napi_value v;
-#define TA(field, key, lenp)                          \
-  v = get_prop(env, o, key);                          \
-  s.field = v ? typed_data(env, v, lenp) : NULL;
-  TA(shape, "shape", &n);
-  TA(size, "size", NULL);
-  TA(density, "density", NULL);
-  TA(friction, "friction", NULL);
-  TA(pos, "pos", NULL);
-  TA(quat, "quat", NULL);
-  TA(vel, "vel", NULL);
-  TA(angvel, "angvel", NULL);
-  TA(scene_of, "sceneOf", NULL);
-  TA(joint_a, "jointA", &nj);
-  TA(joint_b, "jointB", NULL);
-  TA(joint_ra, "jointRA", NULL);
-  TA(joint_rb, "jointRB", NULL);
-  TA(joint_stiff_lin, "jointStiffLin", NULL);
-  TA(joint_stiff_ang, "jointStiffAng", NULL);
-  TA(joint_fracture, "jointFracture", NULL);
-  TA(joint_kind, "jointKind", NULL);
-#undef TA
+  void *d;
+  const napi_typedarray_type I32 = napi_int32_array, F64 = napi_float64_array;
+#define ARR(field, key, type, want, opt)                                            \
+  if (!arr_arg(env, get_prop(env, o, key), type, key, (want), (opt), &d, NULL)) \
+    return NULL;                                                                    \
+  s.field = d;
+  if (!arr_arg(env, get_prop(env, o, "shape"), I32, "shape", -1, 0, &d, &n)) return NULL;
+  s.shape = d;
+  const long long N = (long long)n;
+  ARR(size, "size", F64, 3 * N, 0);
+  ARR(density, "density", F64, N, 0);
+  ARR(friction, "friction", F64, N, 0);
+  ARR(pos, "pos", F64, 3 * N, 0);
+  ARR(quat, "quat", F64, 4 * N, 1);
+  ARR(vel, "vel", F64, 3 * N, 1);
+  ARR(angvel, "angvel", F64, 3 * N, 1);
+  ARR(scene_of, "sceneOf", I32, N, 1);
+  if (!arr_arg(env, get_prop(env, o, "jointA"), I32, "jointA", -1, 1, &d, &nj)) return NULL;
+  s.joint_a = d;
+  if (s.joint_a) {
+    const long long NJ = (long long)nj;
+    ARR(joint_b, "jointB", I32, NJ, 0);
+    ARR(joint_ra, "jointRA", F64, 3 * NJ, 0);
+    ARR(joint_rb, "jointRB", F64, 3 * NJ, 0);
+    ARR(joint_stiff_lin, "jointStiffLin", F64, NJ, 0);
+    ARR(joint_stiff_ang, "jointStiffAng", F64, NJ, 0);
+    ARR(joint_fracture, "jointFracture", F64, NJ, 0);
+    ARR(joint_kind, "jointKind", I32, NJ, 1);
+  }
+#undef ARR
   s.num_bodies = (int32_t)n;
   s.num_joints = s.joint_a ? (int32_t)nj : 0;
   /* pack(ctx, scene, oldIndex Int32Array): avbd_repack, keeping state */
-  size_t no = 0;
-  const int32_t *old_index = argc > 2 ? (const int32_t *)typed_data(env, argv[2], &no) : NULL;
-  if (old_index && no != n) {
-    napi_throw_range_error(env, NULL, "oldIndex needs one entry per body");
-    return NULL;
-  }
+  if (!arr_arg(env, argc > 2 ? argv[2] : NULL, I32, "oldIndex", N, 1, &d, NULL)) return NULL;
+  const int32_t *old_index = (const int32_t *)d;
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   int rc = old_index ? avbd_repack(ctx, &s, old_index) : avbd_pack(ctx, &s);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
@@ -177,20 +274,26 @@ static napi_value js_pack(napi_env env, napi_callback_info info) {
 /* step(ctx, n) (GpuBackend.step, backend.js:1393-1620) */
 static napi_value js_step(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
   int32_t n = 1;
   if (argc > 1) napi_get_value_int32(env, argv[1], &n);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   int rc = avbd_step(ctx, n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
 
-/* sync(ctx) -> Promise (GpuBackend.idle, backend.js:1804-1818) */
+/* sync(ctx) -> Promise (GpuBackend.idle, backend.js:1804-1818). The job holds
+ * a reference on the context external, so that it outlives a host that drops
+ * the context meanwhile, and counts itself in the handle, so that every other
+ * call on the context throws until it has completed. */
 typedef struct {
-  avbd_ctx *ctx;
+  Handle *h;
+  napi_ref ref;
   int rc;
+  char err[448];
   napi_deferred deferred;
   napi_async_work work;
 } SyncJob;
@@ -198,54 +301,109 @@ typedef struct {
 static void sync_execute(napi_env env, void *data) {
   (void)env;
   SyncJob *j = (SyncJob *)data;
-  j->rc = avbd_sync(j->ctx);
+  j->rc = avbd_sync(j->h->ctx);
+  if (j->rc) snprintf(j->err, sizeof j->err, "%s", avbd_last_error(j->h->ctx));
 }
 
 static void sync_complete(napi_env env, napi_status status, void *data) {
   SyncJob *j = (SyncJob *)data;
+  j->h->syncs--;
   napi_value undef;
   napi_get_undefined(env, &undef);
   if (status == napi_ok && j->rc == 0) {
     napi_resolve_deferred(env, j->deferred, undef);
   } else {
     char msg[512];
-    snprintf(msg, sizeof msg, "avbd error %d: %s", j->rc, avbd_last_error(j->ctx));
+    snprintf(msg, sizeof msg, "avbd error %d: %s", j->rc, j->err);
     napi_value m, err;
     napi_create_string_utf8(env, msg, NAPI_AUTO_LENGTH, &m);
     napi_create_error(env, NULL, m, &err);
     napi_reject_deferred(env, j->deferred, err);
   }
   napi_delete_async_work(env, j->work);
+  napi_delete_reference(env, j->ref);
   free(j);
 }
 
 static napi_value js_sync(napi_env env, napi_callback_info info) {
   size_t argc = 1;
-  napi_value argv[1];
+  napi_value argv[1] = {NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  Handle *h = get_handle(env, argv[0]);
+  if (!h || !h->ctx) {
+    napi_throw_type_error(env, NULL, "the first argument must be a context from create()");
+    return NULL;
+  }
   SyncJob *j = (SyncJob *)calloc(1, sizeof(SyncJob));
-  j->ctx = get_ctx(env, argv[0]);
+  if (!j) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  j->h = h;
   napi_value promise, name;
-  CHECK(env, napi_create_promise(env, &j->deferred, &promise));
-  CHECK(env, napi_create_string_utf8(env, "avbd_sync", NAPI_AUTO_LENGTH, &name));
-  CHECK(env, napi_create_async_work(env, NULL, name, sync_execute, sync_complete, j, &j->work));
-  CHECK(env, napi_queue_async_work(env, j->work));
+  if (napi_create_reference(env, argv[0], 1, &j->ref) != napi_ok) {
+    free(j);
+    napi_throw_error(env, NULL, "N-API call failed: napi_create_reference");
+    return NULL;
+  }
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "avbd_sync", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, NULL, name, sync_execute, sync_complete, j, &j->work) != napi_ok) {
+    napi_delete_reference(env, j->ref);
+    free(j);
+    napi_throw_error(env, NULL, "N-API call failed: creating the sync job");
+    return NULL;
+  }
+  if (napi_queue_async_work(env, j->work) != napi_ok) {
+    napi_delete_async_work(env, j->work);
+    napi_delete_reference(env, j->ref);
+    free(j);
+    napi_throw_error(env, NULL, "N-API call failed: napi_queue_async_work");
+    return NULL;
+  }
+  h->syncs++;
   return promise;
+}
+
+/* Body arrays of readState / writeState: pos (n*3, required; gives n), then
+ * quat (n*4), vel, angvel, prevVel (n*3 each), each optional. */
+static int body_arrays(napi_env env, size_t argc, napi_value *argv, double **pos, double **quat, double **vel,
+                       double **angvel, double **prev, int32_t *n) {
+  const napi_typedarray_type F64 = napi_float64_array;
+  void *d;
+  size_t np = 0;
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, F64, "pos", -1, 0, &d, &np)) return 0;
+  if (np % 3) {
+    napi_throw_range_error(env, NULL, "pos must hold 3 numbers per body");
+    return 0;
+  }
+  *pos = d;
+  const long long N = (long long)(np / 3);
+  if (!arr_arg(env, argc > 2 ? argv[2] : NULL, F64, "quat", 4 * N, 1, &d, NULL)) return 0;
+  *quat = d;
+  if (vel) {
+    if (!arr_arg(env, argc > 3 ? argv[3] : NULL, F64, "vel", 3 * N, 1, &d, NULL)) return 0;
+    *vel = d;
+    if (!arr_arg(env, argc > 4 ? argv[4] : NULL, F64, "angvel", 3 * N, 1, &d, NULL)) return 0;
+    *angvel = d;
+    if (!arr_arg(env, argc > 5 ? argv[5] : NULL, F64, "prevVel", 3 * N, 1, &d, NULL)) return 0;
+    *prev = d;
+  }
+  *n = (int32_t)N;
+  return 1;
 }
 
 /* readState(ctx, pos, quat, vel, angvel, prevVel) into Float64Arrays (GpuBackend.readState, backend.js:2154-2192) */
 static napi_value js_read_state(napi_env env, napi_callback_info info) {
   size_t argc = 6;
-  napi_value argv[6];
+  napi_value argv[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  double *pos, *quat, *vel, *angvel, *prev;
+  int32_t n;
+  if (!body_arrays(env, argc, argv, &pos, &quat, &vel, &angvel, &prev, &n)) return NULL;
   avbd_ctx *ctx = get_ctx(env, argv[0]);
-  size_t n = 0;
-  double *pos = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
-  double *quat = argc > 2 ? (double *)typed_data(env, argv[2], NULL) : NULL;
-  double *vel = argc > 3 ? (double *)typed_data(env, argv[3], NULL) : NULL;
-  double *angvel = argc > 4 ? (double *)typed_data(env, argv[4], NULL) : NULL;
-  double *prev = argc > 5 ? (double *)typed_data(env, argv[5], NULL) : NULL;
-  int rc = avbd_read_state(ctx, pos, quat, vel, angvel, prev, (int32_t)(n / 3));
+  if (!ctx) return NULL;
+  int rc = avbd_read_state(ctx, pos, quat, vel, angvel, prev, n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -253,16 +411,14 @@ static napi_value js_read_state(napi_env env, napi_callback_info info) {
 /* writeState(ctx, pos, quat, vel, angvel, prevVel) */
 static napi_value js_write_state(napi_env env, napi_callback_info info) {
   size_t argc = 6;
-  napi_value argv[6];
+  napi_value argv[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  double *pos, *quat, *vel, *angvel, *prev;
+  int32_t n;
+  if (!body_arrays(env, argc, argv, &pos, &quat, &vel, &angvel, &prev, &n)) return NULL;
   avbd_ctx *ctx = get_ctx(env, argv[0]);
-  size_t n = 0;
-  double *pos = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
-  double *quat = argc > 2 ? (double *)typed_data(env, argv[2], NULL) : NULL;
-  double *vel = argc > 3 ? (double *)typed_data(env, argv[3], NULL) : NULL;
-  double *angvel = argc > 4 ? (double *)typed_data(env, argv[4], NULL) : NULL;
-  double *prev = argc > 5 ? (double *)typed_data(env, argv[5], NULL) : NULL;
-  int rc = avbd_write_state(ctx, pos, quat, vel, angvel, prev, (int32_t)(n / 3));
+  if (!ctx) return NULL;
+  int rc = avbd_write_state(ctx, pos, quat, vel, angvel, prev, n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -271,12 +427,18 @@ static napi_value js_write_state(napi_env env, napi_callback_info info) {
  * readStepStart(ctx, pos, quat): the rest of syncToCPU (backend.js:1984-2039) */
 static napi_value js_read_joints(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  void *d;
   size_t n = 0;
-  double *out = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
-  int rc = avbd_read_joints(ctx, out, (int32_t)(n / 16));
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, napi_float64_array, "out", -1, 0, &d, &n)) return NULL;
+  if (n % 16) {
+    napi_throw_range_error(env, NULL, "out must hold 16 numbers per joint");
+    return NULL;
+  }
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_read_joints(ctx, (double *)d, (int32_t)(n / 16));
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -284,31 +446,29 @@ static napi_value js_read_joints(napi_env env, napi_callback_info info) {
 /* setJointAnchor(ctx, index, rA Float64Array[3]) (GpuBackend.updateJointAnchor) */
 static napi_value js_set_joint_anchor(napi_env env, napi_callback_info info) {
   size_t argc = 3;
-  napi_value argv[3];
+  napi_value argv[3] = {NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
   int32_t idx = -1;
   if (argc > 1) CHECK(env, napi_get_value_int32(env, argv[1], &idx));
-  size_t n = 0;
-  const double *rA = argc > 2 ? (const double *)typed_data(env, argv[2], &n) : NULL;
-  if (!rA || n < 3) {
-    napi_throw_range_error(env, NULL, "rA needs 3 numbers");
-    return NULL;
-  }
-  int rc = avbd_set_joint_anchor(ctx, idx, rA);
+  void *d;
+  if (!arr_arg(env, argc > 2 ? argv[2] : NULL, napi_float64_array, "rA", 3, 0, &d, NULL)) return NULL;
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_set_joint_anchor(ctx, idx, (const double *)d);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
 
 static napi_value js_read_step_start(napi_env env, napi_callback_info info) {
   size_t argc = 3;
-  napi_value argv[3];
+  napi_value argv[3] = {NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  double *pos, *quat;
+  int32_t n;
+  if (!body_arrays(env, argc, argv, &pos, &quat, NULL, NULL, NULL, &n)) return NULL;
   avbd_ctx *ctx = get_ctx(env, argv[0]);
-  size_t n = 0;
-  double *pos = argc > 1 ? (double *)typed_data(env, argv[1], &n) : NULL;
-  double *quat = argc > 2 ? (double *)typed_data(env, argv[2], NULL) : NULL;
-  int rc = avbd_read_step_start(ctx, pos, quat, (int32_t)(n / 3));
+  if (!ctx) return NULL;
+  int rc = avbd_read_step_start(ctx, pos, quat, n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -317,14 +477,20 @@ static napi_value js_read_step_start(napi_env env, napi_callback_info info) {
  * renderer draws at a fraction alpha of the last step (renderer.js:1327-1383) */
 static napi_value js_export_poses(napi_env env, napi_callback_info info) {
   size_t argc = 3;
-  napi_value argv[3];
+  napi_value argv[3] = {NULL, NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
   double alpha = 1.0;
   if (argc > 1) CHECK(env, napi_get_value_double(env, argv[1], &alpha));
+  void *d;
   size_t n = 0;
-  float *out = argc > 2 ? (float *)typed_data(env, argv[2], &n) : NULL;
-  int rc = avbd_export_poses(ctx, alpha, out, (int32_t)(n / 8), 0);
+  if (!arr_arg(env, argc > 2 ? argv[2] : NULL, napi_float32_array, "out", -1, 0, &d, &n)) return NULL;
+  if (n % 8) {
+    napi_throw_range_error(env, NULL, "out must hold 8 numbers per body");
+    return NULL;
+  }
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_export_poses(ctx, alpha, (float *)d, (int32_t)(n / 8), 0);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -332,13 +498,15 @@ static napi_value js_export_poses(napi_env env, napi_callback_info info) {
 /* readColoring(ctx, colorOf Int32Array) -> numColors (GpuBackend.readColoring, backend.js:2053-2151) */
 static napi_value js_read_coloring(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  void *d;
   size_t n = 0;
-  int32_t *col = (int32_t *)typed_data(env, argv[1], &n);
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, napi_int32_array, "colorOf", -1, 0, &d, &n)) return NULL;
   int32_t k = 0;
-  int rc = avbd_read_coloring(ctx, col, (int32_t)n, &k);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_read_coloring(ctx, (int32_t *)d, (int32_t)n, &k);
   if (rc) return throw_avbd(env, ctx, rc);
   napi_value out;
   CHECK(env, napi_create_int32(env, k, &out));
@@ -349,14 +517,14 @@ static napi_value js_read_coloring(napi_env env, napi_callback_info info) {
  * the following steps, or none (GpuBackend.uploadColoring, backend.js:1915-1931) */
 static napi_value js_upload_coloring(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
-  napi_valuetype t = napi_null;
-  if (argc > 1) CHECK(env, napi_typeof(env, argv[1], &t));
+  void *d;
   size_t n = 0;
-  int32_t *col = (argc > 1 && t != napi_null && t != napi_undefined) ? (int32_t *)typed_data(env, argv[1], &n) : NULL;
-  int rc = avbd_upload_coloring(ctx, col, (int32_t)n);
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, napi_int32_array, "colorOf", -1, 1, &d, &n)) return NULL;
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_upload_coloring(ctx, (const int32_t *)d, (int32_t)n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -365,14 +533,14 @@ static napi_value js_upload_coloring(napi_env env, napi_callback_info info) {
  * (the counterpart of Solver.bodyOrder, solver.js:225-233) */
 static napi_value js_set_body_order(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
-  napi_valuetype t = napi_null;
-  if (argc > 1) CHECK(env, napi_typeof(env, argv[1], &t));
+  void *d;
   size_t n = 0;
-  int32_t *rank = (argc > 1 && t != napi_null && t != napi_undefined) ? (int32_t *)typed_data(env, argv[1], &n) : NULL;
-  int rc = avbd_set_body_order(ctx, rank, (int32_t)n);
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, napi_int32_array, "rank", -1, 1, &d, &n)) return NULL;
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_set_body_order(ctx, (const int32_t *)d, (int32_t)n);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
 }
@@ -380,13 +548,19 @@ static napi_value js_set_body_order(napi_env env, napi_callback_info info) {
 /* readPairs(ctx, Int32Array|null) -> count */
 static napi_value js_read_pairs(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  void *d;
   size_t n = 0;
-  int32_t *buf = argc > 1 ? (int32_t *)typed_data(env, argv[1], &n) : NULL;
+  if (!arr_arg(env, argc > 1 ? argv[1] : NULL, napi_int32_array, "pairs", -1, 1, &d, &n)) return NULL;
+  if (n % 2) {
+    napi_throw_range_error(env, NULL, "pairs must hold 2 numbers per pair");
+    return NULL;
+  }
   int32_t count = 0;
-  int rc = avbd_read_pairs(ctx, buf, (int32_t)(n / 2), &count);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_read_pairs(ctx, (int32_t *)d, (int32_t)(n / 2), &count);
   if (rc) return throw_avbd(env, ctx, rc);
   napi_value out;
   CHECK(env, napi_create_int32(env, count, &out));
@@ -396,9 +570,10 @@ static napi_value js_read_pairs(napi_env env, napi_callback_info info) {
 /* readCounters(ctx) -> object (GpuBackend.lastStats, backend.js:309-311) */
 static napi_value js_read_counters(napi_env env, napi_callback_info info) {
   size_t argc = 1;
-  napi_value argv[1];
+  napi_value argv[1] = {NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   avbd_counters c;
   int rc = avbd_read_counters(ctx, &c);
   if (rc) return throw_avbd(env, ctx, rc);
@@ -425,11 +600,12 @@ static napi_value js_read_counters(napi_env env, napi_callback_info info) {
 /* setProfiling(ctx, on); readProfile(ctx) -> Float32Array of phase ms (backend.js:1836-1891) */
 static napi_value js_set_profiling(napi_env env, napi_callback_info info) {
   size_t argc = 2;
-  napi_value argv[2];
+  napi_value argv[2] = {NULL, NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  avbd_ctx *ctx = get_ctx(env, argv[0]);
   bool on = false;
-  napi_get_value_bool(env, argv[1], &on);
+  if (argc > 1) napi_get_value_bool(env, argv[1], &on);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   int rc = avbd_set_profiling(ctx, on ? 1 : 0);
   if (rc) return throw_avbd(env, ctx, rc);
   return NULL;
@@ -437,9 +613,10 @@ static napi_value js_set_profiling(napi_env env, napi_callback_info info) {
 
 static napi_value js_read_profile(napi_env env, napi_callback_info info) {
   size_t argc = 1;
-  napi_value argv[1];
+  napi_value argv[1] = {NULL};
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
   napi_value ab, arr;
   void *data;
   CHECK(env, napi_create_arraybuffer(env, sizeof(float) * AVBD_NUM_PHASES, &data, &ab));

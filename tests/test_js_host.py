@@ -69,3 +69,41 @@ def test_js_host_exact_matches_oracle(tmp_path):
     assert np.array_equal(g, o), f"max |diff| {np.abs(g - o).max():.3e}"
     col, k = ref.last_coloring()
     assert d["numColors"] == k and d["colorOf"] == col.tolist()
+
+
+def run_arg_checks(*extra):
+    r = subprocess.run([NODE, "tests/js/check_args.mjs", *extra], capture_output=True, text=True, cwd=ROOT,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    return {d["name"]: d for d in json.loads(r.stdout)}
+
+
+@needs_node
+def test_addon_rejects_malformed_arrays_before_the_library():
+    # Every array's element type and length is checked against the body (or
+    # joint) count before the call reaches the C ABI: with no context, a
+    # malformed array is reported, a well-formed call gets as far as the
+    # context check.
+    res = run_arg_checks()
+    for name in ("pack good", "readState good"):
+        assert res[name]["type"] == "TypeError" and "context" in res[name]["message"], res[name]
+    for name, d in res.items():
+        if name.endswith("good"):
+            continue
+        assert not d["ok"] and d["type"] in ("TypeError", "RangeError"), d
+        assert "context" not in d["message"], d
+
+
+@needs_node
+@pytest.mark.gpu
+def test_addon_argument_checks_on_a_live_context():
+    res = run_arg_checks("--ctx", "exact")
+    assert res["pack good"]["ok"] and res["readState good"]["ok"], res
+    for name, d in res.items():
+        if name.endswith("good") or "sync" in name:
+            continue
+        assert not d["ok"] and d["type"] in ("TypeError", "RangeError"), d
+    # a sync in flight blocks every other call on the context; once it has
+    # settled the context steps again
+    assert not res["step during sync"]["ok"] and "in flight" in res["step during sync"]["message"]
+    assert res["step after sync"]["ok"], res["step after sync"]
