@@ -206,6 +206,15 @@ struct avbd_ctx {
   std::vector<Checkpoint> ckpts;
   Checkpoint endState;
   int adjSteps = 0, adjEvery = 0, adjSmallMax = 64;
+  // Body state was overwritten since the last pack (avbd_write_state): the
+  // velocity and the previous velocity may then differ, and the adjoint's
+  // initial-velocity gradient (their sum) would be wrong.
+  bool stateWritten = false;
+  // A call that changes what a recorded forward stepped through (parameters,
+  // state, schedule, anchors) or takes further steps drops the recording:
+  // the backward then reports AVBD_E_STATE instead of reversing a trajectory
+  // the context no longer holds.
+  void dropAdjoint() { adjSteps = 0; }
   long adjTapeBudget = 4096L << 20;  // bytes the whole forward tape may take
   bool tapeResident = false;         // the last adjoint forward kept every step's tape
   DBuf<double> aXb, aX0b, aYb, aVb, aVbS, aLK[2], aC0b, aRWb, aMUb, aMu;
@@ -870,16 +879,16 @@ struct avbd_ctx {
            (T.src.n + T.live.n + T.color.n + T.cstart.n) * sizeof(int) + T.items.n * sizeof(uint2);
   }
 
-  // Checkpoints every `every` steps. When the first step's tape, times the
-  // step count with a factor 2 for contacts that appear later, fits the tape
-  // budget, every step is recorded as it runs and the tape stays resident:
-  // the backward then reverses it directly instead of re-running each
-  // segment from its checkpoint to record it.
+  // Checkpoints every `every` steps. While the tape recorded so far fits the
+  // tape budget, every step is recorded as it runs and the tape stays
+  // resident: the backward then reverses it directly instead of re-running
+  // each segment from its checkpoint to record it.
   void adjointForward(int nsteps, int every) {
     for (auto &c : ckpts) c.release();
     ckpts.clear();
     ckpts.resize(size_t((nsteps + every - 1) / every));
     tapeResident = adjTapeBudget > 0 && nsteps > 0;
+    size_t held = 0;
     struct Reset {
       avbd_ctx *c;
       ~Reset() { c->rec = nullptr; }
@@ -893,10 +902,17 @@ struct avbd_ctx {
       rec = tapeResident ? &tape[size_t(s)] : nullptr;
       step();
       rec = nullptr;
-      if (s == 0 && tapeResident && double(tapeBytes(tape[0])) * 2.0 * nsteps > double(adjTapeBudget)) {
-        for (auto &t : tape) t.release();
-        tape.clear();
-        tapeResident = false;
+      if (tapeResident) {
+        // the tape held so far, counted after each step (contacts that appear
+        // later make later steps' tapes larger): past the budget it is
+        // released and the backward re-runs the segments from their
+        // checkpoints, which are kept either way
+        held += tapeBytes(tape[size_t(s)]);
+        if (double(held) > double(adjTapeBudget)) {
+          for (auto &t : tape) t.release();
+          tape.clear();
+          tapeResident = false;
+        }
       }
     }
     saveCheckpoint(endState);
@@ -1189,6 +1205,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.srcSlot.reserve(size_t(c.manCap) * MAXC);
   c.lastNTot = 0;
   c.adjSteps = 0;
+  c.stateWritten = false;
 
   // joints (joint.js:78-113)
   std::vector<JointDev> JD(c.J);
@@ -1370,6 +1387,7 @@ int avbd_set_params(avbd_ctx *ctx, const avbd_params *p) {
     if (!p->cached_contact_jacobians)
       throw ApiError(AVBD_E_INVALID, "only cached contact Jacobians are implemented on the GPU");
     ctx->prm = *p;
+    ctx->dropAdjoint();
     ctx->dropGraphs();  // parameters are baked into the captured launches
   });
 }
@@ -1455,6 +1473,8 @@ int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index
     // the force lists the next broad phase tells existing pairs by
     c.buildAdjacency();
     HIPCHECK(hipStreamSynchronize(c.stream));
+    // survivors carry device state, not the description's
+    c.stateWritten = true;
     dOld.release();
     dRemap.release();
   });
@@ -1464,6 +1484,7 @@ int avbd_step(avbd_ctx *ctx, int32_t nsteps) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     HIPCHECK(hipSetDevice(ctx->device));
+    if (nsteps > 0) ctx->dropAdjoint();
     for (int s = 0; s < nsteps; s++) ctx->step();
   });
 }
@@ -1544,6 +1565,8 @@ int avbd_write_state(avbd_ctx *ctx, const double *pos, const double *quat, const
                      const double *prev_vel, int32_t n) {
   return guarded(ctx, [&] {
     if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    ctx->dropAdjoint();
+    ctx->stateWritten = true;
     std::vector<double> X(size_t(n) * XS), V(size_t(n) * VS);
     HIPCHECK(hipMemcpyAsync(X.data(), ctx->X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipMemcpyAsync(V.data(), ctx->V.p, sizeof(double) * V.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -1628,6 +1651,7 @@ int avbd_set_joint_anchor(avbd_ctx *ctx, int32_t joint, const double *rA) {
     if (joint < 0 || joint >= ctx->J) throw ApiError(AVBD_E_INVALID, "joint index out of range");
     if (ctx->hjointKind[size_t(joint)] != AVBD_JOINT_BALL) throw ApiError(AVBD_E_INVALID, "not a ball joint");
     if (!rA) throw ApiError(AVBD_E_INVALID, "rA is NULL");
+    ctx->dropAdjoint();
     // stream-ordered: the next step reads the new anchor
     ctx->anchorStage.resize(size_t(ctx->J) * 3);
     double *h = &ctx->anchorStage[size_t(joint) * 3];
@@ -1789,6 +1813,7 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     if (ctx->prec == AVBD_PREC_EXACT) throw ApiError(AVBD_E_STATE, "the exact precision has a fixed team of 1");
     HIPCHECK(hipSetDevice(ctx->device));
+    ctx->dropAdjoint();
     ctx->tuneRequest = true;
     ctx->step();
     if (team) *team = ctx->team;
@@ -1799,6 +1824,7 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
 int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    ctx->dropAdjoint();
     if (!rank) {
       ctx->useRank = false;
       return;
@@ -1814,6 +1840,7 @@ int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n) {
 int avbd_upload_coloring(avbd_ctx *ctx, const int32_t *color_of, int32_t n) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    ctx->dropAdjoint();
     if (!color_of) {
       ctx->useUpload = false;
       return;
@@ -1834,6 +1861,10 @@ int avbd_adjoint_forward(avbd_ctx *ctx, int32_t nsteps, int32_t ckpt_every) {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     if (nsteps <= 0 || ckpt_every <= 0) throw ApiError(AVBD_E_INVALID, "nsteps and ckpt_every must be positive");
     ctx->checkAdjointScene();
+    // the gradient with respect to the initial velocity is that of the
+    // velocity pack set (and the previous velocity pack set equal to it)
+    if (ctx->steps != 0 || ctx->stateWritten)
+      throw ApiError(AVBD_E_STATE, "the adjoint forward starts from the packed scene: pack it again first");
     HIPCHECK(hipSetDevice(ctx->device));
     ctx->adjointForward(nsteps, ckpt_every);
   });

@@ -326,7 +326,16 @@ int avbd_upload_coloring(avbd_ctx *ctx, const int32_t *color_of, int32_t n);
  * features, anchors) held piecewise constant and only the normal row of C0
  * differentiated for contacts whose anchors were re-derived that step.
  * Requires AVBD_PREC_EXACT or AVBD_PREC_F64, box contacts, no joints and no
- * post-stabilisation (AVBD_E_STATE otherwise). */
+ * post-stabilisation (AVBD_E_STATE otherwise). The forward must start from
+ * the scene as avbd_pack set it (no step, avbd_write_state or avbd_repack
+ * since; AVBD_E_STATE otherwise). A call that changes what the forward
+ * stepped through or steps further (avbd_set_params, avbd_step,
+ * avbd_write_state, avbd_set_joint_anchor, avbd_set_body_order,
+ * avbd_upload_coloring, avbd_autotune, a pack or repack) drops the
+ * recording: the backward then returns AVBD_E_STATE before any launch. The
+ * tape is kept in device memory while what has been recorded fits
+ * avbd_config.adjoint_tape_mb; past it the tape is released and the backward
+ * re-runs each segment from its checkpoint. */
 int avbd_adjoint_forward(avbd_ctx *ctx, int32_t nsteps, int32_t ckpt_every);
 int avbd_adjoint_backward(avbd_ctx *ctx, const double *dL_dpos, double *dL_dmu, double *dL_dv0);
 

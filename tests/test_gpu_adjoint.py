@@ -157,3 +157,58 @@ def test_tape_buffers_reused_across_packs():
     m1, v1 = grads(reused, 40)
     m0, v0 = grads(avbd.HipBackend(precision=avbd.PREC_F64), 40)
     assert np.array_equal(m0, m1) and np.array_equal(v0, v1)
+
+
+def test_recording_dropped_by_state_changes():
+    # a recorded forward is reversed only while the context still holds what
+    # it stepped through: parameters, state or further steps in between make
+    # the backward fail before any launch, and the forward itself starts from
+    # the packed scene (its v0 gradient is that of the packed velocity)
+    scene, params, _ = avbd.scenes.ramp_lanes(n=4)
+    g = avbd.HipBackend(precision=avbd.PREC_F64)
+    dL = np.zeros((len(scene["bodies"]), 3))
+    changes = {
+        "set_params": lambda: g.set_params(iterations=12),
+        "step": lambda: g.step(1),
+        "write_state": lambda: g.write_state(**{k: v for k, v in g.read_state().items()}),
+        "upload_coloring": lambda: g.upload_coloring(g.read_coloring()[0]),
+    }
+    for name, change in changes.items():
+        g.pack(scene, params)
+        g.adjoint_forward(10, 5)
+        change()
+        with pytest.raises(AvbdError) as e:
+            g.adjoint_backward(dL)
+        assert e.value.code == -4, name
+    g.pack(scene, params)
+    g.step(1)
+    with pytest.raises(AvbdError) as e:
+        g.adjoint_forward(10, 5)
+    assert e.value.code == -4
+    # after a fresh pack the same context records and reverses again
+    g.pack(scene, params)
+    g.adjoint_forward(10, 5)
+    mu, v0 = g.adjoint_backward(dL)
+    assert np.isfinite(mu).all() and np.isfinite(v0).all()
+
+
+def test_tape_budget_counted_as_the_forward_runs():
+    # boxes dropped onto the ramp: the first steps hold no contacts, so the
+    # tape grows as they land. A budget the growing tape crosses part-way
+    # releases it and the backward re-runs the segments; the gradients are
+    # the bits of the all-re-run path
+    scene, params, meta = avbd.scenes.ramp(16)
+    for b in scene["bodies"][2:]:
+        b["pos"] = [b["pos"][0], b["pos"][1], b["pos"][2] + 1.5]
+    grads = []
+    for tape_mb in (1, -1, 0):
+        g = avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=tape_mb)
+        g.pack(scene, params)
+        g.adjoint_forward(60, 10)
+        x = g.read_state()["pos"]
+        dL = 2 * x
+        dL[:2] = 0
+        grads.append(g.adjoint_backward(dL))
+    (m0, v0), (m1, v1), (m2, v2) = grads
+    assert np.array_equal(m0, m1) and np.array_equal(v0, v1)
+    assert np.array_equal(m0, m2) and np.array_equal(v0, v2)

@@ -126,21 +126,46 @@ def same_forces(a, b):
 
 # Per configuration and precision: (steps compared, first force-list
 # divergence no earlier than, pose error before it at most, pose error over
-# the window at most). The floors and bounds are the measured values with
-# margin (the measured ones are stated with the error table of the design
-# notes); None = no divergence allowed within the window.
+# the whole window at most). The floors and bounds are the measured values
+# (MI355X) with margin; the measured ones are tabulated in the design notes.
+# floor None: no divergence within the window. bound None: after the first
+# divergence the trajectories are different valid ones and only invariants are
+# checked: the batched chains are chaotic pendulums (the check is that their
+# sockets stay as closed as exact mode's), and in the rain the oracle's own
+# trajectory ejects light bodies squeezed between heavy ones (densities
+# 1:1000) at hundreds of m/s, so any difference there grows without bound.
 VARIANT = {
-    ("stack10", "mixed"): (300, 2, 5e-4, 0.5),
-    ("stack10", "f64"): (300, 2, 1e-6, 0.5),
-    ("ramp", "mixed"): (120, 2, 5e-4, 0.5),
-    ("ramp", "f64"): (120, 2, 1e-6, 0.5),
-    ("batched", "mixed"): (300, 2, 5e-4, 1.0),
-    ("batched", "f64"): (300, 2, 1e-6, 1.0),
-    ("pyramid", "mixed"): (120, 2, 5e-4, 1.0),
-    ("pyramid", "f64"): (120, 2, 1e-6, 1.0),
-    ("rain", "mixed"): (150, 2, 5e-3, 5.0),
-    ("rain", "f64"): (150, 2, 1e-6, 5.0),
+    ("stack10", "mixed"): (300, 4, 1e-9, 5.0),
+    ("stack10", "f64"): (300, None, 1e-10, 1e-10),
+    ("ramp", "mixed"): (120, None, 5e-5, 5e-5),
+    ("ramp", "f64"): (120, None, 1e-10, 1e-10),
+    ("batched", "mixed"): (300, 3, 1e-8, None),
+    ("batched", "f64"): (300, 3, 1e-12, None),
+    ("pyramid", "mixed"): (120, 2, 1e-8, 2e-2),
+    ("pyramid", "f64"): (120, 4, 1e-6, 2e-2),
+    ("rain", "mixed"): (150, 10, 1e-5, None),
+    ("rain", "f64"): (150, 30, 2e-2, None),
 }
+
+
+def qrot(q, v):
+    """Rotates v (n, 3) by unit quaternions q (n, 4), xyzw."""
+    u, w = q[:, :3], q[:, 3:4]
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def socket_gaps(st, joints):
+    """Distance between the two anchors of every ball joint (m)."""
+    if not joints:
+        return np.zeros(0)
+    a = np.array([j["a"] for j in joints])
+    b = np.array([j["b"] for j in joints])
+    rA = np.array([j["rA"] for j in joints], float)
+    rB = np.array([j["rB"] for j in joints], float)
+    pa = np.where((a >= 0)[:, None], st["pos"][a] + qrot(st["quat"][a], rA), rA)
+    pb = st["pos"][b] + qrot(st["quat"][b], rB)
+    return np.linalg.norm(pa - pb, axis=1)
 
 
 @pytest.mark.parametrize("precision", ["mixed", "f64"])
@@ -176,8 +201,10 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
         if spec is not None and s == spec["step"]:
             launch(ex, spec)
             launch(va, spec)
+    gap_ex, gap_va = socket_gaps(a, scene["joints"]), socket_gaps(b, scene["joints"])
     rec = dict(config=name, precision=precision, steps=steps, first_divergence=diverged,
                error_before_divergence=before, worst_error=worst, worst_step=worst_step,
+               socket_gap=dict(exact=float(gap_ex.max(initial=0.0)), variant=float(gap_va.max(initial=0.0))),
                error_at=dict((str(k), per_step[k - 1]) for k in (10, 30, 60, 90, 120, 150, 300) if k <= steps))
     print("[configs] " + json.dumps(rec), flush=True)
     if REPORT:
@@ -188,4 +215,7 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
     else:
         assert diverged is None or diverged >= floor, f"{name} {precision}: force lists diverged at step {diverged}"
     assert before <= tol_before, f"{name} {precision}: pose error {before:.3e} before the first divergence"
-    assert worst <= bound, f"{name} {precision}: pose error {worst:.3e} at step {worst_step}"
+    if bound is not None:
+        assert worst <= bound, f"{name} {precision}: pose error {worst:.3e} at step {worst_step}"
+    if len(gap_ex):
+        assert gap_va.max() <= 2.0 * gap_ex.max() + 1e-3, f"{name} {precision}: socket gap {gap_va.max():.3e}"
