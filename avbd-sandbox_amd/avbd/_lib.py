@@ -15,6 +15,7 @@ AVBD_OK, AVBD_E_INVALID, AVBD_E_HIP, AVBD_E_CAPACITY, AVBD_E_STATE = 0, -1, -2, 
 PREC_EXACT, PREC_F64, PREC_MIXED = 0, 1, 2
 JOINT_BALL, JOINT_SPRING = 0, 1
 SHAPE_BOX, SHAPE_SPHERE = 0, 1
+STATIC_POSE_BODIES = 65536  # AVBD_STATIC_POSE_BODIES
 NUM_PHASES = 6
 PHASES = ("broad", "narrow", "topology", "colour", "solve", "total")
 
@@ -25,7 +26,8 @@ SYMBOLS = (
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
     "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
-    "avbd_upload_coloring", "avbd_export_poses", "avbd_pose_source", "avbd_repack", "avbd_read_joints",
+    "avbd_upload_coloring", "avbd_export_poses", "avbd_export_poses_static", "avbd_pose_source", "avbd_repack",
+    "avbd_read_joints",
     "avbd_read_step_start", "avbd_set_joint_anchor",
 )
 K_PRIMAL, K_DUAL, K_NARROW, K_POSES = 0, 1, 2, 3
@@ -102,6 +104,7 @@ def lib(path=None):
         "avbd_set_body_order": (C.c_int, [ctx, pi, i32]),
         "avbd_upload_coloring": (C.c_int, [ctx, pi, i32]),
         "avbd_export_poses": (C.c_int, [ctx, d, vp, i32, i32]),
+        "avbd_export_poses_static": (C.c_int, [ctx, d, vp, i32]),
         "avbd_repack": (C.c_int, [ctx, P(Scene), pi]),
         "avbd_read_joints": (C.c_int, [ctx, pd, i32]),
         "avbd_read_step_start": (C.c_int, [ctx, pd, pd, i32]),

@@ -230,6 +230,18 @@ class HipBackend:
         self._ck(self.L.avbd_export_poses(self.ctx, float(alpha), out.ctypes.data_as(C.c_void_p), n, 0))
         return out
 
+    def export_poses_static(self, alpha=1.0, out=None):
+        """export_poses into host memory through the library's static staging
+        array (avbd_export_poses_static): allocates nothing; at most
+        AVBD_STATIC_POSE_BODIES bodies."""
+        n = self.n
+        if out is None:
+            out = np.zeros((n, 8), np.float32)
+        if out.dtype != np.float32 or out.shape != (n, 8) or not out.flags["C_CONTIGUOUS"]:
+            raise AvbdError(L.AVBD_E_INVALID, "out must be a contiguous float32 (n, 8) array")
+        self._ck(self.L.avbd_export_poses_static(self.ctx, float(alpha), out.ctypes.data_as(C.c_void_p), n))
+        return out
+
     def pose_source(self):
         """(pose, start_pose, stride_bytes): device addresses of the current
         and step-start f64 pose records {p[3], q[4], pad}."""

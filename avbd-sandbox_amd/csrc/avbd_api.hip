@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,15 @@
 #include "avbd_kernels.hpp"
 
 using namespace avbd;
+
+// Staging for avbd_export_poses_static: the poses of up to
+// AVBD_STATIC_POSE_BODIES bodies (two float4 each), in device memory that
+// every GPU the library runs on holds from module load until the process
+// ends, so that the entry point allocates nothing. One copy per device, shared
+// by every call and context on it; it keeps the last call's contents. The
+// name is only a stand-in in host code: kernels and copies use the device
+// address looked up per device (staticPoseScratch), never the name.
+static __device__ __align__(256) float4 scratch[2 * AVBD_STATIC_POSE_BODIES];
 
 namespace {
 
@@ -92,6 +102,29 @@ struct Phase {
   hipEvent_t ev[AVBD_NUM_PHASES];
 };
 
+// Per device: the address of `scratch` on that device, looked up once with
+// the device current (each GPU holds its own copy at its own address).
+std::mutex scratchMutex;
+std::map<int, float4 *> scratchAddr;
+
+float4 *staticPoseScratch(int device) {
+  std::lock_guard<std::mutex> lock(scratchMutex);
+  auto it = scratchAddr.find(device);
+  if (it != scratchAddr.end()) return it->second;
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  scratchAddr[device] = static_cast<float4 *>(ptr);
+  return static_cast<float4 *>(ptr);
+}
+
 }  // namespace
 
 struct avbd_ctx {
@@ -116,6 +149,7 @@ struct avbd_ctx {
   DBuf<double> X, X0, Y, V;
   DBuf<char> DQ;
   DBuf<float4> poseStage;  // avbd_export_poses to host memory
+  float4 *poseScratch = nullptr;  // this device's `scratch` (avbd_export_poses_static)
   std::vector<double> anchorStage;  // avbd_set_joint_anchor's host copies
   std::vector<int> hjointKind;      // AVBD_JOINT_* of each packed joint
   std::vector<BodyProp> hprops;
@@ -1365,6 +1399,13 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
     c->simds = 4 * cus;
+  // the static staging array's address on this device, for the entry point
+  // that allocates nothing
+  c->poseScratch = staticPoseScratch(c->device);
+  if (!c->poseScratch) {
+    delete c;
+    return AVBD_E_HIP;
+  }
   *out = c;
   return AVBD_OK;
 }
@@ -1549,6 +1590,31 @@ int avbd_export_poses(avbd_ctx *ctx, double alpha, float *dst, int32_t n, int32_
       HIPCHECK(hipMemcpyAsync(dst, out, sizeof(float4) * 2 * size_t(n), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHECK(hipStreamSynchronize(ctx->stream));
     }
+  });
+}
+
+int avbd_export_poses_static(avbd_ctx *ctx, double alpha, float *dst, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed || n != ctx->N) throw ApiError(AVBD_E_INVALID, "body count mismatch");
+    if (!dst) throw ApiError(AVBD_E_INVALID, "dst is NULL");
+    if (n > AVBD_STATIC_POSE_BODIES)
+      throw ApiError(AVBD_E_INVALID, "avbd_export_poses_static takes at most " +
+                                         std::to_string(AVBD_STATIC_POSE_BODIES) + " bodies (" + std::to_string(n) +
+                                         " packed); use avbd_export_poses");
+    if (!ctx->poseScratch) throw ApiError(AVBD_E_STATE, "no static staging address for this device");
+    const double a = std::isnan(alpha) ? 1.0 : std::min(1.0, std::max(0.0, alpha));
+    if (n == 0) return;
+    // the kernel writes all 2n float4 of the staging prefix the copy reads,
+    // so nothing needs clearing first; the launch and the copy are ordered on
+    // the context's stream
+    const int lerp = a < 1.0 ? 1 : 0;
+    hipLaunchKernelGGL(k_export_poses, dim3(blocks(n)), dim3(256), 0, ctx->stream, n, a, lerp,
+                       reinterpret_cast<const double2 *>(ctx->X.p), reinterpret_cast<const double2 *>(ctx->X0.p),
+                       ctx->poseScratch);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(dst, ctx->poseScratch, sizeof(float4) * 2 * size_t(n), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
   });
 }
 

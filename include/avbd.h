@@ -216,6 +216,16 @@ int avbd_read_state(avbd_ctx *ctx, double *pos, double *quat, double *vel, doubl
  * device, written in stream order (avbd_sync waits for it). n must equal the
  * packed body count. */
 int avbd_export_poses(avbd_ctx *ctx, double alpha, float *dst, int32_t n, int32_t dst_on_device);
+/* The same poses into host memory without allocating anything (no
+ * counterpart in the reference): for callers that cannot allocate device
+ * memory, such as a render or serving loop that forbids it. The poses are
+ * staged in a static device array that each GPU holds for the life of the
+ * process; n (the packed body count) must not exceed
+ * AVBD_STATIC_POSE_BODIES (AVBD_E_INVALID). Calls on one GPU share that
+ * array: calls that overlap on one GPU, from several threads or contexts,
+ * race. */
+#define AVBD_STATIC_POSE_BODIES 65536
+int avbd_export_poses_static(avbd_ctx *ctx, double alpha, float *dst, int32_t n);
 /* The body arena for a renderer on the same device that reads poses in place
  * (setPoseSourceExternal(buffer, posBase, quatBase, prevPosBase, prevQuatBase),
  * renderer_webgpu.js:1627): the device addresses of the current pose and of
