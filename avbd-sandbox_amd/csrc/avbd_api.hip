@@ -130,6 +130,9 @@ float4 *staticPoseScratch(int device) {
 struct avbd_ctx {
   std::string err;
   int device = 0, prec = AVBD_PREC_MIXED, team = 16;
+  // exact precision: a team of lanes per body with the additions in the
+  // reference's order (k_primal_xt), or one lane per body (config team 1)
+  bool exactTeam = true;
   int simds = 1024;  // SIMDs of the device (4 per CU), read at creation
   int manCapCfg = 0, pairCapCfg = 0;
   int needPairs = 0, needMan = 0;  // sizes an overflow asked for, for the next pack
@@ -549,7 +552,24 @@ struct avbd_ctx {
   template <class R, bool EXACT>
   void primal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int tm, int dual,
               double dualAlpha) {
-    if (EXACT) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha, dual, dualAlpha); return; }
+    if constexpr (EXACT) {
+      if (!exactTeam) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha, dual, dualAlpha); return; }
+      if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
+      // a team of lanes per body, the additions kept in the reference's
+      // order; bodies with joints then take the one-lane sweep
+      const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
+      timedLaunch(AVBD_K_PRIMAL, bytes, k_primal_xt, dim3(primalBlocks(c, XT_TEAM)), dim3(256), c,
+                  (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (double *)DQ.p, P, alpha, dual,
+                  dualAlpha);
+      if (J > 0) {
+        SolveArgs sj = sa;
+        sj.jointBodiesOnly = 1;
+        timedLaunch(AVBD_K_PRIMAL, 0.0, k_primal<double, true, 1>, dim3(primalBlocks(c, 1)), dim3(256), c,
+                    (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sj, mb, (double *)DQ.p, P, alpha, dual,
+                    dualAlpha);
+      }
+      return;
+    }
     switch (tm) {
       case 1: launchPrimal<R, false, 1>(c, sa, mb, P, alpha, dual, dualAlpha); break;
       case 4: launchPrimal<R, false, 4>(c, sa, mb, P, alpha, dual, dualAlpha); break;
@@ -762,7 +782,7 @@ struct avbd_ctx {
     ManBuf<R> mb = store[cur].view<R>();
     SolveArgs sa;
     sa.props = props.p; sa.X = X.p; sa.X0 = X0.p; sa.Y = Y.p; sa.itemStart = itemStart.p; sa.items = items.p;
-    sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N;
+    sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N; sa.jointBodiesOnly = 0;
     if (tuneRequest && !EXACT) {
       hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
                          (R *)DQ.p, P);
@@ -1379,6 +1399,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
     c->device = cfg->device;
     c->prec = cfg->precision;
     c->team = cfg->team > 0 ? cfg->team : 16;
+    c->exactTeam = cfg->team != 1;
     c->manCapCfg = cfg->manifold_cap;
     c->graphsCfg = cfg->graphs;
     c->adjSmallMax = cfg->adjoint_small == 0 ? 64 : cfg->adjoint_small;

@@ -1241,6 +1241,7 @@ struct SolveArgs {
   const int4 *head;
   const JointDev *joints;
   int nBodies;
+  int jointBodiesOnly;  // sweep only the bodies with joints (the exact team sweep leaves them)
 };
 
 template <class R>
@@ -1285,6 +1286,7 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
   const double dt2 = P.dt * P.dt;
   for (int team = gt / TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
+    if (S.jointBodiesOnly && w.w == 0) continue;
     const int i = w.x, b0 = w.y, b1 = w.z;
     const BodyProp &bp = S.props[i];
     double *x = S.X + size_t(i) * XS;
@@ -1474,6 +1476,161 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
     R *d = DQ + size_t(i) * 8;
     d[0] = R(xn[0] - x0s[0]); d[1] = R(xn[1] - x0s[1]); d[2] = R(xn[2] - x0s[2]);
     d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
+  }
+}
+
+// ------------------------------------------------- exact mode, team-parallel
+// A contact's 27 contributions to its body's system (manifold.js:411-445),
+// each rounded as stampContact rounds it before adding: the Lin and Ang lower
+// triangles, the full Cross block, then the linear and angular right-hand
+// sides, in that order.
+__device__ __forceinline__ void contactTerms(double *o, const double *jl, const double *ja, const double *k,
+                                             const double *F) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const int r = LT_R[i], c = LT_C[i];
+    o[i] = (jl[r] * k[0]) * jl[c] + (jl[3 + r] * k[1]) * jl[3 + c] + (jl[6 + r] * k[2]) * jl[6 + c];
+    o[6 + i] = (ja[r] * k[0]) * ja[c] + (ja[3 + r] * k[1]) * ja[3 + c] + (ja[6 + r] * k[2]) * ja[6 + c];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      o[12 + r * 3 + c] = (ja[r] * k[0]) * jl[c] + (ja[3 + r] * k[1]) * jl[3 + c] + (ja[6 + r] * k[2]) * jl[6 + c];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    o[21 + r] = jl[r] * F[0] + jl[3 + r] * F[1] + jl[6 + r] * F[2];
+    o[24 + r] = ja[r] * F[0] + ja[3 + r] * F[1] + ja[6 + r] * F[2];
+  }
+}
+
+// Orders LDS accesses of the lanes of one wave (the team lives in one wave):
+// the writes before it are seen by the reads after it.
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int XT_TEAM = 32;   // lanes per body
+constexpr int XT_TERMS = 27;  // numbers of the 6x6 system
+constexpr int XT_ROW = 28;    // doubles per item row in LDS (27 used)
+
+// One colour of the exact sweep with a team of XT_TEAM lanes per body, giving
+// the same bits as the one-lane exact sweep (k_primal<double, true, 1>): the
+// reference adds each contact's contribution to the body's system in force
+// order onto the inertial terms (solver.js:419-458), and floating-point
+// addition is not associative, so only the additions are kept in that order.
+//   1. Each lane evaluates one contact of a chunk of XT_TEAM (C, the clamped
+//      force, the folded dual update of a first visitor: all per contact and
+//      order-free) and writes its 27 rounded contributions to a row in LDS.
+//   2. Lane e (< 27) adds entry e of the rows in item order onto its running
+//      sum, which started from the inertial term e.
+// Lane 0 then gathers the 27 sums, solves the 6x6 system in f64 with the
+// reference's LDL^T and applies Eq. 21. A body with joints or springs
+// (cwork.w > 0) is left to the one-lane sweep (skipped here): joint rows add
+// into the system several times per force (jointStamp).
+__global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, const int4 *cwork, SolveArgs S,
+                                                   ManBuf<double> mb, double *DQ, StepParams P, double alpha,
+                                                   int dual, double dualAlpha) {
+  __shared__ double rows[256 / XT_TEAM][XT_TEAM * XT_ROW];
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = gt % XT_TEAM, teams = gridDim.x * blockDim.x / XT_TEAM;
+  double *L = rows[threadIdx.x / XT_TEAM];
+  const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
+  const double dt2 = P.dt * P.dt;
+  const double beta = P.betaLin, pmax = P.pmax, stickT = P.stick;
+  for (int team = gt / XT_TEAM; team < cnt; team += teams) {
+    const int4 w = cwork[c0 + team];
+    if (w.w > 0) continue;  // joints: the one-lane sweep
+    const int i = w.x, b0 = w.y, b1 = w.z;
+    const BodyProp &bp = S.props[i];
+    double *x = S.X + size_t(i) * XS;
+    const double *y = S.Y + size_t(i) * XS;
+    const double *x0 = S.X0 + size_t(i) * XS;
+    double dqi[8];
+    vload<8>(dqi, DQ + size_t(i) * 8);
+    // inertial terms, the reference's first contributions (lane 0 into row 0)
+    if (lane == 0) {
+      double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
+      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+      for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
+      double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], rl[3], ra[3];
+      mulv(rl, Ml, t);
+      qsub(dq, x + 3, y + 3);
+      mulv(ra, Ma, dq);
+      const double lin[6] = {Ml[0], Ml[3], Ml[4], Ml[6], Ml[7], Ml[8]};
+      const double ang[6] = {Ma[0], Ma[3], Ma[4], Ma[6], Ma[7], Ma[8]};
+      for (int k = 0; k < 6; k++) { L[k] = lin[k]; L[6 + k] = ang[k]; }
+      for (int k = 0; k < 9; k++) L[12 + k] = 0.0;
+      for (int k = 0; k < 3; k++) { L[21 + k] = rl[k]; L[24 + k] = ra[k]; }
+    }
+    waveSync();
+    double acc = lane < XT_TERMS ? L[lane] : 0.0;
+    waveSync();
+    for (int t0 = b0; t0 < b1; t0 += XT_TEAM) {
+      const int t = t0 + lane;
+      if (t < b1) {
+        const uint2 iw = S.items[t];
+        const bool isA = iw.x & 1u;
+        const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
+        double bas[BAS], hot[HOT], dqo[8], negb[9];
+        vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
+        vload<HOT>(hot, mb.hot + si * HOT);
+        vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
+        for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+        const double *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+        Eval<double> e;
+        if (dual && (iw.y & ITEM_FIRST)) {
+          evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, dualAlpha, e);
+          double kl[6];
+          const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+          vstoreKL(mb.hot + si * HOT, kl);
+          if (st >= 0) mb.stick[si] = st;
+#pragma unroll
+          for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];
+          if (dualAlpha != alpha) evalC<double, true>(bas, negb, hot, dA, dB, alpha, e);
+          evalF<double, true>(hot + HK, hot + HL, bas[9], e);
+        } else {
+          evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, alpha, e);
+        }
+        double o[XT_TERMS];
+        contactTerms(o, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
+        double *row = L + lane * XT_ROW;
+#pragma unroll
+        for (int k = 0; k < XT_TERMS; k++) row[k] = o[k];
+      }
+      waveSync();
+      const int m = min(XT_TEAM, b1 - t0);
+      if (lane < XT_TERMS)
+        for (int k = 0; k < m; k++) acc += L[k * XT_ROW + lane];
+      waveSync();
+    }
+    // the 27 sums to lane 0
+    if (lane < XT_TERMS) L[lane] = acc;
+    waveSync();
+    if (lane == 0) {
+      Sys<double> s;
+      for (int k = 0; k < 6; k++) { s.lin[k] = L[k]; s.ang[k] = L[6 + k]; }
+      for (int k = 0; k < 9; k++) s.cross[k] = L[12 + k];
+      for (int k = 0; k < 3; k++) { s.rl[k] = L[21 + k]; s.ra[k] = L[24 + k]; }
+      double xs[7], x0s[7];
+      for (int k = 0; k < 7; k++) { xs[k] = x[k]; x0s[k] = x0[k]; }
+      double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
+      double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
+      double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
+      double dxl[3], dxa[3], xn[7];
+      solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+      qint(xn + 3, xs + 3, dxa);
+      for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
+      for (int k = 0; k < 7; k++) x[k] = xn[k];
+      double dq[3];
+      qsub(dq, xn + 3, x0s + 3);
+      double *d = DQ + size_t(i) * 8;
+      d[0] = xn[0] - x0s[0]; d[1] = xn[1] - x0s[1]; d[2] = xn[2] - x0s[2];
+      d[3] = dq[0]; d[4] = dq[1]; d[5] = dq[2];
+    }
+    waveSync();  // row 0 is the next body's
   }
 }
 

@@ -38,8 +38,10 @@ extern "C" {
 #define AVBD_E_STATE (-4)     /* call not valid in the context's state      */
 
 /* Precision of the step.
- *  EXACT: every quantity in f64, each body's system assembled sequentially in
- *         the reference's force order: results equal the f64 CPU engine's.
+ *  EXACT: every quantity in f64; the contributions to each body's system are
+ *         added in the reference's force order (a team of lanes evaluates a
+ *         body's contacts in parallel, one lane per matrix entry adds them in
+ *         order): results equal the f64 CPU engine's, bit for bit.
  *  F64:   f64 throughout, lanes of a team assemble a body's system in
  *         parallel (tree-reduced sums).
  *  MIXED: contact records and contact arithmetic in f32, body state, world
@@ -60,7 +62,10 @@ typedef struct avbd_ctx avbd_ctx;
 typedef struct avbd_config {
   int32_t device;          /* HIP device ordinal                            */
   int32_t precision;       /* AVBD_PREC_*                                   */
-  int32_t team;            /* lanes per body in the primal solve (0: auto)  */
+  int32_t team;            /* lanes per body in the primal solve (0: auto);
+                              AVBD_PREC_EXACT: 1 assembles each body's
+                              system on one lane, anything else uses the
+                              team sweep (the same bits)                    */
   int32_t manifold_cap;    /* max live + new manifolds (0: 32 per body); a
                               step that overflows it fails with
                               AVBD_E_CAPACITY and the next avbd_pack or

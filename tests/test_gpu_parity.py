@@ -148,3 +148,29 @@ def test_exact_counters_and_manifold_records():
         # rA rB C0 k lambda, stick
         assert np.array_equal(g["contacts"][:, :15], r["contacts"][:, :15])
         assert np.array_equal(g["contacts"][:, 15], r["contacts"][:, 18])
+
+
+@pytest.mark.parametrize("scene_name", ["pyramid12", "chain", "batched6"])
+def test_exact_team_sweep_equals_one_lane_sweep(scene_name):
+    # The exact mode's team sweep (a team of lanes per body, each contact's
+    # contributions added in force order by one lane per matrix entry) and the
+    # one-lane sweep (team 1, every body's system assembled by one lane) give
+    # the same bits, step after step; scenes with joints mix the two (bodies
+    # with joints take the one-lane path).
+    if scene_name == "pyramid12":
+        scene, params, _ = avbd.scenes.pyramid(layers=12)
+    elif scene_name == "chain":
+        doc = json.load(open(os.path.join(GOLDEN, "chain_color.json")))
+        scene, params = doc["scene"], doc["params"]
+    else:
+        scene, params, _ = avbd.scenes.batched(num_scenes=6, seed=11)
+    params = dict(params, colorOrder=1)
+    team = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    lane = avbd.HipBackend(precision=avbd.PREC_EXACT, team=1)
+    team.pack(scene, params)
+    lane.pack(scene, params)
+    for s in range(1, 16):
+        team.step(1)
+        lane.step(1)
+        a, b = gpu_state13(team), gpu_state13(lane)
+        assert np.array_equal(a, b), f"step {s}: max |diff| {np.abs(a - b).max():.3e}"

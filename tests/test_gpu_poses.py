@@ -37,6 +37,9 @@ def test_export_matches_the_oracle_blend(alpha):
     host = g.export_poses(alpha)
     assert np.array_equal(host, want)
     dev = torch.full((g.n, 8), float("nan"), dtype=torch.float32, device="cuda:0")
+    # the fill runs on torch's stream and the export on the library's: the
+    # destination must be ready before the export is ordered after it
+    torch.cuda.synchronize()
     g.export_poses(alpha, out=dev)
     g.idle()
     assert np.array_equal(dev.cpu().numpy(), want)
