@@ -305,6 +305,19 @@ class HipBackend:
         self._ck(self.L.avbd_set_body_order(self.ctx, _ip(r), len(r)))
 
     # --- metrics / profiling (backend.js:309-324, 1836-1891)
+    def read_manifolds(self):
+        """Every live manifold in force-list order: head (m, 4) {A, B, contact
+        count, flags}, basis_mu (m, 10), and contacts (c, 16) per contact,
+        manifold after manifold: rA rB C0 penalty lambda stick."""
+        m, c = C.c_int32(), C.c_int32()
+        self._ck(self.L.avbd_read_manifolds(self.ctx, None, None, None, 0, 0, C.byref(m), C.byref(c)))
+        head = np.zeros((m.value, 4), np.int32)
+        bm = np.zeros((m.value, 10))
+        con = np.zeros((c.value, 16))
+        self._ck(self.L.avbd_read_manifolds(self.ctx, _ip(head), _dp(bm), _dp(con), m.value, c.value, C.byref(m),
+                                            C.byref(c)))
+        return head, bm, con
+
     def counters(self):
         c = L.Counters()
         self._ck(self.L.avbd_read_counters(self.ctx, C.byref(c)))

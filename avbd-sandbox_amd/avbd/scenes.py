@@ -164,7 +164,10 @@ def batched(num_scenes=256, seed=3):
 def ramp(n=64, seed=5, angle_deg=30.0):
     """Config 5: n boxes (0.5 m, rho 500) on a static 30-degree ramp, friction
     U[0.1, 0.9], initial speed U[0, 2] m/s down-slope; 10 iterations, 120
-    steps."""
+    steps. meta['targets'] holds the loss's seeded targets, one per body
+    (the statics' are their own positions): each box's start moved U[0, 3] m
+    down the slope and U[-0.5, 0.5] m across it; meta['slope'] is the unit
+    down-slope direction."""
     r = lcg(seed)
     a = math.radians(angle_deg)
     q = axis_angle(0, 1, 0, a)
@@ -178,7 +181,13 @@ def ramp(n=64, seed=5, angle_deg=30.0):
         pos = [c * u + s * hgt, w, 10 - s * u + c * hgt]
         v0 = r() * 2
         bodies.append(box([0.5, 0.5, 0.5], 500, 0.1 + r() * 0.8, pos, quat=q, vel=[c * v0, 0, -s * v0]))
-    return dict(bodies=bodies, joints=[]), dict(gravity=-9.81, iterations=10), dict(steps=120, seed=seed)
+    t = lcg(seed * 7919 + 1)
+    targets = [list(b["pos"]) for b in bodies[:2]]
+    for b in bodies[2:]:
+        d, w = 3.0 * t(), t() - 0.5
+        targets.append([b["pos"][0] + c * d, b["pos"][1] + w, b["pos"][2] - s * d])
+    return (dict(bodies=bodies, joints=[]), dict(gravity=-9.81, iterations=10),
+            dict(steps=120, seed=seed, targets=targets, slope=[c, 0.0, -s]))
 
 
 def ramp_lanes(n=8, seed=3, angle_deg=30.0):

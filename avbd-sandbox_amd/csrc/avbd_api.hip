@@ -1846,6 +1846,74 @@ int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *c
   });
 }
 
+int avbd_read_manifolds(avbd_ctx *ctx, int32_t *head, double *basis_mu, double *contacts, int32_t cap,
+                        int32_t contact_cap, int32_t *count, int32_t *num_contacts) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    const int nl = ctx->nLive;
+    if (count) *count = nl;
+    std::vector<int> live(size_t(std::max(nl, 0)));
+    if (nl) {
+      HIPCHECK(hipMemcpyAsync(live.data(), ctx->live[ctx->cur].p, sizeof(int) * nl, hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    const size_t span = nl ? size_t(*std::max_element(live.begin(), live.end())) + 1 : 0;
+    ManStore &st = ctx->store[ctx->cur];
+    const size_t rs = ctx->rsize();
+    std::vector<int4> h(span);
+    if (span) {
+      HIPCHECK(hipMemcpyAsync(h.data(), st.head.p, sizeof(int4) * span, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    long nc = 0;
+    for (int t = 0; t < nl; t++) nc += h[size_t(live[t])].z;
+    if (num_contacts) *num_contacts = int32_t(nc);
+    if (!head && !basis_mu && !contacts) return;
+    std::vector<char> bas, hot;
+    std::vector<double> anc;
+    std::vector<int32_t> stick;
+    if (span && basis_mu) {
+      bas.resize(span * BAS * rs);
+      HIPCHECK(hipMemcpyAsync(bas.data(), st.basis.p, bas.size(), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (span && contacts) {
+      hot.resize(span * MAXC * HOT * rs);
+      anc.resize(span * MAXC * 6);
+      stick.resize(span * MAXC);
+      HIPCHECK(hipMemcpyAsync(hot.data(), st.hot.p, hot.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipMemcpyAsync(anc.data(), st.anchor.p, sizeof(double) * anc.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HIPCHECK(hipMemcpyAsync(stick.data(), st.stick.p, sizeof(int32_t) * stick.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto rd = [&](const std::vector<char> &v, size_t i) {
+      return rs == sizeof(float) ? double(((const float *)v.data())[i]) : ((const double *)v.data())[i];
+    };
+    long k = 0;
+    for (int t = 0; t < nl && t < cap; t++) {
+      const size_t m = size_t(live[t]);
+      const int4 hd = h[m];
+      if (head) { head[t * 4] = hd.x; head[t * 4 + 1] = hd.y; head[t * 4 + 2] = hd.z; head[t * 4 + 3] = hd.w; }
+      if (basis_mu)
+        for (int e = 0; e < 10; e++) basis_mu[size_t(t) * 10 + e] = rd(bas, m * BAS + e);
+      if (contacts)
+        for (int sl = 0; sl < hd.z && k < contact_cap; sl++, k++) {
+          double *o = contacts + size_t(k) * 16;
+          const size_t si = m * MAXC + size_t(sl);
+          for (int e = 0; e < 6; e++) o[e] = anc[si * 6 + e];
+          for (int e = 0; e < 3; e++) {
+            o[6 + e] = rd(hot, si * HOT + HC0 + e);
+            o[9 + e] = rd(hot, si * HOT + HK + e);
+            o[12 + e] = rd(hot, si * HOT + HL + e);
+          }
+          o[15] = stick[si];
+        }
+    }
+  });
+}
+
 int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out) {
   return guarded(ctx, [&] {
     if (!out) throw ApiError(AVBD_E_INVALID, "null output");

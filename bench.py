@@ -122,6 +122,52 @@ def cpu_sample(args, scene, params, meta):
             f"{secs:.1f} s (+{build_s:.1f} s scene build)")
 
 
+def settle_and_strike(gpu, spec):
+    """The pyramid's set-up: `spec['step']` settling steps, then the sphere
+    launched at the apex (position, velocity and previous velocity set,
+    angular velocity 0)."""
+    gpu.step(spec["step"])
+    st = gpu.read_state()
+    b = spec["body"]
+    st["pos"][b] = spec["pos"]
+    st["vel"][b] = spec["vel"]
+    st["prev_vel"][b] = spec["vel"]
+    st["angvel"][b] = 0.0
+    gpu.write_state(st["pos"], st["quat"], st["vel"], st["angvel"], st["prev_vel"])
+
+
+def exact_leg(args, scene, params, spec, timed_end):
+    """The same workload in the exact precision (f64, bit-identical to the
+    oracle: the error-free line) over the same steps: its throughput, and the
+    pose error of the measured precision against it at the end of the timed
+    region (the largest position and quaternion-component difference over the
+    dynamic bodies)."""
+    import avbd
+    import numpy as np
+    ex = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    ex.pack(scene, params)
+    if spec is not None:
+        settle_and_strike(ex, spec)
+    ex.step(args.warmup)
+    ex.idle()
+    t0 = time.perf_counter()
+    ex.step(args.steps)
+    ex.idle()
+    secs = time.perf_counter() - t0
+    st = ex.read_state()
+    nd = ex.counters()["dynamic_bodies"]
+    dyn = np.array([b["density"] > 0 for b in scene["bodies"]])
+    qa, qb = st["quat"][dyn], timed_end["quat"][dyn]
+    sgn = np.where(np.sum(qa * qb, axis=1) < 0, -1.0, 1.0)[:, None]
+    err_p = float(np.abs(st["pos"][dyn] - timed_end["pos"][dyn]).max())
+    err_q = float(np.abs(qa - sgn * qb).max())
+    return {"value": nd * params["iterations"] * args.steps / secs, "unit": "body-iterations/s",
+            "ms_per_step": secs / args.steps * 1e3, "dtype": "f64",
+            "pose_error_of_measured_precision_m": err_p, "quat_error_of_measured_precision": err_q,
+            "note": "exact precision: f64, every body's system summed in the reference's order, bit-identical "
+                    "to the f64 oracle; same steps as the timed region"}
+
+
 def host_cpu():
     """The host CPU's model name and the logical CPUs this process may run
     on (the CPU baseline uses one of them)."""
@@ -206,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch the solve loop directly, without graph replay")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="pyramid: skip the exact-precision leg (its throughput and the pose error against it)")
     ap.add_argument("--no-launch", action="store_true",
                     help="pyramid: time the first steps after pack (the settling pile) instead of the strike")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -232,19 +280,11 @@ def main():
     # at the apex by the sphere: both are part of the workload's set-up, so
     # the warm-up and timed steps come after the strike and cover the impact
     # and the pile's response.
-    settle = 0
     spec = meta.get("launch") if args.workload == "pyramid" and not args.no_launch else None
+    settle = spec["step"] if spec is not None else 0
     if spec is not None:
-        settle = spec["step"]
         t0 = time.perf_counter()
-        gpu.step(settle)
-        st = gpu.read_state()
-        b = spec["body"]
-        st["pos"][b] = spec["pos"]
-        st["vel"][b] = spec["vel"]
-        st["prev_vel"][b] = spec["vel"]
-        st["angvel"][b] = 0.0
-        gpu.write_state(st["pos"], st["quat"], st["vel"], st["angvel"], st["prev_vel"])
+        settle_and_strike(gpu, spec)
         print(f"[bench] settled {settle} steps and launched the sphere at the apex: "
               f"{time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -265,6 +305,7 @@ def main():
     gpu.step(args.steps)
     gpu.idle()
     secs = time.perf_counter() - t0
+    timed_end = gpu.read_state() if args.workload == "pyramid" and prec != avbd.PREC_EXACT else None
 
     # the same number of further steps with per-kernel timing, for the
     # roofline and the phase split (direct launches)
@@ -351,6 +392,8 @@ def main():
         gpu.n = len(scene2["bodies"])
         c2 = gpu.counters()
         line["kernels"]["repack_kept_manifolds"] = c2["manifolds"]
+    if timed_end is not None and not args.no_exact:
+        line["exact"] = exact_leg(args, scene, params, spec, timed_end)
     if not args.no_adjoint and args.workload == "pyramid":
         line["adjoint"] = adjoint_leg()
     if not args.no_cpu:

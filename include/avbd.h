@@ -278,6 +278,18 @@ int avbd_read_forces(avbd_ctx *ctx, int32_t *kind_a_b_nc, uint32_t *features, in
 int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *contacts,
                        int32_t *num_contacts);
 
+/* Every live contact manifold at once, in force-list order (the manifolds of
+ * avbd_read_forces' list, after its joints), the slots GpuBackend.readColoring
+ * returns as slotInfo (backend.js:2053-2151): per manifold `head` {body A, body
+ * B, contact count, flags (1: a sphere is involved)} and `basis_mu` (the basis,
+ * row 0 the normal B to A, then the friction coefficient; 10 doubles); per
+ * contact, manifold after manifold, `contacts` 16 doubles as in
+ * avbd_read_manifold. At most cap manifolds and contact_cap contacts are
+ * written; *count and *num_contacts receive the live totals. Any output may
+ * be NULL. */
+int avbd_read_manifolds(avbd_ctx *ctx, int32_t *head, double *basis_mu, double *contacts, int32_t cap,
+                        int32_t contact_cap, int32_t *count, int32_t *num_contacts);
+
 /* --- metrics: GpuBackend.lastStats (backend.js:309-311, 1622-1792) */
 int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out);
 

@@ -118,6 +118,17 @@ def pose_error(a, b, dyn):
     return max(dp, dq)
 
 
+def lambda_error(a, b):
+    """Largest difference of any contact multiplier (normal and friction
+    rows) between two contexts holding the same force list, and the largest
+    multiplier (N: lambda is a force, manifold.js:447-491)."""
+    _, _, ca = a.read_manifolds()
+    _, _, cb = b.read_manifolds()
+    if len(ca) == 0:
+        return 0.0, 0.0
+    return float(np.abs(ca[:, 12:15] - cb[:, 12:15]).max()), float(np.abs(ca[:, 12:15]).max())
+
+
 def same_forces(a, b):
     ka, fa = a.read_forces()
     kb, fb = b.read_forces()
@@ -182,6 +193,7 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
     va.pack(scene, params)
     dyn = np.array([b["density"] > 0 for b in scene["bodies"]])
     diverged, before, worst, worst_step = None, 0.0, 0.0, 0
+    lam_err, lam_max = 0.0, 0.0
     per_step = []
     for s in range(1, steps + 1):
         ex.step(1)
@@ -196,6 +208,8 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
         if diverged is None:
             if same_forces(ex, va):
                 before = max(before, err)
+                dl, lm = lambda_error(ex, va)
+                lam_err, lam_max = max(lam_err, dl), max(lam_max, lm)
             else:
                 diverged = s
         if spec is not None and s == spec["step"]:
@@ -204,6 +218,7 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
     gap_ex, gap_va = socket_gaps(a, scene["joints"]), socket_gaps(b, scene["joints"])
     rec = dict(config=name, precision=precision, steps=steps, first_divergence=diverged,
                error_before_divergence=before, worst_error=worst, worst_step=worst_step,
+               lambda_error_before_divergence=lam_err, largest_lambda=lam_max,
                socket_gap=dict(exact=float(gap_ex.max(initial=0.0)), variant=float(gap_va.max(initial=0.0))),
                error_at=dict((str(k), per_step[k - 1]) for k in (10, 30, 60, 90, 120, 150, 300) if k <= steps))
     print("[configs] " + json.dumps(rec), flush=True)
@@ -215,6 +230,10 @@ def test_throughput_precisions_track_exact_at_full_size(name, precision):
     else:
         assert diverged is None or diverged >= floor, f"{name} {precision}: force lists diverged at step {diverged}"
     assert before <= tol_before, f"{name} {precision}: pose error {before:.3e} before the first divergence"
+    # the multipliers before the divergence: relative to the largest, within
+    # the pose tolerance's order (the mixed mode's k and lambda are f32)
+    assert lam_err <= max(1e-3 if precision == "mixed" else 1e-9, tol_before) * max(lam_max, 1.0), \
+        f"{name} {precision}: lambda error {lam_err:.3e} of {lam_max:.3e}"
     if bound is not None:
         assert worst <= bound, f"{name} {precision}: pose error {worst:.3e} at step {worst_step}"
     if len(gap_ex):

@@ -95,3 +95,23 @@ def test_custom_claim_order_is_greedy_in_that_order(seed):
             c += 1
         want[i] = c
     assert np.array_equal(col, want) and k == want.max() + 1
+
+
+def test_read_manifolds_equals_one_by_one_reads():
+    scene, params, _ = avbd.scenes.pyramid(layers=5)
+    for prec in (avbd.PREC_EXACT, avbd.PREC_MIXED):
+        g = avbd.HipBackend(precision=prec)
+        g.pack(scene, params)
+        g.step(6)
+        kab, feats = g.read_forces()
+        head, bm, con = g.read_manifolds()
+        nj = int((kab[:, 0] != 0).sum())
+        assert len(head) == len(kab) - nj and np.array_equal(head[:, :3], kab[nj:, 1:4])
+        k = 0
+        for t in range(len(head)):
+            m = g.read_manifold(nj + t)
+            assert np.array_equal(bm[t, :9], m["basis"].reshape(9)) and bm[t, 9] == m["mu"]
+            nc = len(m["contacts"])
+            assert np.array_equal(con[k:k + nc], m["contacts"])
+            k += nc
+        assert k == len(con)
