@@ -39,7 +39,8 @@ pd, pi, pu = P(d), P(i32), P(u32)
 
 class Config(C.Structure):
     _fields_ = [("device", i32), ("precision", i32), ("team", i32), ("manifold_cap", i32),
-                ("pair_cap", i32), ("graphs", i32), ("adjoint_small", i32), ("adjoint_tape_mb", i32)]
+                ("pair_cap", i32), ("graphs", i32), ("adjoint_small", i32), ("adjoint_tape_mb", i32),
+                ("scene_solve", i32)]
 
 
 class Params(C.Structure):

@@ -98,12 +98,12 @@ class HipBackend:
     """One device context (stands in for GpuBackend; backend.js:243-335)."""
 
     def __init__(self, precision=L.PREC_MIXED, team=16, device=0, manifold_cap=0, pair_cap=0, graphs=0,
-                 adjoint_small=0, adjoint_tape_mb=0):
+                 adjoint_small=0, adjoint_tape_mb=0, scene_solve=0):
         self.L = L.lib()
         self.ctx = C.c_void_p()
         cfg = L.Config(device=device, precision=precision, team=team, manifold_cap=manifold_cap,
                        pair_cap=pair_cap, graphs=graphs, adjoint_small=adjoint_small,
-                       adjoint_tape_mb=adjoint_tape_mb)
+                       adjoint_tape_mb=adjoint_tape_mb, scene_solve=scene_solve)
         rc = self.L.avbd_create(C.byref(self.ctx), C.byref(cfg))
         if rc != 0:
             raise AvbdError(rc, "avbd_create failed (no GPU, or bad configuration)")

@@ -162,6 +162,11 @@ struct avbd_ctx {
   DBuf<int> bucketCnt, bucketStart, bucketCursor, entries, bpScratch;
   DBuf<int4> entryCell;
   DBuf<int> sceneLargeStart, sceneLarges, sceneStart, largeList;
+  // batches: the solve as one launch with a workgroup per scene
+  DBuf<int> sceneJointStart;
+  int sceneSolveCfg = 0;     // avbd_config.scene_solve
+  bool sceneSolveOK = false; // the packed batch qualifies (2+ scenes, small, joints grouped)
+  int sceneTeam = 0;         // lanes per body in it (0: from team)
   DBuf<int> countRev, startRev, countNewRev, startNewRev;
   DBuf<int2> pairs, newPairs;
   int nPairs = 0, nNew = 0;
@@ -351,6 +356,7 @@ struct avbd_ctx {
     cellOf.release(); bucketCnt.release(); bucketStart.release(); bucketCursor.release(); entries.release();
     bpScratch.release(); entryCell.release();
     sceneLargeStart.release(); sceneLarges.release(); sceneStart.release(); largeList.release();
+    sceneJointStart.release();
     countRev.release(); startRev.release(); countNewRev.release(); startNewRev.release(); cmeta.release(); pairs.release(); newPairs.release();
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
@@ -615,6 +621,100 @@ struct avbd_ctx {
     }
   }
 
+  // The batch's solve in one launch (k_solve_scenes), then the contact dual
+  // of the last iteration (k_dual, which also records C). Used when the
+  // packed batch qualifies, post-stabilisation is off and no adjoint step is
+  // being recorded.
+  bool useScenes() const { return sceneSolveOK && sceneSolveCfg >= 0 && !prm.post_stabilize && !rec; }
+
+  int sceneTeamShape() const {
+    const int t = sceneTeam > 0 ? sceneTeam : team;
+    return t <= 4 ? 4 : t <= 8 ? 8 : 16;
+  }
+
+  template <class R, bool EXACT>
+  void launchScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, int tm) {
+    SceneSolve Q;
+    Q.sceneStart = sceneStart.p; Q.sceneJointStart = sceneJointStart.p; Q.cmeta = cmeta.p; Q.cwork = cwork.p;
+    Q.V = V.p; Q.joints = joints.p; Q.iterations = prm.iterations; Q.alpha = prm.alpha;
+    // algorithmic bytes: every sweep of the step (the folded dual update in
+    // all but the first iteration)
+    double bytes = 0.0;
+    if (profiling)
+      for (int c = 0; c < numColors; c++)
+        bytes += double(prm.iterations) * double(hcbytes[c]) + double(prm.iterations - 1) * double(hcbytes[MAX_COLORS + c]);
+    R *dq = (R *)DQ.p;
+    if constexpr (EXACT) {
+      timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, true, 1>, dim3(S), dim3(256), Q, sa, mb, dq, P);
+    } else {
+      switch (tm) {
+        case 4: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 4>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
+        case 8: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 8>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
+        default: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 16>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
+      }
+    }
+  }
+
+  template <class R, bool EXACT>
+  void enqueueScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
+    launchScenes<R, EXACT>(sa, mb, P, sceneTeamShape());
+    if (graphCapturing || nLive > 0)
+      timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
+                  (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, prm.alpha, 1);
+  }
+
+  // The autotuner for the one-launch batch solve: the whole solve of this
+  // step timed for each team shape (lanes per body: 4, 8, 16; the tile of
+  // bodies a workgroup updates at once is 256 / lanes), restoring everything
+  // it writes between trials; the fastest is kept.
+  template <class R>
+  void autotuneScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
+    static const int cands[3] = {4, 8, 16};
+    const size_t xb = sizeof(double) * size_t(N) * XS, vb = sizeof(double) * size_t(N) * VS,
+                 qb = size_t(N) * 8 * rsize(), jb = sizeof(JointDev) * size_t(J);
+    const size_t nT = size_t(std::max(lastNTot, 1)), hb = nT * MAXC * HOT * rsize(), sb = nT * MAXC * sizeof(int32_t);
+    ManStore &m = store[cur];
+    DBuf<char> bX, bX0, bY, bV, bQ, bJ, bH, bS;
+    bX.reserve(xb); bX0.reserve(xb); bY.reserve(xb); bV.reserve(vb); bQ.reserve(qb); bJ.reserve(std::max<size_t>(jb, 1));
+    bH.reserve(hb); bS.reserve(sb);
+    auto copy = [&](void *d, const void *src, size_t n) {
+      if (n) HIPCHECK(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, stream));
+    };
+    auto save = [&]() {
+      copy(bX.p, X.p, xb); copy(bX0.p, X0.p, xb); copy(bY.p, Y.p, xb); copy(bV.p, V.p, vb); copy(bQ.p, DQ.p, qb);
+      copy(bJ.p, joints.p, jb); copy(bH.p, m.hot.p, hb); copy(bS.p, m.stick.p, sb);
+    };
+    auto restore = [&]() {
+      copy(X.p, bX.p, xb); copy(X0.p, bX0.p, xb); copy(Y.p, bY.p, xb); copy(V.p, bV.p, vb); copy(DQ.p, bQ.p, qb);
+      copy(joints.p, bJ.p, jb); copy(m.hot.p, bH.p, hb); copy(m.stick.p, bS.p, sb);
+    };
+    save();
+    hipEvent_t a = takeEvent(), b = takeEvent();
+    const bool prof = profiling;
+    profiling = false;
+    int best = sceneTeamShape();
+    float bestMs = 1e30f;
+    for (int k = 0; k < 6; k++) tuneMs[k] = 0.0f;
+    for (int k = 0; k < 3; k++) {
+      HIPCHECK(hipEventRecord(a, stream));
+      launchScenes<R, false>(sa, mb, P, cands[k]);
+      HIPCHECK(hipEventRecord(b, stream));
+      HIPCHECK(hipEventSynchronize(b));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, a, b));
+      tuneMs[k + 1] = ms;
+      if (ms < bestMs) { bestMs = ms; best = cands[k]; }
+      restore();
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+    bX.release(); bX0.release(); bY.release(); bV.release(); bQ.release(); bJ.release(); bH.release(); bS.release();
+    profiling = prof;
+    evPool.push_back(a);
+    evPool.push_back(b);
+    if (best != sceneTeam) dropGraphs();
+    sceneTeam = best;
+  }
+
   void dropGraphs() {
     for (auto &g : graphs) {
       (void)hipGraphExecDestroy(g.second.second);
@@ -635,6 +735,36 @@ struct avbd_ctx {
       graphCapturing = true;
       try {
         enqueueSolve<R, EXACT>(sa, mb, P, true);
+      } catch (...) {
+        graphCapturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      graphCapturing = false;
+      HIPCHECK(hipStreamEndCapture(stream, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) (void)hipGraphDestroy(g);
+      HIPCHECK(e);
+      it = graphs.emplace(key, std::make_pair(g, ex)).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second.second, stream));
+  }
+
+  // The one-launch batch solve as a replayed graph: its launches take every
+  // count from device memory, so one graph per manifold buffer parity serves
+  // every step (keys above 4 * (MAX_COLORS + 1) are these).
+  template <class R, bool EXACT>
+  void sceneGraph(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
+    const long long key = 4LL * (MAX_COLORS + 1) + cur;
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      graphCapturing = true;
+      try {
+        enqueueScenes<R, EXACT>(sa, mb, P);
       } catch (...) {
         graphCapturing = false;
         (void)hipStreamEndCapture(stream, &g);
@@ -783,7 +913,14 @@ struct avbd_ctx {
     SolveArgs sa;
     sa.props = props.p; sa.X = X.p; sa.X0 = X0.p; sa.Y = Y.p; sa.itemStart = itemStart.p; sa.items = items.p;
     sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N; sa.jointBodiesOnly = 0;
-    if (tuneRequest && !EXACT) {
+    if (useScenes()) {
+      if (tuneRequest && !EXACT) {
+        autotuneScenes<R>(sa, mb, P);
+        tuneRequest = false;
+      }
+      if (graphsCfg >= 0 && !profiling) sceneGraph<R, EXACT>(sa, mb, P);
+      else enqueueScenes<R, EXACT>(sa, mb, P);
+    } else if (tuneRequest && !EXACT) {
       hipLaunchKernelGGL((k_body_init<R>), dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, X0.p, Y.p, V.p,
                          (R *)DQ.p, P);
       autotuneT<R>(sa, mb, P);
@@ -1296,6 +1433,24 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   }
   c.hjointKind.resize(size_t(c.J));
   for (int j = 0; j < c.J; j++) c.hjointKind[size_t(j)] = JD[size_t(j)].kind;
+  // the one-launch solve of a batch: joints grouped by scene in scene order
+  {
+    std::vector<int> sjs(S + 1, 0);
+    bool grouped = true;
+    int last = 0;
+    for (int j = 0; j < c.J; j++) {
+      const int sc = sceneOf[JD[size_t(j)].b];
+      if (sc < last) grouped = false;
+      last = sc;
+      sjs[sc + 1]++;
+    }
+    for (int k = 0; k < S; k++) sjs[k + 1] += sjs[k];
+    int biggest = 0;
+    for (int k = 0; k < S; k++) biggest = std::max(biggest, sceneStart[k + 1] - sceneStart[k]);
+    c.sceneSolveOK = S >= 2 && grouped && biggest <= 4096;
+    c.sceneJointStart.reserve(S + 1);
+    HIPCHECK(hipMemcpy(c.sceneJointStart.p, sjs.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
+  }
   c.joints.reserve(std::max(c.J, 1));
   if (c.J) HIPCHECK(hipMemcpy(c.joints.p, JD.data(), sizeof(JointDev) * c.J, hipMemcpyHostToDevice));
 
@@ -1407,6 +1562,7 @@ int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
                        : cfg->adjoint_tape_mb < 0 ? -1L
                                                   : long(cfg->adjoint_tape_mb) << 20;
     c->pairCapCfg = cfg->pair_cap;
+    c->sceneSolveCfg = cfg->scene_solve;
   }
   if (c->prec < AVBD_PREC_EXACT || c->prec > AVBD_PREC_MIXED) {
     delete c;
@@ -1971,7 +2127,7 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
     ctx->dropAdjoint();
     ctx->tuneRequest = true;
     ctx->step();
-    if (team) *team = ctx->team;
+    if (team) *team = ctx->useScenes() ? ctx->sceneTeamShape() : ctx->team;
     if (ms) for (int k = 0; k < 6; k++) ms[k] = ctx->tuneMs[k];
   });
 }

@@ -776,10 +776,7 @@ __device__ void jointStamp(Sys<double> &s, const JointDev &j, bool isA, const do
 }
 
 // Joint.updateDual (joint.js:255-317)
-__global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParams P, double alpha) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= J) return;
-  JointDev &j = joints[i];
+__device__ __forceinline__ void jointDualOne(JointDev &j, const double *X, const StepParams &P, double alpha) {
   if (j.broken == 2) return;
   if (j.kind == AVBD_JOINT_SPRING) {
     // Spring.updateDual (spring.js:271-288): the constraint value and
@@ -821,6 +818,11 @@ __global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParam
     for (int k = 0; k < 3; k++) j.kLin[k] = j.kAng[k] = j.lamLin[k] = j.lamAng[k] = 0.0;
     j.broken = 1;
   }
+}
+
+__global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParams P, double alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < J) jointDualOne(joints[i], X, P, alpha);
 }
 
 // ---------------------------------------------------------------- topology
@@ -1073,16 +1075,23 @@ __global__ void k_color_starts(const int *ctotal, int *cstart) {
 // Place the bodies: cbodies (the body list of each colour) and cwork, the
 // same list with each body's work-item range and its number of joint items,
 // which is all a primal team needs to find its work.
+// Within a colour the bodies stay in index order (a body's place in its block
+// is the number of lower-indexed bodies of its colour there), so the bodies of
+// one scene of a batch form one run of each colour's list. 256 threads.
 __global__ void k_color_scatter(int n, const int *color, const int *offs, const int *cstart, int *cbodies,
                                 const int *itemStart, const int *itemJoints, int4 *cwork) {
-  __shared__ int cur[MAX_COLORS];
+  __shared__ int base[MAX_COLORS];
+  __shared__ int col[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = threadIdx.x; k < MAX_COLORS; k += blockDim.x)
-    cur[k] = cstart[k] + offs[size_t(blockIdx.x) * MAX_COLORS + k];
-  __syncthreads();
+    base[k] = cstart[k] + offs[size_t(blockIdx.x) * MAX_COLORS + k];
   const int c = i < n ? color[i] : -1;
+  col[threadIdx.x] = c;
+  __syncthreads();
   if (c < 0) return;
-  const int pos = atomicAdd(&cur[c], 1);
+  int rank = 0;
+  for (int j = 0; j < int(threadIdx.x); j++) rank += col[j] == c;
+  const int pos = base[c] + rank;
   cbodies[pos] = i;
   cwork[pos] = make_int4(i, itemStart[i], itemStart[i + 1], itemJoints[i]);
 }
@@ -1194,10 +1203,8 @@ __global__ void k_color_bytes(int n, int J, const BodyProp *props, const int *co
 // Body initialisation (solver.js:366-401), plus the per-body delta record
 // dq = (p - p0, q (-) q0) that neighbours read.
 template <class R>
-__global__ void k_body_init(int n, const BodyProp *props, double *X, double *X0, double *Y, const double *V, R *DQ,
-                            StepParams P) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void bodyInitOne(int i, const BodyProp *props, double *X, double *X0, double *Y,
+                                            const double *V, R *DQ, const StepParams &P) {
   const double dt = P.dt;
   const BodyProp &b = props[i];
   double *x = X + size_t(i) * XS, *x0 = X0 + size_t(i) * XS, *y = Y + size_t(i) * XS;
@@ -1230,6 +1237,13 @@ __global__ void k_body_init(int n, const BodyProp *props, double *X, double *X0,
   d[0] = R(x[0] - x0[0]); d[1] = R(x[1] - x0[1]); d[2] = R(x[2] - x0[2]);
   d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
   d[6] = R(0); d[7] = R(0);
+}
+
+template <class R>
+__global__ void k_body_init(int n, const BodyProp *props, double *X, double *X0, double *Y, const double *V, R *DQ,
+                            StepParams P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bodyInitOne<R>(i, props, X, X0, Y, V, DQ, P);
 }
 
 struct SolveArgs {
@@ -1275,207 +1289,215 @@ constexpr int PRIMAL_BATCH = 8;
 // contact terms are computed in R), the team reduces in f64, and lane 0 adds
 // the inertial terms and the joints (cwork.w of them) in f64: joint rows can
 // be stiff enough to swamp a light body's inertia in f32.
+// One body's primal update (the body of the team loop below; TEAM lanes of
+// one wave call it together for the same work entry w).
+template <class R, bool EXACT, int TEAM>
+__device__ __forceinline__ void primalBody(const int4 w, const int lane, const SolveArgs &S, ManBuf<R> mb, R *DQ,
+                                           const StepParams &P, double alpha, int dual, double dualAlpha) {
+  using Acc = typename std::conditional<EXACT, double, R>::type;
+  const double dt2 = P.dt * P.dt;
+  const int i = w.x, b0 = w.y, b1 = w.z;
+  const BodyProp &bp = S.props[i];
+  double *x = S.X + size_t(i) * XS;
+  const double *y = S.Y + size_t(i) * XS;
+  const double *x0 = S.X0 + size_t(i) * XS;
+  R dqi[8];
+  vload<8>(dqi, DQ + size_t(i) * 8);
+  // own pose and the step's start and inertial target, for lane 0's solve
+  // (requested with the body's other records, so that they arrive during
+  // the contact gathers; the exact mode, short of registers, asks after
+  // them)
+  double xs[7], ys[7], x0s[7];
+  if constexpr (!EXACT)
+    if (lane == 0)
+      for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
+  // lane 0's inertial terms (f64, team modes): M/dt^2 (rotated inertia) and
+  // M/dt^2 (x - y). Computed while the first contact records are in flight
+  // (the wave would otherwise only wait for them), or after the gathers for
+  // a body without contacts.
+  double inMl = 0, inAng[6], inRl[3], inRa[3];
+  bool inDone = false;
+  auto inertia = [&]() {
+    double MAng[9], Ma9[9], dq[3];
+    worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+    const double idt2 = frcp(dt2);
+    inMl = bp.mass * idt2;
+    for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] * idt2;
+    qsub(dq, xs + 3, ys + 3);
+    mulv(inRa, Ma9, dq);
+    inAng[0] = Ma9[0]; inAng[1] = Ma9[3]; inAng[2] = Ma9[4];
+    inAng[3] = Ma9[6]; inAng[4] = Ma9[7]; inAng[5] = Ma9[8];
+    for (int k = 0; k < 3; k++) inRl[k] = inMl * (xs[k] - ys[k]);
+    inDone = true;
+  };
+  Sys<Acc> sl;
+  sl.zero();
+  if constexpr (EXACT) {
+    // inertial terms first, as the reference assembles them
+    double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
+    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+    for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
+    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
+    mulv(sl.rl, Ml, t);
+    qsub(dq, x + 3, y + 3);
+    mulv(sl.ra, Ma, dq);
+    sl.lin[0] = Ml[0]; sl.lin[1] = Ml[3]; sl.lin[2] = Ml[4]; sl.lin[3] = Ml[6]; sl.lin[4] = Ml[7]; sl.lin[5] = Ml[8];
+    sl.ang[0] = Ma[0]; sl.ang[1] = Ma[3]; sl.ang[2] = Ma[4]; sl.ang[3] = Ma[6]; sl.ang[4] = Ma[7]; sl.ang[5] = Ma[8];
+  }
+  const R ralpha = R(alpha), rdalpha = R(dualAlpha);
+  const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
+  constexpr int IB = PRIMAL_BATCH;
+  for (int t0 = b0 + lane; t0 < b1; t0 += TEAM * IB) {
+    uint2 itw[IB];
+#pragma unroll
+    for (int k = 0; k < IB; k++) {
+      const int tk = t0 + k * TEAM;
+      itw[k] = tk < b1 ? S.items[tk] : make_uint2(0xffffffffu, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < IB; k++) {
+      const uint2 iw = itw[k];
+      if (iw.x == 0xffffffffu) break;
+      const bool isA = iw.x & 1u;
+      if (iw.x & 0x80000000u) {
+        if constexpr (EXACT) jointStamp(sl, S.joints[int((iw.x & 0x7fffffffu) >> 1)], isA, S.X, alpha);
+        continue;
+      }
+      const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
+      R bas[BAS], hot[HOT], dqo[8];
+      vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
+      vload<HOT>(hot, mb.hot + si * HOT);
+      vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
+      if constexpr (!EXACT && sizeof(R) == 4)  // f64 records leave no registers for it
+        if (k == 0 && lane == 0 && !inDone) inertia();
+      if constexpr (!EXACT) {
+        const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+        R rW[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) rW[q] = isA ? hot[HRA + q] : hot[HRB + q];
+        FEval<R> e;
+        if (dual && (iw.y & ITEM_FIRST)) {
+          fevalC(bas, hot, dA, dB, rdalpha, e);
+          fevalF(hot + HK, hot + HL, bas[9], e);
+          R kl[6];
+          const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+          vstoreKL(mb.hot + si * HOT, kl);
+          if (st >= 0) mb.stick[si] = st;
+          if (rdalpha != ralpha) fevalCAlpha(bas, hot, ralpha, e);
+          fevalF(kl, kl + 3, bas[9], e);
+          fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, kl, e.F);
+        } else {
+          fevalC(bas, hot, dA, dB, ralpha, e);
+          fevalF(hot + HK, hot + HL, bas[9], e);
+          fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, hot + HK, e.F);
+        }
+        continue;
+      }
+      R negb[9];
+      for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+      const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+      Eval<R> e;
+      if (dual && (iw.y & ITEM_FIRST)) {
+        evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, rdalpha, e);
+        R kl[6];
+        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+        vstoreKL(mb.hot + si * HOT, kl);
+        if (st >= 0) mb.stick[si] = st;
+#pragma unroll
+        for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];  // k then lambda
+        if (rdalpha != ralpha) evalC<R, EXACT>(bas, negb, hot, dA, dB, ralpha, e);
+        evalF<R, EXACT>(hot + HK, hot + HL, bas[9], e);
+      } else {
+        evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, ralpha, e);
+      }
+      stampContact<R, Acc>(sl, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
+    }
+  }
+  if constexpr (EXACT)
+    if (lane == 0)
+      for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
+  Sys<double> s;
+  if constexpr (TEAM > 1 && std::is_same<Acc, float>::value) {
+    // f32 lane sums reduce in f32: the tree adds four roundings to the
+    // lanes' own sums, at a third of the instructions of an f64 tree
+#pragma unroll
+    for (int k = 0; k < 6; k++) { sl.lin[k] = teamSumF<TEAM>(sl.lin[k]); sl.ang[k] = teamSumF<TEAM>(sl.ang[k]); }
+#pragma unroll
+    for (int k = 0; k < 9; k++) sl.cross[k] = teamSumF<TEAM>(sl.cross[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) { sl.rl[k] = teamSumF<TEAM>(sl.rl[k]); sl.ra[k] = teamSumF<TEAM>(sl.ra[k]); }
+  }
+  for (int k = 0; k < 6; k++) { s.lin[k] = double(sl.lin[k]); s.ang[k] = double(sl.ang[k]); }
+  for (int k = 0; k < 9; k++) s.cross[k] = double(sl.cross[k]);
+  for (int k = 0; k < 3; k++) { s.rl[k] = double(sl.rl[k]); s.ra[k] = double(sl.ra[k]); }
+  if constexpr (TEAM > 1 && !std::is_same<Acc, float>::value) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
+#pragma unroll
+    for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
+  }
+  if constexpr (!EXACT) {
+    // joint and spring rows in f64, spread over the team: each lane stamps
+    // its share (lane 0 on top of the reduced contact terms, the others
+    // from zero) and an f64 reduction brings them to lane 0 (w.w, the
+    // body's joint-item count, is the same on every lane of the team)
+    if (w.w > 0) {
+      if (lane != 0) s.zero();
+      for (int t = b0 + lane; t < b1; t += TEAM) {
+        const uint2 itw = S.items[t];
+        if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
+      }
+      if constexpr (TEAM > 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
+#pragma unroll
+        for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
+      }
+    }
+  }
+  if (lane != 0) return;
+  if constexpr (!EXACT) {
+    if (!inDone) inertia();
+    s.lin[0] += inMl; s.lin[2] += inMl; s.lin[5] += inMl;
+    for (int k = 0; k < 6; k++) s.ang[k] += inAng[k];
+    for (int k = 0; k < 3; k++) { s.rl[k] += inRl[k]; s.ra[k] += inRa[k]; }
+  }
+  double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
+  double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
+  double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
+  double dxl[3], dxa[3];
+  double xn[7];
+  if constexpr (EXACT) {
+    solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+    qint(xn + 3, xs + 3, dxa);
+  } else {
+    solve6f(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+    qintf(xn + 3, xs + 3, dxa);
+  }
+  for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
+  for (int k = 0; k < 7; k++) x[k] = xn[k];
+  double dq[3];
+  qsub(dq, xn + 3, x0s + 3);
+  R *d = DQ + size_t(i) * 8;
+  d[0] = R(xn[0] - x0s[0]); d[1] = R(xn[1] - x0s[1]); d[2] = R(xn[2] - x0s[2]);
+  d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
+}
+
 template <class R, bool EXACT, int TEAM>
 __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                 ManBuf<R> mb, R *DQ, StepParams P, double alpha, int dual,
                                                 double dualAlpha) {
-  using Acc = typename std::conditional<EXACT, double, R>::type;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
   const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
-  const double dt2 = P.dt * P.dt;
   for (int team = gt / TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
     if (S.jointBodiesOnly && w.w == 0) continue;
-    const int i = w.x, b0 = w.y, b1 = w.z;
-    const BodyProp &bp = S.props[i];
-    double *x = S.X + size_t(i) * XS;
-    const double *y = S.Y + size_t(i) * XS;
-    const double *x0 = S.X0 + size_t(i) * XS;
-    R dqi[8];
-    vload<8>(dqi, DQ + size_t(i) * 8);
-    // own pose and the step's start and inertial target, for lane 0's solve
-    // (requested with the body's other records, so that they arrive during
-    // the contact gathers; the exact mode, short of registers, asks after
-    // them)
-    double xs[7], ys[7], x0s[7];
-    if constexpr (!EXACT)
-      if (lane == 0)
-        for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
-    // lane 0's inertial terms (f64, team modes): M/dt^2 (rotated inertia) and
-    // M/dt^2 (x - y). Computed while the first contact records are in flight
-    // (the wave would otherwise only wait for them), or after the gathers for
-    // a body without contacts.
-    double inMl = 0, inAng[6], inRl[3], inRa[3];
-    bool inDone = false;
-    auto inertia = [&]() {
-      double MAng[9], Ma9[9], dq[3];
-      worldMoment(MAng, xs + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-      const double idt2 = frcp(dt2);
-      inMl = bp.mass * idt2;
-      for (int k = 0; k < 9; k++) Ma9[k] = MAng[k] * idt2;
-      qsub(dq, xs + 3, ys + 3);
-      mulv(inRa, Ma9, dq);
-      inAng[0] = Ma9[0]; inAng[1] = Ma9[3]; inAng[2] = Ma9[4];
-      inAng[3] = Ma9[6]; inAng[4] = Ma9[7]; inAng[5] = Ma9[8];
-      for (int k = 0; k < 3; k++) inRl[k] = inMl * (xs[k] - ys[k]);
-      inDone = true;
-    };
-    Sys<Acc> sl;
-    sl.zero();
-    if constexpr (EXACT) {
-      // inertial terms first, as the reference assembles them
-      double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
-      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-      for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
-      double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3];
-      mulv(sl.rl, Ml, t);
-      qsub(dq, x + 3, y + 3);
-      mulv(sl.ra, Ma, dq);
-      sl.lin[0] = Ml[0]; sl.lin[1] = Ml[3]; sl.lin[2] = Ml[4]; sl.lin[3] = Ml[6]; sl.lin[4] = Ml[7]; sl.lin[5] = Ml[8];
-      sl.ang[0] = Ma[0]; sl.ang[1] = Ma[3]; sl.ang[2] = Ma[4]; sl.ang[3] = Ma[6]; sl.ang[4] = Ma[7]; sl.ang[5] = Ma[8];
-    }
-    const R ralpha = R(alpha), rdalpha = R(dualAlpha);
-    const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
-    constexpr int IB = PRIMAL_BATCH;
-    for (int t0 = b0 + lane; t0 < b1; t0 += TEAM * IB) {
-      uint2 itw[IB];
-#pragma unroll
-      for (int k = 0; k < IB; k++) {
-        const int tk = t0 + k * TEAM;
-        itw[k] = tk < b1 ? S.items[tk] : make_uint2(0xffffffffu, 0u);
-      }
-#pragma unroll
-      for (int k = 0; k < IB; k++) {
-        const uint2 iw = itw[k];
-        if (iw.x == 0xffffffffu) break;
-        const bool isA = iw.x & 1u;
-        if (iw.x & 0x80000000u) {
-          if constexpr (EXACT) jointStamp(sl, S.joints[int((iw.x & 0x7fffffffu) >> 1)], isA, S.X, alpha);
-          continue;
-        }
-        const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
-        R bas[BAS], hot[HOT], dqo[8];
-        vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
-        vload<HOT>(hot, mb.hot + si * HOT);
-        vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
-        if constexpr (!EXACT && sizeof(R) == 4)  // f64 records leave no registers for it
-          if (k == 0 && lane == 0 && !inDone) inertia();
-        if constexpr (!EXACT) {
-          const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
-          R rW[3];
-#pragma unroll
-          for (int q = 0; q < 3; q++) rW[q] = isA ? hot[HRA + q] : hot[HRB + q];
-          FEval<R> e;
-          if (dual && (iw.y & ITEM_FIRST)) {
-            fevalC(bas, hot, dA, dB, rdalpha, e);
-            fevalF(hot + HK, hot + HL, bas[9], e);
-            R kl[6];
-            const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
-            vstoreKL(mb.hot + si * HOT, kl);
-            if (st >= 0) mb.stick[si] = st;
-            if (rdalpha != ralpha) fevalCAlpha(bas, hot, ralpha, e);
-            fevalF(kl, kl + 3, bas[9], e);
-            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, kl, e.F);
-          } else {
-            fevalC(bas, hot, dA, dB, ralpha, e);
-            fevalF(hot + HK, hot + HL, bas[9], e);
-            fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, hot + HK, e.F);
-          }
-          continue;
-        }
-        R negb[9];
-        for (int e = 0; e < 9; e++) negb[e] = -bas[e];
-        const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
-        Eval<R> e;
-        if (dual && (iw.y & ITEM_FIRST)) {
-          evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, rdalpha, e);
-          R kl[6];
-          const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
-          vstoreKL(mb.hot + si * HOT, kl);
-          if (st >= 0) mb.stick[si] = st;
-#pragma unroll
-          for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];  // k then lambda
-          if (rdalpha != ralpha) evalC<R, EXACT>(bas, negb, hot, dA, dB, ralpha, e);
-          evalF<R, EXACT>(hot + HK, hot + HL, bas[9], e);
-        } else {
-          evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, ralpha, e);
-        }
-        stampContact<R, Acc>(sl, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
-      }
-    }
-    if constexpr (EXACT)
-      if (lane == 0)
-        for (int k = 0; k < 7; k++) { xs[k] = x[k]; ys[k] = y[k]; x0s[k] = x0[k]; }
-    Sys<double> s;
-    if constexpr (TEAM > 1 && std::is_same<Acc, float>::value) {
-      // f32 lane sums reduce in f32: the tree adds four roundings to the
-      // lanes' own sums, at a third of the instructions of an f64 tree
-#pragma unroll
-      for (int k = 0; k < 6; k++) { sl.lin[k] = teamSumF<TEAM>(sl.lin[k]); sl.ang[k] = teamSumF<TEAM>(sl.ang[k]); }
-#pragma unroll
-      for (int k = 0; k < 9; k++) sl.cross[k] = teamSumF<TEAM>(sl.cross[k]);
-#pragma unroll
-      for (int k = 0; k < 3; k++) { sl.rl[k] = teamSumF<TEAM>(sl.rl[k]); sl.ra[k] = teamSumF<TEAM>(sl.ra[k]); }
-    }
-    for (int k = 0; k < 6; k++) { s.lin[k] = double(sl.lin[k]); s.ang[k] = double(sl.ang[k]); }
-    for (int k = 0; k < 9; k++) s.cross[k] = double(sl.cross[k]);
-    for (int k = 0; k < 3; k++) { s.rl[k] = double(sl.rl[k]); s.ra[k] = double(sl.ra[k]); }
-    if constexpr (TEAM > 1 && !std::is_same<Acc, float>::value) {
-#pragma unroll
-      for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
-#pragma unroll
-      for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
-#pragma unroll
-      for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
-    }
-    if constexpr (!EXACT) {
-      // joint and spring rows in f64, spread over the team: each lane stamps
-      // its share (lane 0 on top of the reduced contact terms, the others
-      // from zero) and an f64 reduction brings them to lane 0 (w.w, the
-      // body's joint-item count, is the same on every lane of the team)
-      if (w.w > 0) {
-        if (lane != 0) s.zero();
-        for (int t = b0 + lane; t < b1; t += TEAM) {
-          const uint2 itw = S.items[t];
-          if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
-        }
-        if constexpr (TEAM > 1) {
-#pragma unroll
-          for (int k = 0; k < 6; k++) { s.lin[k] = teamSum<TEAM>(s.lin[k]); s.ang[k] = teamSum<TEAM>(s.ang[k]); }
-#pragma unroll
-          for (int k = 0; k < 9; k++) s.cross[k] = teamSum<TEAM>(s.cross[k]);
-#pragma unroll
-          for (int k = 0; k < 3; k++) { s.rl[k] = teamSum<TEAM>(s.rl[k]); s.ra[k] = teamSum<TEAM>(s.ra[k]); }
-        }
-      }
-    }
-    if (lane != 0) continue;
-    if constexpr (!EXACT) {
-      if (!inDone) inertia();
-      s.lin[0] += inMl; s.lin[2] += inMl; s.lin[5] += inMl;
-      for (int k = 0; k < 6; k++) s.ang[k] += inAng[k];
-      for (int k = 0; k < 3; k++) { s.rl[k] += inRl[k]; s.ra[k] += inRa[k]; }
-    }
-    double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
-    double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
-    double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
-    double dxl[3], dxa[3];
-    double xn[7];
-    if constexpr (EXACT) {
-      solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
-      qint(xn + 3, xs + 3, dxa);
-    } else {
-      solve6f(aLin, aAng, s.cross, bl, ba, dxl, dxa);
-      qintf(xn + 3, xs + 3, dxa);
-    }
-    for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
-    for (int k = 0; k < 7; k++) x[k] = xn[k];
-    double dq[3];
-    qsub(dq, xn + 3, x0s + 3);
-    R *d = DQ + size_t(i) * 8;
-    d[0] = R(xn[0] - x0s[0]); d[1] = R(xn[1] - x0s[1]); d[2] = R(xn[2] - x0s[2]);
-    d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
+    primalBody<R, EXACT, TEAM>(w, lane, S, mb, DQ, P, alpha, dual, dualAlpha);
   }
 }
 
@@ -1530,6 +1552,104 @@ constexpr int XT_ROW = 28;    // doubles per item row in LDS (27 used)
 // reference's LDL^T and applies Eq. 21. A body with joints or springs
 // (cwork.w > 0) is left to the one-lane sweep (skipped here): joint rows add
 // into the system several times per force (jointStamp).
+// One body's exact primal update by a team of XT_TEAM lanes of one wave,
+// with L the team's LDS rows (XT_TEAM * XT_ROW doubles); w.w must be 0 (no
+// joint items).
+__device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, double *L, const SolveArgs &S,
+                                             ManBuf<double> mb, double *DQ, const StepParams &P, double alpha,
+                                             int dual, double dualAlpha) {
+  const double dt2 = P.dt * P.dt;
+  const double beta = P.betaLin, pmax = P.pmax, stickT = P.stick;
+  const int i = w.x, b0 = w.y, b1 = w.z;
+  const BodyProp &bp = S.props[i];
+  double *x = S.X + size_t(i) * XS;
+  const double *y = S.Y + size_t(i) * XS;
+  const double *x0 = S.X0 + size_t(i) * XS;
+  double dqi[8];
+  vload<8>(dqi, DQ + size_t(i) * 8);
+  // inertial terms, the reference's first contributions (lane 0 into row 0)
+  if (lane == 0) {
+    double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
+    worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
+    for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
+    double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], rl[3], ra[3];
+    mulv(rl, Ml, t);
+    qsub(dq, x + 3, y + 3);
+    mulv(ra, Ma, dq);
+    const double lin[6] = {Ml[0], Ml[3], Ml[4], Ml[6], Ml[7], Ml[8]};
+    const double ang[6] = {Ma[0], Ma[3], Ma[4], Ma[6], Ma[7], Ma[8]};
+    for (int k = 0; k < 6; k++) { L[k] = lin[k]; L[6 + k] = ang[k]; }
+    for (int k = 0; k < 9; k++) L[12 + k] = 0.0;
+    for (int k = 0; k < 3; k++) { L[21 + k] = rl[k]; L[24 + k] = ra[k]; }
+  }
+  waveSync();
+  double acc = lane < XT_TERMS ? L[lane] : 0.0;
+  waveSync();
+  for (int t0 = b0; t0 < b1; t0 += XT_TEAM) {
+    const int t = t0 + lane;
+    if (t < b1) {
+      const uint2 iw = S.items[t];
+      const bool isA = iw.x & 1u;
+      const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
+      double bas[BAS], hot[HOT], dqo[8], negb[9];
+      vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
+      vload<HOT>(hot, mb.hot + si * HOT);
+      vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
+      for (int e = 0; e < 9; e++) negb[e] = -bas[e];
+      const double *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
+      Eval<double> e;
+      if (dual && (iw.y & ITEM_FIRST)) {
+        evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, dualAlpha, e);
+        double kl[6];
+        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+        vstoreKL(mb.hot + si * HOT, kl);
+        if (st >= 0) mb.stick[si] = st;
+#pragma unroll
+        for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];
+        if (dualAlpha != alpha) evalC<double, true>(bas, negb, hot, dA, dB, alpha, e);
+        evalF<double, true>(hot + HK, hot + HL, bas[9], e);
+      } else {
+        evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, alpha, e);
+      }
+      double o[XT_TERMS];
+      contactTerms(o, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
+      double *row = L + lane * XT_ROW;
+#pragma unroll
+      for (int k = 0; k < XT_TERMS; k++) row[k] = o[k];
+    }
+    waveSync();
+    const int m = min(XT_TEAM, b1 - t0);
+    if (lane < XT_TERMS)
+      for (int k = 0; k < m; k++) acc += L[k * XT_ROW + lane];
+    waveSync();
+  }
+  // the 27 sums to lane 0
+  if (lane < XT_TERMS) L[lane] = acc;
+  waveSync();
+  if (lane == 0) {
+    Sys<double> s;
+    for (int k = 0; k < 6; k++) { s.lin[k] = L[k]; s.ang[k] = L[6 + k]; }
+    for (int k = 0; k < 9; k++) s.cross[k] = L[12 + k];
+    for (int k = 0; k < 3; k++) { s.rl[k] = L[21 + k]; s.ra[k] = L[24 + k]; }
+    double xs[7], x0s[7];
+    for (int k = 0; k < 7; k++) { xs[k] = x[k]; x0s[k] = x0[k]; }
+    double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
+    double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
+    double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
+    double dxl[3], dxa[3], xn[7];
+    solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
+    qint(xn + 3, xs + 3, dxa);
+    for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
+    for (int k = 0; k < 7; k++) x[k] = xn[k];
+    double dq[3];
+    qsub(dq, xn + 3, x0s + 3);
+    double *d = DQ + size_t(i) * 8;
+    d[0] = xn[0] - x0s[0]; d[1] = xn[1] - x0s[1]; d[2] = xn[2] - x0s[2];
+    d[3] = dq[0]; d[4] = dq[1]; d[5] = dq[2];
+  }
+  waveSync();  // row 0 is the next body's
+}
+
 __global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                    ManBuf<double> mb, double *DQ, StepParams P, double alpha,
                                                    int dual, double dualAlpha) {
@@ -1538,100 +1658,112 @@ __global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, 
   const int lane = gt % XT_TEAM, teams = gridDim.x * blockDim.x / XT_TEAM;
   double *L = rows[threadIdx.x / XT_TEAM];
   const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
-  const double dt2 = P.dt * P.dt;
-  const double beta = P.betaLin, pmax = P.pmax, stickT = P.stick;
   for (int team = gt / XT_TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
     if (w.w > 0) continue;  // joints: the one-lane sweep
-    const int i = w.x, b0 = w.y, b1 = w.z;
-    const BodyProp &bp = S.props[i];
-    double *x = S.X + size_t(i) * XS;
-    const double *y = S.Y + size_t(i) * XS;
-    const double *x0 = S.X0 + size_t(i) * XS;
-    double dqi[8];
-    vload<8>(dqi, DQ + size_t(i) * 8);
-    // inertial terms, the reference's first contributions (lane 0 into row 0)
-    if (lane == 0) {
-      double MLin[9] = {bp.mass, 0, 0, 0, bp.mass, 0, 0, 0, bp.mass}, MAng[9], Ml[9], Ma[9];
-      worldMoment(MAng, x + 3, bp.m0, bp.m1, bp.m2, !P.rotated || bp.shape == 1);
-      for (int k = 0; k < 9; k++) { Ml[k] = MLin[k] / dt2; Ma[k] = MAng[k] / dt2; }
-      double t[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]}, dq[3], rl[3], ra[3];
-      mulv(rl, Ml, t);
-      qsub(dq, x + 3, y + 3);
-      mulv(ra, Ma, dq);
-      const double lin[6] = {Ml[0], Ml[3], Ml[4], Ml[6], Ml[7], Ml[8]};
-      const double ang[6] = {Ma[0], Ma[3], Ma[4], Ma[6], Ma[7], Ma[8]};
-      for (int k = 0; k < 6; k++) { L[k] = lin[k]; L[6 + k] = ang[k]; }
-      for (int k = 0; k < 9; k++) L[12 + k] = 0.0;
-      for (int k = 0; k < 3; k++) { L[21 + k] = rl[k]; L[24 + k] = ra[k]; }
-    }
-    waveSync();
-    double acc = lane < XT_TERMS ? L[lane] : 0.0;
-    waveSync();
-    for (int t0 = b0; t0 < b1; t0 += XT_TEAM) {
-      const int t = t0 + lane;
-      if (t < b1) {
-        const uint2 iw = S.items[t];
-        const bool isA = iw.x & 1u;
-        const size_t si = size_t(iw.x >> 4) * MAXC + ((iw.x >> 1) & 7u);
-        double bas[BAS], hot[HOT], dqo[8], negb[9];
-        vload<BAS>(bas, mb.basis + size_t(iw.x >> 4) * BAS);
-        vload<HOT>(hot, mb.hot + si * HOT);
-        vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
-        for (int e = 0; e < 9; e++) negb[e] = -bas[e];
-        const double *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
-        Eval<double> e;
-        if (dual && (iw.y & ITEM_FIRST)) {
-          evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, dualAlpha, e);
-          double kl[6];
-          const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
-          vstoreKL(mb.hot + si * HOT, kl);
-          if (st >= 0) mb.stick[si] = st;
-#pragma unroll
-          for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];
-          if (dualAlpha != alpha) evalC<double, true>(bas, negb, hot, dA, dB, alpha, e);
-          evalF<double, true>(hot + HK, hot + HL, bas[9], e);
-        } else {
-          evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, alpha, e);
-        }
-        double o[XT_TERMS];
-        contactTerms(o, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
-        double *row = L + lane * XT_ROW;
-#pragma unroll
-        for (int k = 0; k < XT_TERMS; k++) row[k] = o[k];
-      }
-      waveSync();
-      const int m = min(XT_TEAM, b1 - t0);
-      if (lane < XT_TERMS)
-        for (int k = 0; k < m; k++) acc += L[k * XT_ROW + lane];
-      waveSync();
-    }
-    // the 27 sums to lane 0
-    if (lane < XT_TERMS) L[lane] = acc;
-    waveSync();
-    if (lane == 0) {
-      Sys<double> s;
-      for (int k = 0; k < 6; k++) { s.lin[k] = L[k]; s.ang[k] = L[6 + k]; }
-      for (int k = 0; k < 9; k++) s.cross[k] = L[12 + k];
-      for (int k = 0; k < 3; k++) { s.rl[k] = L[21 + k]; s.ra[k] = L[24 + k]; }
-      double xs[7], x0s[7];
-      for (int k = 0; k < 7; k++) { xs[k] = x[k]; x0s[k] = x0[k]; }
-      double aLin[9] = {s.lin[0], 0, 0, s.lin[1], s.lin[2], 0, s.lin[3], s.lin[4], s.lin[5]};
-      double aAng[9] = {s.ang[0], 0, 0, s.ang[1], s.ang[2], 0, s.ang[3], s.ang[4], s.ang[5]};
-      double bl[3] = {-s.rl[0], -s.rl[1], -s.rl[2]}, ba[3] = {-s.ra[0], -s.ra[1], -s.ra[2]};
-      double dxl[3], dxa[3], xn[7];
-      solve6(aLin, aAng, s.cross, bl, ba, dxl, dxa);
-      qint(xn + 3, xs + 3, dxa);
-      for (int k = 0; k < 3; k++) xn[k] = xs[k] + dxl[k];
-      for (int k = 0; k < 7; k++) x[k] = xn[k];
-      double dq[3];
-      qsub(dq, xn + 3, x0s + 3);
-      double *d = DQ + size_t(i) * 8;
-      d[0] = xn[0] - x0s[0]; d[1] = xn[1] - x0s[1]; d[2] = xn[2] - x0s[2];
-      d[3] = dq[0]; d[4] = dq[1]; d[5] = dq[2];
-    }
-    waveSync();  // row 0 is the next body's
+    primalBodyXT(w, lane, L, S, mb, DQ, P, alpha, dual, dualAlpha);
   }
+}
+
+// BDF1 velocities (solver.js:476-488).
+__device__ __forceinline__ void velocityOne(int i, const BodyProp *props, const double *X, const double *X0,
+                                            double *V, double dt) {
+  double *v = V + size_t(i) * VS;
+  v[6] = v[0]; v[7] = v[1]; v[8] = v[2];
+  if (props[i].mass > 0) {
+    const double *x = X + size_t(i) * XS, *x0 = X0 + size_t(i) * XS;
+    for (int k = 0; k < 3; k++) v[k] = (x[k] - x0[k]) / dt;
+    double w[3];
+    qsub(w, x + 3, x0 + 3);
+    for (int k = 0; k < 3; k++) v[3 + k] = w[k] / dt;
+  }
+}
+
+__global__ void k_velocity(int n, const BodyProp *props, const double *X, const double *X0, double *V, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) velocityOne(i, props, X, X0, V, dt);
+}
+
+// ------------------------------------------------ batched scenes, one launch
+// The solve of a batch of independent scenes in one launch per step: one
+// workgroup per scene runs its bodies' initialisation, every iteration's
+// colours one after another with a workgroup barrier between them (bodies of
+// one colour share no force, so a colour's bodies run in parallel, and the
+// barrier orders the colours as the per-colour launches do), the joints' dual
+// update after each iteration and the velocities (solver.js:366-489). A
+// scene's bodies of colour c are one run of the colour's work list (k_color_
+// scatter keeps index order), found by binary search. Each body's update is the
+// per-colour sweep's own (primalBody, primalBodyXT), so the exact precision
+// gives the same bits. The contact dual of the last iteration, which also
+// records C, runs after this kernel as k_dual: nothing here reads it.
+struct SceneSolve {
+  const int *sceneStart;       // [S + 1] body ranges
+  const int *sceneJointStart;  // [S + 1] joint ranges (joints grouped by scene)
+  const int *cmeta;            // [4] colour count, [8 ...] colour starts
+  const int4 *cwork;
+  double *V;
+  JointDev *joints;
+  int iterations;
+  double alpha;
+};
+
+// First index in [lo, hi) whose work entry's body is >= body.
+__device__ __forceinline__ int firstBodyAtLeast(const int4 *cwork, int lo, int hi, int body) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cwork[mid].x < body) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class R, bool EXACT, int TEAM>
+__global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S, ManBuf<R> mb, R *DQ,
+                                                     StepParams P) {
+  constexpr int T = EXACT ? XT_TEAM : TEAM;
+  __shared__ double rows[EXACT ? 256 / XT_TEAM : 1][EXACT ? XT_TEAM * XT_ROW : 1];
+  __shared__ int rlo[MAX_COLORS], rhi[MAX_COLORS];
+  const int sc = blockIdx.x, tid = threadIdx.x;
+  const int n0 = Q.sceneStart[sc], n1 = Q.sceneStart[sc + 1];
+  const int j0 = Q.sceneJointStart[sc], j1 = Q.sceneJointStart[sc + 1];
+  const int lane = tid % T, team = tid / T;
+  constexpr int nteams = 256 / T;
+  double *X0 = const_cast<double *>(S.X0), *Y = const_cast<double *>(S.Y);
+  for (int i = n0 + tid; i < n1; i += 256) bodyInitOne<R>(i, S.props, S.X, X0, Y, Q.V, DQ, P);
+  // the scene's run of each colour's work list, found once per step
+  const int ncol = Q.cmeta[4];
+  const int *cst = Q.cmeta + 8;
+  for (int c = tid; c < ncol; c += 256) {
+    const int lo = firstBodyAtLeast(Q.cwork, cst[c], cst[c + 1], n0);
+    rlo[c] = lo;
+    rhi[c] = firstBodyAtLeast(Q.cwork, lo, cst[c + 1], n1);
+  }
+  __syncthreads();
+  for (int it = 0; it < Q.iterations; it++) {
+    // from the second iteration on, the first visitor of each contact folds
+    // in the previous iteration's dual update
+    const int fused = it >= 1 ? 1 : 0;
+    for (int c = 0; c < ncol; c++) {
+      const int lo = rlo[c], hi = rhi[c];
+      if (lo == hi) continue;  // no body of this colour in the scene (uniform over the workgroup)
+      for (int t = lo + team; t < hi; t += nteams) {
+        const int4 w = Q.cwork[t];
+        if constexpr (EXACT) {
+          if (w.w > 0) {
+            if (lane == 0) primalBody<double, true, 1>(w, 0, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+          } else {
+            primalBodyXT(w, lane, rows[team], S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+          }
+        } else {
+          primalBody<R, false, TEAM>(w, lane, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+        }
+      }
+      __syncthreads();
+    }
+    for (int j = j0 + tid; j < j1; j += 256) jointDualOne(Q.joints[j], S.X, P, Q.alpha);
+    __syncthreads();
+  }
+  for (int i = n0 + tid; i < n1; i += 256) velocityOne(i, S.props, S.X, S.X0, Q.V, P.dt);
 }
 
 // Dual update of every live manifold (manifold.js:447-491); the live count
@@ -1669,20 +1801,6 @@ __global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *li
   }
 }
 
-// BDF1 velocities (solver.js:476-488).
-__global__ void k_velocity(int n, const BodyProp *props, const double *X, const double *X0, double *V, double dt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double *v = V + size_t(i) * VS;
-  v[6] = v[0]; v[7] = v[1]; v[8] = v[2];
-  if (props[i].mass > 0) {
-    const double *x = X + size_t(i) * XS, *x0 = X0 + size_t(i) * XS;
-    for (int k = 0; k < 3; k++) v[k] = (x[k] - x0[k]) / dt;
-    double w[3];
-    qsub(w, x + 3, x0 + 3);
-    for (int k = 0; k < 3; k++) v[3 + k] = w[k] / dt;
-  }
-}
 
 // ------------------------------------------------------------ repack
 // A topology edit that keeps the survivors' state (the identity-preserving

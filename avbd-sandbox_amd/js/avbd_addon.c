@@ -156,6 +156,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.team = (int32_t)get_num(env, argv[0], "team", 0);
     cfg.manifold_cap = (int32_t)get_num(env, argv[0], "manifoldCap", 0);
     cfg.pair_cap = (int32_t)get_num(env, argv[0], "pairCap", 0);
+    cfg.scene_solve = (int32_t)get_num(env, argv[0], "sceneSolve", 0);
   }
   avbd_ctx *ctx = NULL;
   int rc = avbd_create(&ctx, &cfg);

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 2
+#define AVBD_ABI_VERSION 3
 
 /* Error codes. */
 #define AVBD_OK 0
@@ -82,6 +82,13 @@ typedef struct avbd_config {
                               the backward needs no segment re-runs (0:
                               4096; -1: never, re-run every checkpoint
                               segment)                                      */
+  int32_t scene_solve;     /* batches of scenes (scene_of): the solve of each
+                              step as one launch with one workgroup per
+                              scene (0: auto, when there are 2 or more scenes
+                              of at most 4096 bodies each, the joints are
+                              grouped by scene and post-stabilisation is off;
+                              -1: never, one launch per colour and
+                              iteration)                                    */
 } avbd_config;
 
 /* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform
@@ -318,8 +325,11 @@ int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int6
  * (lanes per body: 1, 4, 8, 16, 32, 64) on the step's own data, restoring the
  * body state between trials; the step then completes with the fastest shape,
  * which the context keeps. `team` receives it and `ms` (may be NULL) the six
- * sweep times in that order. Not available in AVBD_PREC_EXACT, whose
- * assembly order is fixed. */
+ * sweep times in that order. A batch solved in one launch per step
+ * (avbd_config.scene_solve) has the whole solve timed instead for lanes per
+ * body 4, 8 and 16 (a workgroup of 256 lanes then updates 64, 32 or 16 bodies
+ * of a colour at once); the other entries of `ms` are 0. Not available in
+ * AVBD_PREC_EXACT, whose assembly order is fixed. */
 int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
 
 /* Verification hook: the claim order of the greedy colouring, and so the

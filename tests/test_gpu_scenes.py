@@ -298,3 +298,32 @@ def test_js_step_collects_errors_and_regrows(tmp_path):
     ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
     ref.step(20)  # 21 calls, one of which overflowed
     assert np.array_equal(np.array(out["pos"]), ref.state()[:, :3])
+
+
+@pytest.mark.parametrize("precision", [avbd.PREC_EXACT, avbd.PREC_F64, avbd.PREC_MIXED], ids=["exact", "f64", "mixed"])
+def test_one_launch_batch_solve_equals_per_colour_launches(precision):
+    # A batch's solve as one launch with a workgroup per scene (colours
+    # separated by workgroup barriers) against one launch per colour and
+    # iteration: the same per-body updates in the same colour order. Exact:
+    # the same bits. The team modes sum each body's contacts over a team of
+    # lanes whose width the two paths choose separately, so they agree to
+    # rounding until a tie flips a contact (the batch's stacks flip at step 3
+    # in either path), and stay finite and close after.
+    scene, params, meta = avbd.scenes.batched(num_scenes=20, seed=5)
+    params = dict(params, colorOrder=1)
+    one = avbd.HipBackend(precision=precision, team=8)
+    per = avbd.HipBackend(precision=precision, team=8, scene_solve=-1)
+    one.pack(scene, params)
+    per.pack(scene, params)
+    for s in range(1, 41):
+        one.step(1)
+        per.step(1)
+        a, b = one.read_state(), per.read_state()
+        if precision == avbd.PREC_EXACT:
+            for k in ("pos", "quat", "vel", "angvel"):
+                assert np.array_equal(a[k], b[k]), f"step {s}: {k}"
+        else:
+            assert np.isfinite(a["pos"]).all()
+            if s <= 2:
+                assert np.abs(a["pos"] - b["pos"]).max() <= (1e-4 if precision == avbd.PREC_MIXED else 1e-9)
+    assert one.counters()["overflow"] == 0
