@@ -1192,8 +1192,17 @@ int newForce(World &W, int kind, int a, int b) {
   f.a = a;
   f.b = b;
   f.seq = W.nextSeq++;
-  W.pool.push_back(f);
-  const int id = int(W.pool.size()) - 1;
+  // a dropped contact's slot is taken again (the reference's manifolds are
+  // garbage-collected objects; ids are only handles, never ordered)
+  int id;
+  if (!W.freeForces.empty()) {
+    id = W.freeForces.back();
+    W.freeForces.pop_back();
+    W.pool[size_t(id)] = f;
+  } else {
+    W.pool.push_back(f);
+    id = int(W.pool.size()) - 1;
+  }
   W.forces.push_back(id);
   if (a >= 0) W.bodies[a].forces.push_back(id);
   if (b >= 0) W.bodies[b].forces.push_back(id);
@@ -1364,8 +1373,10 @@ int World::removeBodies(const std::vector<char> &doomed) {
   if (kept == n) return 0;
   auto retained = [&](const Force &f) { return (f.a < 0 || !doomed[f.a]) && !doomed[f.b]; };
   int write = 0;
-  for (size_t k = 0; k < forces.size(); k++)
+  for (size_t k = 0; k < forces.size(); k++) {
     if (retained(pool[forces[k]])) forces[write++] = forces[k];
+    else if (pool[forces[k]].kind == MANIFOLD) freeForces.push_back(forces[k]);
+  }
   forces.resize(write);
   std::vector<Body> survivors;
   survivors.reserve(kept);
@@ -1425,6 +1436,9 @@ void World::step() {
     } else {
       if (f.a >= 0) removeFromBody(bodies[f.a].forces, id);
       if (f.b >= 0) removeFromBody(bodies[f.b].forces, id);
+      // only contact slots are taken again: joints and springs keep theirs,
+      // whose order numbers them (orc_set_joint_anchor)
+      if (f.kind == MANIFOLD) freeForces.push_back(id);
     }
   }
   forces.resize(write);

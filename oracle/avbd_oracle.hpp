@@ -100,7 +100,8 @@ struct Coloring {                // coloring.js:43-118
 struct World {
   Params prm;
   std::vector<Body> bodies;
-  std::vector<Force> pool;       // all forces ever created (ids index here)
+  std::vector<Force> pool;       // every force's record (ids index here)
+  std::vector<int> freeForces;   // pool slots of dropped contact manifolds, reused by newForce
   std::vector<int> forces;       // live force ids, creation order
   int64_t nextSeq = 0;
   Schedule schedule = SCHED_REVERSE;
