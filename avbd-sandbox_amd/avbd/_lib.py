@@ -28,7 +28,8 @@ SYMBOLS = (
     "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
     "avbd_upload_coloring", "avbd_export_poses", "avbd_export_poses_static", "avbd_pose_source", "avbd_repack",
     "avbd_read_joints",
-    "avbd_read_step_start", "avbd_set_joint_anchor", "avbd_read_manifolds",
+    "avbd_read_step_start", "avbd_set_joint_anchor", "avbd_read_manifolds", "avbd_set_coloring_limits",
+    "avbd_read_inertial_target",
 )
 K_PRIMAL, K_DUAL, K_NARROW, K_POSES = 0, 1, 2, 3
 
@@ -112,6 +113,8 @@ def lib(path=None):
         "avbd_set_joint_anchor": (C.c_int, [ctx, i32, pd]),
         "avbd_pose_source": (C.c_int, [ctx, P(vp), P(vp), pi]),
         "avbd_read_manifolds": (C.c_int, [ctx, pi, pd, pd, i32, i32, pi, pi]),
+        "avbd_set_coloring_limits": (C.c_int, [ctx, i32, i32]),
+        "avbd_read_inertial_target": (C.c_int, [ctx, pd, pd, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -22,6 +22,8 @@ struct StepParams {
 };
 
 // ------------------------------------------------------------------ utils
+__global__ void k_or_flag(int *p, int bits) { atomicOr(p, bits); }
+
 __global__ void k_fill(int *p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1029,6 +1031,19 @@ __global__ void k_color_hist(int n, const int *color, int *hist, int *numColors)
   if (threadIdx.x == 0 && top > 0) atomicMax(numColors, top);
 }
 
+// A cap on the colour count (GpuBackend.encColors, backend.js:1403-1430): a
+// body whose colour is at or above the cap takes the last allowed colour,
+// which may put two bodies of one force in one colour (reported in bit 32 of
+// the overflow word).
+__global__ void k_color_cap(int n, int *color, int cap, int *overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (color[i] >= cap) {
+    color[i] = cap - 1;
+    atomicOr(overflow, 32);
+  }
+}
+
 // One 256-thread block per colour: offs[block][colour], the colour's bodies in
 // earlier blocks (each thread scans a contiguous run of blocks, then the runs
 // are offset by a block-wide scan), and the colour's total.
@@ -1108,8 +1123,10 @@ __global__ void k_item_flags(int n, const int *color, const int *itemStart, uint
   for (int t = itemStart[i] + lane; t < itemStart[i + 1]; t += 8) {
     const uint2 w = items[t];
     if (w.x & 0x80000000u) continue;
-    const int co = color[w.y & ITEM_BODY];
-    items[t].y = (w.y & ~ITEM_FIRST) | ((co < 0 || co > c) ? ITEM_FIRST : 0u);
+    const int o = int(w.y & ITEM_BODY), co = color[o];
+    // a capped colouring may leave two bodies of one force in one colour (the
+    // conflicts the reference tolerates): the lower index is the first visitor
+    items[t].y = (w.y & ~ITEM_FIRST) | ((co < 0 || co > c || (co == c && o > i)) ? ITEM_FIRST : 0u);
   }
 }
 

@@ -36,23 +36,31 @@ HBM_PEAK_GBS = 8000.0
 POSE_LAUNCHES = 100
 
 
-def measured_traffic(precision, team):
-    """HBM bytes per k_primal launch (read + written) from the committed
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the same workload,
-    scaled by the copy-probe calibration: the dispatch-weighted mean over the
-    team shapes the sweep used (small colours get wider teams); None when no
-    summary matches."""
-    path = os.path.join(ROOT, "profiles", f"k_primal_pyramid51k_{precision}_traffic.json")
+def measured_traffic(workload, precision, kernel_prefixes):
+    """HBM bytes per launch (read + written) of the dominant kernel from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the same
+    workload (profiles/<workload>_<precision>_kernels.md, already scaled by the
+    copy-probe calibration): the dispatch-weighted mean over the rows whose
+    kernel name starts with one of `kernel_prefixes` (the team shapes a sweep
+    used). None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_kernels.md")
     if not os.path.exists(path):
         return None, None
+    n, total = 0, 0.0
     with open(path) as f:
-        doc = json.load(f)
-    prefix = f"k_primal<{'float' if precision == 'mixed' else 'double'}, {'true' if precision == 'exact' else 'false'}, "
-    ks = [k for name, k in doc["kernels"].items() if name.startswith(prefix)]
-    n = sum(k["dispatches"] for k in ks)
-    if doc.get("team") != team or n == 0:
+        for ln in f:
+            if not ln.startswith("| ") or ln.startswith("| kernel"):
+                continue
+            cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+            name = cells[0].replace("void ", "").replace("avbd::", "")
+            if not any(name.startswith(p) for p in kernel_prefixes) or cells[5] == "-":
+                continue
+            calls = int(cells[1])
+            n += calls
+            total += calls * (float(cells[5]) + float(cells[6])) * 1e6
+    if n == 0:
         return None, None
-    return sum((k["read_bytes"] + k["write_bytes"]) * k["dispatches"] for k in ks) / n, os.path.relpath(path, ROOT)
+    return total / n, os.path.relpath(path, ROOT)
 
 
 def copy_probe(mib):
@@ -331,7 +339,10 @@ def main():
     avg_ms = pms / max(plaunch, 1)
     achieved = (pbytes / max(plaunch, 1)) / (avg_ms * 1e-3) / 1e9 if pms > 0 else 0.0
     team = tune[0] if tune else args.team
-    traffic, traffic_src = measured_traffic(args.precision, team) if args.workload == "pyramid" else (None, None)
+    kern = {"exact": ("k_primal_xt",), "f64": ("k_primal<double",), "mixed": ("k_primal<float",)}[args.precision]
+    if args.workload == "batched":
+        kern = ("k_solve_scenes<" + ("double" if args.precision != "mixed" else "float"),)
+    traffic, traffic_src = measured_traffic(args.workload, args.precision, kern)
     line = {
         "metric": METRIC,
         "value": value,

@@ -175,7 +175,8 @@ typedef struct avbd_counters {
   int32_t color_rounds;    /* fixed-point rounds the colouring took         */
   int32_t overflow;        /* bit 1 pairs, 2 manifolds (layout.js:174), 4
                               colours, 8 work items, 16 an uploaded
-                              colouring was not proper for a step           */
+                              colouring was not proper for a step, 32 the
+                              colouring limits left it improper            */
 } avbd_counters;
 
 /* --- lifecycle: GpuBackend.init / initDevice / initPipelines (backend.js:337-469) */
@@ -349,6 +350,19 @@ int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n);
  * avbd_counters.overflow. AVBD_E_INVALID for a dynamic body's colour outside
  * [0, 256). */
 int avbd_upload_coloring(avbd_ctx *ctx, const int32_t *color_of, int32_t n);
+
+/* Limits on the colouring (GpuBackend.colorRounds and encColors,
+ * backend.js:256-270, 1403-1430): at most `rounds` relaxation rounds per step
+ * and at most `colors` colours (bodies above take the last one). 0 = no limit,
+ * the default: the colouring is then the exact greedy fixed point. A limited
+ * colouring may give two bodies of one force the same colour; they are then
+ * updated in the same sweep, the conflict the reference's GPU tolerates
+ * (layout.js:26-43), and bit 32 of avbd_counters.overflow is set. */
+int avbd_set_coloring_limits(avbd_ctx *ctx, int32_t rounds, int32_t colors);
+
+/* The inertial target y of the last step (the reference GPU's bInerP /
+ * bInerQ, layout.js BODY_SECTIONS; solver.js:366-401). */
+int avbd_read_inertial_target(avbd_ctx *ctx, double *pos, double *quat, int32_t n);
 
 /* Adjoint of the step (no counterpart in the reference; the build plan's
  * adjoint boundary). avbd_adjoint_forward steps the scene nsteps times, as
