@@ -305,6 +305,17 @@ class HipBackend:
         self._ck(self.L.avbd_set_body_order(self.ctx, _ip(r), len(r)))
 
     # --- metrics / profiling (backend.js:309-324, 1836-1891)
+    def set_coloring_limits(self, rounds=0, colors=0):
+        """Relaxation rounds and colours per step (GpuBackend.colorRounds /
+        encColors); 0 = no limit, the exact greedy colouring."""
+        self._ck(self.L.avbd_set_coloring_limits(self.ctx, int(rounds), int(colors)))
+
+    def read_inertial_target(self):
+        """The inertial target y of the last step: (pos, quat)."""
+        pos, quat = np.zeros((self.n, 3)), np.zeros((self.n, 4))
+        self._ck(self.L.avbd_read_inertial_target(self.ctx, _dp(pos), _dp(quat), self.n))
+        return pos, quat
+
     def read_manifolds(self):
         """Every live manifold in force-list order: head (m, 4) {A, B, contact
         count, flags}, basis_mu (m, 10), and contacts (c, 16) per contact,

@@ -628,6 +628,86 @@ static napi_value js_read_profile(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+/* Typed array of `count` elements of `type` (element size `es`) whose data is
+ * returned in *data. */
+static napi_value new_typed(napi_env env, napi_typedarray_type type, size_t es, size_t count, void **data) {
+  napi_value ab, arr;
+  if (napi_create_arraybuffer(env, es * (count ? count : 1), data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, type, count, ab, 0, &arr) != napi_ok) return NULL;
+  return arr;
+}
+
+/* readManifolds(ctx) -> {head Int32Array (4 per manifold: A, B, contacts,
+ * flags), basisMu Float64Array (10 per manifold), contacts Float64Array (16 per
+ * contact), features Uint32Array (8 per manifold)}: every live manifold in
+ * force-list order (avbd_read_manifolds, avbd_read_forces), for
+ * GpuBackend.readColoring's slots (backend.js:2053-2151). */
+static napi_value js_read_manifolds(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1] = {NULL};
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int32_t m = 0, c = 0, nf = 0;
+  int rc = avbd_read_manifolds(ctx, NULL, NULL, NULL, 0, 0, &m, &c);
+  if (rc) return throw_avbd(env, ctx, rc);
+  rc = avbd_read_forces(ctx, NULL, NULL, 0, &nf);
+  if (rc) return throw_avbd(env, ctx, rc);
+  void *ph, *pb, *pc, *pk, *pf;
+  napi_value head = new_typed(env, napi_int32_array, 4, (size_t)m * 4, &ph);
+  napi_value bm = new_typed(env, napi_float64_array, 8, (size_t)m * 10, &pb);
+  napi_value con = new_typed(env, napi_float64_array, 8, (size_t)c * 16, &pc);
+  napi_value kab = new_typed(env, napi_int32_array, 4, (size_t)nf * 4, &pk);
+  napi_value ft = new_typed(env, napi_uint32_array, 4, (size_t)nf * 8, &pf);
+  if (!head || !bm || !con || !kab || !ft) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  rc = avbd_read_manifolds(ctx, (int32_t *)ph, (double *)pb, (double *)pc, m, c, &m, &c);
+  if (rc) return throw_avbd(env, ctx, rc);
+  rc = avbd_read_forces(ctx, (int32_t *)pk, (uint32_t *)pf, nf, &nf);
+  if (rc) return throw_avbd(env, ctx, rc);
+  napi_value o;
+  CHECK(env, napi_create_object(env, &o));
+  CHECK(env, napi_set_named_property(env, o, "head", head));
+  CHECK(env, napi_set_named_property(env, o, "basisMu", bm));
+  CHECK(env, napi_set_named_property(env, o, "contacts", con));
+  CHECK(env, napi_set_named_property(env, o, "forces", kab));
+  CHECK(env, napi_set_named_property(env, o, "features", ft));
+  return o;
+}
+
+/* setColoringLimits(ctx, rounds, colors): GpuBackend.colorRounds / encColors
+ * (0: none, the greedy fixed point) */
+static napi_value js_set_coloring_limits(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3] = {NULL, NULL, NULL};
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  int32_t rounds = 0, colors = 0;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &rounds);
+  if (argc > 2) napi_get_value_int32(env, argv[2], &colors);
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_set_coloring_limits(ctx, rounds, colors);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
+/* readInertialTarget(ctx, pos, quat): the inertial target y (bInerP / bInerQ) */
+static napi_value js_read_inertial_target(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3] = {NULL, NULL, NULL};
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  double *pos, *quat;
+  int32_t n;
+  if (!body_arrays(env, argc, argv, &pos, &quat, NULL, NULL, NULL, &n)) return NULL;
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_read_inertial_target(ctx, pos, quat, n);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 static napi_value js_abi_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -656,6 +736,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readCounters", NULL, js_read_counters, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setProfiling", NULL, js_set_profiling, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readProfile", NULL, js_read_profile, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readManifolds", NULL, js_read_manifolds, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setColoringLimits", NULL, js_set_coloring_limits, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readInertialTarget", NULL, js_read_inertial_target, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

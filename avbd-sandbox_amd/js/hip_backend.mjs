@@ -13,6 +13,30 @@ const addon = require(join(here, 'avbd.node'));
 
 export const PRECISION = { exact: 0, f64: 1, mixed: 2 };
 const PHASE_NAMES = ['broad', 'narrow', 'topology', 'colour', 'solve', 'total'];
+export const MAX_COLORS = 256;
+
+// The body arena readState returns: the sections of the reference GPU's
+// BODY_F32 buffer (layout.js BODY_SECTIONS), 4 floats per body each except
+// bStatic (16), one section after the other, so `data[layout.body.bPos + i * 4
+// + k]` addresses a body's position as GpuBackend's readState does.
+const BODY_SECTIONS = [
+  ['bPos', 4], ['bQuat', 4], ['bPosB', 4], ['bQuatB', 4], ['bVel', 4], ['bVelA', 4], ['bPrevV', 4],
+  ['bInitP', 4], ['bInitQ', 4], ['bInerP', 4], ['bInerQ', 4], ['bStatic', 16],
+];
+// bStatic element indices (layout.js BS_*)
+export const BS = {
+  MASS: 0, MOMENT: 1, SIZE: 4, STATIC_FRICTION: 7, DYNAMIC_FRICTION: 8, RESTITUTION: 9, RADIUS: 10, FLAGS: 11,
+};
+
+export function bodyLayout(n) {
+  const body = {};
+  let o = 0;
+  for (const [name, per] of BODY_SECTIONS) {
+    body[name] = o;
+    o += per * n;
+  }
+  return { body, bodyF32Len: o };
+}
 
 export class HipBackend {
   constructor({ precision = 'mixed', team = 0, device = 0, manifoldCap = 0, pairCap = 0 } = {}) {
@@ -34,6 +58,28 @@ export class HipBackend {
     this.lastProfile = null;
     this._profiling = false;
     this._queued = 0;
+    // Colouring controls of GpuBackend (backend.js:256, 270, 299): rounds of
+    // refinement per step and colours per step; null = none (the exact greedy
+    // colouring, this backend's default; the reference ships 5 rounds and 16
+    // colours and tolerates the conflicts they leave).
+    this.colorRounds = null;
+    this.encColors = null;
+    this._limits = '0,0';
+    this._override = null;
+    this._uploaded = null;
+  }
+
+  // A truthy value makes the following steps use the colouring given to
+  // uploadColoring (backend.js:299, 1508-1516); null returns to the greedy one.
+  get overrideColoring() {
+    return this._override;
+  }
+
+  set overrideColoring(v) {
+    this._override = v || null;
+    if (!this.ctx || this.numBodies === 0) return;  // applied by the next uploadColoring
+    if (this._override && this._uploaded) addon.uploadColoring(this.ctx, this._uploaded);
+    else addon.uploadColoring(this.ctx, null);
   }
 
   // --- lifecycle (backend.js:337-469)
@@ -170,6 +216,8 @@ export class HipBackend {
     });
     if (oldIndex) addon.pack(this.ctx, A, oldIndex);
     else addon.pack(this.ctx, A);
+    this._packed = A;
+    this._uploaded = null;
     this.lastPackPreservedContacts = oldIndex !== null;
     this.forceRealloc = false;
     this.packedJoints = joints;
@@ -228,6 +276,11 @@ export class HipBackend {
     const t0 = Date.now();
     try {
       this.flushTopology(solver);
+      const limits = `${this.colorRounds || 0},${this.encColors || 0}`;
+      if (limits !== this._limits) {
+        addon.setColoringLimits(this.ctx, this.colorRounds || 0, this.encColors || 0);
+        this._limits = limits;
+      }
       addon.step(this.ctx, 1);
       this._queued++;
       if (this._profiling) this._readProfile();
@@ -248,15 +301,80 @@ export class HipBackend {
   }
 
   // --- state (backend.js:2154-2192, 1984-2039)
-  async readState() {
+  // Body state in f64, straight from the device.
+  async readBodies() {
     await this.idle();
     const n = this.numBodies;
-    const data = {
+    const d = {
       pos: new Float64Array(n * 3), quat: new Float64Array(n * 4), vel: new Float64Array(n * 3),
       angvel: new Float64Array(n * 3), prevVel: new Float64Array(n * 3),
     };
-    addon.readState(this.ctx, data.pos, data.quat, data.vel, data.angvel, data.prevVel);
-    return { data, numBodies: n, counters: addon.readCounters(this.ctx) };
+    addon.readState(this.ctx, d.pos, d.quat, d.vel, d.angvel, d.prevVel);
+    return d;
+  }
+
+  // GpuBackend.readState's result (backend.js:2154-2192): `data` is the body
+  // arena in the reference's BODY_F32 layout (f32, addressed through
+  // `layout.body`), with `counters` {pairs, slots, adj, overflow, contacts} and
+  // `dispatch`; `f64` carries the same state at full precision.
+  async readState() {
+    const f64 = await this.readBodies();
+    const n = this.numBodies;
+    const L = bodyLayout(n);
+    const data = new Float32Array(L.bodyF32Len);
+    const put = (sec, src, w) => {
+      for (let i = 0; i < n; i++) for (let k = 0; k < w; k++) data[L.body[sec] + i * 4 + k] = src[i * w + k];
+    };
+    put('bPos', f64.pos, 3); put('bQuat', f64.quat, 4);
+    put('bPosB', f64.pos, 3); put('bQuatB', f64.quat, 4);
+    put('bVel', f64.vel, 3); put('bVelA', f64.angvel, 3); put('bPrevV', f64.prevVel, 3);
+    const p0 = new Float64Array(n * 3), q0 = new Float64Array(n * 4);
+    addon.readStepStart(this.ctx, p0, q0);
+    put('bInitP', p0, 3); put('bInitQ', q0, 4);
+    const py = new Float64Array(n * 3), qy = new Float64Array(n * 4);
+    addon.readInertialTarget(this.ctx, py, qy);
+    put('bInerP', py, 3); put('bInerQ', qy, 4);
+    const A = this._packed;
+    if (A) {
+      for (let i = 0; i < n; i++) {
+        const o = L.body.bStatic + i * 16;
+        const sx = A.size[i * 3], sy = A.size[i * 3 + 1], sz = A.size[i * 3 + 2];
+        const sphere = A.shape[i] === 1;
+        let mass, m0, m1, m2, radius;
+        if (sphere) {  // rigid.js:105-133
+          const r = sx * 0.5;
+          mass = (4 / 3) * Math.PI * r * r * r * A.density[i];
+          m0 = m1 = m2 = 0.4 * mass * r * r;
+          radius = r;
+        } else {  // rigid.js:29-98
+          mass = sx * sy * sz * A.density[i];
+          m0 = ((sy * sy + sz * sz) / 12) * mass; m1 = ((sx * sx + sz * sz) / 12) * mass;
+          m2 = ((sx * sx + sy * sy) / 12) * mass;
+          radius = Math.sqrt(sx * sx / 4 + sy * sy / 4 + sz * sz / 4);
+        }
+        data[o + BS.MASS] = mass;
+        data[o + BS.MOMENT] = m0; data[o + BS.MOMENT + 1] = m1; data[o + BS.MOMENT + 2] = m2;
+        data[o + BS.SIZE] = sx; data[o + BS.SIZE + 1] = sy; data[o + BS.SIZE + 2] = sz;
+        data[o + BS.STATIC_FRICTION] = A.friction[i];
+        data[o + BS.DYNAMIC_FRICTION] = A.friction[i];
+        data[o + BS.RADIUS] = radius;
+        data[o + BS.FLAGS] = sphere ? 1 : 0;
+      }
+    }
+    const c = addon.readCounters(this.ctx);
+    const joints = this.packedJoints || [];
+    const springs = joints.filter((j) => this.springIndex.has(j)).length;
+    return {
+      data, layout: L, numBodies: n, f64,
+      counters: {
+        pairs: c.pairs, slots: c.manifolds, adj: 2 * (c.manifolds + c.joints), overflow: c.overflow,
+        contacts: c.contacts,
+      },
+      dispatch: {
+        numBodies: n, numJoints: joints.length - springs, numSprings: springs,
+        encColors: this.encColors || c.colors,
+      },
+    };
   }
 
   // Device state back into the scene graph (backend.js:1984-2039): bodies by
@@ -264,7 +382,7 @@ export class HipBackend {
   // initialAng), and the packed joints' penalties, multipliers and fracture,
   // and the springs' penalty, tearing and peak strain.
   async syncToCPU(solver) {
-    const { data } = await this.readState();
+    const data = await this.readBodies();
     const n = this.numBodies;
     const p0 = new Float64Array(n * 3);
     const q0 = new Float64Array(n * 4);
@@ -313,29 +431,82 @@ export class HipBackend {
   }
 
   // --- verification (backend.js:2053-2151)
+  // GpuBackend.readColoring's result: the colouring the last step used
+  // (colorOf, -1 for static bodies; counts and offsets per colour; entries,
+  // the bodies grouped by colour, by index within a colour) and the contact
+  // slots it coloured around: contactPairs ([A, B] per manifold), slots,
+  // manifolds (distinct pairs) and slotInfo (per manifold: contact count,
+  // friction, basis, and per contact the body-frame anchors, C0, penalty,
+  // lambda, stick flag and feature key). numColors and broadPhasePairs (the
+  // last broad phase's pairs, A > B) are this backend's additions.
   async readColoring() {
     await this.idle();
-    const colorOf = new Int32Array(this.numBodies);
+    const n = this.numBodies;
+    const colorOf = new Int32Array(n);
     const numColors = addon.readColoring(this.ctx, colorOf);
+    const counts = new Uint32Array(MAX_COLORS);
+    for (let i = 0; i < n; i++) if (colorOf[i] >= 0) counts[colorOf[i]]++;
+    const offsets = new Uint32Array(MAX_COLORS + 1);
+    for (let c = 0; c < MAX_COLORS; c++) offsets[c + 1] = offsets[c] + counts[c];
+    const entries = new Uint32Array(n);
+    const cursor = offsets.slice(0, MAX_COLORS);
+    for (let i = 0; i < n; i++) if (colorOf[i] >= 0) entries[cursor[colorOf[i]]++] = i;
     const count = addon.readPairs(this.ctx, null);
-    const contactPairs = new Int32Array(count * 2);
-    addon.readPairs(this.ctx, contactPairs);
-    return { colorOf, numColors, contactPairs };
+    const broadPhasePairs = new Int32Array(count * 2);
+    addon.readPairs(this.ctx, broadPhasePairs);
+    const M = addon.readManifolds(this.ctx);
+    const m = M.head.length / 4;
+    const nj = M.forces.length / 4 - m;
+    const contactPairs = [];
+    const slotInfo = [];
+    const pairSet = new Set();
+    let k = 0;
+    for (let t = 0; t < m; t++) {
+      const a = M.head[t * 4], b = M.head[t * 4 + 1], nc = M.head[t * 4 + 2];
+      contactPairs.push([a, b]);
+      pairSet.add(`${Math.min(a, b)}:${Math.max(a, b)}`);
+      const bm = M.basisMu.subarray(t * 10, t * 10 + 10);
+      const contacts = [];
+      let sticking = 0, maxPenalty = 0;
+      for (let s = 0; s < nc; s++, k++) {
+        const r = M.contacts.subarray(k * 16, k * 16 + 16);
+        const stick = r[15] > 0.5;
+        sticking += stick ? 1 : 0;
+        maxPenalty = Math.max(maxPenalty, r[9], r[10], r[11]);
+        contacts.push({
+          rA: [r[0], r[1], r[2]], rB: [r[3], r[4], r[5]], constraint: [r[6], r[7], r[8]],
+          penalty: [r[9], r[10], r[11]], lambda: [r[12], r[13], r[14]], sticking: stick,
+          feature: M.features[(nj + t) * 8 + s], restitutionBias: 0,
+        });
+      }
+      slotInfo.push({
+        numContacts: nc, chainLen: 1, staticFriction: bm[9], dynamicFriction: bm[9],
+        basis: { normal: [bm[0], bm[1], bm[2]], tangent1: [bm[3], bm[4], bm[5]], tangent2: [bm[6], bm[7], bm[8]] },
+        contacts, stickingContacts: sticking, maxPenalty,
+      });
+    }
+    return {
+      colorOf, counts, offsets, entries, contactPairs, slots: m, manifolds: pairSet.size, slotInfo,
+      numColors, broadPhasePairs,
+    };
   }
 
   // Verification hook (backend.js:1915-1931): install a colouring given as
   // body indices grouped by colour (entries, offsets[c]..offsets[c+1]) for
   // the following steps; clearColoring() returns to the greedy colouring.
+  // The colouring is used while overrideColoring is truthy (backend.js:299).
   uploadColoring(entries, offsets, counts) {
     const colorOf = new Int32Array(this.numBodies).fill(-1);
     for (let c = 0; c + 1 < offsets.length; c++)
       for (let k = offsets[c]; k < offsets[c] + (counts ? counts[c] : offsets[c + 1] - offsets[c]); k++)
         colorOf[entries[k]] = c;
+    this._uploaded = colorOf;
+    if (this._override === null) this._override = true;
     addon.uploadColoring(this.ctx, colorOf);
   }
 
   clearColoring() {
-    addon.uploadColoring(this.ctx, null);
+    this.overrideColoring = null;
   }
 
   // The colouring's claim order (the device counterpart of Solver.bodyOrder,

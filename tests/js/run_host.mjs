@@ -19,7 +19,7 @@ async function main() {
   const col = await backend.readColoring();
   const state = solver.bodies.map((b) => [...b.positionLin, ...b.positionAng, ...b.velocityLin, ...b.velocityAng]);
   process.stdout.write(JSON.stringify({
-    state, numColors: col.numColors, colorOf: Array.from(col.colorOf), pairs: col.contactPairs.length / 2,
+    state, numColors: col.numColors, colorOf: Array.from(col.colorOf), pairs: col.broadPhasePairs.length / 2,
     stats: backend.lastStats, profile: backend.lastProfile, adapter: backend.adapterInfo,
   }));
 }

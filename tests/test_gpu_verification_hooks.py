@@ -115,3 +115,31 @@ def test_read_manifolds_equals_one_by_one_reads():
             assert np.array_equal(con[k:k + nc], m["contacts"])
             k += nc
         assert k == len(con)
+
+
+def test_coloring_limits():
+    # GpuBackend.colorRounds / encColors: a cap on the colours leaves none at
+    # or above it and reports the possible conflicts (bit 32); a cap on the
+    # relaxation rounds stops it early on a chain linked in claim order (each
+    # round settles one more link); lifting both returns to the greedy
+    # colouring, equal to the oracle's
+    scene, params, _ = avbd.scenes.pyramid(layers=6)
+    params = dict(params, colorOrder=1)
+    g = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    g.pack(scene, params)
+    g.step(3)
+    _, k = g.read_coloring()
+    assert k > 3
+    g.set_coloring_limits(colors=3)
+    g.step(1)
+    col, k3 = g.read_coloring()
+    assert col.max() <= 2 and k3 <= 3 and g.counters()["overflow"] & 32
+    assert np.isfinite(g.read_state()["pos"]).all()
+    g.set_coloring_limits(rounds=1)
+    g.step(1)
+    assert g.counters()["color_rounds"] <= 1
+    g.set_coloring_limits()
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=1)
+    # the inertial target is readable and finite
+    y, qy = g.read_inertial_target()
+    assert np.isfinite(y).all() and np.abs(np.linalg.norm(qy, axis=1) - 1).max() < 1e-9

@@ -107,3 +107,63 @@ def test_addon_argument_checks_on_a_live_context():
     # settled the context steps again
     assert not res["step during sync"]["ok"] and "in flight" in res["step during sync"]["message"]
     assert res["step after sync"]["ok"], res["step after sync"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_verification_readbacks_match_the_oracle(tmp_path):
+    # GpuBackend's verification surface (readState, readColoring,
+    # encColors / colorRounds / overrideColoring) on a 5-layer pyramid in
+    # exact precision, against the oracle's state, colouring and manifolds
+    import avbd
+    import pyoracle
+    scene, params, _ = avbd.scenes.pyramid(layers=5)
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": scene, "params": params}))
+    steps = 12
+    r = subprocess.run([NODE, "tests/js/run_verification.mjs", str(path), str(steps)], capture_output=True,
+                       text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    ref.step(steps)
+    o = ref.state()
+    n = len(scene["bodies"])
+    assert d["numBodies"] == n
+    assert np.array_equal(np.array(d["pos"], np.float32), o[:, :3].astype(np.float32))
+    assert np.array_equal(np.array(d["quat"], np.float32), o[:, 3:7].astype(np.float32))
+    assert np.array_equal(np.array(d["mass"], np.float32), ref.mass()[:, 0].astype(np.float32))
+    # colouring: colorOf as the oracle's, counts / offsets / entries its buckets
+    col, k = ref.last_coloring()
+    assert d["colorOf"] == col.tolist() and d["numColors"] == k
+    counts = np.bincount(col[col >= 0], minlength=256)
+    assert d["counts"] == counts.tolist()
+    assert d["offsets"] == np.concatenate([[0], np.cumsum(counts)]).tolist()
+    assert d["entries"][:int(counts.sum())] == np.argsort(np.where(col >= 0, col, 10**6), kind="stable")[
+        :int(counts.sum())].tolist()
+    # slots: the oracle's manifolds in force-list order
+    kab, _ = ref.forces()
+    mans = [i for i in range(len(kab)) if kab[i][0] == 0]
+    assert d["slots"] == len(mans) == d["counters"]["slots"]
+    assert d["manifolds"] == len({(min(kab[i][1], kab[i][2]), max(kab[i][1], kab[i][2])) for i in mans})
+    assert d["counters"]["contacts"] == int(kab[mans, 3].sum())
+    assert d["dispatch"]["numBodies"] == n
+    for t, i in enumerate(mans):
+        m = ref.manifold(i)
+        s = d["slotInfo"][t]
+        assert d["contactPairs"][t] == [int(kab[i][1]), int(kab[i][2])]
+        assert s["numContacts"] == len(m["contacts"]) and s["staticFriction"] == m["mu"]
+        assert s["basis"]["normal"] == m["basis"][0].tolist() and s["basis"]["tangent2"] == m["basis"][2].tolist()
+        for c, rc in zip(s["contacts"], m["contacts"]):
+            assert c["rA"] == rc[0:3].tolist() and c["constraint"] == rc[6:9].tolist()
+            assert c["penalty"] == rc[9:12].tolist() and c["lambda"] == rc[12:15].tolist()
+            assert c["sticking"] == bool(rc[18])
+        assert [c["feature"] for c in s["contacts"]] == m["features"].tolist()
+    # the colour cap: no colour above 1, and the overflow word says so
+    assert d["capped"]["maxColor"] <= 1 and d["capped"]["overflow"] & 32
+    # the uploaded colouring is used while overrideColoring is set, and the
+    # greedy one again once it is cleared
+    forced = np.array(d["forced"]["colorOf"])
+    dyn = np.array([b["density"] > 0 for b in scene["bodies"]])
+    assert np.array_equal(forced[dyn], np.arange(dyn.sum()))
+    assert max(d["released"]["colorOf"]) < dyn.sum() - 1
