@@ -293,7 +293,7 @@ struct ContactSink {
 
 // collide.js:291-324
 bool testAxis(const OBB &A, const OBB &B, const double *delta, const double *axis, int type,
-              int ia, int ib, SatAxis &t) {
+              int ia, int ib, SatAxis &t, double skin) {
   const double lenSq = v_lenSq(axis);
   if (lenSq < SAT_EPS) return true;
   const double invLen = 1.0 / std::sqrt(lenSq);
@@ -306,7 +306,8 @@ bool testAxis(const OBB &A, const OBB &B, const double *delta, const double *axi
   const double rB = B.h[0] * std::fabs(v_dot(n, B.ax[0])) + B.h[1] * std::fabs(v_dot(n, B.ax[1])) +
                     B.h[2] * std::fabs(v_dot(n, B.ax[2]));
   const double sep = dist - (rA + rB);
-  if (sep > 0) return false;
+  // skin: the GPU contact model's speculative contact distance (0 otherwise)
+  if (sep > skin) return false;
   if (!t.valid || sep > t.sep) {
     t.valid = true; t.type = type; t.ia = ia; t.ib = ib; t.sep = sep; v_copy(t.n, n);
   }
@@ -356,7 +357,8 @@ void closestSeg(const double *p0, const double *p1, const double *q0, const doub
 }
 
 // collide.js:403-472
-int faceManifold(const OBB &oA, const OBB &oB, ContactSink &sink, bool refIsA, int refAxis, const double *nAB) {
+int faceManifold(const OBB &oA, const OBB &oB, ContactSink &sink, bool refIsA, int refAxis, const double *nAB,
+                 double skin) {
   const OBB &ref = refIsA ? oA : oB;
   const OBB &inc = refIsA ? oB : oA;
   double on[3];
@@ -413,7 +415,7 @@ int faceManifold(const OBB &oA, const OBB &oB, ContactSink &sink, bool refIsA, i
     v_copy(p, c0 + i * 3);
     v_sub(t, p, fc);
     const double dist = v_dot(t, fn);
-    if (dist > PLANE_EPS) continue;
+    if (dist > skin + PLANE_EPS) continue;
     v_copy(t, p);
     v_addScaled(t, fn, -dist);
     if (refIsA) sink.add(t, p, prefix | uint32_t(i & 0xff));
@@ -448,11 +450,11 @@ int edgeContact(const OBB &oA, const OBB &oB, ContactSink &sink, int axA, int ax
 }
 
 // collide.js:496-520
-int sphereSphere(const Body &A, const Body &B, ContactSink &sink, double *basis) {
+int sphereSphere(const Body &A, const Body &B, ContactSink &sink, double *basis, double skin) {
   double d[3], n[3], t[3], xA[3], xB[3];
   v_sub(d, B.p, A.p);
   const double dsq = v_lenSq(d);
-  const double rs = A.radius + B.radius;
+  const double rs = A.radius + B.radius + skin;
   if (dsq > rs * rs) return 0;
   if (dsq > SPHERE_EPS_SQ) v_scale(n, d, 1 / std::sqrt(dsq));
   else v_set(n, 1, 0, 0);
@@ -466,7 +468,7 @@ int sphereSphere(const Body &A, const Body &B, ContactSink &sink, double *basis)
 }
 
 // collide.js:526-600
-int sphereBox(const Body &A, const Body &B, ContactSink &sink, double *basis, bool sphereIsA) {
+int sphereBox(const Body &A, const Body &B, ContactSink &sink, double *basis, bool sphereIsA, double skin) {
   const Body &sph = sphereIsA ? A : B;
   const Body &bx = sphereIsA ? B : A;
   OBB o;
@@ -486,7 +488,7 @@ int sphereBox(const Body &A, const Body &B, ContactSink &sink, double *basis, bo
   int fa = 0;
   if (dsq > SPHERE_EPS_SQ) {
     const double dist = std::sqrt(dsq);
-    if (dist > sph.radius) return 0;
+    if (dist > sph.radius + skin) return 0;
     v_scale(n, sv, 1 / dist);
   } else {
     const double gx = o.h[0] - std::fabs(lx), gy = o.h[1] - std::fabs(ly), gz = o.h[2] - std::fabs(lz);
@@ -526,11 +528,10 @@ int sphereBox(const Body &A, const Body &B, ContactSink &sink, double *basis, bo
 }  // namespace
 
 // collide.js:612-685
-int collide(const Body &A, const Body &B, Contact *contacts, double basis[9]) {
-  ContactSink sink(A, B, contacts);
+int collideInto(const Body &A, const Body &B, ContactSink &sink, double basis[9], double skin) {
   const bool sA = A.shape == SPHERE, sB = B.shape == SPHERE;
-  if (sA && sB) return sphereSphere(A, B, sink, basis);
-  if (sA || sB) return sphereBox(A, B, sink, basis, sA);
+  if (sA && sB) return sphereSphere(A, B, sink, basis, skin);
+  if (sA || sB) return sphereBox(A, B, sink, basis, sA, skin);
 
   OBB oA, oB;
   makeOBB(oA, A);
@@ -540,14 +541,14 @@ int collide(const Body &A, const Body &B, Contact *contacts, double basis[9]) {
   SatAxis face{0, -1, -1, -std::numeric_limits<double>::infinity(), {0, 0, 0}, false};
   SatAxis edge = face;
   for (int i = 0; i < 3; i++)
-    if (!testAxis(oA, oB, delta, oA.ax[i], AXIS_FACE_A, i, -1, face)) return 0;
+    if (!testAxis(oA, oB, delta, oA.ax[i], AXIS_FACE_A, i, -1, face, skin)) return 0;
   for (int i = 0; i < 3; i++)
-    if (!testAxis(oA, oB, delta, oB.ax[i], AXIS_FACE_B, -1, i, face)) return 0;
+    if (!testAxis(oA, oB, delta, oB.ax[i], AXIS_FACE_B, -1, i, face, skin)) return 0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
       double ax[3];
       v_cross(ax, oA.ax[i], oB.ax[j]);
-      if (!testAxis(oA, oB, delta, ax, AXIS_EDGE, i, j, edge)) return 0;
+      if (!testAxis(oA, oB, delta, ax, AXIS_EDGE, i, j, edge, skin)) return 0;
     }
   if (!face.valid) return 0;
   SatAxis best = face;
@@ -559,8 +560,16 @@ int collide(const Body &A, const Body &B, Contact *contacts, double basis[9]) {
   v_neg(t, best.n);
   m_orthonormal(basis, t);
   if (best.type == AXIS_EDGE) return edgeContact(oA, oB, sink, best.ia, best.ib, best.n);
-  if (best.type == AXIS_FACE_A) return faceManifold(oA, oB, sink, true, best.ia, best.n);
-  return faceManifold(oA, oB, sink, false, best.ib, best.n);
+  if (best.type == AXIS_FACE_A) return faceManifold(oA, oB, sink, true, best.ia, best.n, skin);
+  return faceManifold(oA, oB, sink, false, best.ib, best.n, skin);
+}
+
+int collide(const Body &A, const Body &B, Contact *contacts, double basis[9], double skin, double *mids) {
+  ContactSink sink(A, B, contacts);
+  const int n = collideInto(A, B, sink, basis, skin);
+  if (mids)
+    for (int i = 0; i < 3 * n; i++) mids[i] = sink.mid[i];
+  return n;
 }
 
 // maths.js:660-725 — unrolled LDL^T on the lower triangle
@@ -646,6 +655,16 @@ void angularBlocks(const double *jALin, const double *jBLin, const double *rAw, 
   }
 }
 
+// A body's contact and rest offsets in the GPU contact model: its own, else
+// the solver's, with the contact offset at least 0 and the rest offset at
+// most the contact offset (bodyContactSignature, backend.js:176-185).
+double bodyContactOffset(const Params &P, const Body &b) {
+  return jmax(0.0, std::isnan(b.contactOffset) ? P.contactOffset : b.contactOffset);
+}
+double bodyRestOffset(const Params &P, const Body &b) {
+  return jmin(std::isnan(b.restOffset) ? P.restOffset : b.restOffset, bodyContactOffset(P, b));
+}
+
 // manifold.js:214-318
 bool manifoldInitialize(World &W, Force &f) {
   const Params &P = W.prm;
@@ -654,17 +673,93 @@ bool manifoldInitialize(World &W, Force &f) {
   const bool hasSphere = A.shape == SPHERE || B.shape == SPHERE;
   const bool hadContacts = f.nc > 0;
   double oldBasis[9] = {0};
-  if (hasSphere && hadContacts) m_copy(oldBasis, f.basis);
-  f.mu = std::sqrt(A.friction * B.friction);
+  const bool gpu = P.contactModel == 1;
+  if ((hasSphere || gpu) && hadContacts) m_copy(oldBasis, f.basis);
+  // GPU contact model (shaders.js:1105-1149): material of the pair, the
+  // contact skin and the rest distance are sums of the bodies' offsets
+  double restDistance = 0.0, restitution = 0.0;
+  if (gpu) {
+    f.mu = std::sqrt(A.staticFriction * B.staticFriction);
+    f.muD = std::sqrt(A.dynamicFriction * B.dynamicFriction);
+    f.contactDistance = bodyContactOffset(P, A) + bodyContactOffset(P, B);
+    restDistance = bodyRestOffset(P, A) + bodyRestOffset(P, B);
+    restitution = jmax(A.restitution, B.restitution);
+  } else {
+    f.mu = std::sqrt(A.friction * B.friction);
+    f.muD = f.mu;
+    f.contactDistance = 0.0;
+  }
 
   Contact nw[MAX_CONTACTS];
-  for (int i = 0; i < MAX_CONTACTS; i++) nw[i].reset();
-  const int nn = collide(A, B, nw, f.basis);
+  bool found[MAX_CONTACTS];
+  for (int i = 0; i < MAX_CONTACTS; i++) {
+    nw[i].reset();
+    nw[i].stick = gpu;  // GPU model: a new contact starts in the static regime (shaders.js:1207-1210)
+    found[i] = false;
+  }
+  double mids[3 * MAX_CONTACTS];
+  const int nn = collide(A, B, nw, f.basis, f.contactDistance, mids);
   const bool compatible = !hasSphere || !hadContacts ||
                           (oldBasis[0] * f.basis[0] + oldBasis[1] * f.basis[1] + oldBasis[2] * f.basis[2]) > 0.85;
-  for (int i = 0; i < nn; i++) {
+  if (gpu) {
+    // shaders.js:1174-1319: an old contact matches a new one by feature key
+    // or, failing that, geometrically: the old anchors, placed at the current
+    // poses, lie within the persistence distance of the new contact (normal
+    // gap, tangential offset and midpoint); the nearest such one wins, an
+    // exact feature match ends the search (for a pair with a sphere at once),
+    // and each old contact is claimed once. k and lambda are carried into the
+    // new basis for every pair.
+    const bool basisCompatible =
+        hadContacts && (oldBasis[0] * f.basis[0] + oldBasis[1] * f.basis[1] + oldBasis[2] * f.basis[2]) > 0.85;
+    const double minExtent = 2.0 * jmin(jmin(A.size[0] * 0.5, jmin(A.size[1] * 0.5, A.size[2] * 0.5)),
+                                        jmin(B.size[0] * 0.5, jmin(B.size[1] * 0.5, B.size[2] * 0.5)));
+    const double persistence = jmax(P.contactPersistence, jmax(f.contactDistance, minExtent * 0.02));
+    const double persistenceSq = persistence * persistence;
+    uint32_t oldUsed = 0;
+    for (int i = 0; i < nn; i++) {
+      int match = -1;
+      bool near = false, exact = false;
+      double best = persistenceSq;
+      for (int j = 0; basisCompatible && j < f.nc; j++) {
+        if (oldUsed & (1u << j)) continue;
+        const bool same = f.c[j].feature == nw[i].feature;
+        if (hasSphere && same) { match = j; exact = true; break; }
+        double oxA[3], oxB[3], od[3], tan[3], md[3];
+        transformPoint(oxA, A.p, A.q, f.c[j].rA);
+        transformPoint(oxB, B.p, B.q, f.c[j].rB);
+        v_sub(od, oxA, oxB);
+        const double sep = v_dot(f.basis, od);
+        for (int k = 0; k < 3; k++) tan[k] = od[k] - f.basis[k] * sep;
+        if (sep > f.contactDistance + persistence) continue;
+        if (v_lenSq(tan) > persistenceSq) continue;
+        for (int k = 0; k < 3; k++) md[k] = mids[i * 3 + k] - (oxA[k] + oxB[k]) * 0.5;
+        const double mdSq = v_lenSq(md);
+        if (mdSq > persistenceSq) continue;
+        if (same) { match = j; exact = true; break; }
+        if (!near || mdSq < best) { near = true; match = j; best = mdSq; }
+      }
+      if (!near && !exact) continue;
+      if (!exact) W.geometricMatches++;
+      oldUsed |= 1u << match;
+      found[i] = true;
+      double rA[3], rB[3];
+      v_copy(rA, nw[i].rA);
+      v_copy(rB, nw[i].rB);
+      const uint32_t feature = nw[i].feature;  // the new contact's key is kept
+      nw[i] = f.c[match];
+      nw[i].feature = feature;
+      transportComponents(nw[i].lam, nw[i].lam, oldBasis, f.basis);
+      transportPenalty(nw[i].k, nw[i].k, oldBasis, f.basis);
+      if (!f.c[match].stick || hasSphere) {
+        v_copy(nw[i].rA, rA);
+        v_copy(nw[i].rB, rB);
+      }
+    }
+  }
+  for (int i = 0; i < nn && !gpu; i++) {
     for (int j = 0; j < f.nc; j++) {
       if (compatible && nw[i].feature == f.c[j].feature) {
+        found[i] = true;
         double rA[3], rB[3];
         v_copy(rA, nw[i].rA);
         v_copy(rB, nw[i].rB);
@@ -689,8 +784,35 @@ bool manifoldInitialize(World &W, Force &f) {
     transformPoint(xA, A.p, A.q, c.rA);
     transformPoint(xB, B.p, B.q, c.rB);
     v_sub(d, xA, xB);
-    m_mulVec(c.C0, f.basis, d);
-    c.C0[0] += P.collisionMargin;
+    if (gpu) {
+      // shaders.js:1322-1350: C0 against the rest distance; restitution is a
+      // one-step target for a new impact (or one a speculative-only contact
+      // kept armed), when the closing motion crosses the rest surface this
+      // step faster than the threshold
+      const bool priorOutsideRest = c.C0[0] > 0.0, priorHadBias = c.bias > 0.0;
+      m_mulVec(c.C0, f.basis, d);
+      c.C0[0] = c.C0[0] - restDistance;
+      double bias = 0.0;
+      const bool armed = !found[i] || (priorOutsideRest && !priorHadBias);
+      if (armed && restitution > 0.0) {
+        double ra[3], rb[3], va[3], vb[3], rel[3];
+        q_rotate(ra, A.q, c.rA);
+        q_rotate(rb, B.q, c.rB);
+        v_cross(va, A.w, ra);
+        v_add(va, A.v, va);
+        v_cross(vb, B.w, rb);
+        v_add(vb, B.v, vb);
+        v_sub(rel, va, vb);
+        const double vn = v_dot(f.basis, rel);
+        const double closing = jmax(0.0, -vn * P.dt);
+        const double toRest = jmax(0.0, c.C0[0]);
+        if (vn < -P.restitutionThreshold && closing > toRest) bias = restitution * (closing - toRest);
+      }
+      c.bias = bias;
+    } else {
+      m_mulVec(c.C0, f.basis, d);
+      c.C0[0] += P.collisionMargin;
+    }
     if (!P.postStabilize) {
       v_scale(c.lam, c.lam, P.alpha);
       v_scale(c.lam, c.lam, P.gamma);
@@ -742,6 +864,13 @@ void evaluate(World &W, Force &f, const PassState &s, int i, double alpha, EvalO
   double K[9], acc[3];
   m_diag(K, c.k[0], c.k[1], c.k[2]);
   v_scale(e.C, c.C0, 1 - alpha);
+  const bool gpu = W.prm.contactModel == 1;
+  if (gpu) {
+    // shaders.js:1805-1819: a positive speculative gap is kept whole, and the
+    // restitution target is subtracted outside the projection pass
+    if (f.contactDistance > 0.0 && c.C0[0] > 0.0) e.C[0] = c.C0[0];
+    if (!(W.prm.postStabilize && alpha < 0.5)) e.C[0] = e.C[0] - c.bias;
+  }
   m_mulVec(acc, s.jALin, s.dqALin); v_add(e.C, e.C, acc);
   m_mulVec(acc, s.jBLin, s.dqBLin); v_add(e.C, e.C, acc);
   m_mulVec(acc, e.jAAng, s.dqAAng); v_add(e.C, e.C, acc);
@@ -749,6 +878,19 @@ void evaluate(World &W, Force &f, const PassState &s, int i, double alpha, EvalO
   m_mulVec(e.F, K, e.C);
   v_add(e.F, e.F, c.lam);
   e.F[0] = jmin(e.F[0], 0);
+  if (gpu) {
+    // shaders.js:1824-1844: the static cone while the contact sticks; a
+    // force beyond it is clamped to the dynamic cone at once
+    const double fs = std::sqrt(e.F[1] * e.F[1] + e.F[2] * e.F[2]);
+    const double active = std::fabs(e.F[0]) * (c.stick ? f.mu : f.muD);
+    e.within = fs <= active;
+    if (!e.within && fs > 0.0) {
+      const double dyn = std::fabs(e.F[0]) * f.muD;
+      e.F[1] = e.F[1] * dyn / fs;
+      e.F[2] = e.F[2] * dyn / fs;
+    }
+    return;
+  }
   const double bounds = std::fabs(e.F[0]) * f.mu;
   const double fs = std::sqrt(e.F[1] * e.F[1] + e.F[2] * e.F[2]);
   if (fs > bounds && fs > 0) {
@@ -799,7 +941,15 @@ void manifoldDual(World &W, Force &f, double alpha) {
     v_copy(c.lam, e.F);
     v_copy(c.C, e.C);
     if (e.F[0] < 0) c.k[0] = jmin(c.k[0] + P.betaLin * std::fabs(e.C[0]), P.penaltyMax);
-    if (e.within) {
+    if (P.contactModel == 1) {
+      // shaders.js:2248-2258: the tangential ramp inside the cone, and the
+      // contact sticks exactly when the force is inside it
+      if (e.within) {
+        c.k[1] = jmin(c.k[1] + P.betaLin * std::fabs(e.C[1]), P.penaltyMax);
+        c.k[2] = jmin(c.k[2] + P.betaLin * std::fabs(e.C[2]), P.penaltyMax);
+      }
+      c.stick = e.within;
+    } else if (e.within) {
       c.k[1] = jmin(c.k[1] + P.betaLin * std::fabs(e.C[1]), P.penaltyMax);
       c.k[2] = jmin(c.k[2] + P.betaLin * std::fabs(e.C[2]), P.penaltyMax);
       c.stick = std::sqrt(e.C[1] * e.C[1] + e.C[2] * e.C[2]) < P.stickThreshold;
@@ -1123,6 +1273,8 @@ void removeFromBody(std::vector<int> &fl, int id) {
 // broadphase.js:236-243). A uniform grid over the bodies' centres keeps the
 // cost near-linear; bodies far larger than the rest are tested against all
 // others. The set visited is exactly the brute-force set.
+double bodyContactOffset(const Params &P, const Body &b);
+
 template <class Visit>
 void forEachPair(const World &W, Visit visit, bool brute) {
   const int n = int(W.bodies.size());
@@ -1142,7 +1294,14 @@ void forEachPair(const World &W, Visit visit, bool brute) {
     if (W.bodies[i].radius > large) { isLarge[i] = 1; larges.push_back(i); }
     else rmax = std::max(rmax, W.bodies[i].radius);
   }
-  const double h = std::max(2.0 * rmax * (1.0 + 1e-9), 1e-6);
+  // the GPU contact model grows every pair's reach by the two contact offsets
+  double reach = rmax;
+  if (W.prm.contactModel == 1) {
+    double off = 0.0;
+    for (int i = 0; i < n; i++) off = std::max(off, bodyContactOffset(W.prm, W.bodies[i]));
+    reach = rmax + off;
+  }
+  const double h = std::max(2.0 * reach * (1.0 + 1e-9), 1e-6);
   struct Key { int64_t cell; int idx; };
   std::vector<Key> keys;
   keys.reserve(n);
@@ -1226,6 +1385,7 @@ int World::addBox(const double size[3], double density, double friction, const d
   b.qy[0] = b.qy[1] = b.qy[2] = 0; b.qy[3] = 1;
   b.density = density;
   b.friction = friction;
+  b.staticFriction = b.dynamicFriction = friction;  // rigid.js:56-58
   b.mass = size[0] * size[1] * size[2] * density;
   b.moment[0] = ((size[1] * size[1] + size[2] * size[2]) / 12.0) * b.mass;
   b.moment[1] = ((size[0] * size[0] + size[2] * size[2]) / 12.0) * b.mass;
@@ -1413,7 +1573,11 @@ void World::step() {
   forEachPair(*this, [&](int i, int j) {
     const Body &A = bodies[i], &B = bodies[j];
     const double dx = A.p[0] - B.p[0], dy = A.p[1] - B.p[1], dz = A.p[2] - B.p[2];
-    const double r = A.radius + B.radius;
+    // GPU contact model: bounding spheres grown by the pair's contact skin
+    // (shaders.js:440-441)
+    const double r = prm.contactModel == 1
+                         ? A.radius + B.radius + (bodyContactOffset(prm, A) + bodyContactOffset(prm, B))
+                         : A.radius + B.radius;
     if (dx * dx + dy * dy + dz * dz <= r * r) {
       if (recordPairs && (A.mass > 0 || B.mass > 0) && !jointConnected(*this, i, j)) {
         lastPairs.push_back(i);

@@ -11,6 +11,7 @@
 // bit. Per-function citations are in avbd_oracle.cpp.
 #pragma once
 #include <cstdint>
+#include <limits>
 #include <vector>
 
 namespace orc {
@@ -32,6 +33,15 @@ struct Params {                  // solver.js:92-236 (defaultParams)
   bool postStabilize = false;
   bool rotatedInertia = true;
   bool cachedContactJacobians = true;
+  // The reference GPU backend's contact model (shaders.js:440-454,
+  // 821-1055, 1105-1367, 1777-1850, 2229-2264; solver.js:142-162;
+  // rigid.js:144-214): 0 = the CPU engine's (the default), 1 = speculative
+  // contact skins, rest offsets, restitution, static and dynamic friction.
+  int contactModel = 0;
+  double contactOffset = 0.005;        // solver.js:151
+  double restOffset = -0.005;          // solver.js:152
+  double restitutionThreshold = 1.0;   // solver.js:155
+  double contactPersistence = 0.002;   // solver.js:162
 };
 
 enum Shape : int { BOX = 0, SPHERE = 1 };
@@ -40,6 +50,11 @@ struct Body {                    // rigid.js:29-98
   int shape = BOX;
   double size[3];
   double density, friction, mass, moment[3], radius;
+  // contact material of the GPU contact model (Rigid.setMaterial,
+  // rigid.js:144-214); offsets NaN = the solver's
+  double staticFriction = 0, dynamicFriction = 0, restitution = 0;
+  double contactOffset = std::numeric_limits<double>::quiet_NaN();
+  double restOffset = std::numeric_limits<double>::quiet_NaN();
   double p[3], q[4];             // positionLin / positionAng
   double p0[3], q0[4];           // initialLin / initialAng (x_t)
   double py[3], qy[4];           // inertialLin / inertialAng (y)
@@ -52,10 +67,12 @@ struct Contact {                 // manifold.js:34-70
   double rA[3] = {0, 0, 0}, rB[3] = {0, 0, 0};
   double C0[3] = {0, 0, 0}, k[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, C[3] = {0, 0, 0};
   bool stick = false;
+  double bias = 0.0;             // GPU contact model: one-step restitution target
   void reset() {
     feature = 0;
     for (int i = 0; i < 3; i++) k[i] = lam[i] = C0[i] = C[i] = 0.0;
     stick = false;
+    bias = 0.0;
   }
 };
 
@@ -68,7 +85,9 @@ struct Force {
   // --- manifold (manifold.js:165-199)
   int nc = 0;
   double basis[9] = {0};
-  double mu = 0.0;
+  double mu = 0.0;               // CPU model: sqrt(mu_A mu_B); GPU model: the static coefficient
+  double muD = 0.0;              // GPU model: the dynamic coefficient
+  double contactDistance = 0.0;  // GPU model: the pair's contact skin
   Contact c[MAX_CONTACTS];
   double jac[MAX_CONTACTS * 18] = {0};
   bool jacValid = false;
@@ -111,6 +130,9 @@ struct World {
   bool recordPairs = false;
   int broadphase = 0;            // 0 grid (exactly the brute-force pair set), 1 brute force
   int colorOrder = 0;            // 0 body index, 1 lowbias32 priority
+  // GPU contact model: contacts warm-started from an old contact with another
+  // feature key (the geometric persistence match), counted over all steps
+  int64_t geometricMatches = 0;
 
   int addBox(const double size[3], double density, double friction,
              const double pos[3], const double vel[3]);
@@ -131,7 +153,10 @@ uint32_t colorPriority(uint32_t x);
 
 // Narrow phase on its own, for tests: fills up to MAX_CONTACTS contacts
 // (feature, rA, rB) and the basis; returns the contact count.
-int collide(const Body &A, const Body &B, Contact *contacts, double basis[9]);
+// skin: the GPU contact model's contact distance; mids, when given, receives
+// each contact's world midpoint (xA + xB) / 2 (3 per contact).
+int collide(const Body &A, const Body &B, Contact *contacts, double basis[9], double skin = 0.0,
+            double *mids = nullptr);
 
 // 6x6 LDL^T solve (maths.js:660-725).
 void solve6(const double aLin[9], const double aAng[9], const double aCross[9],

@@ -6,6 +6,7 @@
 // src/physics/rigid.js (box and sphere constructors).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -32,6 +33,45 @@ void orc_set_params(void *w, const double *d, const int *i) {
   P.stickThreshold = d[9];
   P.iterations = i[0]; P.postStabilize = i[1] != 0; P.rotatedInertia = i[2] != 0;
   P.cachedContactJacobians = i[3] != 0;
+}
+
+// The reference GPU backend's contact model (include/avbd.h
+// AVBD_CONTACTS_GPU): model 1 switches it on, with the solver's contact
+// offset, rest offset, restitution threshold and persistence distance
+// (solver.js:142-162).
+void orc_set_contact_model(void *w, int model, double contactOffset, double restOffset, double restitutionThreshold,
+                           double contactPersistence) {
+  Params &P = static_cast<World *>(w)->prm;
+  P.contactModel = model;
+  P.contactOffset = contactOffset;
+  P.restOffset = restOffset;
+  P.restitutionThreshold = restitutionThreshold;
+  P.contactPersistence = contactPersistence;
+}
+
+// Rigid.setMaterial (rigid.js:144-214) for body i: static and dynamic
+// friction, restitution, and the contact and rest offsets (NaN: the
+// solver's). Returns -1 for an index out of range and -2 for a material
+// setMaterial rejects (a negative or non-finite coefficient, restitution
+// outside [0, 1], dynamic above static friction, a rest offset above the
+// contact offset when both are given); the body is then unchanged.
+int orc_set_material(void *w, int i, double staticFriction, double dynamicFriction, double restitution,
+                     double contactOffset, double restOffset) {
+  World &W = *static_cast<World *>(w);
+  if (i < 0 || i >= int(W.bodies.size())) return -1;
+  auto nonNeg = [](double v) { return std::isfinite(v) && v >= 0; };
+  if (!nonNeg(staticFriction) || !nonNeg(dynamicFriction) || dynamicFriction > staticFriction) return -2;
+  if (!std::isfinite(restitution) || restitution < 0 || restitution > 1) return -2;
+  if (!std::isnan(contactOffset) && !nonNeg(contactOffset)) return -2;
+  if (!std::isnan(restOffset) && !std::isfinite(restOffset)) return -2;
+  if (!std::isnan(contactOffset) && !std::isnan(restOffset) && restOffset > contactOffset) return -2;
+  Body &b = W.bodies[i];
+  b.staticFriction = staticFriction;
+  b.dynamicFriction = dynamicFriction;
+  b.restitution = restitution;
+  b.contactOffset = contactOffset;
+  b.restOffset = restOffset;
+  return 0;
 }
 
 static void setPose(Body &b, const double *q, const double *w) {
@@ -284,6 +324,22 @@ int orc_get_manifold(void *w, int idx, double *basis_mu, unsigned *features, dou
   }
   return f.nc;
 }
+
+// The GPU contact model's fields of the manifold at list position `idx`:
+// extra = {dynamic friction, contact distance}, bias[c] = each contact's
+// restitution target. Returns the contact count.
+int orc_get_manifold_gpu(void *w, int idx, double *extra, double *bias) {
+  const World &W = *static_cast<World *>(w);
+  if (idx < 0 || idx >= int(W.forces.size())) return -1;
+  const Force &f = W.pool[W.forces[idx]];
+  if (f.kind != MANIFOLD) return -1;
+  extra[0] = f.muD;
+  extra[1] = f.contactDistance;
+  for (int i = 0; i < f.nc; i++) bias[i] = f.c[i].bias;
+  return f.nc;
+}
+
+long long orc_geometric_matches(void *w) { return static_cast<World *>(w)->geometricMatches; }
 
 // Joint at list position `idx`: C0Lin kLin lamLin CLin C0Ang kAng lamAng CAng (24) + broken.
 int orc_get_joint(void *w, int idx, double *out) {

@@ -62,6 +62,10 @@ def lib():
             "orc_get_spring": (i, [v, i, pd]),
             "orc_set_joint_anchor": (i, [v, i, pd]),
             "orc_format_state": (C.c_long, [v, i, C.c_char_p, C.c_long]),
+            "orc_set_contact_model": (None, [v, i, d, d, d, d]),
+            "orc_set_material": (i, [v, i, d, d, d, d, d]),
+            "orc_get_manifold_gpu": (i, [v, i, pd, pd]),
+            "orc_geometric_matches": (C.c_longlong, [v]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -93,6 +97,25 @@ DEFAULT_PARAMS = dict(dt=1.0 / 60.0, gravity=-10.0, iterations=10, betaLin=1e4, 
                       alpha=0.95, gamma=0.999, penaltyMin=1.0, penaltyMax=1e10,
                       collisionMargin=0.01, stickThreshold=1e-5, postStabilize=False,
                       rotatedInertia=True, cachedContactJacobians=True)
+# the reference GPU backend's contact model (solver.js:142-162); contactModel
+# 0 is the CPU engine's, 1 the GPU backend's
+CONTACT_PARAMS = dict(contactModel=0, contactOffset=0.005, restOffset=-0.005, restitutionThreshold=1.0,
+                      contactPersistence=0.002)
+
+
+def _material(b):
+    """The body's GPU contact material (Rigid.setMaterial, rigid.js:144-214),
+    or None when the body sets none: static and dynamic friction default to
+    its friction (the constructor's, rigid.js:56-58), restitution to 0, the
+    offsets (None) to the solver's."""
+    keys = ("staticFriction", "dynamicFriction", "restitution", "contactOffset", "restOffset")
+    if not any(b.get(k) is not None for k in keys):
+        return None
+    mu = float(b["friction"])
+    sf = float(b["staticFriction"]) if b.get("staticFriction") is not None else mu
+    df = float(b["dynamicFriction"]) if b.get("dynamicFriction") is not None else mu
+    off = [math.nan if b.get(k) is None else float(b[k]) for k in ("contactOffset", "restOffset")]
+    return sf, df, float(b.get("restitution") or 0.0), off[0], off[1]
 
 
 class Oracle:
@@ -104,12 +127,12 @@ class Oracle:
         p = dict(DEFAULT_PARAMS)
         p.update({k: v for k, v in (params or {}).items() if k in DEFAULT_PARAMS})
         self.params = p
-        dv = np.array([p["dt"], p["gravity"], p["betaLin"], p["betaAng"], p["alpha"], p["gamma"],
-                       p["penaltyMin"], p["penaltyMax"], p["collisionMargin"], p["stickThreshold"]],
-                      dtype=np.float64)
-        iv = np.array([p["iterations"], int(p["postStabilize"]), int(p["rotatedInertia"]),
-                       int(p["cachedContactJacobians"])], dtype=np.int32)
-        self.L.orc_set_params(self.h, _dp(dv), _ip(iv))
+        self._params_to_lib()
+        cm = dict(CONTACT_PARAMS)
+        cm.update({k: v for k, v in (params or {}).items() if k in CONTACT_PARAMS})
+        self.L.orc_set_contact_model(self.h, int(cm["contactModel"]), float(cm["contactOffset"]),
+                                     float(cm["restOffset"]), float(cm["restitutionThreshold"]),
+                                     float(cm["contactPersistence"]))
         if isinstance(scene, (str, os.PathLike)):
             # a .scene file, read by the oracle's C restatement of the
             # reference loader (test/parity/reference_main.cpp:45-124)
@@ -129,6 +152,8 @@ class Oracle:
                 size = _vec(b["size"], 3)
                 self.L.orc_add_box(self.h, _dp(size), float(b["density"]), float(b["friction"]),
                                    _dp(pos), _dp(vel), _dp(q), _dp(w))
+        for i, b in enumerate(scene["bodies"]):
+            self._set_material(i, b)
         for j in scene.get("joints", []):
             rA, rB = _vec(j["rA"], 3), _vec(j["rB"], 3)
             if j.get("type") == "spring":
@@ -145,6 +170,34 @@ class Oracle:
         self.set_schedule(schedule)
         self.L.orc_set_broadphase(self.h, 1 if broadphase == "brute" else 0)
         self.L.orc_set_color_order(self.h, int(color_order))
+
+    def _params_to_lib(self):
+        p = self.params
+        dv = np.array([p["dt"], p["gravity"], p["betaLin"], p["betaAng"], p["alpha"], p["gamma"],
+                       p["penaltyMin"], p["penaltyMax"], p["collisionMargin"], p["stickThreshold"]],
+                      dtype=np.float64)
+        iv = np.array([p["iterations"], int(p["postStabilize"]), int(p["rotatedInertia"]),
+                       int(p["cachedContactJacobians"])], dtype=np.int32)
+        self.L.orc_set_params(self.h, _dp(dv), _ip(iv))
+
+    def set_params(self, **kw):
+        """Changes solver parameters between steps (solver.iterations = 1 and
+        the like)."""
+        unknown = set(kw) - set(DEFAULT_PARAMS)
+        if unknown:
+            raise KeyError(sorted(unknown))
+        self.params.update(kw)
+        self._params_to_lib()
+
+    def geometric_matches(self):
+        """GPU contact model: contacts warm-started by the geometric
+        persistence match (another feature key), over all steps so far."""
+        return int(self.L.orc_geometric_matches(self.h))
+
+    def _set_material(self, i, b):
+        m = _material(b)
+        if m is not None and self.L.orc_set_material(self.h, int(i), *m) < 0:
+            raise ValueError(f"body {i}: material {m} rejected (rigid.js:144-214)")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -182,6 +235,7 @@ class Oracle:
                 size = _vec(b["size"], 3)
                 ids.append(self.L.orc_add_box(self.h, _dp(size), float(b["density"]), float(b["friction"]),
                                               _dp(pos), _dp(vel), _dp(q), _dp(w)))
+            self._set_material(ids[-1], b)
         return ids
 
     def record_pairs(self, on=True):
@@ -234,7 +288,11 @@ class Oracle:
                                      _dp(con))
         if nc < 0:
             raise IndexError(idx)
-        return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], features=feat[:nc], contacts=con[:nc])
+        ex = np.zeros(2, dtype=np.float64)
+        bias = np.zeros(8, dtype=np.float64)
+        self.L.orc_get_manifold_gpu(self.h, int(idx), _dp(ex), _dp(bias))
+        return dict(basis=bm[:9].reshape(3, 3), mu=bm[9], features=feat[:nc], contacts=con[:nc],
+                    mu_dynamic=ex[0], contact_distance=ex[1], bias=bias[:nc])
 
     def set_joint_anchor(self, joint, rA):
         """New rA of the joint created joint-th among joints and springs."""
