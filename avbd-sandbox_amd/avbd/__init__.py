@@ -1,5 +1,6 @@
 """MI355X (gfx950) AVBD rigid-body step: Python binding of include/avbd.h."""
-from ._lib import (PREC_EXACT, PREC_F64, PREC_MIXED, SYMBOLS, LIB_PATH, lib)  # noqa: F401
+from ._lib import (CONTACTS_CPU, CONTACTS_GPU, PREC_EXACT, PREC_F64, PREC_MIXED, SYMBOLS, LIB_PATH,  # noqa: F401
+                   lib)
 from .backend import AvbdError, HipBackend, scene_arrays  # noqa: F401
 from . import scenes  # noqa: F401
 from . import sceneio  # noqa: F401

@@ -29,7 +29,7 @@ SYMBOLS = (
     "avbd_upload_coloring", "avbd_export_poses", "avbd_export_poses_static", "avbd_pose_source", "avbd_repack",
     "avbd_read_joints",
     "avbd_read_step_start", "avbd_set_joint_anchor", "avbd_read_manifolds", "avbd_set_coloring_limits",
-    "avbd_read_inertial_target",
+    "avbd_read_inertial_target", "avbd_read_contact_model",
 )
 K_PRIMAL, K_DUAL, K_NARROW, K_POSES = 0, 1, 2, 3
 
@@ -48,7 +48,12 @@ class Params(C.Structure):
     _fields_ = [("dt", d), ("gravity", d), ("beta_lin", d), ("beta_ang", d), ("alpha", d), ("gamma", d),
                 ("penalty_min", d), ("penalty_max", d), ("collision_margin", d), ("stick_threshold", d),
                 ("iterations", i32), ("post_stabilize", i32), ("rotated_inertia", i32),
-                ("cached_contact_jacobians", i32), ("color_order", i32), ("reserved", i32)]
+                ("cached_contact_jacobians", i32), ("color_order", i32), ("contact_model", i32),
+                ("contact_offset", d), ("rest_offset", d), ("restitution_threshold", d),
+                ("contact_persistence", d)]
+
+
+CONTACTS_CPU, CONTACTS_GPU = 0, 1
 
 
 ORDER_INDEX, ORDER_PRIORITY, ORDER_CUSTOM = 0, 1, 2
@@ -58,7 +63,9 @@ class Scene(C.Structure):
     _fields_ = [("num_bodies", i32), ("shape", pi), ("size", pd), ("density", pd), ("friction", pd),
                 ("pos", pd), ("quat", pd), ("vel", pd), ("angvel", pd), ("scene_of", pi),
                 ("num_joints", i32), ("joint_a", pi), ("joint_b", pi), ("joint_ra", pd), ("joint_rb", pd),
-                ("joint_stiff_lin", pd), ("joint_stiff_ang", pd), ("joint_fracture", pd), ("joint_kind", pi)]
+                ("joint_stiff_lin", pd), ("joint_stiff_ang", pd), ("joint_fracture", pd), ("joint_kind", pi),
+                ("static_friction", pd), ("dynamic_friction", pd), ("restitution", pd), ("contact_offset", pd),
+                ("rest_offset", pd)]
 
 
 class Counters(C.Structure):
@@ -115,6 +122,7 @@ def lib(path=None):
         "avbd_read_manifolds": (C.c_int, [ctx, pi, pd, pd, i32, i32, pi, pi]),
         "avbd_set_coloring_limits": (C.c_int, [ctx, i32, i32]),
         "avbd_read_inertial_target": (C.c_int, [ctx, pd, pd, i32]),
+        "avbd_read_contact_model": (C.c_int, [ctx, pd, pd, i32, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -4,10 +4,13 @@ readState / readColoring / lastStats.
 
 A scene is a dict with ``bodies`` (each: shape 'box' | 'sphere', size (full
 widths; a sphere's diameter three times), density (0 = static), friction, pos,
-and optional vel, quat (xyzw), angvel, scene) and ``joints`` (each: a (-1 =
-world anchor), b, rA, rB, kLin, kAng, fracture; 'inf' or math.inf for
-infinite). Parameters use the names of the reference's Solver
-(solver.js:92-236).
+and optional vel, quat (xyzw), angvel, scene, and the contact material of
+the GPU contact law: staticFriction, dynamicFriction, restitution,
+contactOffset, restOffset, as Rigid.setMaterial takes them, rigid.js:144-214)
+and ``joints`` (each: a (-1 = world anchor), b, rA, rB, kLin, kAng, fracture;
+'inf' or math.inf for infinite). Parameters use the names of the reference's
+Solver (solver.js:92-236); contactModel 1 selects the GPU backend's contact
+law (contactOffset, restOffset, restitutionThreshold, contactPersistence).
 """
 import ctypes as C
 import math
@@ -22,7 +25,10 @@ PARAM_NAMES = {
     "collisionMargin": "collision_margin", "stickThreshold": "stick_threshold", "iterations": "iterations",
     "postStabilize": "post_stabilize", "rotatedInertia": "rotated_inertia",
     "cachedContactJacobians": "cached_contact_jacobians", "colorOrder": "color_order",
+    "contactModel": "contact_model", "contactOffset": "contact_offset", "restOffset": "rest_offset",
+    "restitutionThreshold": "restitution_threshold", "contactPersistence": "contact_persistence",
 }
+MATERIAL_KEYS = ("staticFriction", "dynamicFriction", "restitution", "contactOffset", "restOffset")
 
 
 class AvbdError(RuntimeError):
@@ -72,6 +78,17 @@ def scene_arrays(scene):
         "angvel": _rows(bodies, "angvel", (0, 0, 0), 3),
         "scene_of": np.fromiter((b.get("scene", 0) for b in bodies), np.int32, n),
     }
+    # the GPU contact law's material, when any body sets one: static and
+    # dynamic friction default to the body's friction, restitution to 0, the
+    # offsets to the solver's (NaN)
+    if any(b.get(k) is not None for b in bodies for k in MATERIAL_KEYS):
+        def col(key, default):
+            return np.fromiter((default(b) if b.get(key) is None else float(b[key]) for b in bodies), np.float64, n)
+        A["static_friction"] = col("staticFriction", lambda b: float(b["friction"]))
+        A["dynamic_friction"] = col("dynamicFriction", lambda b: float(b["friction"]))
+        A["restitution"] = col("restitution", lambda b: 0.0)
+        A["contact_offset"] = col("contactOffset", lambda b: math.nan)
+        A["rest_offset"] = col("restOffset", lambda b: math.nan)
     # joints and springs, in creation order; a spring ({"type": "spring", a,
     # b, rA, rB, stiffness, rest (negative or absent: current length),
     # tearStrain (absent: never)}) rides the joint arrays (include/avbd.h)
@@ -175,6 +192,10 @@ class HipBackend:
             s.joint_stiff_lin, s.joint_stiff_ang = _dp(A["joint_stiff_lin"]), _dp(A["joint_stiff_ang"])
             s.joint_fracture = _dp(A["joint_fracture"])
             s.joint_kind = _ip(A["joint_kind"])
+        if "static_friction" in A:
+            s.static_friction, s.dynamic_friction = _dp(A["static_friction"]), _dp(A["dynamic_friction"])
+            s.restitution, s.contact_offset = _dp(A["restitution"]), _dp(A["contact_offset"])
+            s.rest_offset = _dp(A["rest_offset"])
         return s
 
     # --- stepping (backend.js:1393-1620, 1804-1818)
@@ -328,6 +349,17 @@ class HipBackend:
         self._ck(self.L.avbd_read_manifolds(self.ctx, _ip(head), _dp(bm), _dp(con), m.value, c.value, C.byref(m),
                                             C.byref(c)))
         return head, bm, con
+
+    def read_contact_model(self):
+        """The GPU contact law's fields of the slots read_manifolds returns, in
+        the same order: material (m, 3) {static friction, dynamic friction,
+        contact distance} and each contact's restitution target bias (c,)."""
+        m, c = C.c_int32(), C.c_int32()
+        self._ck(self.L.avbd_read_manifolds(self.ctx, None, None, None, 0, 0, C.byref(m), C.byref(c)))
+        mat = np.zeros((m.value, 3))
+        bias = np.zeros(c.value)
+        self._ck(self.L.avbd_read_contact_model(self.ctx, _dp(mat), _dp(bias), m.value, c.value))
+        return mat, bias
 
     def counters(self):
         c = L.Counters()

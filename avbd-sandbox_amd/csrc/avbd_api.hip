@@ -156,6 +156,13 @@ struct avbd_ctx {
   std::vector<double> anchorStage;  // avbd_set_joint_anchor's host copies
   std::vector<int> hjointKind;      // AVBD_JOINT_* of each packed joint
   std::vector<BodyProp> hprops;
+  // the GPU contact model's material: per body as given (static, dynamic
+  // friction, restitution, contact and rest offset, NaN = the solver's), and
+  // resolved on the device with the contact offsets the pair test reads
+  std::vector<double> hmatRaw;
+  DBuf<BodyMat> mat;
+  DBuf<double> skinOff;
+  double rmaxSmall = 0;  // largest radius of a body on the broad-phase grid
 
   // broad phase
   DBuf<int4> cellOf;
@@ -351,6 +358,7 @@ struct avbd_ctx {
     dropGraphs();
     Xbk.release(); DQbk.release();
     props.release(); X.release(); X0.release(); Y.release(); V.release(); DQ.release(); poseStage.release();
+    mat.release(); skinOff.release();
     cbytes.release();
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     pending.clear();
@@ -396,6 +404,9 @@ struct avbd_ctx {
     P.alpha = prm.alpha; P.gamma = prm.gamma; P.pmin = prm.penalty_min; P.pmax = prm.penalty_max;
     P.margin = prm.collision_margin; P.stick = prm.stick_threshold; P.iterations = prm.iterations;
     P.postStab = prm.post_stabilize; P.rotated = prm.rotated_inertia;
+    P.contactModel = prm.contact_model;
+    P.restThreshold = prm.restitution_threshold;
+    P.persistence = prm.contact_persistence;
     return P;
   }
 
@@ -856,6 +867,7 @@ struct avbd_ctx {
     ba.countNewRev = countNewRev.p; ba.startNewRev = startNewRev.p;
     ba.pairs = pairs.p; ba.newPairs = newPairs.p; ba.pairCap = pairCap;
     ba.entryCell = entryCell.p; ba.scratch = bpScratch.p;
+    ba.skin = prm.contact_model ? skinOff.p : nullptr;
     HIPCHECK(hipMemsetAsync(countRev.p, 0, sizeof(int) * N, stream));
     HIPCHECK(hipMemsetAsync(countNewRev.p, 0, sizeof(int) * N, stream));
     hipLaunchKernelGGL(k_bp_team, dim3(blocks(N, 256 / BP_TEAM)), dim3(256), 0, stream, ba);
@@ -889,7 +901,8 @@ struct avbd_ctx {
     if (nTot > 0)
       timedLaunch(AVBD_K_NARROW, 0.0, k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), nOld, nNew,
                   (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p, (const BodyProp *)props.p,
-                  (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p);
+                  (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p, (const BodyMat *)mat.p,
+                  (const double *)V.p);
     if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
     scan(keep.p, keepPos.p, nTot, scal.p + 2);
     if (nTot > 0)
@@ -1076,6 +1089,7 @@ struct avbd_ctx {
       throw ApiError(AVBD_E_STATE, "the adjoint needs f64 records (precision exact or f64)");
     if (J > 0) throw ApiError(AVBD_E_STATE, "the adjoint covers contacts only; the scene has joints");
     if (prm.post_stabilize) throw ApiError(AVBD_E_STATE, "the adjoint does not cover post-stabilisation");
+    if (prm.contact_model) throw ApiError(AVBD_E_STATE, "the adjoint covers the CPU contact law only");
     for (const auto &b : hprops)
       if (b.shape == 1) throw ApiError(AVBD_E_STATE, "the adjoint covers box contacts only; the scene has spheres");
   }
@@ -1271,6 +1285,66 @@ template <class F> int guarded(avbd_ctx *c, F &&f) {
   }
 }
 
+// The scene's contact material (avbd_scene static_friction .. rest_offset),
+// validated as Rigid.setMaterial validates it (rigid.js:144-214), into
+// c.hmatRaw: 5 numbers per body, offsets NaN where the solver's apply.
+void readMaterials(avbd_ctx &c, const avbd_scene &s) {
+  const int N = s.num_bodies;
+  c.hmatRaw.assign(size_t(N) * 5, 0.0);
+  const bool given = s.static_friction || s.dynamic_friction || s.restitution || s.contact_offset || s.rest_offset;
+  auto nonNeg = [](double v) { return std::isfinite(v) && v >= 0; };
+  for (int i = 0; i < N; i++) {
+    double *r = &c.hmatRaw[size_t(i) * 5];
+    r[0] = s.static_friction ? s.static_friction[i] : s.friction[i];
+    r[1] = s.dynamic_friction ? s.dynamic_friction[i] : s.friction[i];
+    r[2] = s.restitution ? s.restitution[i] : 0.0;
+    r[3] = s.contact_offset ? s.contact_offset[i] : std::nan("");
+    r[4] = s.rest_offset ? s.rest_offset[i] : std::nan("");
+    if (!given) continue;
+    const std::string at = "body " + std::to_string(i) + ": ";
+    if (!nonNeg(r[0]) || !nonNeg(r[1]))
+      throw ApiError(AVBD_E_INVALID, at + "static and dynamic friction must be finite and non-negative");
+    if (r[1] > r[0]) throw ApiError(AVBD_E_INVALID, at + "dynamic friction must not exceed static friction");
+    if (!(r[2] >= 0 && r[2] <= 1)) throw ApiError(AVBD_E_INVALID, at + "restitution must be in [0, 1]");
+    if (!std::isnan(r[3]) && !nonNeg(r[3]))
+      throw ApiError(AVBD_E_INVALID, at + "contact offset must be finite and non-negative");
+    if (std::isinf(r[4])) throw ApiError(AVBD_E_INVALID, at + "rest offset must be finite");
+    if (!std::isnan(r[3]) && !std::isnan(r[4]) && r[4] > r[3])
+      throw ApiError(AVBD_E_INVALID, at + "rest offset must not exceed contact offset");
+  }
+}
+
+// The material resolved against the solver's offsets as
+// bodyContactSignature does (backend.js:156-185): contact offset at least 0,
+// rest offset at most the contact offset. Uploads it with the pair test's
+// contact offsets and sets the broad-phase grid cell to at least twice the
+// largest reach of a grid body: its radius plus, under the GPU contact model,
+// the largest contact offset of any body (a pair within reach of each other
+// then has its centres in neighbouring cells).
+void resolveMaterials(avbd_ctx &c) {
+  const int N = c.N;
+  std::vector<BodyMat> M(static_cast<size_t>(N));
+  std::vector<double> off(static_cast<size_t>(N));
+  double offMax = 0;
+  for (int i = 0; i < N; i++) {
+    const double *r = &c.hmatRaw[size_t(i) * 5];
+    BodyMat &m = M[size_t(i)];
+    m.staticFriction = r[0];
+    m.dynamicFriction = r[1];
+    m.restitution = r[2];
+    m.contactOffset = jmax(0.0, std::isnan(r[3]) ? c.prm.contact_offset : r[3]);
+    m.restOffset = jmin(std::isnan(r[4]) ? c.prm.rest_offset : r[4], m.contactOffset);
+    m.pad = 0;
+    off[size_t(i)] = m.contactOffset;
+    offMax = std::max(offMax, m.contactOffset);
+  }
+  if (c.stream) HIPCHECK(hipStreamSynchronize(c.stream));  // no step may be reading the old values
+  HIPCHECK(hipMemcpy(c.mat.p, M.data(), sizeof(BodyMat) * N, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c.skinOff.p, off.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  const double reach = c.prm.contact_model ? c.rmaxSmall + offMax : c.rmaxSmall;
+  c.invh = 1.0 / std::max(2.0 * reach * (1.0 + 1e-9), 1e-6);
+}
+
 void packScene(avbd_ctx &c, const avbd_scene &s) {
   if (s.num_bodies <= 0) throw ApiError(AVBD_E_INVALID, "num_bodies must be positive");
   if (!s.shape || !s.size || !s.density || !s.friction || !s.pos)
@@ -1366,13 +1440,16 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   }
   sls[S] = int(slist.size());
   c.nLarge = int(largeList.size());
-  const double h = std::max(2.0 * rmax * (1.0 + 1e-9), 1e-6);
-  c.invh = 1.0 / h;
+  c.rmaxSmall = rmax;
+  readMaterials(c, s);
   uint32_t tsz = 64;
   while (tsz < uint32_t(2 * N)) tsz <<= 1;
   c.tableMask = tsz - 1;
 
   c.hprops = P;
+  c.mat.reserve(N);
+  c.skinOff.reserve(N);
+  resolveMaterials(c);  // and the grid cell
   c.nDynamic = 0;
   for (const BodyProp &b : P) c.nDynamic += b.mass > 0;
   c.props.reserve(N);
@@ -1558,7 +1635,11 @@ void avbd_default_params(avbd_params *p) {
   p->rotated_inertia = 1;
   p->cached_contact_jacobians = 1;
   p->color_order = AVBD_ORDER_INDEX;
-  p->reserved = 0;
+  p->contact_model = AVBD_CONTACTS_CPU;
+  p->contact_offset = 0.005;
+  p->rest_offset = -0.005;
+  p->restitution_threshold = 1.0;
+  p->contact_persistence = 0.002;
 }
 
 int avbd_create(avbd_ctx **out, const avbd_config *cfg) {
@@ -1621,9 +1702,17 @@ int avbd_set_params(avbd_ctx *ctx, const avbd_params *p) {
       throw ApiError(AVBD_E_INVALID, "color_order must be AVBD_ORDER_INDEX or AVBD_ORDER_PRIORITY");
     if (!p->cached_contact_jacobians)
       throw ApiError(AVBD_E_INVALID, "only cached contact Jacobians are implemented on the GPU");
+    if (p->contact_model != AVBD_CONTACTS_CPU && p->contact_model != AVBD_CONTACTS_GPU)
+      throw ApiError(AVBD_E_INVALID, "contact_model must be AVBD_CONTACTS_CPU or AVBD_CONTACTS_GPU");
+    if (!std::isfinite(p->contact_offset) || !std::isfinite(p->rest_offset) ||
+        !std::isfinite(p->restitution_threshold) || !(p->contact_persistence >= 0) ||
+        !std::isfinite(p->contact_persistence))
+      throw ApiError(AVBD_E_INVALID, "contact offsets, restitution threshold and persistence must be finite "
+                                     "(persistence >= 0)");
     ctx->prm = *p;
     ctx->dropAdjoint();
     ctx->dropGraphs();  // parameters are baked into the captured launches
+    if (ctx->packed) resolveMaterials(*ctx);  // offsets and the grid follow the solver's
   });
 }
 
@@ -2083,6 +2172,42 @@ int avbd_read_manifolds(avbd_ctx *ctx, int32_t *head, double *basis_mu, double *
           }
           o[15] = stick[si];
         }
+    }
+  });
+}
+
+int avbd_read_contact_model(avbd_ctx *ctx, double *material, double *bias, int32_t cap, int32_t contact_cap) {
+  return guarded(ctx, [&] {
+    if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
+    if (cap < 0 || contact_cap < 0) throw ApiError(AVBD_E_INVALID, "negative capacity");
+    const int nl = ctx->nLive;
+    if (nl == 0 || (!material && !bias)) return;
+    std::vector<int> live(static_cast<size_t>(nl));
+    HIPCHECK(hipMemcpyAsync(live.data(), ctx->live[ctx->cur].p, sizeof(int) * nl, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const size_t span = size_t(*std::max_element(live.begin(), live.end())) + 1;
+    ManStore &st = ctx->store[ctx->cur];
+    const size_t rs = ctx->rsize();
+    std::vector<int4> h(span);
+    std::vector<char> bas(span * BAS * rs), hot(span * MAXC * HOT * rs);
+    HIPCHECK(hipMemcpyAsync(h.data(), st.head.p, sizeof(int4) * span, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(bas.data(), st.basis.p, bas.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(hot.data(), st.hot.p, hot.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto rd = [&](const std::vector<char> &v, size_t i) {
+      return rs == sizeof(float) ? double(((const float *)v.data())[i]) : ((const double *)v.data())[i];
+    };
+    long k = 0;
+    for (int t = 0; t < nl && t < cap; t++) {
+      const size_t m = size_t(live[t]);
+      if (material) {
+        material[size_t(t) * 3] = rd(bas, m * BAS + BMU);
+        material[size_t(t) * 3 + 1] = rd(bas, m * BAS + BMUD);
+        material[size_t(t) * 3 + 2] = rd(bas, m * BAS + BCD);
+      }
+      for (int sl = 0; sl < h[m].z && k < contact_cap; sl++, k++)
+        if (bias) bias[k] = rd(hot, (m * MAXC + size_t(sl)) * HOT + HB);
     }
   });
 }

@@ -20,6 +20,14 @@ struct BodyProp {
   int32_t shape, scene, large, pad;
 };
 
+// Contact material of the GPU contact model, per body, resolved when it is
+// uploaded (bodyContactSignature, backend.js:156-185): static and dynamic
+// friction, restitution, the contact offset (at least 0) and the rest offset
+// (at most the contact offset), each its own or else the solver's.
+struct BodyMat {
+  double staticFriction, dynamicFriction, restitution, contactOffset, restOffset, pad;
+};
+
 // Body state rows, double, 8 per body: X = [p0 p1 p2 q0 q1 q2 q3 -].
 constexpr int XS = 8;
 // Velocities, 12 per body: v[3] w[3] vprev[3] -.
@@ -59,6 +67,19 @@ constexpr int BAS = 12;
 // Offsets in a hot contact record. k and lambda lead so that the dual update
 // writes them as aligned vectors.
 constexpr int HK = 0, HL = 3, HC0 = 6, HRA = 9, HRB = 12;
+// GPU contact model: hot[HB] the contact's restitution target; basis[BMU]
+// the static friction coefficient (the CPU engine's only one), basis[BMUD]
+// the dynamic one, basis[BCD] the pair's contact distance (skin).
+constexpr int HB = 15, BMU = 9, BMUD = 10, BCD = 11;
+
+// The contact law an evaluation follows: the CPU engine's (manifold.js:
+// 358-491), the reference GPU backend's (shaders.js:1777-1850, 2229-2264),
+// or the GPU backend's in a post-stabilisation projection pass (alpha below
+// 0.5), which leaves the restitution target out.
+constexpr int LAW_CPU = 0, LAW_GPU = 1, LAW_GPU_PROJ = 2;
+__host__ __device__ inline int contactLaw(int model, int postStab, double alpha) {
+  return model ? (postStab && alpha < 0.5 ? LAW_GPU_PROJ : LAW_GPU) : LAW_CPU;
+}
 
 // Aligned 16-byte loads of N consecutive numbers (N a multiple of 4 floats or
 // 2 doubles; the records are 16-byte aligned).
@@ -225,6 +246,9 @@ struct Sink {
   AliasDouble *mid;
   int midStride;
   int count;
+  // the contact skin of the GPU contact model (0 otherwise): pairs within it
+  // produce speculative contacts (shaders.js:821-1055)
+  double skin;
   // collide.js:254-285
   __device__ bool add(const double *xA, const double *xB, uint32_t key) {
     const double mx = (xA[0] + xB[0]) * 0.5, my = (xA[1] + xB[1]) * 0.5, mz = (xA[2] + xB[2]) * 0.5;
@@ -256,7 +280,7 @@ struct Sat { int type, ia, ib; double sep; double n[3]; bool valid; };
 
 // collide.js:291-324
 __device__ inline bool satAxis(const Obb &A, const Obb &B, const double *delta, const double *axis, int type,
-                               int ia, int ib, Sat &t) {
+                               int ia, int ib, Sat &t, double skin) {
   const double lenSq = dot3(axis, axis);
   if (lenSq < SAT_EPS) return true;
   const double invLen = 1.0 / xsqrt(lenSq);
@@ -266,7 +290,7 @@ __device__ inline bool satAxis(const Obb &A, const Obb &B, const double *delta, 
   const double rA = A.h[0] * xabs(dot3(n, A.ax[0])) + A.h[1] * xabs(dot3(n, A.ax[1])) + A.h[2] * xabs(dot3(n, A.ax[2]));
   const double rB = B.h[0] * xabs(dot3(n, B.ax[0])) + B.h[1] * xabs(dot3(n, B.ax[1])) + B.h[2] * xabs(dot3(n, B.ax[2]));
   const double sep = dist - (rA + rB);
-  if (sep > 0) return false;
+  if (sep > skin) return false;
   if (!t.valid || sep > t.sep) {
     t.valid = true; t.type = type; t.ia = ia; t.ib = ib; t.sep = sep;
     t.n[0] = n[0]; t.n[1] = n[1]; t.n[2] = n[2];
@@ -388,7 +412,7 @@ __device__ inline int faceManifoldT(const Obb &oA, const Obb &oB, Sink &sink, bo
     for (int k = 0; k < 3; k++) p[k] = c0[(i * 3 + k) * STRIDE];
     for (int k = 0; k < 3; k++) t[k] = p[k] - fc[k];
     const double dist = dot3(t, fn);
-    if (dist > PLANE_EPS) continue;
+    if (dist > sink.skin + PLANE_EPS) continue;
     for (int k = 0; k < 3; k++) { t[k] = p[k]; t[k] += fn[k] * -dist; }
     if (refIsA) sink.add(t, p, prefix | uint32_t(i & 0xff));
     else sink.add(p, t, prefix | uint32_t(i & 0xff));
@@ -438,7 +462,7 @@ __device__ inline int edgeContact(const Obb &oA, const Obb &oB, Sink &sink, int 
 __device__ inline int sphereSphere(const BodyView &A, const BodyView &B, Sink &sink, double *basis) {
   double d[3] = {B.p[0] - A.p[0], B.p[1] - A.p[1], B.p[2] - A.p[2]}, n[3], t[3], xA[3], xB[3];
   const double dsq = dot3(d, d);
-  const double rs = A.radius + B.radius;
+  const double rs = A.radius + B.radius + sink.skin;
   if (dsq > rs * rs) return 0;
   if (dsq > SPHERE_EPS_SQ) {
     const double inv = 1 / xsqrt(dsq);
@@ -482,7 +506,7 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
   int fa = 0;
   if (dsq > SPHERE_EPS_SQ) {
     const double dist = xsqrt(dsq);
-    if (dist > sph.radius) return 0;
+    if (dist > sph.radius + sink.skin) return 0;
     const double inv = 1 / dist;
     for (int k = 0; k < 3; k++) n[k] = sv[k] * inv;
   } else {
@@ -520,9 +544,11 @@ __device__ inline int sphereBox(const BodyView &A, const BodyView &B, Sink &sink
 }
 
 // collide.js:612-685. Writes the basis only where the reference does.
+// skin: the GPU contact model's contact distance, 0 for the CPU engine's
+// (the same comparisons then, bit for bit).
 __device__ inline int collide(const BodyView &A, const BodyView &B, double *anchor, uint32_t *feat, AliasDouble *mid,
-                              int midStride, double *basis, double *lds = nullptr) {
-  Sink sink{A.p, A.q, B.p, B.q, anchor, feat, mid, midStride, 0};
+                              int midStride, double *basis, double *lds = nullptr, double skin = 0.0) {
+  Sink sink{A.p, A.q, B.p, B.q, anchor, feat, mid, midStride, 0, skin};
   const bool sA = A.shape == 1, sB = B.shape == 1;
   if (sA && sB) return sphereSphere(A, B, sink, basis);
   if (sA || sB) return sphereBox(A, B, sink, basis, sA);
@@ -534,17 +560,17 @@ __device__ inline int collide(const BodyView &A, const BodyView &B, double *anch
   Sat edge = face;
 #pragma unroll
   for (int i = 0; i < 3; i++)
-    if (!satAxis(oA, oB, delta, oA.ax[i], 0, i, -1, face)) return 0;
+    if (!satAxis(oA, oB, delta, oA.ax[i], 0, i, -1, face, skin)) return 0;
 #pragma unroll
   for (int i = 0; i < 3; i++)
-    if (!satAxis(oA, oB, delta, oB.ax[i], 1, -1, i, face)) return 0;
+    if (!satAxis(oA, oB, delta, oB.ax[i], 1, -1, i, face, skin)) return 0;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       double ax[3];
       cross3(ax, oA.ax[i], oB.ax[j]);
-      if (!satAxis(oA, oB, delta, ax, 2, i, j, edge)) return 0;
+      if (!satAxis(oA, oB, delta, ax, 2, i, j, edge, skin)) return 0;
     }
   if (!face.valid) return 0;
   Sat best = face;
@@ -574,7 +600,7 @@ template <class R> struct Eval {
 // Jacobian at x_t (manifold.js:325-410), and the angular Jacobian blocks.
 template <class R, bool EXACT>
 __device__ __forceinline__ void evalC(const R *bas, const R *negb, const R *hot, const R *dqA, const R *dqB, R alpha,
-                                      Eval<R> &e) {
+                                      Eval<R> &e, int law = LAW_CPU) {
   // angular blocks from the cached world anchors
 #pragma unroll
   for (int r = 0; r < 3; r++) {
@@ -588,6 +614,12 @@ __device__ __forceinline__ void evalC(const R *bas, const R *negb, const R *hot,
   R acc[3];
   const R oma = R(1) - alpha;
   e.C[0] = C0[0] * oma; e.C[1] = C0[1] * oma; e.C[2] = C0[2] * oma;
+  if (law != LAW_CPU) {
+    // a positive speculative gap is kept whole; the restitution target is
+    // subtracted outside a projection pass (shaders.js:1805-1819)
+    if (bas[BCD] > R(0) && C0[0] > R(0)) e.C[0] = C0[0];
+    if (law == LAW_GPU) e.C[0] = e.C[0] - hot[HB];
+  }
   mulv(acc, bas, dqA);
   e.C[0] = e.C[0] + acc[0]; e.C[1] = e.C[1] + acc[1]; e.C[2] = e.C[2] + acc[2];
   mulv(acc, negb, dqB);
@@ -599,8 +631,12 @@ __device__ __forceinline__ void evalC(const R *bas, const R *negb, const R *hot,
 }
 
 // Force F = clamp(K C + lambda) into the friction cone (manifold.js:411-430).
+// Under the GPU law (shaders.js:1824-1844) the cone is the static one while
+// the contact sticks, the dynamic one otherwise, and a force outside it is
+// scaled onto the dynamic cone.
 template <class R, bool EXACT>
-__device__ __forceinline__ void evalF(const R *k, const R *lam, R mu, Eval<R> &e) {
+__device__ __forceinline__ void evalF(const R *k, const R *lam, R mu, Eval<R> &e, int law = LAW_CPU, R muD = R(0),
+                                      int stick = 0) {
   if (EXACT) {
     // K C with the diagonal K written out as the reference's full product
     e.F[0] = k[0] * e.C[0] + R(0) * e.C[1] + R(0) * e.C[2];
@@ -611,6 +647,17 @@ __device__ __forceinline__ void evalF(const R *k, const R *lam, R mu, Eval<R> &e
   }
   e.F[0] = e.F[0] + lam[0]; e.F[1] = e.F[1] + lam[1]; e.F[2] = e.F[2] + lam[2];
   e.F[0] = jmin(e.F[0], R(0));
+  if (law != LAW_CPU) {
+    const R fs = xsqrt(e.F[1] * e.F[1] + e.F[2] * e.F[2]);
+    const R active = xabs(e.F[0]) * (stick ? mu : muD);
+    e.within = fs <= active;
+    if (!e.within && fs > R(0)) {
+      const R dyn = xabs(e.F[0]) * muD;
+      e.F[1] = e.F[1] * dyn / fs;
+      e.F[2] = e.F[2] * dyn / fs;
+    }
+    return;
+  }
   const R bounds = xabs(e.F[0]) * mu;
   const R fs = xsqrt(e.F[1] * e.F[1] + e.F[2] * e.F[2]);
   if (fs > bounds && fs > R(0)) {
@@ -622,20 +669,29 @@ __device__ __forceinline__ void evalF(const R *k, const R *lam, R mu, Eval<R> &e
 
 template <class R, bool EXACT>
 __device__ __forceinline__ void evalContact(const R *bas, const R *negb, R mu, const R *hot, const R *dqA,
-                                            const R *dqB, R alpha, Eval<R> &e) {
-  evalC<R, EXACT>(bas, negb, hot, dqA, dqB, alpha, e);
-  evalF<R, EXACT>(hot + HK, hot + HL, mu, e);
+                                            const R *dqB, R alpha, Eval<R> &e, int law = LAW_CPU, int stick = 0) {
+  evalC<R, EXACT>(bas, negb, hot, dqA, dqB, alpha, e, law);
+  evalF<R, EXACT>(hot + HK, hot + HL, mu, e, law, bas[BMUD], stick);
 }
 
 // Manifold.updateDual for one contact (manifold.js:447-491) from its
 // evaluation e with the current k: the new k and lambda into kl[6] (k then
 // lambda), and the stick flag to store, or -1 to leave it as it is.
+// Under the GPU law the contact sticks exactly when its force is inside the
+// cone (shaders.js:2248-2258).
 template <class R>
 __device__ __forceinline__ int dualUpdate(const R *k, const Eval<R> &e, R beta, R pmax, R stickT, bool hasSphere,
-                                          R *kl) {
+                                          R *kl, int law = LAW_CPU) {
   kl[0] = k[0]; kl[1] = k[1]; kl[2] = k[2];
   kl[3] = e.F[0]; kl[4] = e.F[1]; kl[5] = e.F[2];
   if (e.F[0] < R(0)) kl[0] = jmin(k[0] + beta * xabs(e.C[0]), pmax);
+  if (law != LAW_CPU) {
+    if (e.within) {
+      kl[1] = jmin(k[1] + beta * xabs(e.C[1]), pmax);
+      kl[2] = jmin(k[2] + beta * xabs(e.C[2]), pmax);
+    }
+    return e.within ? 1 : 0;
+  }
   int stick = -1;
   if (e.within) {
     kl[1] = jmin(k[1] + beta * xabs(e.C[1]), pmax);
@@ -837,30 +893,50 @@ __device__ __forceinline__ void fcross(R *o, const R *a, const R *b) {
 // one vector: (rAw x B_r) . wA = B_r . (wA x rAw) and (B_r x rBw) . wB =
 // B_r . (rBw x wB), so every row is B_r . v.
 template <class R>
-__device__ __forceinline__ void fevalC(const R *B, const R *hot, const R *dqA, const R *dqB, R alpha, FEval<R> &e) {
+__device__ __forceinline__ void fevalC(const R *B, const R *hot, const R *dqA, const R *dqB, R alpha, FEval<R> &e,
+                                       int law = LAW_CPU) {
   R wa[3], wb[3];
   fcross(wa, dqA + 3, hot + HRA);
   fcross(wb, hot + HRB, dqB + 3);
 #pragma unroll
   for (int k = 0; k < 3; k++) e.v[k] = (dqA[k] - dqB[k]) + (wa[k] + wb[k]);
-  fevalCAlpha(B, hot, alpha, e);
+  fevalCAlpha(B, hot, alpha, e, law);
 }
 
 // C from the stored v under another alpha.
 template <class R>
-__device__ __forceinline__ void fevalCAlpha(const R *B, const R *hot, R alpha, FEval<R> &e) {
+__device__ __forceinline__ void fevalCAlpha(const R *B, const R *hot, R alpha, FEval<R> &e, int law = LAW_CPU) {
   const R oma = R(1) - alpha;
+  R base[3] = {oma * hot[HC0], oma * hot[HC0 + 1], oma * hot[HC0 + 2]};
+  if (law != LAW_CPU) {
+    if (B[BCD] > R(0) && hot[HC0] > R(0)) base[0] = hot[HC0];
+    if (law == LAW_GPU) base[0] = base[0] - hot[HB];
+  }
 #pragma unroll
   for (int r = 0; r < 3; r++)
-    e.C[r] = fmaR(B[r * 3], e.v[0], fmaR(B[r * 3 + 1], e.v[1], fmaR(B[r * 3 + 2], e.v[2], oma * hot[HC0 + r])));
+    e.C[r] = fmaR(B[r * 3], e.v[0], fmaR(B[r * 3 + 1], e.v[1], fmaR(B[r * 3 + 2], e.v[2], base[r])));
 }
 
 template <class R>
-__device__ __forceinline__ void fevalF(const R *k, const R *lam, R mu, FEval<R> &e) {
+__device__ __forceinline__ void fevalF(const R *k, const R *lam, R mu, FEval<R> &e, int law = LAW_CPU,
+                                       R muD = R(0), int stick = 0) {
   const R z0 = fmaR(k[0], e.C[0], lam[0]);
   const R z1 = fmaR(k[1], e.C[1], lam[1]);
   const R z2 = fmaR(k[2], e.C[2], lam[2]);
   const R f0 = z0 < R(0) ? z0 : R(0);
+  if (law != LAW_CPU) {
+    // the GPU law: static cone while sticking, a force outside it scaled
+    // onto the dynamic cone
+    const R active = -f0 * (stick ? mu : muD), dyn = -f0 * muD;
+    const R fs2 = fmaR(z1, z1, z2 * z2);
+    e.F[0] = f0;
+    e.within = fs2 <= active * active;
+    const R tiny = sizeof(R) == 4 ? R(1.1754944e-38f) : R(2.2250738585072014e-308);
+    const R scale = e.within ? R(1) : (dyn > R(0) ? dyn * rsqrtR(fs2 > tiny ? fs2 : tiny) : R(0));
+    e.F[1] = z1 * scale;
+    e.F[2] = z2 * scale;
+    return;
+  }
   const R bounds = -f0 * mu;
   const R fs2 = fmaR(z1, z1, z2 * z2);
   e.F[0] = f0;
@@ -928,10 +1004,17 @@ __device__ __forceinline__ void fstamp(Sys<A> &sy, const R *B, R s, const R *rW,
 // The dual update of a contact (dualUpdate) for the fused evaluation.
 template <class R>
 __device__ __forceinline__ int fdualUpdate(const R *k, const FEval<R> &e, R beta, R pmax, R stickT, bool hasSphere,
-                                           R *kl) {
+                                           R *kl, int law = LAW_CPU) {
   kl[0] = k[0]; kl[1] = k[1]; kl[2] = k[2];
   kl[3] = e.F[0]; kl[4] = e.F[1]; kl[5] = e.F[2];
   if (e.F[0] < R(0)) kl[0] = jmin(fmaR(beta, xabs(e.C[0]), k[0]), pmax);
+  if (law != LAW_CPU) {
+    if (e.within) {
+      kl[1] = jmin(fmaR(beta, xabs(e.C[1]), k[1]), pmax);
+      kl[2] = jmin(fmaR(beta, xabs(e.C[2]), k[2]), pmax);
+    }
+    return e.within ? 1 : 0;
+  }
   int stick = -1;
   if (e.within) {
     kl[1] = jmin(fmaR(beta, xabs(e.C[1]), k[1]), pmax);

@@ -19,6 +19,11 @@ namespace avbd {
 struct StepParams {
   double dt, gravity, betaLin, betaAng, alpha, gamma, pmin, pmax, margin, stick;
   int iterations, postStab, rotated;
+  // the GPU contact model (shaders.js:1105-1367): on or off, the impact speed
+  // below which no restitution target is set, the warm-start persistence
+  // distance (solver.js:155-162)
+  int contactModel;
+  double restThreshold, persistence;
 };
 
 // ------------------------------------------------------------------ utils
@@ -161,6 +166,7 @@ struct BpArgs {
   int pairCap;
   const int4 *entryCell;                     // cell of each bucket entry
   int *scratch;                              // [n * BP_CAP] per-body sorted lists
+  const double *skin;                        // [n] contact offsets (GPU contact model), or null
 };
 
 // The predicate of solver.js:316-329 for A > B, plus the two flags: is the
@@ -170,7 +176,9 @@ __device__ __forceinline__ int pairTest(const BpArgs &a, int A, int B) {
   const BodyProp &pa = a.props[A], &pb = a.props[B];
   const double *xa = a.X + size_t(A) * XS, *xb = a.X + size_t(B) * XS;
   const double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
-  const double r = pa.radius + pb.radius;
+  double r = pa.radius + pb.radius;
+  // the GPU contact model grows the spheres by the pair's skin (shaders.js:440-441)
+  if (a.skin) r = r + (a.skin[A] + a.skin[B]);
   if (!(dx * dx + dy * dy + dz * dz <= r * r)) return 0;
   if (!(pa.mass > 0 || pb.mass > 0)) return 0;
   // look the link up from the side with the shorter force list
@@ -190,7 +198,8 @@ __device__ __forceinline__ int pairTestList(const BpArgs &a, int A, int B, const
   const BodyProp &pa = a.props[A], &pb = a.props[B];
   const double *xa = a.X + size_t(A) * XS, *xb = a.X + size_t(B) * XS;
   const double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
-  const double r = pa.radius + pb.radius;
+  double r = pa.radius + pb.radius;
+  if (a.skin) r = r + (a.skin[A] + a.skin[B]);
   if (!(dx * dx + dy * dy + dz * dz <= r * r)) return 0;
   if (!(pa.mass > 0 || pb.mass > 0)) return 0;
   uint32_t kind = 0;
@@ -404,11 +413,15 @@ __global__ void k_bp_large(BpArgs a, const int *largeList) {
 
 // ------------------------------------------------------------ narrow phase
 // Manifold.initialize (manifold.js:214-318) for every old manifold (t < nOld)
-// and every new pair, written to slot t of the other buffer.
+// and every new pair, written to slot t of the other buffer. Under the GPU
+// contact model (P.contactModel, mat and V given) it follows the reference
+// GPU backend's narrowphase (shaders.js:1105-1367) instead: pair material,
+// the contact skin, the geometric warm-start fallback, the rest distance and
+// the restitution target.
 template <class R>
 __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
                          const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal,
-                         int *srcSlot) {
+                         int *srcSlot, const BodyMat *mat, const double *V) {
   // clip polygons in LDS, one lane-interleaved column per thread (64 threads)
   __shared__ double poly[2 * POLY_LDS * 3 * LDS_LANES];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,9 +446,23 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     for (int k = 0; k < 9; k++) basis[k] = double(ob.basis[size_t(m) * BAS + k]);
   else
     for (int k = 0; k < 9; k++) basis[k] = 0;
-  if (hasSphere && hadContacts)
+  const bool gpu = P.contactModel != 0;
+  if ((hasSphere || gpu) && hadContacts)
     for (int k = 0; k < 9; k++) oldBasis[k] = basis[k];
-  const double mu = xsqrt(pa.friction * pb.friction);
+  // pair material: the CPU engine's one coefficient, or the GPU model's
+  // static and dynamic coefficients, restitution, skin and rest distance
+  double mu, muD = 0, cdist = 0, rdist = 0, rest = 0;
+  if (gpu) {
+    const BodyMat &ma = mat[a], &mm = mat[b];
+    mu = xsqrt(ma.staticFriction * mm.staticFriction);
+    muD = xsqrt(ma.dynamicFriction * mm.dynamicFriction);
+    cdist = ma.contactOffset + mm.contactOffset;
+    rdist = ma.restOffset + mm.restOffset;
+    rest = jmax(ma.restitution, mm.restitution);
+  } else {
+    mu = xsqrt(pa.friction * pb.friction);
+    muD = mu;
+  }
 
   // the old contacts' feature keys, read once for every new contact's match
   // (requested before the collision test, which hides their latency)
@@ -444,11 +471,23 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
   for (int j = 0; j < MAXC; j++) ofeat[j] = j < ncOld ? ob.feat[size_t(m) * MAXC + j] : 0u;
   // the new contacts go straight to this manifold's output slots
   const size_t nbase = size_t(t) * MAXC;
-  const int n = collide(A, B, nb.anchor + nbase * 6, nb.feat + nbase,
-                        reinterpret_cast<AliasDouble *>(nb.hot + nbase * HOT), int(HOT * sizeof(R) / sizeof(double)),
-                        basis, poly + threadIdx.x);
+  AliasDouble *mids = reinterpret_cast<AliasDouble *>(nb.hot + nbase * HOT);
+  constexpr int MID_STRIDE = int(HOT * sizeof(R) / sizeof(double));
+  const int n = collide(A, B, nb.anchor + nbase * 6, nb.feat + nbase, mids, MID_STRIDE, basis, poly + threadIdx.x,
+                        cdist);
   const bool compatible =
       !hasSphere || !hadContacts || (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
+  // GPU model: an old contact matches by feature key or geometrically, each
+  // old contact once (shaders.js:1174-1319)
+  const bool basisCompatible =
+      hadContacts && (oldBasis[0] * basis[0] + oldBasis[1] * basis[1] + oldBasis[2] * basis[2]) > 0.85;
+  double persistence = 0;
+  if (gpu) {
+    const double minExtent = 2.0 * jmin(jmin(pa.hx, jmin(pa.hy, pa.hz)), jmin(pb.hx, jmin(pb.hy, pb.hz)));
+    persistence = jmax(P.persistence, jmax(cdist, minExtent * 0.02));
+  }
+  const double persistenceSq = persistence * persistence;
+  uint32_t oldUsed = 0;
 
   // merged per-contact state: anchors, k, lambda, C, stick
   for (int i = 0; i < n; i++) {
@@ -457,13 +496,46 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     double rB[3] = {nca[3], nca[4], nca[5]};
     const uint32_t feature = nb.feat[nbase + i];
     double kk[3] = {0, 0, 0}, lam[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
-    int stick = 0, src = -1;
-    // the first old contact with this feature key (manifold.js:241-286)
+    int stick = gpu ? 1 : 0, src = -1;  // a new GPU-model contact starts static
+    bool found = false, priorOutsideRest = false, priorHadBias = false;
     int match = -1;
+    if (!gpu) {
+      // the first old contact with this feature key (manifold.js:241-286)
 #pragma unroll
-    for (int j = MAXC - 1; j >= 0; j--)
-      if (j < ncOld && ofeat[j] == feature) match = j;
-    if (compatible && match >= 0) {
+      for (int j = MAXC - 1; j >= 0; j--)
+        if (j < ncOld && ofeat[j] == feature) match = j;
+      if (!compatible) match = -1;
+    } else if (basisCompatible) {
+      // this contact's midpoint, from the merge test's record in its own
+      // (not yet written) slot
+      const AliasDouble *mp = mids + i * MID_STRIDE;
+      const double mx = mp[0], my = mp[1], mz = mp[2];
+      bool nearOne = false;
+      double best = persistenceSq;
+      for (int j = 0; j < ncOld; j++) {
+        if (oldUsed & (1u << j)) continue;
+        const bool same = ofeat[j] == feature;
+        if (hasSphere && same) { match = j; break; }
+        const double *oa = ob.anchor + (size_t(m) * MAXC + j) * 6;
+        double oxA[3], oxB[3], od[3], tg[3];
+        xform(oxA, A.p, A.q, oa);
+        xform(oxB, B.p, B.q, oa + 3);
+        for (int k = 0; k < 3; k++) od[k] = oxA[k] - oxB[k];
+        const double sep = basis[0] * od[0] + basis[1] * od[1] + basis[2] * od[2];
+        for (int k = 0; k < 3; k++) tg[k] = od[k] - basis[k] * sep;
+        if (sep > cdist + persistence) continue;
+        if (tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2] > persistenceSq) continue;
+        const double dx = mx - (oxA[0] + oxB[0]) * 0.5, dy = my - (oxA[1] + oxB[1]) * 0.5,
+                     dz = mz - (oxA[2] + oxB[2]) * 0.5;
+        const double md = dx * dx + dy * dy + dz * dz;
+        if (md > persistenceSq) continue;
+        if (same) { match = j; break; }
+        if (!nearOne || md < best) { nearOne = true; match = j; best = md; }
+      }
+      if (match >= 0) oldUsed |= 1u << match;
+    }
+    if (match >= 0) {
+      found = true;
       const size_t os = size_t(m) * MAXC + match;
       {
         src = int(os);
@@ -475,7 +547,9 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
         }
         const int oldStick = ob.stick[os];
         stick = oldStick;
-        if (hasSphere) {
+        priorOutsideRest = oh[HC0] > R(0);
+        priorHadBias = oh[HB] > R(0);
+        if (hasSphere || gpu) {
           // transportComponents / transportPenalty (manifold.js:73-162)
           const double x = lam[0], y = lam[1], z = lam[2];
           const double px = kk[0], py = kk[1], pz = kk[2];
@@ -507,7 +581,28 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     xform(xb, B.p, B.q, rB);
     d[0] = xa[0] - xb[0]; d[1] = xa[1] - xb[1]; d[2] = xa[2] - xb[2];
     mulv(C0, basis, d);
-    C0[0] += P.margin;
+    double bias = 0;
+    if (gpu) {
+      // C0 against the rest distance; the restitution target of a new impact
+      // (or of one a speculative-only contact kept armed) whose closing
+      // motion crosses the rest surface this step (shaders.js:1322-1350)
+      C0[0] = C0[0] - rdist;
+      if ((!found || (priorOutsideRest && !priorHadBias)) && rest > 0.0) {
+        const double *va = V + size_t(a) * VS, *vb = V + size_t(b) * VS;
+        double ra[3], rb[3], ca[3], cb[3], rel[3];
+        qrot(ra, A.q, rA);
+        qrot(rb, B.q, rB);
+        cross3(ca, va + 3, ra);
+        cross3(cb, vb + 3, rb);
+        for (int k = 0; k < 3; k++) rel[k] = (va[k] + ca[k]) - (vb[k] + cb[k]);
+        const double vn = basis[0] * rel[0] + basis[1] * rel[1] + basis[2] * rel[2];
+        const double closing = jmax(0.0, -vn * P.dt);
+        const double toRest = jmax(0.0, C0[0]);
+        if (vn < -P.restThreshold && closing > toRest) bias = rest * (closing - toRest);
+      }
+    } else {
+      C0[0] += P.margin;
+    }
     if (!P.postStab) {
       for (int k = 0; k < 3; k++) lam[k] = lam[k] * P.alpha;
       for (int k = 0; k < 3; k++) lam[k] = lam[k] * P.gamma;
@@ -524,7 +619,7 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
       h[HC0 + k] = R(C0[k]); h[HK + k] = R(kk[k]); h[HL + k] = R(lam[k]);
       h[HRA + k] = R(rAw[k]); h[HRB + k] = R(rBw[k]);
     }
-    h[15] = R(0);
+    h[HB] = R(bias);
     double *an = nb.anchor + s * 6;
     for (int k = 0; k < 3; k++) { an[k] = rA[k]; an[3 + k] = rB[k]; }
     nb.stick[s] = stick;
@@ -534,9 +629,9 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
   }
   nb.head[t] = make_int4(a, b, n, hasSphere ? int(MF_SPHERE) : 0);
   for (int k = 0; k < 9; k++) nb.basis[size_t(t) * BAS + k] = R(basis[k]);
-  nb.basis[size_t(t) * BAS + 9] = R(mu);
-  nb.basis[size_t(t) * BAS + 10] = R(0);
-  nb.basis[size_t(t) * BAS + 11] = R(0);
+  nb.basis[size_t(t) * BAS + BMU] = R(mu);
+  nb.basis[size_t(t) * BAS + BMUD] = R(gpu ? muD : 0.0);
+  nb.basis[size_t(t) * BAS + BCD] = R(cdist);
   keep[t] = n > 0 ? 1 : 0;
   if (n > 0) atomicAdd(contactTotal, n);
 }
@@ -1363,6 +1458,8 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
   }
   const R ralpha = R(alpha), rdalpha = R(dualAlpha);
   const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
+  const int lawP = contactLaw(P.contactModel, P.postStab, alpha);
+  const int lawD = contactLaw(P.contactModel, P.postStab, dualAlpha);
   constexpr int IB = PRIMAL_BATCH;
   for (int t0 = b0 + lane; t0 < b1; t0 += TEAM * IB) {
     uint2 itw[IB];
@@ -1387,6 +1484,8 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
       vload<8>(dqo, DQ + size_t(iw.y & ITEM_BODY) * 8);
       if constexpr (!EXACT && sizeof(R) == 4)  // f64 records leave no registers for it
         if (k == 0 && lane == 0 && !inDone) inertia();
+      // the GPU contact law's cone depends on the stick flag
+      int stk = P.contactModel ? mb.stick[si] : 0;
       if constexpr (!EXACT) {
         const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
         R rW[3];
@@ -1394,18 +1493,18 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
         for (int q = 0; q < 3; q++) rW[q] = isA ? hot[HRA + q] : hot[HRB + q];
         FEval<R> e;
         if (dual && (iw.y & ITEM_FIRST)) {
-          fevalC(bas, hot, dA, dB, rdalpha, e);
-          fevalF(hot + HK, hot + HL, bas[9], e);
+          fevalC(bas, hot, dA, dB, rdalpha, e, lawD);
+          fevalF(hot + HK, hot + HL, bas[BMU], e, lawD, bas[BMUD], stk);
           R kl[6];
-          const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+          const int st = fdualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl, lawD);
           vstoreKL(mb.hot + si * HOT, kl);
-          if (st >= 0) mb.stick[si] = st;
-          if (rdalpha != ralpha) fevalCAlpha(bas, hot, ralpha, e);
-          fevalF(kl, kl + 3, bas[9], e);
+          if (st >= 0) { mb.stick[si] = st; stk = st; }
+          if (rdalpha != ralpha) fevalCAlpha(bas, hot, ralpha, e, lawP);
+          fevalF(kl, kl + 3, bas[BMU], e, lawP, bas[BMUD], stk);
           fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, kl, e.F);
         } else {
-          fevalC(bas, hot, dA, dB, ralpha, e);
-          fevalF(hot + HK, hot + HL, bas[9], e);
+          fevalC(bas, hot, dA, dB, ralpha, e, lawP);
+          fevalF(hot + HK, hot + HL, bas[BMU], e, lawP, bas[BMUD], stk);
           fstamp<R, Acc>(sl, bas, isA ? R(1) : R(-1), rW, hot + HK, e.F);
         }
         continue;
@@ -1415,17 +1514,17 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
       const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
       Eval<R> e;
       if (dual && (iw.y & ITEM_FIRST)) {
-        evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, rdalpha, e);
+        evalContact<R, EXACT>(bas, negb, bas[BMU], hot, dA, dB, rdalpha, e, lawD, stk);
         R kl[6];
-        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl, lawD);
         vstoreKL(mb.hot + si * HOT, kl);
-        if (st >= 0) mb.stick[si] = st;
+        if (st >= 0) { mb.stick[si] = st; stk = st; }
 #pragma unroll
         for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];  // k then lambda
-        if (rdalpha != ralpha) evalC<R, EXACT>(bas, negb, hot, dA, dB, ralpha, e);
-        evalF<R, EXACT>(hot + HK, hot + HL, bas[9], e);
+        if (rdalpha != ralpha) evalC<R, EXACT>(bas, negb, hot, dA, dB, ralpha, e, lawP);
+        evalF<R, EXACT>(hot + HK, hot + HL, bas[BMU], e, lawP, bas[BMUD], stk);
       } else {
-        evalContact<R, EXACT>(bas, negb, bas[9], hot, dA, dB, ralpha, e);
+        evalContact<R, EXACT>(bas, negb, bas[BMU], hot, dA, dB, ralpha, e, lawP, stk);
       }
       stampContact<R, Acc>(sl, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
     }
@@ -1577,6 +1676,8 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
                                              int dual, double dualAlpha) {
   const double dt2 = P.dt * P.dt;
   const double beta = P.betaLin, pmax = P.pmax, stickT = P.stick;
+  const int lawP = contactLaw(P.contactModel, P.postStab, alpha);
+  const int lawD = contactLaw(P.contactModel, P.postStab, dualAlpha);
   const int i = w.x, b0 = w.y, b1 = w.z;
   const BodyProp &bp = S.props[i];
   double *x = S.X + size_t(i) * XS;
@@ -1615,18 +1716,19 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
       for (int e = 0; e < 9; e++) negb[e] = -bas[e];
       const double *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
       Eval<double> e;
+      int stk = P.contactModel ? mb.stick[si] : 0;
       if (dual && (iw.y & ITEM_FIRST)) {
-        evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, dualAlpha, e);
+        evalContact<double, true>(bas, negb, bas[BMU], hot, dA, dB, dualAlpha, e, lawD, stk);
         double kl[6];
-        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl);
+        const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, iw.y & ITEM_SPHERE, kl, lawD);
         vstoreKL(mb.hot + si * HOT, kl);
-        if (st >= 0) mb.stick[si] = st;
+        if (st >= 0) { mb.stick[si] = st; stk = st; }
 #pragma unroll
         for (int q = 0; q < 6; q++) hot[HK + q] = kl[q];
-        if (dualAlpha != alpha) evalC<double, true>(bas, negb, hot, dA, dB, alpha, e);
-        evalF<double, true>(hot + HK, hot + HL, bas[9], e);
+        if (dualAlpha != alpha) evalC<double, true>(bas, negb, hot, dA, dB, alpha, e, lawP);
+        evalF<double, true>(hot + HK, hot + HL, bas[BMU], e, lawP, bas[BMUD], stk);
       } else {
-        evalContact<double, true>(bas, negb, bas[9], hot, dA, dB, alpha, e);
+        evalContact<double, true>(bas, negb, bas[BMU], hot, dA, dB, alpha, e, lawP, stk);
       }
       double o[XT_TERMS];
       contactTerms(o, isA ? bas : negb, isA ? e.jA : e.jB, hot + HK, e.F);
@@ -1790,6 +1892,7 @@ __global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *li
                        int writeC) {
   const int nLive = *nLivePtr;
   const R ralpha = R(alpha), beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
+  const int law = contactLaw(P.contactModel, P.postStab, alpha);
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nLive; k += gridDim.x * blockDim.x) {
     const int m = live[k];
     const int4 h = mb.head[m];
@@ -1805,13 +1908,13 @@ __global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *li
       R hot[HOT];
       vload<HOT>(hot, hp);
       Eval<R> e;
-      evalContact<R, EXACT>(bas, negb, bas[9], hot, dqA, dqB, ralpha, e);
+      evalContact<R, EXACT>(bas, negb, bas[BMU], hot, dqA, dqB, ralpha, e, law, P.contactModel ? mb.stick[si] : 0);
       if (writeC) {
         R *cd = mb.Cd + si * 4;
         cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];
       }
       R kl[6];
-      const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, hasSphere, kl);
+      const int st = dualUpdate(hot + HK, e, beta, pmax, stickT, hasSphere, kl, law);
       if (st >= 0) mb.stick[si] = st;
       vstoreKL(hp, kl);
     }

@@ -206,6 +206,12 @@ static napi_value js_set_params(napi_env env, napi_callback_info info) {
   p.rotated_inertia = (int32_t)get_num(env, o, "rotatedInertia", p.rotated_inertia);
   p.cached_contact_jacobians = (int32_t)get_num(env, o, "cachedContactJacobians", p.cached_contact_jacobians);
   p.color_order = (int32_t)get_num(env, o, "colorOrder", p.color_order);
+  /* the GPU contact law (solver.js:142-162) */
+  p.contact_model = (int32_t)get_num(env, o, "contactModel", p.contact_model);
+  p.contact_offset = get_num(env, o, "contactOffset", p.contact_offset);
+  p.rest_offset = get_num(env, o, "restOffset", p.rest_offset);
+  p.restitution_threshold = get_num(env, o, "restitutionThreshold", p.restitution_threshold);
+  p.contact_persistence = get_num(env, o, "contactPersistence", p.contact_persistence);
   avbd_ctx *ctx = get_ctx(env, argv[0]);
   if (!ctx) return NULL;
   int rc = avbd_set_params(ctx, &p);
@@ -247,6 +253,12 @@ static napi_value js_pack(napi_env env, napi_callback_info info) {
   ARR(vel, "vel", F64, 3 * N, 1);
   ARR(angvel, "angvel", F64, 3 * N, 1);
   ARR(scene_of, "sceneOf", I32, N, 1);
+  /* the GPU contact law's material (Rigid.setMaterial, rigid.js:144-214) */
+  ARR(static_friction, "staticFriction", F64, N, 1);
+  ARR(dynamic_friction, "dynamicFriction", F64, N, 1);
+  ARR(restitution, "restitution", F64, N, 1);
+  ARR(contact_offset, "contactOffset", F64, N, 1);
+  ARR(rest_offset, "restOffset", F64, N, 1);
   if (!arr_arg(env, get_prop(env, o, "jointA"), I32, "jointA", -1, 1, &d, &nj)) return NULL;
   s.joint_a = d;
   if (s.joint_a) {
@@ -639,9 +651,12 @@ static napi_value new_typed(napi_env env, napi_typedarray_type type, size_t es, 
 
 /* readManifolds(ctx) -> {head Int32Array (4 per manifold: A, B, contacts,
  * flags), basisMu Float64Array (10 per manifold), contacts Float64Array (16 per
- * contact), features Uint32Array (8 per manifold)}: every live manifold in
- * force-list order (avbd_read_manifolds, avbd_read_forces), for
- * GpuBackend.readColoring's slots (backend.js:2053-2151). */
+ * contact), features Uint32Array (8 per manifold), material Float64Array (3
+ * per manifold: static, dynamic friction, contact distance), bias Float64Array
+ * (1 per contact: the restitution target)}: every live manifold in
+ * force-list order (avbd_read_manifolds, avbd_read_forces,
+ * avbd_read_contact_model), for GpuBackend.readColoring's slots
+ * (backend.js:2053-2151). */
 static napi_value js_read_manifolds(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1] = {NULL};
@@ -653,19 +668,23 @@ static napi_value js_read_manifolds(napi_env env, napi_callback_info info) {
   if (rc) return throw_avbd(env, ctx, rc);
   rc = avbd_read_forces(ctx, NULL, NULL, 0, &nf);
   if (rc) return throw_avbd(env, ctx, rc);
-  void *ph, *pb, *pc, *pk, *pf;
+  void *ph, *pb, *pc, *pk, *pf, *pm, *pr;
   napi_value head = new_typed(env, napi_int32_array, 4, (size_t)m * 4, &ph);
   napi_value bm = new_typed(env, napi_float64_array, 8, (size_t)m * 10, &pb);
   napi_value con = new_typed(env, napi_float64_array, 8, (size_t)c * 16, &pc);
   napi_value kab = new_typed(env, napi_int32_array, 4, (size_t)nf * 4, &pk);
   napi_value ft = new_typed(env, napi_uint32_array, 4, (size_t)nf * 8, &pf);
-  if (!head || !bm || !con || !kab || !ft) {
+  napi_value mat = new_typed(env, napi_float64_array, 8, (size_t)m * 3, &pm);
+  napi_value bias = new_typed(env, napi_float64_array, 8, (size_t)c, &pr);
+  if (!head || !bm || !con || !kab || !ft || !mat || !bias) {
     napi_throw_error(env, NULL, "out of host memory");
     return NULL;
   }
   rc = avbd_read_manifolds(ctx, (int32_t *)ph, (double *)pb, (double *)pc, m, c, &m, &c);
   if (rc) return throw_avbd(env, ctx, rc);
   rc = avbd_read_forces(ctx, (int32_t *)pk, (uint32_t *)pf, nf, &nf);
+  if (rc) return throw_avbd(env, ctx, rc);
+  rc = avbd_read_contact_model(ctx, (double *)pm, (double *)pr, m, c);
   if (rc) return throw_avbd(env, ctx, rc);
   napi_value o;
   CHECK(env, napi_create_object(env, &o));
@@ -674,6 +693,8 @@ static napi_value js_read_manifolds(napi_env env, napi_callback_info info) {
   CHECK(env, napi_set_named_property(env, o, "contacts", con));
   CHECK(env, napi_set_named_property(env, o, "forces", kab));
   CHECK(env, napi_set_named_property(env, o, "features", ft));
+  CHECK(env, napi_set_named_property(env, o, "material", mat));
+  CHECK(env, napi_set_named_property(env, o, "bias", bias));
   return o;
 }
 

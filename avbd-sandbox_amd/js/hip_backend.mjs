@@ -123,7 +123,11 @@ export class HipBackend {
       shape: new Int32Array(n), size: new Float64Array(n * 3), density: new Float64Array(n),
       friction: new Float64Array(n), pos: new Float64Array(n * 3), quat: new Float64Array(n * 4),
       vel: new Float64Array(n * 3), angvel: new Float64Array(n * 3),
+      // the GPU contact law's material (Rigid.setMaterial, rigid.js:144-214)
+      staticFriction: new Float64Array(n), dynamicFriction: new Float64Array(n), restitution: new Float64Array(n),
+      contactOffset: new Float64Array(n), restOffset: new Float64Array(n),
     };
+    const num = (v, d) => (typeof v === 'number' ? v : d);
     this.bodyIndex = new Map();
     bodies.forEach((b, i) => {
       this.bodyIndex.set(b, i);
@@ -135,6 +139,11 @@ export class HipBackend {
       A.quat.set(b.positionAng, i * 4);
       A.vel.set(b.velocityLin, i * 3);
       A.angvel.set(b.velocityAng, i * 3);
+      A.staticFriction[i] = num(b.staticFriction, b.friction);
+      A.dynamicFriction[i] = num(b.dynamicFriction, b.friction);
+      A.restitution[i] = num(b.restitution, 0);
+      A.contactOffset[i] = num(b.contactOffset, NaN);  // null: the solver's
+      A.restOffset[i] = num(b.restOffset, NaN);
     });
     // joints and springs (spring.js: stiffness, rest, tearStrain), in the
     // force list's creation order
@@ -213,7 +222,14 @@ export class HipBackend {
       stickThreshold: solver.stickThreshold, postStabilize: solver.postStabilize ? 1 : 0,
       rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
       colorOrder: solver.colorOrder === 'priority' || solver.colorOrder === 1 ? 1 : 0,
+      // the GPU backend's contact law when the solver asks for it (this
+      // backend's addition: contactModel 'gpu'), with the solver's offsets
+      contactModel: solver.contactModel === 'gpu' || solver.contactModel === 1 ? 1 : 0,
+      contactOffset: num(solver.contactOffset, 0.005), restOffset: num(solver.restOffset, -0.005),
+      restitutionThreshold: num(solver.restitutionThreshold, 1.0),
+      contactPersistence: num(solver.contactPersistence, 0.002),
     });
+    this._contactModel = solver.contactModel === 'gpu' || solver.contactModel === 1 ? 1 : 0;
     if (oldIndex) addon.pack(this.ctx, A, oldIndex);
     else addon.pack(this.ctx, A);
     this._packed = A;
@@ -455,6 +471,7 @@ export class HipBackend {
     const broadPhasePairs = new Int32Array(count * 2);
     addon.readPairs(this.ctx, broadPhasePairs);
     const M = addon.readManifolds(this.ctx);
+    const contactModel = this._contactModel;
     const m = M.head.length / 4;
     const nj = M.forces.length / 4 - m;
     const contactPairs = [];
@@ -476,11 +493,12 @@ export class HipBackend {
         contacts.push({
           rA: [r[0], r[1], r[2]], rB: [r[3], r[4], r[5]], constraint: [r[6], r[7], r[8]],
           penalty: [r[9], r[10], r[11]], lambda: [r[12], r[13], r[14]], sticking: stick,
-          feature: M.features[(nj + t) * 8 + s], restitutionBias: 0,
+          feature: M.features[(nj + t) * 8 + s], restitutionBias: M.bias[k],
         });
       }
       slotInfo.push({
-        numContacts: nc, chainLen: 1, staticFriction: bm[9], dynamicFriction: bm[9],
+        numContacts: nc, chainLen: 1, staticFriction: M.material[t * 3],
+        dynamicFriction: contactModel ? M.material[t * 3 + 1] : M.material[t * 3],
         basis: { normal: [bm[0], bm[1], bm[2]], tangent1: [bm[3], bm[4], bm[5]], tangent2: [bm[6], bm[7], bm[8]] },
         contacts, stickingContacts: sticking, maxPenalty,
       });

@@ -26,6 +26,13 @@ export class Solver {
     this.stickThreshold = 0.00001;
     this.rotatedInertia = true;
     this.cachedContactJacobians = true;
+    // the GPU backend's contact law (solver.js:142-162); contactModel 'gpu'
+    // selects it on this backend ('cpu', the CPU engine's, by default)
+    this.contactModel = 'cpu';
+    this.contactOffset = 0.005;
+    this.restOffset = -0.005;
+    this.restitutionThreshold = 1.0;
+    this.contactPersistence = 0.002;
   }
 
   /**
@@ -59,6 +66,13 @@ export class Rigid {
     this.size = Float64Array.from(size);
     this.density = density;
     this.friction = friction;
+    // contact material of the GPU law (rigid.js:56-70): null offsets inherit
+    // the solver's
+    this.staticFriction = friction;
+    this.dynamicFriction = friction;
+    this.restitution = 0;
+    this.contactOffset = null;
+    this.restOffset = null;
     this.positionLin = Float64Array.from(position);
     this.positionAng = Float64Array.from([0, 0, 0, 1]);
     this.velocityLin = Float64Array.from(velocity);
@@ -81,6 +95,46 @@ export class Rigid {
 
   get isStatic() {
     return this.mass <= 0;
+  }
+
+  /**
+   * Rigid.setMaterial (rigid.js:144-214): the whole candidate material is
+   * validated before any field changes; `friction` sets both coefficients.
+   */
+  setMaterial({ friction, staticFriction, dynamicFriction, restitution, contactOffset, restOffset } = {}) {
+    const nonNegative = (name, v) => {
+      if (!Number.isFinite(v) || v < 0) throw new RangeError(`${name} must be a finite non-negative number`);
+      return v;
+    };
+    let f = this.friction, sf = this.staticFriction, df = this.dynamicFriction, e = this.restitution;
+    let co = this.contactOffset, ro = this.restOffset;
+    if (friction !== undefined) f = sf = df = nonNegative('friction', friction);
+    if (staticFriction !== undefined) sf = nonNegative('staticFriction', staticFriction);
+    if (dynamicFriction !== undefined) df = nonNegative('dynamicFriction', dynamicFriction);
+    if (restitution !== undefined) {
+      if (!Number.isFinite(restitution) || restitution < 0 || restitution > 1) {
+        throw new RangeError('restitution must be a finite number in [0, 1]');
+      }
+      e = restitution;
+    }
+    if (contactOffset !== undefined) co = contactOffset === null ? null : nonNegative('contactOffset', contactOffset);
+    if (restOffset !== undefined) {
+      if (restOffset !== null && !Number.isFinite(restOffset)) {
+        throw new RangeError('restOffset must be null or a finite number');
+      }
+      ro = restOffset;
+    }
+    const resolvedContact = co === null ? this.solver.contactOffset : co;
+    const resolvedRest = ro === null ? this.solver.restOffset : ro;
+    if (df > sf) throw new RangeError('dynamicFriction must not exceed staticFriction');
+    if (resolvedRest > resolvedContact) throw new RangeError('restOffset must not exceed contactOffset');
+    this.friction = f;
+    this.staticFriction = sf;
+    this.dynamicFriction = df;
+    this.restitution = e;
+    this.contactOffset = co;
+    this.restOffset = ro;
+    return this;
   }
 }
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 3
+#define AVBD_ABI_VERSION 4
 
 /* Error codes. */
 #define AVBD_OK 0
@@ -115,8 +115,24 @@ typedef struct avbd_params {
    * shaders.js:1571-1585), whose dependency chains are short. Either way the
    * colouring is the exact greedy fixed point. */
   int32_t color_order;
-  int32_t reserved;
+  /* Contact law. AVBD_CONTACTS_CPU: the CPU engine's (manifold.js:214-491),
+   * bit for bit in AVBD_PREC_EXACT. AVBD_CONTACTS_GPU: the reference GPU
+   * backend's (shaders.js:1105-1367, 1777-1850, 2229-2264): per-body static
+   * and dynamic friction with a stick-selected cone, a one-step restitution
+   * target for new impacts, speculative contacts inside the pair's contact
+   * skin (the sum of the bodies' contact offsets) with C0 measured against the
+   * rest distance (the sum of their rest offsets) instead of the collision
+   * margin, and warm starts matched geometrically when feature keys change
+   * (the persistence fallback). The fields below configure it. */
+  int32_t contact_model;
+  double contact_offset;        /* 0.005: a body's skin unless it sets its own (solver.js:151) */
+  double rest_offset;           /* -0.005: a body's rest offset unless it sets its own (solver.js:152) */
+  double restitution_threshold; /* 1.0 m/s: slower impacts set no restitution target (solver.js:155) */
+  double contact_persistence;   /* 0.002 m: least warm-start matching distance (solver.js:162) */
 } avbd_params;
+
+#define AVBD_CONTACTS_CPU 0
+#define AVBD_CONTACTS_GPU 1
 
 #define AVBD_ORDER_INDEX 0
 #define AVBD_ORDER_PRIORITY 1
@@ -157,6 +173,21 @@ typedef struct avbd_scene {
    * tears (INFINITY: never). Joints and springs are forces in the order given,
    * which is their creation order. */
   const int32_t *joint_kind;
+  /* The contact material of AVBD_CONTACTS_GPU (Rigid.setMaterial,
+   * rigid.js:144-214), each [n] or NULL: static and dynamic friction (NULL:
+   * the body's friction), restitution in [0, 1] (NULL: 0), the contact offset
+   * and the rest offset (NULL or NaN: the solver's contact_offset /
+   * rest_offset). As setMaterial, pack rejects a negative or non-finite
+   * coefficient, dynamic above static friction, a negative contact offset and
+   * a rest offset above the body's own contact offset. A rest offset resolved
+   * above the resolved contact offset is lowered to it, and a contact offset
+   * is at least 0 (bodyContactSignature, backend.js:156-185). The CPU law
+   * reads friction alone. */
+  const double *static_friction;
+  const double *dynamic_friction;
+  const double *restitution;
+  const double *contact_offset;
+  const double *rest_offset;
 } avbd_scene;
 
 #define AVBD_JOINT_BALL 0
@@ -298,6 +329,14 @@ int avbd_read_manifold(avbd_ctx *ctx, int32_t index, double *basis_mu, double *c
 int avbd_read_manifolds(avbd_ctx *ctx, int32_t *head, double *basis_mu, double *contacts, int32_t cap,
                         int32_t contact_cap, int32_t *count, int32_t *num_contacts);
 
+/* The fields AVBD_CONTACTS_GPU adds to the slots of avbd_read_manifolds,
+ * in the same order (readColoring's slotInfo staticFriction, dynamicFriction
+ * and each contact's restitutionBias, backend.js:2053-2151): per manifold
+ * `material` {static friction, dynamic friction, contact distance} (3
+ * doubles), per contact its restitution target `bias`. Under
+ * AVBD_CONTACTS_CPU they read {friction, 0, 0} and 0. */
+int avbd_read_contact_model(avbd_ctx *ctx, double *material, double *bias, int32_t cap, int32_t contact_cap);
+
 /* --- metrics: GpuBackend.lastStats (backend.js:309-311, 1622-1792) */
 int avbd_read_counters(avbd_ctx *ctx, avbd_counters *out);
 
@@ -376,8 +415,8 @@ int avbd_read_inertial_target(avbd_ctx *ctx, double *pos, double *quat, int32_t 
  * trajectory (a fixed iteration count), with the contact geometry (normals,
  * features, anchors) held piecewise constant and only the normal row of C0
  * differentiated for contacts whose anchors were re-derived that step.
- * Requires AVBD_PREC_EXACT or AVBD_PREC_F64, box contacts, no joints and no
- * post-stabilisation (AVBD_E_STATE otherwise). The forward must start from
+ * Requires AVBD_PREC_EXACT or AVBD_PREC_F64, box contacts, no joints, no
+ * post-stabilisation and AVBD_CONTACTS_CPU (AVBD_E_STATE otherwise). The forward must start from
  * the scene as avbd_pack set it (no step, avbd_write_state or avbd_repack
  * since; AVBD_E_STATE otherwise). A call that changes what the forward
  * stepped through or steps further (avbd_set_params, avbd_step,

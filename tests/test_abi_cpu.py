@@ -29,7 +29,7 @@ def test_abi_version_and_defaults():
     import avbd
     from avbd import _lib as L
     lib = avbd.lib()
-    assert lib.avbd_abi_version() == 3
+    assert lib.avbd_abi_version() == 4
     p = L.Params()
     lib.avbd_default_params(ctypes.byref(p))
     # solver.js:92-236
@@ -37,6 +37,9 @@ def test_abi_version_and_defaults():
     assert p.beta_lin == 1e4 and p.beta_ang == 100.0 and p.alpha == 0.95 and p.gamma == 0.999
     assert p.penalty_min == 1.0 and p.penalty_max == 1e10 and p.collision_margin == 0.01
     assert p.stick_threshold == 1e-5 and p.post_stabilize == 0 and p.rotated_inertia == 1
+    # the GPU contact law's parameters (solver.js:151-162), the CPU law selected
+    assert p.contact_model == L.CONTACTS_CPU and p.contact_offset == 0.005 and p.rest_offset == -0.005
+    assert p.restitution_threshold == 1.0 and p.contact_persistence == 0.002
 
 
 def test_null_and_invalid_arguments():
