@@ -397,6 +397,8 @@ struct avbd_ctx {
   }
 
   size_t rsize() const { return prec == AVBD_PREC_MIXED ? sizeof(float) : sizeof(double); }
+  // the GPU contact law: its kernels are separate instantiations
+  bool gl() const { return prm.contact_model != AVBD_CONTACTS_CPU; }
 
   StepParams sp() const {
     StepParams P;
@@ -558,7 +560,8 @@ struct avbd_ctx {
                     double dualAlpha) {
     if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
     const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
-    timedLaunch(AVBD_K_PRIMAL, bytes, k_primal<R, EXACT, TEAM>, dim3(primalBlocks(c, TEAM)), dim3(256), c,
+    timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal<R, EXACT, TEAM, true> : k_primal<R, EXACT, TEAM, false>,
+                dim3(primalBlocks(c, TEAM)), dim3(256), c,
                 (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
   }
 
@@ -589,13 +592,15 @@ struct avbd_ctx {
       // a team of lanes per body, the additions kept in the reference's
       // order; bodies with joints then take the one-lane sweep
       const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
-      timedLaunch(AVBD_K_PRIMAL, bytes, k_primal_xt, dim3(primalBlocks(c, XT_TEAM)), dim3(256), c,
+      timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal_xt<true> : k_primal_xt<false>,
+                  dim3(primalBlocks(c, XT_TEAM)), dim3(256), c,
                   (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (double *)DQ.p, P, alpha, dual,
                   dualAlpha);
       if (J > 0) {
         SolveArgs sj = sa;
         sj.jointBodiesOnly = 1;
-        timedLaunch(AVBD_K_PRIMAL, 0.0, k_primal<double, true, 1>, dim3(primalBlocks(c, 1)), dim3(256), c,
+        timedLaunch(AVBD_K_PRIMAL, 0.0, gl() ? k_primal<double, true, 1, true> : k_primal<double, true, 1, false>,
+                    dim3(primalBlocks(c, 1)), dim3(256), c,
                     (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sj, mb, (double *)DQ.p, P, alpha, dual,
                     dualAlpha);
       }
@@ -635,9 +640,9 @@ struct avbd_ctx {
       if (it < prm.iterations) {
         const bool last = it == prm.iterations - 1;
         if (last && (graphCapturing || nLive > 0))
-          timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
-                      (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, alpha,
-                      last ? 1 : 0);
+          timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, gl() ? k_dual<R, EXACT, true> : k_dual<R, EXACT, false>,
+                      dim3(dualBlocks()), dim3(128), (const int *)(scal.p + 2), (const int *)live[cur].p, mb,
+                      (const R *)DQ.p, P, alpha, last ? 1 : 0);
         if (J > 0) hipLaunchKernelGGL(k_joint_dual, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P, alpha);
       }
       if (rec && it == prm.iterations - 1) recordSnapshot(-1, it + 1);
@@ -673,13 +678,13 @@ struct avbd_ctx {
         bytes += double(prm.iterations) * double(hcbytes[c]) + double(prm.iterations - 1) * double(hcbytes[MAX_COLORS + c]);
     R *dq = (R *)DQ.p;
     if constexpr (EXACT) {
-      timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, true, 1>, dim3(S), dim3(256), Q, sa, mb, dq, P);
+      timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_solve_scenes<R, true, 1, true> : k_solve_scenes<R, true, 1, false>,
+                  dim3(S), dim3(256), Q, sa, mb, dq, P);
     } else {
-      switch (tm) {
-        case 4: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 4>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
-        case 8: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 8>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
-        default: timedLaunch(AVBD_K_PRIMAL, bytes, k_solve_scenes<R, false, 16>, dim3(S), dim3(256), Q, sa, mb, dq, P); break;
-      }
+      auto k = gl() ? k_solve_scenes<R, false, 16, true> : k_solve_scenes<R, false, 16, false>;
+      if (tm == 4) k = gl() ? k_solve_scenes<R, false, 4, true> : k_solve_scenes<R, false, 4, false>;
+      if (tm == 8) k = gl() ? k_solve_scenes<R, false, 8, true> : k_solve_scenes<R, false, 8, false>;
+      timedLaunch(AVBD_K_PRIMAL, bytes, k, dim3(S), dim3(256), Q, sa, mb, dq, P);
     }
   }
 
@@ -687,8 +692,9 @@ struct avbd_ctx {
   void enqueueScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
     launchScenes<R, EXACT>(sa, mb, P, sceneTeamShape());
     if (graphCapturing || nLive > 0)
-      timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, k_dual<R, EXACT>, dim3(dualBlocks()), dim3(128),
-                  (const int *)(scal.p + 2), (const int *)live[cur].p, mb, (const R *)DQ.p, P, prm.alpha, 1);
+      timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, gl() ? k_dual<R, EXACT, true> : k_dual<R, EXACT, false>,
+                  dim3(dualBlocks()), dim3(128), (const int *)(scal.p + 2), (const int *)live[cur].p, mb,
+                  (const R *)DQ.p, P, prm.alpha, 1);
   }
 
   // The autotuner for the one-launch batch solve: the whole solve of this
@@ -899,7 +905,8 @@ struct avbd_ctx {
     ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
     HIPCHECK(hipMemsetAsync(scal.p + 8, 0, sizeof(int), stream));
     if (nTot > 0)
-      timedLaunch(AVBD_K_NARROW, 0.0, k_narrow<R>, dim3(blocks(nTot, 64)), dim3(64), nOld, nNew,
+      timedLaunch(AVBD_K_NARROW, 0.0, gl() ? k_narrow<R, true> : k_narrow<R, false>, dim3(blocks(nTot, 64)), dim3(64),
+                  nOld, nNew,
                   (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p, (const BodyProp *)props.p,
                   (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p, (const BodyMat *)mat.p,
                   (const double *)V.p);

@@ -418,7 +418,7 @@ __global__ void k_bp_large(BpArgs a, const int *largeList) {
 // GPU backend's narrowphase (shaders.js:1105-1367) instead: pair material,
 // the contact skin, the geometric warm-start fallback, the rest distance and
 // the restitution target.
-template <class R>
+template <class R, bool GL>
 __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
                          const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal,
                          int *srcSlot, const BodyMat *mat, const double *V) {
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
     for (int k = 0; k < 9; k++) basis[k] = double(ob.basis[size_t(m) * BAS + k]);
   else
     for (int k = 0; k < 9; k++) basis[k] = 0;
-  const bool gpu = P.contactModel != 0;
+  constexpr bool gpu = GL;  // the GPU contact law (P.contactModel), compiled in only where launched for it
   if ((hasSphere || gpu) && hadContacts)
     for (int k = 0; k < 9; k++) oldBasis[k] = basis[k];
   // pair material: the CPU engine's one coefficient, or the GPU model's
@@ -1403,7 +1403,9 @@ constexpr int PRIMAL_BATCH = 8;
 // be stiff enough to swamp a light body's inertia in f32.
 // One body's primal update (the body of the team loop below; TEAM lanes of
 // one wave call it together for the same work entry w).
-template <class R, bool EXACT, int TEAM>
+// GL: the GPU contact law (P.contactModel set), a template parameter so that
+// the CPU law's sweep carries none of its code.
+template <class R, bool EXACT, int TEAM, bool GL>
 __device__ __forceinline__ void primalBody(const int4 w, const int lane, const SolveArgs &S, ManBuf<R> mb, R *DQ,
                                            const StepParams &P, double alpha, int dual, double dualAlpha) {
   using Acc = typename std::conditional<EXACT, double, R>::type;
@@ -1458,8 +1460,8 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
   }
   const R ralpha = R(alpha), rdalpha = R(dualAlpha);
   const R beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
-  const int lawP = contactLaw(P.contactModel, P.postStab, alpha);
-  const int lawD = contactLaw(P.contactModel, P.postStab, dualAlpha);
+  const int lawP = GL ? contactLaw(1, P.postStab, alpha) : LAW_CPU;
+  const int lawD = GL ? contactLaw(1, P.postStab, dualAlpha) : LAW_CPU;
   constexpr int IB = PRIMAL_BATCH;
   for (int t0 = b0 + lane; t0 < b1; t0 += TEAM * IB) {
     uint2 itw[IB];
@@ -1485,7 +1487,7 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
       if constexpr (!EXACT && sizeof(R) == 4)  // f64 records leave no registers for it
         if (k == 0 && lane == 0 && !inDone) inertia();
       // the GPU contact law's cone depends on the stick flag
-      int stk = P.contactModel ? mb.stick[si] : 0;
+      int stk = GL ? mb.stick[si] : 0;
       if constexpr (!EXACT) {
         const R *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
         R rW[3];
@@ -1603,7 +1605,7 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
   d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
 }
 
-template <class R, bool EXACT, int TEAM>
+template <class R, bool EXACT, int TEAM, bool GL>
 __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                 ManBuf<R> mb, R *DQ, StepParams P, double alpha, int dual,
                                                 double dualAlpha) {
@@ -1613,7 +1615,7 @@ __global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, con
   for (int team = gt / TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
     if (S.jointBodiesOnly && w.w == 0) continue;
-    primalBody<R, EXACT, TEAM>(w, lane, S, mb, DQ, P, alpha, dual, dualAlpha);
+    primalBody<R, EXACT, TEAM, GL>(w, lane, S, mb, DQ, P, alpha, dual, dualAlpha);
   }
 }
 
@@ -1671,13 +1673,14 @@ constexpr int XT_ROW = 28;    // doubles per item row in LDS (27 used)
 // One body's exact primal update by a team of XT_TEAM lanes of one wave,
 // with L the team's LDS rows (XT_TEAM * XT_ROW doubles); w.w must be 0 (no
 // joint items).
+template <bool GL>
 __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, double *L, const SolveArgs &S,
                                              ManBuf<double> mb, double *DQ, const StepParams &P, double alpha,
                                              int dual, double dualAlpha) {
   const double dt2 = P.dt * P.dt;
   const double beta = P.betaLin, pmax = P.pmax, stickT = P.stick;
-  const int lawP = contactLaw(P.contactModel, P.postStab, alpha);
-  const int lawD = contactLaw(P.contactModel, P.postStab, dualAlpha);
+  const int lawP = GL ? contactLaw(1, P.postStab, alpha) : LAW_CPU;
+  const int lawD = GL ? contactLaw(1, P.postStab, dualAlpha) : LAW_CPU;
   const int i = w.x, b0 = w.y, b1 = w.z;
   const BodyProp &bp = S.props[i];
   double *x = S.X + size_t(i) * XS;
@@ -1716,7 +1719,7 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
       for (int e = 0; e < 9; e++) negb[e] = -bas[e];
       const double *dA = isA ? dqi : dqo, *dB = isA ? dqo : dqi;
       Eval<double> e;
-      int stk = P.contactModel ? mb.stick[si] : 0;
+      int stk = GL ? mb.stick[si] : 0;
       if (dual && (iw.y & ITEM_FIRST)) {
         evalContact<double, true>(bas, negb, bas[BMU], hot, dA, dB, dualAlpha, e, lawD, stk);
         double kl[6];
@@ -1769,6 +1772,7 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
   waveSync();  // row 0 is the next body's
 }
 
+template <bool GL>
 __global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                    ManBuf<double> mb, double *DQ, StepParams P, double alpha,
                                                    int dual, double dualAlpha) {
@@ -1780,7 +1784,7 @@ __global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, 
   for (int team = gt / XT_TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
     if (w.w > 0) continue;  // joints: the one-lane sweep
-    primalBodyXT(w, lane, L, S, mb, DQ, P, alpha, dual, dualAlpha);
+    primalBodyXT<GL>(w, lane, L, S, mb, DQ, P, alpha, dual, dualAlpha);
   }
 }
 
@@ -1836,7 +1840,7 @@ __device__ __forceinline__ int firstBodyAtLeast(const int4 *cwork, int lo, int h
   return lo;
 }
 
-template <class R, bool EXACT, int TEAM>
+template <class R, bool EXACT, int TEAM, bool GL>
 __global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S, ManBuf<R> mb, R *DQ,
                                                      StepParams P) {
   constexpr int T = EXACT ? XT_TEAM : TEAM;
@@ -1869,12 +1873,12 @@ __global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S,
         const int4 w = Q.cwork[t];
         if constexpr (EXACT) {
           if (w.w > 0) {
-            if (lane == 0) primalBody<double, true, 1>(w, 0, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+            if (lane == 0) primalBody<double, true, 1, GL>(w, 0, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
           } else {
-            primalBodyXT(w, lane, rows[team], S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+            primalBodyXT<GL>(w, lane, rows[team], S, mb, DQ, P, Q.alpha, fused, Q.alpha);
           }
         } else {
-          primalBody<R, false, TEAM>(w, lane, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
+          primalBody<R, false, TEAM, GL>(w, lane, S, mb, DQ, P, Q.alpha, fused, Q.alpha);
         }
       }
       __syncthreads();
@@ -1887,12 +1891,12 @@ __global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S,
 
 // Dual update of every live manifold (manifold.js:447-491); the live count
 // comes from device memory, threads stride over it.
-template <class R, bool EXACT>
+template <class R, bool EXACT, bool GL>
 __global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *live, ManBuf<R> mb, const R *DQ, StepParams P, double alpha,
                        int writeC) {
   const int nLive = *nLivePtr;
   const R ralpha = R(alpha), beta = R(P.betaLin), pmax = R(P.pmax), stickT = R(P.stick);
-  const int law = contactLaw(P.contactModel, P.postStab, alpha);
+  const int law = GL ? contactLaw(1, P.postStab, alpha) : LAW_CPU;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nLive; k += gridDim.x * blockDim.x) {
     const int m = live[k];
     const int4 h = mb.head[m];
@@ -1908,7 +1912,7 @@ __global__ void __launch_bounds__(128) k_dual(const int *nLivePtr, const int *li
       R hot[HOT];
       vload<HOT>(hot, hp);
       Eval<R> e;
-      evalContact<R, EXACT>(bas, negb, bas[BMU], hot, dqA, dqB, ralpha, e, law, P.contactModel ? mb.stick[si] : 0);
+      evalContact<R, EXACT>(bas, negb, bas[BMU], hot, dqA, dqB, ralpha, e, law, GL ? mb.stick[si] : 0);
       if (writeC) {
         R *cd = mb.Cd + si * 4;
         cd[0] = e.C[0]; cd[1] = e.C[1]; cd[2] = e.C[2];

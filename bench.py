@@ -176,6 +176,33 @@ def exact_leg(args, scene, params, spec, timed_end):
                     "to the f64 oracle; same steps as the timed region"}
 
 
+def contact_law_leg(args, scene, params, spec, prec):
+    """The same workload under the reference GPU backend's contact law
+    (contactModel 1: static/dynamic friction, restitution targets, a 5 mm
+    contact skin per body, rest offsets, geometric warm-start persistence;
+    include/avbd.h AVBD_CONTACTS_GPU) in the measured precision, over the same
+    steps after the same set-up: its throughput and the contact counts it
+    carries (the skin adds speculative contacts)."""
+    import avbd
+    g = avbd.HipBackend(precision=prec, team=args.team, graphs=-1 if args.no_graphs else 0)
+    g.pack(scene, dict(params, contactModel=1))
+    if spec is not None:
+        settle_and_strike(g, spec)
+    g.step(max(args.warmup - 1, 0))
+    tune = g.autotune() if args.team == 0 and prec != avbd.PREC_EXACT else None
+    if tune is None and args.warmup > 0:
+        g.step(1)
+    g.idle()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.idle()
+    secs = time.perf_counter() - t0
+    c = g.counters()
+    return {"value": c["dynamic_bodies"] * params["iterations"] * args.steps / secs, "unit": "body-iterations/s",
+            "ms_per_step": secs / args.steps * 1e3, "team": tune[0] if tune else args.team,
+            "manifolds": c["manifolds"], "contacts": c["contacts"], "pairs": c["pairs"]}
+
+
 def host_cpu():
     """The host CPU's model name and the logical CPUs this process may run
     on (the CPU baseline uses one of them)."""
@@ -264,6 +291,11 @@ def main():
                     help="pyramid: skip the exact-precision leg (its throughput and the pose error against it)")
     ap.add_argument("--no-launch", action="store_true",
                     help="pyramid: time the first steps after pack (the settling pile) instead of the strike")
+    ap.add_argument("--contact-model", default="cpu", choices=["cpu", "gpu"],
+                    help="contact law of the timed region: the CPU engine's (the headline) or the reference GPU "
+                         "backend's (static/dynamic friction, restitution, contact skin and rest offsets)")
+    ap.add_argument("--no-contact-law", action="store_true",
+                    help="pyramid: skip the leg timing the same steps under the GPU backend's contact law")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -281,6 +313,9 @@ def main():
 
     prec = {"mixed": avbd.PREC_MIXED, "f64": avbd.PREC_F64, "exact": avbd.PREC_EXACT}[args.precision]
     scene, params, meta, desc = build_workload(args)
+    if args.contact_model == "gpu":
+        params = dict(params, contactModel=1)
+        desc += ", the GPU backend's contact law"
     gpu = avbd.HipBackend(precision=prec, team=args.team, graphs=-1 if args.no_graphs else 0)
     gpu.pack(scene, params)
     iters = params["iterations"]
@@ -361,7 +396,7 @@ def main():
                                          f"apex at 20 m/s; " if settle else "") +
                          f"timed steps {settle + args.warmup + 1}-{settle + args.warmup + args.steps} after pack"),
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
-            "precision": args.precision, "team": team, "color_order": args.order,
+            "precision": args.precision, "team": team, "color_order": args.order, "contact_law": args.contact_model,
             "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],
             "manifolds": c["manifolds"], "contacts": c["contacts"], "joints": c["joints"], "colors": c["colors"],
@@ -405,6 +440,8 @@ def main():
         line["kernels"]["repack_kept_manifolds"] = c2["manifolds"]
     if timed_end is not None and not args.no_exact:
         line["exact"] = exact_leg(args, scene, params, spec, timed_end)
+    if args.workload == "pyramid" and args.contact_model == "cpu" and not args.no_contact_law:
+        line["contact_law_gpu"] = contact_law_leg(args, scene, params, spec, prec)
     if not args.no_adjoint and args.workload == "pyramid":
         line["adjoint"] = adjoint_leg()
     if not args.no_cpu:
