@@ -104,30 +104,51 @@ def build_workload(args):
     return scene, params, meta, desc
 
 
-def cpu_sample(args, scene, params, meta):
-    """The f64 oracle (one thread) on a bounded sample of the same workload:
-    returns (body-iterations/s, description)."""
+def cpu_sample(args, scene, params, meta, start):
+    """The f64 oracle (one thread) on a bounded sample of the same workload,
+    from the bodies' state where the timed region starts (`start`: the GPU's
+    poses and velocities after the set-up and warm-up; the oracle's contacts
+    then start cold, as a pack's do): steps one at a time until about
+    `args.cpu_seconds` of CPU work. Returns (body-iterations/s, description)."""
     import pyoracle
     iters = params["iterations"]
+    budget = args.cpu_seconds
+
+    def resumed(part, lo):
+        bodies = []
+        for i, b in enumerate(part["bodies"]):
+            j = lo + i
+            bodies.append(dict(b, pos=[float(x) for x in start["pos"][j]], quat=[float(x) for x in start["quat"][j]],
+                               vel=[float(x) for x in start["vel"][j]], angvel=[float(x) for x in start["angvel"][j]]))
+        return dict(part, bodies=bodies)
+
     if args.workload == "batched":
-        # scene by scene (the oracle has no notion of independent scenes)
-        bi, secs, k = 0, 0.0, 0
+        # scene by scene (the oracle has no notion of independent scenes):
+        # every scene one step per round, rounds until the budget is spent
+        worlds, lo = [], 0
         for part in meta["parts"]:
-            o = pyoracle.Oracle(part, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
-            secs += o.step(args.cpu_steps)
-            bi += o.n_dynamic * iters * args.cpu_steps
-            k += 1
-            if secs > 15:
-                break
-        return bi / secs, (f"steps 1-{args.cpu_steps} of the first {k} scenes, f64 oracle (C++ -O2, single thread); "
-                           f"{secs:.1f} s")
+            worlds.append(pyoracle.Oracle(resumed(part, lo), params, schedule=pyoracle.SCHED_COLOR,
+                                          color_order=params["colorOrder"]))
+            lo += len(part["bodies"])
+        nd = sum(o.n_dynamic for o in worlds)
+        secs, rounds = 0.0, 0
+        while rounds < 1000 and (rounds < 1 or secs < budget):
+            secs += sum(o.step(1) for o in worlds)
+            rounds += 1
+        return nd * iters * rounds / secs, (f"all {len(worlds)} scenes from their state at the timed region's "
+                                            f"start (the oracle's contacts starting cold), {rounds} steps each, "
+                                            f"f64 oracle (C++ -O2, single thread); {secs:.1f} s")
     t0 = time.perf_counter()
-    o = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
+    o = pyoracle.Oracle(resumed(scene, 0), params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
     build_s = time.perf_counter() - t0
-    secs = o.step(args.cpu_steps)
-    return (o.n_dynamic * iters * args.cpu_steps / secs,
-            f"steps 1-{args.cpu_steps} of the same scene, {iters} iterations, f64 oracle (C++ -O2, single thread); "
-            f"{secs:.1f} s (+{build_s:.1f} s scene build)")
+    secs, steps = 0.0, 0
+    while steps < 1000 and (steps < 1 or secs < budget):
+        secs += o.step(1)
+        steps += 1
+    return (o.n_dynamic * iters * steps / secs,
+            f"{steps} steps of the same scene from its state at the timed region's start (the oracle's contacts "
+            f"starting cold), {iters} iterations, f64 oracle (C++ -O2, single thread); {secs:.1f} s "
+            f"(+{build_s:.1f} s scene build)")
 
 
 def settle_and_strike(gpu, spec):
@@ -283,7 +304,8 @@ def main():
     ap.add_argument("--layers", type=int, default=53)
     ap.add_argument("--bodies", type=int, default=50000, help="rain: body count")
     ap.add_argument("--scenes", type=int, default=256, help="batched: scene count")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU baseline: oracle steps until about this many seconds of CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch the solve loop directly, without graph replay")
@@ -343,6 +365,7 @@ def main():
     c = gpu.counters()
     nd = c["dynamic_bodies"]
 
+    start = gpu.read_state() if not args.no_cpu else None  # where the CPU baseline resumes
     # timed region: plain stepping (the solve loop replays a captured graph)
     t0 = time.perf_counter()
     gpu.step(args.steps)
@@ -445,7 +468,7 @@ def main():
     if not args.no_adjoint and args.workload == "pyramid":
         line["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        cpu_rate, sample = cpu_sample(args, scene, params, meta)
+        cpu_rate, sample = cpu_sample(args, scene, params, meta, start)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "body-iterations/s", "cores": 1, "kind": "port",
                                 "sample": sample, "host": host_cpu()}
         line["vs_cpu"] = value / cpu_rate
