@@ -26,7 +26,93 @@ const BODY_SECTIONS = [
 // bStatic element indices (layout.js BS_*)
 export const BS = {
   MASS: 0, MOMENT: 1, SIZE: 4, STATIC_FRICTION: 7, DYNAMIC_FRICTION: 8, RESTITUTION: 9, RADIUS: 10, FLAGS: 11,
+  CONTACT_OFFSET: 12, REST_OFFSET: 13,
 };
+
+const num = (v, d) => (typeof v === 'number' ? v : d);
+// A joint-like force is a spring when it carries spring.js's fields
+// (stiffness, rest) and no joint stiffness.
+const isSpring = (f) => 'stiffness' in f && 'rest' in f && !('stiffnessLin' in f);
+
+/**
+ * What pack hands avbd_pack and avbd_set_params, read from a Solver's scene
+ * graph (GpuBackend.pack's arenas, backend.js:662-1301): body arrays named as
+ * avbd_scene's fields (the GPU law's material from Rigid.setMaterial's fields,
+ * rigid.js:144-214, a null offset as NaN: the solver's), the joints and
+ * springs of forces[] in creation order, the parameters (solver.js:92-236),
+ * and each body's index. Pure: no device call.
+ */
+export function packArrays(solver) {
+  const bodies = solver.bodies;
+  const n = bodies.length;
+  const A = {
+    shape: new Int32Array(n), size: new Float64Array(n * 3), density: new Float64Array(n),
+    friction: new Float64Array(n), pos: new Float64Array(n * 3), quat: new Float64Array(n * 4),
+    vel: new Float64Array(n * 3), angvel: new Float64Array(n * 3),
+    staticFriction: new Float64Array(n), dynamicFriction: new Float64Array(n), restitution: new Float64Array(n),
+    contactOffset: new Float64Array(n), restOffset: new Float64Array(n),
+  };
+  const index = new Map();
+  bodies.forEach((b, i) => {
+    index.set(b, i);
+    A.shape[i] = b.shape === 'sphere' ? 1 : 0;
+    A.size.set(b.size, i * 3);
+    A.density[i] = b.density;
+    A.friction[i] = b.friction;
+    A.pos.set(b.positionLin, i * 3);
+    A.quat.set(b.positionAng, i * 4);
+    A.vel.set(b.velocityLin, i * 3);
+    A.angvel.set(b.velocityAng, i * 3);
+    A.staticFriction[i] = num(b.staticFriction, b.friction);
+    A.dynamicFriction[i] = num(b.dynamicFriction, b.friction);
+    A.restitution[i] = num(b.restitution, 0);
+    A.contactOffset[i] = num(b.contactOffset, NaN);
+    A.restOffset[i] = num(b.restOffset, NaN);
+  });
+  const joints = solver.forces.filter((f) => f.rA && f.rB && ('stiffnessLin' in f || isSpring(f)));
+  const m = joints.length;
+  if (m) {
+    A.jointA = new Int32Array(m);
+    A.jointB = new Int32Array(m);
+    A.jointRA = new Float64Array(m * 3);
+    A.jointRB = new Float64Array(m * 3);
+    A.jointStiffLin = new Float64Array(m);
+    A.jointStiffAng = new Float64Array(m);
+    A.jointFracture = new Float64Array(m);
+    A.jointKind = new Int32Array(m);
+    joints.forEach((j, k) => {
+      A.jointA[k] = j.bodyA ? index.get(j.bodyA) : -1;
+      A.jointB[k] = index.get(j.bodyB);
+      A.jointRA.set(j.rA, k * 3);
+      A.jointRB.set(j.rB, k * 3);
+      if (isSpring(j)) {
+        A.jointKind[k] = 1;
+        A.jointStiffLin[k] = j.stiffness;
+        A.jointStiffAng[k] = j.rest;
+        A.jointFracture[k] = j.tearStrain === undefined ? Infinity : j.tearStrain;
+      } else {
+        A.jointStiffLin[k] = j.stiffnessLin;
+        A.jointStiffAng[k] = j.stiffnessAng;
+        A.jointFracture[k] = j.fracture;
+      }
+    });
+  }
+  const params = {
+    dt: solver.dt, gravity: solver.gravity, iterations: solver.iterations, betaLin: solver.betaLin,
+    betaAng: solver.betaAng, alpha: solver.alpha, gamma: solver.gamma, penaltyMin: solver.penaltyMin,
+    penaltyMax: solver.penaltyMax, collisionMargin: solver.collisionMargin,
+    stickThreshold: solver.stickThreshold, postStabilize: solver.postStabilize ? 1 : 0,
+    rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
+    colorOrder: solver.colorOrder === 'priority' || solver.colorOrder === 1 ? 1 : 0,
+    // the GPU backend's contact law when the solver asks for it (this
+    // backend's addition: contactModel 'gpu'), with the solver's offsets
+    contactModel: solver.contactModel === 'gpu' || solver.contactModel === 1 ? 1 : 0,
+    contactOffset: num(solver.contactOffset, 0.005), restOffset: num(solver.restOffset, -0.005),
+    restitutionThreshold: num(solver.restitutionThreshold, 1.0),
+    contactPersistence: num(solver.contactPersistence, 0.002),
+  };
+  return { arrays: A, index, joints, params };
+}
 
 export function bodyLayout(n) {
   const body = {};
@@ -115,41 +201,12 @@ export class HipBackend {
   // (avbd_repack). Otherwise the scene is packed afresh from the survivors'
   // device state, as the reference does when it cannot keep contact state.
   pack(solver) {
+    const { arrays: A, index, joints, params } = packArrays(solver);
     const bodies = solver.bodies;
     const n = bodies.length;
     const prev = this.bodyIndex;
     const prevJoints = this.packedJoints || [];
-    const A = {
-      shape: new Int32Array(n), size: new Float64Array(n * 3), density: new Float64Array(n),
-      friction: new Float64Array(n), pos: new Float64Array(n * 3), quat: new Float64Array(n * 4),
-      vel: new Float64Array(n * 3), angvel: new Float64Array(n * 3),
-      // the GPU contact law's material (Rigid.setMaterial, rigid.js:144-214)
-      staticFriction: new Float64Array(n), dynamicFriction: new Float64Array(n), restitution: new Float64Array(n),
-      contactOffset: new Float64Array(n), restOffset: new Float64Array(n),
-    };
-    const num = (v, d) => (typeof v === 'number' ? v : d);
-    this.bodyIndex = new Map();
-    bodies.forEach((b, i) => {
-      this.bodyIndex.set(b, i);
-      A.shape[i] = b.shape === 'sphere' ? 1 : 0;
-      A.size.set(b.size, i * 3);
-      A.density[i] = b.density;
-      A.friction[i] = b.friction;
-      A.pos.set(b.positionLin, i * 3);
-      A.quat.set(b.positionAng, i * 4);
-      A.vel.set(b.velocityLin, i * 3);
-      A.angvel.set(b.velocityAng, i * 3);
-      A.staticFriction[i] = num(b.staticFriction, b.friction);
-      A.dynamicFriction[i] = num(b.dynamicFriction, b.friction);
-      A.restitution[i] = num(b.restitution, 0);
-      A.contactOffset[i] = num(b.contactOffset, NaN);  // null: the solver's
-      A.restOffset[i] = num(b.restOffset, NaN);
-    });
-    // joints and springs (spring.js: stiffness, rest, tearStrain), in the
-    // force list's creation order
-    const isSpring = (f) => 'stiffness' in f && 'rest' in f && !('stiffnessLin' in f);
-    const joints = solver.forces.filter((f) => f.rA && f.rB && ('stiffnessLin' in f || isSpring(f)));
-    const m = joints.length;
+    this.bodyIndex = index;
     let oldIndex = null;
     if (prev && this.numBodies > 0 && !this.forceRealloc) {
       oldIndex = new Int32Array(n);
@@ -189,47 +246,9 @@ export class HipBackend {
         oldIndex = null;
       }
     }
-    if (m) {
-      A.jointA = new Int32Array(m);
-      A.jointB = new Int32Array(m);
-      A.jointRA = new Float64Array(m * 3);
-      A.jointRB = new Float64Array(m * 3);
-      A.jointStiffLin = new Float64Array(m);
-      A.jointStiffAng = new Float64Array(m);
-      A.jointFracture = new Float64Array(m);
-      A.jointKind = new Int32Array(m);
-      joints.forEach((j, k) => {
-        A.jointA[k] = j.bodyA ? this.bodyIndex.get(j.bodyA) : -1;
-        A.jointB[k] = this.bodyIndex.get(j.bodyB);
-        A.jointRA.set(j.rA, k * 3);
-        A.jointRB.set(j.rB, k * 3);
-        if (isSpring(j)) {
-          A.jointKind[k] = 1;
-          A.jointStiffLin[k] = j.stiffness;
-          A.jointStiffAng[k] = j.rest;
-          A.jointFracture[k] = j.tearStrain === undefined ? Infinity : j.tearStrain;
-        } else {
-          A.jointStiffLin[k] = j.stiffnessLin;
-          A.jointStiffAng[k] = j.stiffnessAng;
-          A.jointFracture[k] = j.fracture;
-        }
-      });
-    }
-    addon.setParams(this.ctx, {
-      dt: solver.dt, gravity: solver.gravity, iterations: solver.iterations, betaLin: solver.betaLin,
-      betaAng: solver.betaAng, alpha: solver.alpha, gamma: solver.gamma, penaltyMin: solver.penaltyMin,
-      penaltyMax: solver.penaltyMax, collisionMargin: solver.collisionMargin,
-      stickThreshold: solver.stickThreshold, postStabilize: solver.postStabilize ? 1 : 0,
-      rotatedInertia: solver.rotatedInertia === false ? 0 : 1, cachedContactJacobians: 1,
-      colorOrder: solver.colorOrder === 'priority' || solver.colorOrder === 1 ? 1 : 0,
-      // the GPU backend's contact law when the solver asks for it (this
-      // backend's addition: contactModel 'gpu'), with the solver's offsets
-      contactModel: solver.contactModel === 'gpu' || solver.contactModel === 1 ? 1 : 0,
-      contactOffset: num(solver.contactOffset, 0.005), restOffset: num(solver.restOffset, -0.005),
-      restitutionThreshold: num(solver.restitutionThreshold, 1.0),
-      contactPersistence: num(solver.contactPersistence, 0.002),
-    });
-    this._contactModel = solver.contactModel === 'gpu' || solver.contactModel === 1 ? 1 : 0;
+    addon.setParams(this.ctx, params);
+    this._contactModel = params.contactModel;
+    this._offsets = { contact: params.contactOffset, rest: params.restOffset };
     if (oldIndex) addon.pack(this.ctx, A, oldIndex);
     else addon.pack(this.ctx, A);
     this._packed = A;
@@ -371,8 +390,14 @@ export class HipBackend {
         data[o + BS.MASS] = mass;
         data[o + BS.MOMENT] = m0; data[o + BS.MOMENT + 1] = m1; data[o + BS.MOMENT + 2] = m2;
         data[o + BS.SIZE] = sx; data[o + BS.SIZE + 1] = sy; data[o + BS.SIZE + 2] = sz;
-        data[o + BS.STATIC_FRICTION] = A.friction[i];
-        data[o + BS.DYNAMIC_FRICTION] = A.friction[i];
+        // the contact material as the GPU law resolves it (backend.js:156-185)
+        const co = Math.max(0, Number.isFinite(A.contactOffset[i]) ? A.contactOffset[i] : this._offsets.contact);
+        const ro = Math.min(Number.isFinite(A.restOffset[i]) ? A.restOffset[i] : this._offsets.rest, co);
+        data[o + BS.STATIC_FRICTION] = A.staticFriction[i];
+        data[o + BS.DYNAMIC_FRICTION] = A.dynamicFriction[i];
+        data[o + BS.RESTITUTION] = A.restitution[i];
+        data[o + BS.CONTACT_OFFSET] = co;
+        data[o + BS.REST_OFFSET] = ro;
         data[o + BS.RADIUS] = radius;
         data[o + BS.FLAGS] = sphere ? 1 : 0;
       }

@@ -183,13 +183,94 @@ function sceneSprings() {
   return { bodies, joints, params: { gravity: -10, iterations: 10 }, steps: 120 };
 }
 
+function scenePostStab() {
+  // post-stabilisation (solver.js:115-129, 411-417): a 3-cube stack dropped
+  // with a gap, a sphere landing on a box, and a 4-link jointed pendulum
+  const bodies = [ground(0.6)];
+  for (let i = 0; i < 3; i++) bodies.push(box([1, 1, 1], 1, 0.6, [0.02 * i, 0, 0.55 + 1.05 * i]));
+  bodies.push(box([1.5, 1.5, 0.5], 2, 0.5, [4, 0, 0.3]));
+  bodies.push(sphere(0.8, 1, 0.5, [4.1, 0, 1.6], { vel: [0, 0, -2] }));
+  const joints = [];
+  bodies.push(box([0.4, 0.4, 0.4], 0, 0.5, [-4, 0, 6]));
+  let prev = bodies.length - 1;
+  for (let i = 0; i < 4; i++) {
+    bodies.push(box([1, 0.3, 0.3], 1, 0.5, [-4 + 0.7 + i * 1.0, 0, 6]));
+    const cur = bodies.length - 1;
+    joints.push({ a: prev, b: cur, rA: i === 0 ? [0.2, 0, 0] : [0.5, 0, 0], rB: [-0.5, 0, 0],
+                  kLin: Infinity, kAng: 0, fracture: Infinity });
+    prev = cur;
+  }
+  return { bodies, joints, params: { gravity: -10, iterations: 8, postStabilize: true }, steps: 90 };
+}
+
+function sceneBodyInertia() {
+  // rotatedInertia = false (solver.js:130-140): the body-frame inertia in the
+  // mass matrix; tumbling boxes of unequal moments landing on the ground
+  const bodies = [ground(0.5)];
+  bodies.push(box([2, 0.5, 0.25], 1, 0.5, [0, 0, 2], { quat: axisAngle(1, 1, 0, 0.7), angvel: [3, -2, 5] }));
+  bodies.push(box([0.3, 1.2, 0.6], 2, 0.4, [2.5, 0, 1.5], { quat: axisAngle(0, 1, 1, 1.1), angvel: [-4, 1, 2] }));
+  bodies.push(box([1, 1, 1], 1, 0.6, [-2.5, 0, 0.5]));
+  bodies.push(box([0.6, 0.6, 1.8], 1, 0.6, [-2.5, 0.1, 2.0], { quat: axisAngle(1, 0, 0, 0.4), angvel: [0, 2, 0] }));
+  return { bodies, joints: [], params: { gravity: -10, iterations: 10, rotatedInertia: false }, steps: 90 };
+}
+
+function sceneEdits() {
+  // Solver.removeBodies (solver.js:261-300) and bodies created mid-run: a
+  // 4-layer pyramid and a 3-link rope; after step 20 three pyramid cubes and a
+  // rope link go (their contacts and joints with them), after step 30 a cube
+  // and a sphere are dropped in
+  const bodies = [ground(0.6)];
+  for (let l = 0; l < 4; l++) {
+    const n = 4 - l;
+    for (let i = 0; i < n; i++) {
+      for (let j = 0; j < n; j++) bodies.push(box([1, 1, 1], 1, 0.6, [i - (n - 1) / 2, j - (n - 1) / 2, 0.5 + l]));
+    }
+  }
+  const joints = [];
+  bodies.push(box([0.4, 0.4, 0.4], 0, 0.5, [6, 0, 5]));
+  let prev = bodies.length - 1;
+  for (let i = 0; i < 3; i++) {
+    bodies.push(box([0.8, 0.3, 0.3], 1, 0.5, [6.6 + i * 0.9, 0, 5]));
+    const cur = bodies.length - 1;
+    joints.push({ a: prev, b: cur, rA: i === 0 ? [0.2, 0, 0] : [0.4, 0, 0], rB: [-0.4, 0, 0],
+                  kLin: Infinity, kAng: 0, fracture: Infinity });
+    prev = cur;
+  }
+  const edits = [
+    { step: 20, remove: [3, 17, 28, bodies.length - 2] },
+    { step: 30, add: [box([1, 1, 1], 1, 0.6, [0.3, 0.2, 6]), sphere(0.6, 2, 0.5, [-1.5, 0, 3], { vel: [2, 0, 0] })] },
+  ];
+  return { bodies, joints, params: { gravity: -10, iterations: 10 }, steps: 60, edits };
+}
+
 // ---------------------------------------------------------------- runner
+function makeBody(solver, d) {
+  const vel = d.vel || [0, 0, 0];
+  const b = d.shape === 'sphere'
+    ? Rigid.sphere(solver, d.size[0], d.density, d.friction, d.pos, vel)
+    : new Rigid(solver, d.size, d.density, d.friction, d.pos, vel);
+  if (d.quat) { b.positionAng.set(d.quat); b.initialAng.set(d.quat); }
+  if (d.angvel) b.velocityAng.set(d.angvel);
+  return b;
+}
+
+// A topology edit after a step: bodies removed by their index at that moment
+// (Solver.removeBodies), then bodies appended (new Rigid on the running solver).
+function applyEdit(solver, edit) {
+  if (edit.remove) {
+    const doomed = new Set(edit.remove.map((i) => solver.bodies[i]));
+    solver.removeBodies((b) => doomed.has(b));
+  }
+  for (const d of edit.add || []) makeBody(solver, d);
+}
+
 function build(def) {
   const solver = new Solver();
   solver.dt = 1 / 60;
   solver.gravity = def.params.gravity;
   solver.iterations = def.params.iterations;
   if (def.params.postStabilize) solver.postStabilize = true;
+  if (def.params.rotatedInertia === false) solver.rotatedInertia = false;
   solver.broadphase = def.params.broadphase || 'bruteforce';
   const bodies = def.bodies.map((d) => {
     const vel = d.vel || [0, 0, 0];
@@ -297,6 +378,7 @@ function forceListOf(solver) {
 }
 
 function run(name, def, schedule) {
+  if (ONLY && !ONLY.has(name)) return;
   const solver = build(def);
   const sink = {};
   if (schedule === 'color' || schedule === 'prio') installColorSchedule(solver, sink, schedule === 'prio');
@@ -312,6 +394,7 @@ function run(name, def, schedule) {
     if (s <= 3) rec.forceList = Array.from(fl);
     steps.push(rec);
     if (sampleAt.has(s)) samples[s] = Array.from(st);
+    for (const e of def.edits || []) if (e.step === s) applyEdit(solver, e);
   }
   const enc = (x) => (x === Infinity ? 'inf' : x);
   const scene = {
@@ -321,9 +404,13 @@ function run(name, def, schedule) {
       : { ...j, kLin: enc(j.kLin), kAng: enc(j.kAng), fracture: enc(j.fracture) })),
   };
   const doc = { name, schedule, params: { dt: 1 / 60, ...def.params }, steps: def.steps, scene, perStep: steps, samples };
+  if (def.edits) doc.edits = def.edits;
   writeFileSync(join(OUT, `${name}.json`), JSON.stringify(doc));
   console.log(`${name}: ${def.steps} steps, ${def.bodies.length} bodies, final forces ${solver.forces.length}`);
 }
+
+// argv[4], when given: a comma-separated list of the fixtures to write
+const ONLY = process.argv[4] ? new Set(process.argv[4].split(',')) : null;
 
 async function main() {
   ({ Solver } = await imp('src/physics/solver.js'));
@@ -351,6 +438,9 @@ run('pendulum_prio', scenePendulum(), 'prio');
 run('chain_prio', sceneChain(), 'prio');
 run('springs_color', sceneSprings(), 'color');
 run('springs_reverse', sceneSprings(), 'reverse');
+run('poststab_color', scenePostStab(), 'color');
+run('bodyinertia_color', sceneBodyInertia(), 'color');
+run('edits_color', sceneEdits(), 'color');
 }
 
 main().catch((e) => { console.error(e); process.exit(1); });

@@ -1,6 +1,8 @@
 """GPU parity: the HIP step against the oracle and the reference's own vectors.
 
-Exact precision (AVBD_PREC_EXACT) must reproduce, at every step, bit for bit:
+Exact precision (AVBD_PREC_EXACT) must reproduce, at every step, bit for bit
+(post-stabilisation, the body-frame inertia and mid-run removals and additions
+included):
   * the body state hashes the reference engine wrote (tests/golden, colour
     schedule: the reference Solver with its bodyOrder hook fed the greedy
     colouring taken after force initialisation, solver.js:225-233 and
@@ -57,6 +59,7 @@ def test_exact_bit_identical_to_reference(name):
     gpu.pack(doc["scene"], dict(doc["params"], colorOrder=order))
     ref = pyoracle.Oracle(doc["scene"], doc["params"], schedule=pyoracle.SCHED_COLOR, color_order=order)
     ref.record_pairs(True)
+    cur = doc["scene"]
     for s, rec in enumerate(doc["perStep"][:MAX_STEPS], start=1):
         gpu.step(1)
         ref.step(1)
@@ -77,6 +80,33 @@ def test_exact_bit_identical_to_reference(name):
             i, k = np.unravel_index(np.argmax(diff), diff.shape)
             pytest.fail(f"step {s}: state differs from the reference; max |diff| {diff.max():.3e} "
                         f"at body {i} component {k}")
+        cur = apply_edits(gpu, ref, doc, s, cur)
+
+
+def apply_edits(gpu, ref, doc, s, cur):
+    """The fixture's topology edits after step s (Solver.removeBodies and new
+    Rigid, solver.js:261-300): the oracle removes and appends; the GPU keeps
+    the survivors' device state through avbd_repack with each body's old
+    index. Returns the scene description after the edits."""
+    for e in doc.get("edits", []):
+        if e["step"] != s:
+            continue
+        n = len(cur["bodies"])
+        gone = set(e.get("remove", []))
+        keep = [i for i in range(n) if i not in gone]
+        remap = {o: k for k, o in enumerate(keep)}
+        joints = [dict(j, a=remap[j["a"]] if j["a"] >= 0 else -1, b=remap[j["b"]]) for j in cur["joints"]
+                  if j["b"] in remap and (j["a"] < 0 or j["a"] in remap)]
+        added = e.get("add", [])
+        cur = dict(bodies=[cur["bodies"][i] for i in keep] + added, joints=joints)
+        if gone:
+            mask = np.zeros(n, dtype=np.int32)
+            mask[sorted(gone)] = 1
+            ref.remove_bodies(mask)
+        if added:
+            ref.add_bodies(added)
+        gpu.repack(cur, keep + [-1] * len(added))
+    return cur
 
 
 @pytest.mark.parametrize("order", [0, 1], ids=["index", "priority"])

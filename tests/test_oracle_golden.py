@@ -3,8 +3,10 @@
 Fixtures under tests/golden/ were written by the reference engine itself
 (tests/golden/make_golden.mjs): per-step SHA-256 of every body's state and of
 the force list, the colouring used under the colour schedule, and exact states
-at sample steps. Reference: src/physics/solver.js:307-493 and everything it
-calls.
+at sample steps. Among them: post-stabilisation (poststab_color), the
+body-frame inertia (bodyinertia_color, rotatedInertia false) and bodies removed
+and added mid-run (edits_color). Reference: src/physics/solver.js:261-493 and
+everything it calls.
 """
 import glob
 import hashlib
@@ -68,3 +70,19 @@ def test_bit_exact_against_reference(path):
             assert col.tolist() == rec["colorOf"], f"colouring differs at step {s}"
             assert k == rec["numColors"]
         assert _sha(st) == rec["state"], f"state hash differs at step {s}"
+        apply_edits(o, doc, s)
+
+
+def apply_edits(o, doc, s):
+    """The topology edits the fixture made after step s: bodies removed by
+    their index at that moment (Solver.removeBodies, solver.js:261-300), then
+    bodies appended (new Rigid on the running solver)."""
+    for e in doc.get("edits", []):
+        if e["step"] != s:
+            continue
+        if e.get("remove"):
+            mask = np.zeros(o.n, dtype=np.int32)
+            mask[e["remove"]] = 1
+            o.remove_bodies(mask)
+        if e.get("add"):
+            o.add_bodies(e["add"])
