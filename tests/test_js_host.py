@@ -34,6 +34,21 @@ def test_addon_loads_and_exports():
 
 
 @needs_node
+def test_pack_reads_a_reference_solver_as_it_reads_the_scene_graph():
+    # tests/golden/pack_arrays.json holds what HipBackend.pack's extraction
+    # (js/pack_arrays.mjs) read from scenes built with the reference's own
+    # Solver, Rigid, Joint and Spring and Rigid.setMaterial
+    # (make_pack_vectors.mjs); the same constructor calls on this package's
+    # scene graph must give the same arrays and parameters, every number
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "check_pack_arrays.mjs")], capture_output=True,
+                       text=True, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert d["scenes"] == ["shapes", "joints"]
+    assert d["diffs"] == []
+
+
+@needs_node
 def test_backend_module_imports():
     # the GpuBackend surface (backend.js:243-2226) a host calls
     methods = ["init", "initDevice", "initPipelines", "pack", "flushTopology", "prepareTopologyChange", "step",
