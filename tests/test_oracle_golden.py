@@ -19,7 +19,8 @@ import pytest
 import pyoracle
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-FILES = sorted(glob.glob(os.path.join(GOLDEN, "*.json")))
+# trajectory fixtures (pack_arrays.json holds scene-graph extractions instead)
+FILES = sorted(f for f in glob.glob(os.path.join(GOLDEN, "*.json")) if not os.path.basename(f).startswith("pack_"))
 
 
 def _state13(o):
