@@ -45,10 +45,13 @@ def oracle_forces(o):
 
 
 def track(scene, params, steps, precision, order=0):
+    """Steps GPU and oracle side by side: the first step whose force list or
+    feature keys differ, the worst pose error before it (held to TOL), and
+    the worst pose error after it."""
     gpu = avbd.HipBackend(precision=precision)
     gpu.pack(scene, dict(params, colorOrder=order))
     ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=order)
-    diverged, worst = None, 0.0
+    diverged, worst, after = None, 0.0, 0.0
     for s in range(1, steps + 1):
         gpu.step(1)
         ref.step(1)
@@ -56,17 +59,36 @@ def track(scene, params, steps, precision, order=0):
         assert np.isfinite(st["pos"]).all() and np.isfinite(st["quat"]).all(), f"step {s}: non-finite state"
         qn = np.linalg.norm(st["quat"], axis=1)
         assert np.abs(qn - 1).max() < 1e-6, f"step {s}: quaternion norm drifted"
+        o = ref.state()
+        err = max(np.abs(st["pos"] - o[:, :3]).max(), np.abs(st["quat"] - o[:, 3:7]).max())
         if diverged is None:
             gk, gf = gpu.read_forces()
             rk, rf = oracle_forces(ref)
             if not (np.array_equal(gk, rk) and np.array_equal(gf, rf)):
                 diverged = s
+            else:
+                worst = max(worst, err)
+                assert err <= TOL[precision], f"step {s}: pose error {err:.3e} > {TOL[precision]:.0e}"
                 continue
-            o = ref.state()
-            err = max(np.abs(st["pos"] - o[:, :3]).max(), np.abs(st["quat"] - o[:, 3:7]).max())
-            worst = max(worst, err)
-            assert err <= TOL[precision], f"step {s}: pose error {err:.3e} > {TOL[precision]:.0e}"
-    return diverged, worst
+        after = max(after, err)
+    return diverged, worst, after
+
+
+# The first step whose force list or feature keys differ from the oracle's,
+# per scene and precision, over the first 90 steps (None: none), as measured
+# on MI355X with 16 lanes per body: a mode that diverges earlier fails. Every
+# divergence comes from a tie in the separating-axis choice that rounding
+# flips (collide.js:291-324); the f64-team precision meets none in 90 steps.
+FIRST_DIVERGENCE = {
+    "pyramid5_color": 2, "edits_color": 2, "stack10_color": 5, "poststab_color": 8, "springs_color": 83,
+}
+# After a divergence the trajectories are different valid ones; the pose
+# difference to the oracle over the rest of the 90 steps is held to about ten
+# times what was measured (pyramid5 1.9e-3, edits 9.2e-5, stack10 0.33 as the
+# stack falls differently, poststab 3.5e-3, springs 1.3e-2; m or a quaternion
+# component), which a blow-up or a body falling through the ground exceeds.
+AFTER_BOUND = {"pyramid5_color": 0.02, "edits_color": 1e-3, "stack10_color": 2.0, "poststab_color": 0.05,
+               "springs_color": 0.2}
 
 
 @pytest.mark.parametrize("precision", [avbd.PREC_MIXED, avbd.PREC_F64], ids=["mixed", "f64"])
@@ -74,8 +96,15 @@ def track(scene, params, steps, precision, order=0):
 def test_variant_tracks_oracle(name, precision):
     doc = json.load(open(os.path.join(GOLDEN, name + ".json")))
     steps = min(len(doc["perStep"]), 90)
-    diverged, worst = track(doc["scene"], doc["params"], steps, precision)
-    print(f"[{name}] precision {precision}: first divergence {diverged}, worst pose error before it {worst:.2e}")
+    diverged, worst, after = track(doc["scene"], doc["params"], steps, precision)
+    print(f"[{name}] precision {precision}: first divergence {diverged}, worst pose error before it {worst:.2e}, "
+          f"after it {after:.2e}")
+    floor = FIRST_DIVERGENCE.get(name) if precision == avbd.PREC_MIXED else None
+    if floor is None:
+        assert diverged is None, f"{name}: force lists diverged at step {diverged}"
+    else:
+        assert diverged is None or diverged >= floor, f"{name}: force lists diverged at step {diverged} < {floor}"
+        assert after <= AFTER_BOUND[name], f"{name}: pose difference {after:.3e} after the divergence"
 
 
 @pytest.mark.parametrize("order", [0, 1], ids=["index", "priority"])
@@ -327,3 +356,28 @@ def test_one_launch_batch_solve_equals_per_colour_launches(precision):
             if s <= 2:
                 assert np.abs(a["pos"] - b["pos"]).max() <= (1e-4 if precision == avbd.PREC_MIXED else 1e-9)
     assert one.counters()["overflow"] == 0
+
+
+@pytest.mark.parametrize("which", ["pyramid12", "rain3000"])
+def test_mixed_precision_repeats_bit_for_bit_with_a_fixed_team(which):
+    # With the lanes per body fixed (not autotuned by timing), every reduction
+    # of the mixed precision has one order, and two runs give the same bits
+    # (the run-to-run variation of a benchmark line comes from the autotuned
+    # team shape alone)
+    if which == "pyramid12":
+        scene, params, _ = avbd.scenes.pyramid(layers=12)
+    else:
+        scene, params, _ = avbd.scenes.rain(n=3000)
+    params = dict(params, colorOrder=1)
+    runs = []
+    for _ in range(2):
+        g = avbd.HipBackend(precision=avbd.PREC_MIXED, team=16)
+        g.pack(scene, params)
+        out = []
+        for _ in range(40):
+            g.step(1)
+            st = g.read_state()
+            out.append(np.concatenate([st["pos"], st["quat"], st["vel"], st["angvel"]], axis=1))
+        runs.append(out)
+    for s, (a, b) in enumerate(zip(*runs), start=1):
+        assert np.array_equal(a, b), f"step {s}: runs differ by {np.abs(a - b).max():.3e}"
