@@ -16,7 +16,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# AVBD_ORACLE_LIB selects another build of the same library (the sanitized
+# one of oracle/run_sanitized.sh)
+LIB_PATH = os.environ.get("AVBD_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 SCHED_REVERSE, SCHED_ORDER, SCHED_COLOR = 0, 1, 2
 
