@@ -128,6 +128,16 @@ def scenes():
             s["joints"].append(spring(prev, cur, [0.5, 0, 0], [-0.5, 0, 0], 10 if i % 2 == 0 else 10000, 3))
         prev = cur
     S["spring_ratio"] = s
+
+    # a spring of rest -1 takes the anchors' distance when it is made
+    # (spring.js:79-84), beside one of a given rest; off-centre anchors
+    s = ground()
+    s["bodies"].append(body([1, 1, 1], 0, 0.5, [0, 0, 10]))
+    s["bodies"].append(body([1, 0.5, 0.5], 1, 0.5, [0.7, 0.2, 7.3]))
+    s["bodies"].append(body([0.6, 0.6, 0.6], 1, 0.5, [2.1, -0.4, 5.9]))
+    s["joints"].append(spring(1, 2, [0.25, 0, -0.5], [0.5, 0.1, 0], 300, -1))
+    s["joints"].append(spring(2, 3, [0, 0, -0.25], [0, 0, 0.3], 150, 1.5))
+    S["spring_current"] = s
     return S
 
 
