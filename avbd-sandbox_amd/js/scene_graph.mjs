@@ -203,6 +203,12 @@ export function fromDescription(desc, params = {}) {
       : new Rigid(solver, d.size, d.density, d.friction, d.pos, d.vel || [0, 0, 0]);
     if (d.quat) b.positionAng.set(d.quat);
     if (d.angvel) b.velocityAng.set(d.angvel);
+    // the GPU contact law's material, where the description gives one
+    const mat = {};
+    for (const k of ['staticFriction', 'dynamicFriction', 'restitution', 'contactOffset', 'restOffset']) {
+      if (d[k] !== undefined && d[k] !== null) mat[k] = d[k];
+    }
+    if (Object.keys(mat).length) b.setMaterial(mat);
     return b;
   });
   for (const j of desc.joints || []) {

@@ -20,6 +20,8 @@ async function main() {
   const state = solver.bodies.map((b) => [...b.positionLin, ...b.positionAng, ...b.velocityLin, ...b.velocityAng]);
   process.stdout.write(JSON.stringify({
     state, numColors: col.numColors, colorOf: Array.from(col.colorOf), pairs: col.broadPhasePairs.length / 2,
+    materials: col.slotInfo.map((sl) => [sl.staticFriction, sl.dynamicFriction]),
+    bias: col.slotInfo.flatMap((sl) => sl.contacts.map((c) => c.restitutionBias)),
     stats: backend.lastStats, profile: backend.lastProfile, adapter: backend.adapterInfo,
   }));
 }

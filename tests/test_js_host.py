@@ -86,6 +86,33 @@ def test_js_host_exact_matches_oracle(tmp_path):
     assert d["numColors"] == k and d["colorOf"] == col.tolist()
 
 
+@needs_node
+@pytest.mark.gpu
+def test_js_host_exact_matches_oracle_under_the_gpu_contact_law(tmp_path):
+    # the JS host with solver.contactModel = 1 and every body's material set
+    # through Rigid.setMaterial: boxes and spheres of random materials in a pit
+    import avbd  # noqa: F401
+    import pyoracle
+    from contact_model_cases import mixed_scene
+    bodies, params = mixed_scene()
+    scene = dict(bodies=bodies, joints=[])
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": scene, "params": params}))
+    r = subprocess.run([NODE, "tests/js/run_host.mjs", str(path), "60", "exact"], capture_output=True, text=True,
+                       cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=params["colorOrder"])
+    ref.step(60)
+    g = np.array(d["state"])
+    o = ref.state()[:, :13]
+    assert np.array_equal(g, o), f"max |diff| {np.abs(g - o).max():.3e}"
+    kab, _ = ref.forces()
+    ms = [ref.manifold(i) for i, (k, a, b, nc) in enumerate(kab) if k == 0]
+    assert d["materials"] == [[m["mu"], m["mu_dynamic"]] for m in ms]
+    assert d["bias"] == [float(x) for m in ms for x in m["bias"]]
+
+
 def run_arg_checks(*extra):
     r = subprocess.run([NODE, "tests/js/check_args.mjs", *extra], capture_output=True, text=True, cwd=ROOT,
                        timeout=120)
