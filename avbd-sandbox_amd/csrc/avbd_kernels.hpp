@@ -1654,7 +1654,9 @@ __device__ __forceinline__ void waveSync() {
 
 constexpr int XT_TEAM = 32;   // lanes per body
 constexpr int XT_TERMS = 27;  // numbers of the 6x6 system
-constexpr int XT_ROW = 28;    // doubles per item row in LDS (27 used)
+// doubles per item row in LDS (27 used): an odd count, so that the 32 lanes
+// writing their rows (58-dword strides) hit 32 different bank pairs
+constexpr int XT_ROW = 29;
 
 // One colour of the exact sweep with a team of XT_TEAM lanes per body, giving
 // the same bits as the one-lane exact sweep (k_primal<double, true, 1>): the
@@ -1741,8 +1743,20 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
     }
     waveSync();
     const int m = min(XT_TEAM, b1 - t0);
-    if (lane < XT_TERMS)
-      for (int k = 0; k < m; k++) acc += L[k * XT_ROW + lane];
+    if (lane < XT_TERMS) {
+      // the rows in item order; eight reads issued at once, the additions
+      // still one after the other
+      const double *col = L + lane;
+      int k = 0;
+      for (; k + 8 <= m; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = col[(k + u) * XT_ROW];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+      }
+      for (; k < m; k++) acc += col[k * XT_ROW];
+    }
     waveSync();
   }
   // the 27 sums to lane 0
