@@ -55,6 +55,9 @@ class Params(C.Structure):
 
 CONTACTS_CPU, CONTACTS_GPU = 0, 1
 
+# Config.graphs: replay the solve loop of each step (0), whole steps (1), none (-1)
+GRAPHS_SOLVE, GRAPHS_STEP, GRAPHS_NONE = 0, 1, -1
+
 
 ORDER_INDEX, ORDER_PRIORITY, ORDER_CUSTOM = 0, 1, 2
 

@@ -213,6 +213,26 @@ struct avbd_ctx {
   std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   bool graphCapturing = false;
   int graphsCfg = 0;
+  // Whole steps as replayed graphs (avbd_config.graphs = AVBD_GRAPHS_STEP):
+  // no host read inside a step. The counts the host keeps (live manifolds,
+  // colours, overflow) reach it through a ring of pinned host slots that each
+  // step's last kernel fills; a slot is read when it comes round again (at
+  // most PIPE_SLOTS steps in flight) or when a call needs the state.
+  static constexpr int PIPE_SLOTS = 4;
+  DBuf<int> pc;                  // the step's device counts (PC_*)
+  int *pipeRing = nullptr;       // pinned host ring, PIPE_SLOTS x PIPE_STRIDE ints
+  int *pipeRingDev = nullptr;    // its device address
+  hipEvent_t pipeEv[PIPE_SLOTS] = {};
+  int64_t pipeSerial = 0, pipeTaken = 0;  // steps enqueued, ring slots read
+  bool pipeReady = false;        // a step with host reads has run since the pack (shapes known)
+  bool pipeUnsettled = false;    // the host's counts lag the device's
+  bool pipeCapacity = false;     // a graph step exceeded a capacity: the next step call fails
+  int pipeColors = 0, pipeRounds = 0;  // colour launches and relaxation rounds per graph (0: derived)
+  // the counts a step graph's launch shapes are sized for (bodies per colour,
+  // manifolds); a step that outgrows them has the graphs captured again
+  std::vector<long> pipeEst;
+  long pipeTotEst = 0;
+  bool pipeCapturing = false;
   // autotuner
   bool tuneRequest = false;
   float tuneMs[6] = {0};
@@ -392,6 +412,14 @@ struct avbd_ctx {
     evs.clear();
     if (hscal) (void)hipHostFree(hscal);
     hscal = nullptr;
+    if (pipeRing) (void)hipHostFree(pipeRing);
+    pipeRing = pipeRingDev = nullptr;
+    for (auto &e : pipeEv) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+    pipeUnsettled = false;
+    pc.release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -415,17 +443,34 @@ struct avbd_ctx {
   // ---------------------------------------------------------------- scan
   // Exclusive scan of n ints; *total (device) receives the sum.
   // Exclusive scan of n ints in two launches; *total receives the sum.
-  void scan(const int *in, int *out, int n, int *total) {
+  // nDev (or null): a count produced on the device, at most n.
+  void scan(const int *in, int *out, int n, int *total, const int *nDev = nullptr) {
     if (n <= 0) {
       hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, total, 1, 0);
       return;
     }
     const int nb = (n + 1023) / 1024;
     DBuf<int> &sums = scanTmp[0];
-    sums.reserve(nb);
-    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, stream, in, out, n, sums.p);
-    hipLaunchKernelGGL(k_scan_finish, dim3(nb), dim3(256), 0, stream, out, n, (const int *)sums.p, nb, total);
+    if (sums.n < size_t(nb)) {
+      // a replayed graph holds the buffer's address (the pack sizes it for
+      // every scan a step makes, so a step being captured never grows it)
+      if (graphCapturing) throw ApiError(AVBD_E_STATE, "scan buffer too small for a captured step");
+      dropGraphs();
+      sums.reserve(nb);
+    }
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, stream, in, out, n, sums.p, nDev);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nb), dim3(256), 0, stream, out, n, (const int *)sums.p, nb, total, nDev);
   }
+
+  // Zero n ints: a memset, or in a step captured whole a fill kernel (memset
+  // nodes are kept out of replayed graphs).
+  void zero(int *p, long n, bool kernel) {
+    if (n <= 0) return;
+    if (kernel) hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(256), 0, stream, p, int(n), 0);
+    else HIPCHECK(hipMemsetAsync(p, 0, sizeof(int) * size_t(n), stream));
+  }
+  // a grid for threads that stride over a count known only on the device
+  static int strideBlocks(long n) { return std::max(1, std::min(blocks(n), 2048)); }
 
   void readScal(int first, int count) {
     HIPCHECK(hipMemcpyAsync(hscal + first, scal.p + first, sizeof(int) * count, hipMemcpyDeviceToHost, stream));
@@ -440,20 +485,24 @@ struct avbd_ctx {
   // Force lists (CSR) and per-body work lists. Their totals are bounded by
   // 2 (manifolds + joints) and 2 (8 manifolds + joints), for which the
   // buffers are sized at pack, so neither needs a host round trip.
-  void buildAdjacency() {
-    const int nf = J + nLive;
-    HIPCHECK(hipMemsetAsync(deg.p, 0, sizeof(int) * N, stream));
+  // dev: inside a step captured whole, the live count is read on the device
+  // (scal[2]) and the grids stride over it.
+  void buildAdjacency(bool dev = false) {
+    const long nf = dev ? long(J) + manCap : long(J) + nLive;  // dev: an upper bound
+    zero(deg.p, N, dev);
     const int4 *head = store[cur].head.p;
+    const int *nLiveDev = dev ? scal.p + 2 : nullptr;
+    const int gf = dev ? strideBlocks(nf) : blocks(nf), gr = dev ? strideBlocks(2 * nf) : blocks(2 * nf);
     if (nf > 0)
-      hipLaunchKernelGGL(k_adj<0>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
-                         deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
+      hipLaunchKernelGGL(k_adj<0>, dim3(gf), dim3(256), 0, stream, nLive, nLiveDev, J, live[cur].p, head,
+                         joints.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
     scan(deg.p, adjStart.p, N, adjStart.p + N);
     HIPCHECK(hipMemcpyAsync(adjCursor.p, adjStart.p, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
     if (nf > 0) {
-      hipLaunchKernelGGL(k_adj<1>, dim3(blocks(nf)), dim3(256), 0, stream, nLive, J, live[cur].p, head, joints.p,
-                         deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
+      hipLaunchKernelGGL(k_adj<1>, dim3(gf), dim3(256), 0, stream, nLive, nLiveDev, J, live[cur].p, head,
+                         joints.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
       // at most two entries per force
-      hipLaunchKernelGGL(k_adj_rank, dim3(blocks(2L * nf)), dim3(256), 0, stream, 2 * nf, N, props.p, adjStart.p,
+      hipLaunchKernelGGL(k_adj_rank, dim3(gr), dim3(256), 0, stream, int(2 * nf), N, props.p, adjStart.p,
                          (const int *)adjOwner.p, (const int *)adjKeyFill.p, (const int *)adjOtherFill.p, adjKey.p,
                          adjOther.p);
     }
@@ -505,7 +554,8 @@ struct avbd_ctx {
       }
       const bool capped = !uploaded && roundCap > 0 && launched >= roundCap;
       if (colorCap > 0)
-        hipLaunchKernelGGL(k_color_cap, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, colorCap, scal.p + 5);
+        hipLaunchKernelGGL(k_color_cap, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, colorCap, scal.p + 5,
+                           (int *)nullptr);
       HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
       const int nb = blocks(N);
       hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
@@ -540,6 +590,38 @@ struct avbd_ctx {
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
   }
 
+  // The colouring inside a step captured whole: R relaxation rounds (those
+  // after the fixed point return at once), the cap of E colours, the check
+  // that reports a miss, and the buckets; no host read.
+  void colorizeDev(int E, int R, bool ownRounds, bool ownCap) {
+    zero(cmeta.p, 8, true);
+    const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
+    for (int r = 0; r < R; r++)
+      hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjOther.p,
+                         color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
+    hipLaunchKernelGGL(k_color_cap, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, E, scal.p + 5,
+                       pc.p + PC_SEEN);
+    hipLaunchKernelGGL(k_pipe_color_check, dim3(1), dim3(1), 0, stream, (const int *)cmeta.p, pc.p, scal.p + 5, E,
+                       ownRounds ? 1 : 0, ownCap ? 1 : 0);
+    zero(cmeta.p + 4, 1, true);
+    const int nb = blocks(N);
+    hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
+    hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
+    hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
+    hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
+                       cbodies.p, itemStart.p, itemJoints.p, cwork.p);
+    hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
+                       items.p);
+  }
+
+  // bodies of colour c at the last colouring the host read (an estimate for
+  // a graph's launch shapes; 0 for a colour it did not see)
+  long colorCount(int c) const {
+    if (pipeCapturing) return c >= 0 && c < int(pipeEst.size()) ? pipeEst[c] : 0;
+    if (c < 0 || c + 1 >= int(hcstart.size())) return 0;
+    return hcstart[c + 1] - hcstart[c];
+  }
+
   // The solve loop's kernels take their counts from device memory and stride,
   // so one captured graph serves every step with that colour count; the
   // launch shapes are chosen from the colour sizes at capture.
@@ -549,7 +631,7 @@ struct avbd_ctx {
   // may have grown: it gets a quarter more teams (and teams stride over any
   // excess, so a larger colour is still swept whole).
   int primalBlocks(int c, int tm) const {
-    const long cnt = std::max(hcstart[c + 1] - hcstart[c], 1);
+    const long cnt = std::max(colorCount(c), 1L);
     const long teams = graphCapturing ? cnt + cnt / 4 + 64 : cnt;
     return std::max(1, std::min(blocks(teams * tm), 8192));
   }
@@ -558,7 +640,7 @@ struct avbd_ctx {
   template <class R, bool EXACT, int TEAM>
   void launchPrimal(int c, const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, double alpha, int dual,
                     double dualAlpha) {
-    if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
+    if (!graphCapturing && colorCount(c) <= 0) return;
     const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
     timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal<R, EXACT, TEAM, true> : k_primal<R, EXACT, TEAM, false>,
                 dim3(primalBlocks(c, TEAM)), dim3(256), c,
@@ -575,7 +657,7 @@ struct avbd_ctx {
   int teamFor(int c) const {
     int tm = team;
     if (tm < 4) return tm;
-    const long cnt = hcstart[c + 1] - hcstart[c];
+    const long cnt = colorCount(c);
     const double perBody = double(2L * contactsTotal + 2L * J) / double(std::max(nDynamic, 1));
     while (tm < 64 && cnt * long(tm) * 2 <= 2L * simds * 64 && tm * 2 <= perBody) tm *= 2;
     return tm;
@@ -588,7 +670,7 @@ struct avbd_ctx {
               double dualAlpha) {
     if constexpr (EXACT) {
       if (!exactTeam) { launchPrimal<R, true, 1>(c, sa, mb, P, alpha, dual, dualAlpha); return; }
-      if (!graphCapturing && hcstart[c + 1] - hcstart[c] <= 0) return;
+      if (!graphCapturing && colorCount(c) <= 0) return;
       // a team of lanes per body, the additions kept in the reference's
       // order; bodies with joints then take the one-lane sweep
       const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
@@ -852,12 +934,11 @@ struct avbd_ctx {
   }
 
   // ---------------------------------------------------------------- step
-  template <class R, bool EXACT>
-  void stepT() {
-    const StepParams P = sp();
-    record(0);
-    // 1. broad phase
-    HIPCHECK(hipMemsetAsync(bucketCnt.p, 0, sizeof(int) * (tableMask + 1), stream));
+  // 1. broad phase: the pair lists and their counts (scal[0], scal[1]).
+  // dev: inside a step captured whole (fill kernels for the clears).
+  void broadPhase(bool dev) {
+    zero(bucketCnt.p, long(tableMask) + 1, dev);
+
     hipLaunchKernelGGL(k_bp_cells, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, X.p, invh, tableMask, cellOf.p,
                        bucketCnt.p);
     scan(bucketCnt.p, bucketStart.p, int(tableMask + 1), scal.p + 7);
@@ -874,16 +955,23 @@ struct avbd_ctx {
     ba.pairs = pairs.p; ba.newPairs = newPairs.p; ba.pairCap = pairCap;
     ba.entryCell = entryCell.p; ba.scratch = bpScratch.p;
     ba.skin = prm.contact_model ? skinOff.p : nullptr;
-    HIPCHECK(hipMemsetAsync(countRev.p, 0, sizeof(int) * N, stream));
-    HIPCHECK(hipMemsetAsync(countNewRev.p, 0, sizeof(int) * N, stream));
+    zero(countRev.p, N, dev);
+    zero(countNewRev.p, N, dev);
     hipLaunchKernelGGL(k_bp_team, dim3(blocks(N, 256 / BP_TEAM)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<0>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
     scan(countRev.p, startRev.p, N, scal.p + 0);
     scan(countNewRev.p, startNewRev.p, N, scal.p + 1);
     // the write pass keeps every write below pairCap; an excess is reported
-    // below, before anything reads the lists
+    // before anything reads the lists
     hipLaunchKernelGGL(k_bp_emit, dim3(blocks(N)), dim3(256), 0, stream, ba);
     if (nLarge) hipLaunchKernelGGL(k_bp_large<1>, dim3(nLarge), dim3(256), 0, stream, ba, largeList.p);
+  }
+
+  template <class R, bool EXACT>
+  void stepT() {
+    const StepParams P = sp();
+    record(0);
+    broadPhase(false);
     readScal(0, 2);
     nPairs = hscal[0];
     nNew = hscal[1];
@@ -906,15 +994,15 @@ struct avbd_ctx {
     HIPCHECK(hipMemsetAsync(scal.p + 8, 0, sizeof(int), stream));
     if (nTot > 0)
       timedLaunch(AVBD_K_NARROW, 0.0, gl() ? k_narrow<R, true> : k_narrow<R, false>, dim3(blocks(nTot, 64)), dim3(64),
-                  nOld, nNew,
+                  nOld, nNew, (const int *)nullptr,
                   (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p, (const BodyProp *)props.p,
                   (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p, (const BodyMat *)mat.p,
                   (const double *)V.p);
     if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
     scan(keep.p, keepPos.p, nTot, scal.p + 2);
     if (nTot > 0)
-      hipLaunchKernelGGL(k_live_compact, dim3(blocks(nTot)), dim3(256), 0, stream, nTot, keep.p, keepPos.p,
-                         live[1 - cur].p);
+      hipLaunchKernelGGL(k_live_compact, dim3(blocks(nTot)), dim3(256), 0, stream, nTot, (const int *)nullptr,
+                         keep.p, keepPos.p, live[1 - cur].p);
     // how many of last step's manifolds survived (the exclusive scan at nOld)
     if (nNew > 0 && nOld > 0)
       HIPCHECK(hipMemcpyAsync(scal.p + 9, keepPos.p + nOld, sizeof(int), hipMemcpyDeviceToDevice, stream));
@@ -971,6 +1059,7 @@ struct avbd_ctx {
     record(5);
     HIPCHECK(hipGetLastError());
     steps++;
+    pipeReady = true;
     if (profiling) {
       HIPCHECK(hipEventSynchronize(evs[5]));
       float ms;
@@ -987,11 +1076,216 @@ struct avbd_ctx {
     }
   }
 
+  // ------------------------------------------------- whole-step graphs
+  // The step of stepT with every count kept on the device: the narrow phase,
+  // compaction and force lists read the counts k_pipe_counts derived, the
+  // colouring launches a fixed number of rounds and E colour buckets, the
+  // solve E colour launches per iteration (a colour with no body returns at
+  // once). Launch shapes come from the last step the host read. Nothing is
+  // executed here: this is captured, then replayed.
+  template <class R, bool EXACT>
+  void enqueueStepDevice(int E, int rounds, bool ownRounds, bool ownCap) {
+    const StepParams P = sp();
+    broadPhase(true);
+    hipLaunchKernelGGL(k_pipe_counts, dim3(1), dim3(1), 0, stream, scal.p, pc.p, pairCap, manCap);
+    ManBuf<R> ob = store[cur].view<R>(), nb = store[1 - cur].view<R>();
+    const long est = narrowEstimate();
+    auto narrow = gl() ? k_narrow<R, true> : k_narrow<R, false>;
+    hipLaunchKernelGGL(narrow, dim3(blocks(est, 64)), dim3(64), 0, stream, 0, 0, (const int *)pc.p, (const int *)live[cur].p, ob, nb, (const int2 *)newPairs.p,
+                       (const BodyProp *)props.p, (const double *)X.p, P, keep.p, scal.p + 8, srcSlot.p,
+                       (const BodyMat *)mat.p, (const double *)V.p);
+    if (J > 0) hipLaunchKernelGGL(k_joint_init, dim3(blocks(J)), dim3(256), 0, stream, J, joints.p, X.p, P);
+    scan(keep.p, keepPos.p, manCap, scal.p + 2, pc.p + PC_TOT);
+    hipLaunchKernelGGL(k_live_compact, dim3(strideBlocks(est)), dim3(256), 0, stream, 0, (const int *)(pc.p + PC_TOT),
+                       keep.p, keepPos.p, live[1 - cur].p);
+    cur = 1 - cur;
+    buildAdjacency(true);
+    buildItems();
+    colorizeDev(E, rounds, ownRounds, ownCap);
+    ManBuf<R> mb = store[cur].view<R>();
+    SolveArgs sa;
+    sa.props = props.p; sa.X = X.p; sa.X0 = X0.p; sa.Y = Y.p; sa.itemStart = itemStart.p; sa.items = items.p;
+    sa.head = mb.head; sa.joints = joints.p; sa.nBodies = N; sa.jointBodiesOnly = 0;
+    if (useScenes()) enqueueScenes<R, EXACT>(sa, mb, P);
+    else enqueueSolve<R, EXACT>(sa, mb, P, true);
+    hipLaunchKernelGGL(k_pipe_stats, dim3(1), dim3(64), 0, stream, (const int *)scal.p, (const int *)cmeta.p, pc.p,
+                       pipeRingDev, int(PIPE_SLOTS));
+  }
+
+  // threads of the narrow phase and compaction in a step graph: a quarter
+  // above the manifold count it is sized for (more stride)
+  long narrowEstimate() const {
+    return std::min<long>(manCap, std::max<long>(4096, pipeTotEst + pipeTotEst / 4));
+  }
+  // the shapes of the next step graphs from the counts the host last read
+  void pipeShapesFromHost() {
+    pipeEst.assign(MAX_COLORS, 0);
+    for (int c = 0; c < numColors; c++) pipeEst[c] = hcstart[c + 1] - hcstart[c];
+    pipeTotEst = lastNTot;
+  }
+  void dropStepGraphs() {
+    for (auto it = graphs.begin(); it != graphs.end();) {
+      if (it->first >= (1LL << 40)) {
+        (void)hipGraphExecDestroy(it->second.second);
+        (void)hipGraphDestroy(it->second.first);
+        it = graphs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+
+  // colour launches and relaxation rounds of the next graph step: the
+  // caller's limits (GpuBackend.encColors / colorRounds) where set, else one
+  // colour and four rounds above what the last step the host read needed,
+  // raised when a step reports it needed more
+  int pipeColorCount() const {
+    if (useScenes()) return MAX_COLORS;  // one launch reads every colour
+    if (colorCap > 0) return colorCap;
+    const int want = std::max(pipeColors, numColors + 1);
+    return std::max(1, std::min(MAX_COLORS, want));
+  }
+  int pipeRoundCount() const {
+    if (roundCap > 0) return roundCap;
+    return std::min(N + 2, std::max({8, pipeRounds, colorRounds + 4}));
+  }
+
+  // Read ring slot k (its step has completed): misses raise the next graphs'
+  // counts; a colour or manifold count past what the graphs were sized for
+  // has them captured again (their teams and threads would stride).
+  void takeSlot(int64_t k) {
+    const int *o = pipeRing + (k % PIPE_SLOTS) * PIPE_STRIDE;
+    const int miss = o[8];
+    if (miss & 4) {
+      needPairs = std::max(needPairs, o[10]);
+      needMan = std::max(needMan, o[11]);
+      pipeCapacity = true;
+    }
+    if (miss & 1) pipeRounds = std::min(N + 2, 2 * pipeRoundCount());
+    if (miss & 2) pipeColors = std::min(MAX_COLORS, o[9] + 1);
+    bool stale = false;
+    if (long(o[12]) > narrowEstimate()) {
+      pipeTotEst = o[12];
+      stale = true;
+    }
+    if (pipeEst.size() < size_t(MAX_COLORS)) pipeEst.resize(MAX_COLORS, 0);
+    for (int c = 0; c < std::min(o[5], PIPE_COLORS); c++) {
+      const long cnt = o[17 + c] - o[16 + c], est = pipeEst[c];
+      if (cnt > est + est / 4 + 64) {
+        pipeEst[c] = cnt;
+        stale = true;
+      }
+    }
+    if (stale) dropStepGraphs();
+  }
+
+  template <class R, bool EXACT>
+  void stepGraphT() {
+    // the slot this step will fill holds a step PIPE_SLOTS back: wait for it
+    // (at most that many steps in flight) and read it
+    while (pipeSerial - pipeTaken >= PIPE_SLOTS) {
+      HIPCHECK(hipEventSynchronize(pipeEv[pipeTaken % PIPE_SLOTS]));
+      takeSlot(pipeTaken++);
+    }
+    if (pipeCapacity) {
+      pipeCapacity = false;
+      throw ApiError(AVBD_E_CAPACITY, "a replayed step exceeded the pair or manifold capacity (the excess was "
+                                      "dropped, bits 1 and 2 of the overflow counter); repack to grow it");
+    }
+    const int E = pipeColorCount(), rounds = pipeRoundCount();
+    const long long key = (1LL << 40) + (long long)E * (1LL << 24) + (long long)rounds * 4 + cur;
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      const int cur0 = cur, nc0 = numColors;
+      numColors = E;
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      graphCapturing = pipeCapturing = true;
+      try {
+        enqueueStepDevice<R, EXACT>(E, rounds, roundCap <= 0, colorCap <= 0);
+      } catch (...) {
+        graphCapturing = pipeCapturing = false;
+        cur = cur0;
+        numColors = nc0;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      graphCapturing = pipeCapturing = false;
+      cur = cur0;
+      numColors = nc0;
+      HIPCHECK(hipStreamEndCapture(stream, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) (void)hipGraphDestroy(g);
+      HIPCHECK(e);
+      it = graphs.emplace(key, std::make_pair(g, ex)).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second.second, stream));
+    if (!pipeEv[pipeSerial % PIPE_SLOTS]) HIPCHECK(hipEventCreateWithFlags(&pipeEv[pipeSerial % PIPE_SLOTS],
+                                                                            hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(pipeEv[pipeSerial % PIPE_SLOTS], stream));
+    pipeSerial++;
+    cur = 1 - cur;
+    steps++;
+    pipeUnsettled = true;
+  }
+
+  // Bring the host's counts up to the device after graph steps: every call
+  // that reads or changes the state does this first.
+  void settle() {
+    if (!pipeUnsettled) return;
+    HIPCHECK(hipStreamSynchronize(stream));
+    int64_t last = -1;
+    while (pipeTaken < pipeSerial) {
+      last = pipeTaken;
+      takeSlot(pipeTaken++);
+    }
+    readScal(0, 10);
+    // the lists hold at most pairCap entries (an excess was dropped)
+    nPairs = std::min(hscal[0], pairCap);
+    nNew = std::min(hscal[1], pairCap);
+    nLive = hscal[2];
+    contactsTotal = hscal[8];
+    if (last >= 0) lastNTot = pipeRing[(last % PIPE_SLOTS) * PIPE_STRIDE + 12];
+    HIPCHECK(hipMemcpy(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost));
+    numColors = hmeta[4];
+    colorRounds = hmeta[3];
+    hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
+    colorFor = -1;  // relax from these colours on a step with host reads
+    pipeUnsettled = false;
+  }
+
+  // graph steps need a step with host reads since the pack (for the launch
+  // shapes) and none of the paths that read on the host inside a step
+  bool graphStepOK() const {
+    return graphsCfg == AVBD_GRAPHS_STEP && pipeReady && !profiling && !rec && !useUpload && !tuneRequest;
+  }
+
   void step() {
+    if (graphStepOK()) {
+      switch (prec) {
+        case AVBD_PREC_EXACT: stepGraphT<double, true>(); break;
+        case AVBD_PREC_F64: stepGraphT<double, false>(); break;
+        default: stepGraphT<float, false>(); break;
+      }
+      return;
+    }
+    settle();
+    if (pipeCapacity) {
+      pipeCapacity = false;
+      throw ApiError(AVBD_E_CAPACITY, "a replayed step exceeded the pair or manifold capacity (the excess was "
+                                      "dropped, bits 1 and 2 of the overflow counter); repack to grow it");
+    }
     switch (prec) {
       case AVBD_PREC_EXACT: stepT<double, true>(); break;
       case AVBD_PREC_F64: stepT<double, false>(); break;
       default: stepT<float, false>(); break;
+    }
+    if (graphsCfg == AVBD_GRAPHS_STEP) {
+      // the next step graphs are sized from this step's counts
+      pipeShapesFromHost();
+      dropStepGraphs();
     }
   }
 
@@ -1278,9 +1572,12 @@ int fail(avbd_ctx *c, int code, const std::string &msg) {
   return code;
 }
 
-template <class F> int guarded(avbd_ctx *c, F &&f) {
+// Every call but avbd_step first brings the host's counts up to date after
+// replayed steps (settleFirst).
+template <class F> int guarded(avbd_ctx *c, F &&f, bool settleFirst = true) {
   if (!c) return AVBD_E_INVALID;
   try {
+    if (settleFirst) c->settle();
     f();
     return AVBD_OK;
   } catch (const ApiError &e) {
@@ -1361,6 +1658,14 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   HIPCHECK(hipSetDevice(c.device));
   HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   HIPCHECK(hipHostMalloc((void **)&c.hscal, sizeof(int) * 16, hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc((void **)&c.pipeRing, sizeof(int) * c.PIPE_SLOTS * PIPE_STRIDE, hipHostMallocDefault));
+  std::memset(c.pipeRing, 0, sizeof(int) * c.PIPE_SLOTS * PIPE_STRIDE);
+  HIPCHECK(hipHostGetDevicePointer((void **)&c.pipeRingDev, c.pipeRing, 0));
+  c.pipeSerial = c.pipeTaken = 0;
+  c.pipeReady = c.pipeUnsettled = c.pipeCapacity = false;
+  c.pipeColors = c.pipeRounds = 0;
+  c.pipeEst.clear();
+  c.pipeTotEst = 0;
   if (s.num_joints < 0) throw ApiError(AVBD_E_INVALID, "num_joints must not be negative");
   if (s.num_joints > 0 && (!s.joint_a || !s.joint_b || !s.joint_ra || !s.joint_rb || !s.joint_stiff_lin ||
                            !s.joint_stiff_ang || !s.joint_fracture))
@@ -1577,9 +1882,14 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.useRank = c.useUpload = false;
   c.items.reserve(2 * (size_t(MAXC) * c.manCap + c.J) + 1);
   c.scal.reserve(16);
+  c.pc.reserve(PC_N);
+  // block sums of the largest scan a step makes (the manifold keep flags of a
+  // step replayed whole are scanned at capacity)
+  c.scanTmp[0].reserve(size_t(std::max({c.manCap, int(c.tableMask) + 1, N + 1}) + 1023) / 1024 + 1);
   c.cbytes.reserve(2 * MAX_COLORS);
   c.cmeta.reserve(8 + MAX_COLORS + 1);
   HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
+  HIPCHECK(hipMemset(c.pc.p, 0, sizeof(int) * PC_N));
   c.evs.resize(AVBD_NUM_PHASES);
   for (auto &e : c.evs) HIPCHECK(hipEventCreate(&e));
   // force lists of the initial scene (joints only) for the first broad phase
@@ -1817,7 +2127,7 @@ int avbd_step(avbd_ctx *ctx, int32_t nsteps) {
     HIPCHECK(hipSetDevice(ctx->device));
     if (nsteps > 0) ctx->dropAdjoint();
     for (int s = 0; s < nsteps; s++) ctx->step();
-  });
+  }, false);
 }
 
 int avbd_sync(avbd_ctx *ctx) {
@@ -2285,6 +2595,7 @@ int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n) {
   return guarded(ctx, [&] {
     if (!ctx->packed) throw ApiError(AVBD_E_STATE, "no scene packed");
     ctx->dropAdjoint();
+    ctx->dropGraphs();  // a replayed step's colouring is captured with its claim order
     if (!rank) {
       ctx->useRank = false;
       return;
@@ -2304,6 +2615,7 @@ int avbd_set_coloring_limits(avbd_ctx *ctx, int32_t rounds, int32_t colors) {
     ctx->roundCap = rounds;
     ctx->colorCap = colors;
     ctx->dropAdjoint();
+    ctx->dropGraphs();  // a replayed step's colouring is captured with its limits
   });
 }
 

@@ -34,9 +34,11 @@ __global__ void k_fill(int *p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
-// Block-level exclusive scan of 1024 ints (256 threads x 4).
-__global__ void k_scan_block(const int *in, int *out, int n, int *blockSums) {
+// Block-level exclusive scan of 1024 ints (256 threads x 4). With nDev, the
+// count is min(n, *nDev) (a count produced on the device; the grid covers n).
+__global__ void k_scan_block(const int *in, int *out, int n, int *blockSums, const int *nDev) {
   __shared__ int sh[256];
+  if (nDev) n = min(n, *nDev);
   const int base = blockIdx.x * 1024 + threadIdx.x * 4;
   int v[4], s = 0;
   for (int k = 0; k < 4; k++) {
@@ -63,8 +65,9 @@ __global__ void k_scan_block(const int *in, int *out, int n, int *blockSums) {
 // before it, which its 256 threads add up themselves from the block sums (a
 // few hundred at most for the sizes here, so no third pass is needed); the
 // last block also writes the total.
-__global__ void k_scan_finish(int *out, int n, const int *blockSums, int nb, int *total) {
+__global__ void k_scan_finish(int *out, int n, const int *blockSums, int nb, int *total, const int *nDev) {
   __shared__ int red[256];
+  if (nDev) n = min(n, *nDev);
   const int b = blockIdx.x;
   int s = 0;
   for (int j = threadIdx.x; j < b; j += 256) s += blockSums[j];
@@ -418,14 +421,12 @@ __global__ void k_bp_large(BpArgs a, const int *largeList) {
 // GPU backend's narrowphase (shaders.js:1105-1367) instead: pair material,
 // the contact skin, the geometric warm-start fallback, the rest distance and
 // the restitution target.
+// poly: the block's LDS clip polygons (one lane-interleaved column per thread).
 template <class R, bool GL>
-__global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb, const int2 *newPairs,
-                         const BodyProp *props, const double *X, StepParams P, int *keep, int *contactTotal,
-                         int *srcSlot, const BodyMat *mat, const double *V) {
-  // clip polygons in LDS, one lane-interleaved column per thread (64 threads)
-  __shared__ double poly[2 * POLY_LDS * 3 * LDS_LANES];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nOld + nNew) return;
+__device__ __forceinline__ void narrowOne(int t, int nOld, const int *liveOld, ManBuf<R> ob, ManBuf<R> nb,
+                                          const int2 *newPairs, const BodyProp *props, const double *X,
+                                          const StepParams &P, int *keep, int *contactTotal, int *srcSlot,
+                                          const BodyMat *mat, const double *V, double *poly) {
   int a, b, m = -1, ncOld = 0;
   if (t < nOld) {
     m = liveOld[t];
@@ -634,6 +635,21 @@ __global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *li
   nb.basis[size_t(t) * BAS + BCD] = R(cdist);
   keep[t] = n > 0 ? 1 : 0;
   if (n > 0) atomicAdd(contactTotal, n);
+}
+
+// One thread per manifold, 64 threads a block. counts (device, or null): the
+// old and new counts as a step on the device produced them, for a launch
+// whose grid was sized before they were known; the threads then stride.
+template <class R, bool GL>
+__global__ void __launch_bounds__(64) k_narrow(int nOld, int nNew, const int *counts, const int *liveOld, ManBuf<R> ob,
+                                               ManBuf<R> nb, const int2 *newPairs, const BodyProp *props,
+                                               const double *X, StepParams P, int *keep, int *contactTotal,
+                                               int *srcSlot, const BodyMat *mat, const double *V) {
+  __shared__ double poly[2 * POLY_LDS * 3 * LDS_LANES];
+  if (counts) { nOld = counts[0]; nNew = counts[1]; }
+  const int nTot = nOld + nNew;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nTot; t += gridDim.x * blockDim.x)
+    narrowOne<R, GL>(t, nOld, liveOld, ob, nb, newPairs, props, X, P, keep, contactTotal, srcSlot, mat, V, poly);
 }
 
 // ----------------------------------------------------------------- joints
@@ -923,37 +939,41 @@ __global__ void k_joint_dual(int J, JointDev *joints, const double *X, StepParam
 }
 
 // ---------------------------------------------------------------- topology
-__global__ void k_live_compact(int nTot, const int *keep, const int *pos, int *live) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nTot && keep[t]) live[pos[t]] = t;
+// nTotDev (or null): the count as produced on the device; the threads stride.
+__global__ void k_live_compact(int nTot, const int *nTotDev, const int *keep, const int *pos, int *live) {
+  if (nTotDev) nTot = *nTotDev;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nTot; t += gridDim.x * blockDim.x)
+    if (keep[t]) live[pos[t]] = t;
 }
 
 // Per-body force lists. Key = j for joint j, J + k for live manifold k, so
-// ascending keys are the force list's creation order.
+// ascending keys are the force list's creation order. nLiveDev (or null): the
+// live count as produced on the device; the threads stride.
 template <int PASS>
-__global__ void k_adj(int nLive, int J, const int *live, const int4 *head, const JointDev *joints, int *deg,
-                      int *cursor, int *adjKey, int *adjOther, int *owner) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int a, b, key;
-  if (t < J) {
-    if (joints[t].broken == 2) return;
-    a = joints[t].a; b = joints[t].b; key = t;
-  } else if (t < J + nLive) {
-    const int4 h = head[live[t - J]];
-    a = h.x; b = h.y; key = t;
-  } else {
-    return;
-  }
-  if (PASS == 0) {
-    if (a >= 0) atomicAdd(&deg[a], 1);
-    atomicAdd(&deg[b], 1);
-  } else {
-    if (a >= 0) {
-      const int p = atomicAdd(&cursor[a], 1);
-      adjKey[p] = key; adjOther[p] = b; owner[p] = a;
+__global__ void k_adj(int nLive, const int *nLiveDev, int J, const int *live, const int4 *head,
+                      const JointDev *joints, int *deg, int *cursor, int *adjKey, int *adjOther, int *owner) {
+  if (nLiveDev) nLive = *nLiveDev;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < J + nLive; t += gridDim.x * blockDim.x) {
+    int a, b;
+    const int key = t;
+    if (t < J) {
+      if (joints[t].broken == 2) continue;
+      a = joints[t].a; b = joints[t].b;
+    } else {
+      const int4 h = head[live[t - J]];
+      a = h.x; b = h.y;
     }
-    const int p = atomicAdd(&cursor[b], 1);
-    adjKey[p] = key; adjOther[p] = a; owner[p] = b;
+    if (PASS == 0) {
+      if (a >= 0) atomicAdd(&deg[a], 1);
+      atomicAdd(&deg[b], 1);
+    } else {
+      if (a >= 0) {
+        const int p = atomicAdd(&cursor[a], 1);
+        adjKey[p] = key; adjOther[p] = b; owner[p] = a;
+      }
+      const int p = atomicAdd(&cursor[b], 1);
+      adjKey[p] = key; adjOther[p] = a; owner[p] = b;
+    }
   }
 }
 
@@ -961,10 +981,8 @@ __global__ void k_adj(int nLive, int J, const int *live, const int4 *head, const
 // place is the number of smaller keys in its list (keys are distinct within a
 // list), read eight at a time. Static bodies' lists keep their fill order
 // (the ground's can hold thousands of entries).
-__global__ void k_adj_rank(int maxEntries, int n, const BodyProp *props, const int *adjStart, const int *owner,
-                           const int *keyIn, const int *otherIn, int *keyOut, int *otherOut) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= maxEntries || p >= adjStart[n]) return;
+__device__ __forceinline__ void adjRankOne(int p, const BodyProp *props, const int *adjStart, const int *owner,
+                                           const int *keyIn, const int *otherIn, int *keyOut, int *otherOut) {
   const int b = owner[p];
   const int s0 = adjStart[b], s1 = adjStart[b + 1];
   const int key = keyIn[p];
@@ -982,6 +1000,14 @@ __global__ void k_adj_rank(int maxEntries, int n, const BodyProp *props, const i
   }
   keyOut[s0 + rank] = key;
   otherOut[s0 + rank] = otherIn[p];
+}
+
+// Entries up to min(maxEntries, the list total); the threads stride.
+__global__ void k_adj_rank(int maxEntries, int n, const BodyProp *props, const int *adjStart, const int *owner,
+                           const int *keyIn, const int *otherIn, int *keyOut, int *otherOut) {
+  const int total = min(maxEntries, adjStart[n]);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x)
+    adjRankOne(p, props, adjStart, owner, keyIn, otherIn, keyOut, otherOut);
 }
 
 // Greedy colouring (coloring.js:75-92): colour(i) is the smallest colour not
@@ -1101,6 +1127,76 @@ __global__ void k_color_install(int n, const BodyProp *props, const int *src, in
 }
 
 
+// ------------------------------------------------------ whole-step graphs
+// A step replayed as one graph takes every count from device memory. pc:
+// [PC_OLD] live manifolds the step starts with, [PC_NEW] new pairs taken,
+// [PC_TOT] their sum, [PC_MISS] this step's misses (1: the relaxation had not
+// reached its fixed point after the rounds launched, 2: more colours than
+// launched, 4: a capacity was exceeded), [PC_SEEN] the colour count before
+// the cap, [PC_SERIAL] steps written to the host ring, [PC_PAIRS] pairs found,
+// [PC_WANT] manifolds asked for.
+enum { PC_OLD, PC_NEW, PC_TOT, PC_MISS, PC_SEEN, PC_SERIAL, PC_PAIRS, PC_WANT, PC_N };
+
+// After the broad phase: the narrow phase's counts, within the capacities
+// (an excess is dropped and reported: bits 1 and 2 of the overflow word, as
+// the reference's overflow counter reports them, layout.js:171-178).
+__global__ void k_pipe_counts(int *scal, int *pc, int pairCap, int manCap) {
+  const int nOld = scal[2], found = scal[0], fresh = scal[1];
+  int nNew = fresh, miss = 0;
+  if (found > pairCap) {
+    nNew = min(nNew, pairCap);
+    miss = 4;
+    atomicOr(&scal[5], 1);
+  }
+  if (nOld + nNew > manCap) {
+    nNew = manCap - nOld;
+    miss = 4;
+    atomicOr(&scal[5], 2);
+  }
+  pc[PC_OLD] = nOld;
+  pc[PC_NEW] = nNew;
+  pc[PC_TOT] = nOld + nNew;
+  pc[PC_PAIRS] = found;
+  pc[PC_WANT] = nOld + fresh;
+  pc[PC_MISS] |= miss;
+  scal[8] = 0;  // contacts, counted by the narrow phase
+}
+
+// After the relaxation and the cap: own* say whether the round count and the
+// cap are the graph's own (then a miss asks the host for a larger graph) or
+// the caller's limits (GpuBackend.colorRounds / encColors).
+__global__ void k_pipe_color_check(const int *cmeta, int *pc, int *overflow, int cap, int ownRounds, int ownCap) {
+  if (!cmeta[1]) {
+    atomicOr(overflow, 32);
+    if (ownRounds) pc[PC_MISS] |= 1;
+  }
+  if (ownCap && pc[PC_SEEN] > cap) pc[PC_MISS] |= 2;
+}
+
+// The step's counters into the host ring (pinned host memory, PIPE_STRIDE
+// ints per step, slot = step serial mod slots): [0..13] counts, misses and
+// the serial, [16..16+PIPE_COLORS] the first colour starts (the host sizes
+// the next graphs' launches from them); this step's misses are reset.
+constexpr int PIPE_STRIDE = 64, PIPE_COLORS = 32;
+__global__ void k_pipe_stats(const int *scal, const int *cmeta, int *pc, int *ring, int slots) {
+  const int serial = pc[PC_SERIAL];
+  int *o = ring + (serial % slots) * PIPE_STRIDE;
+  const int t = threadIdx.x;
+  if (t <= PIPE_COLORS) o[16 + t] = cmeta[8 + t];
+  if (t == 0) {
+    o[0] = scal[0]; o[1] = scal[1]; o[2] = scal[2]; o[3] = scal[8]; o[4] = scal[5];
+    o[5] = cmeta[4]; o[6] = cmeta[3]; o[7] = cmeta[1];
+    o[8] = pc[PC_MISS]; o[9] = pc[PC_SEEN]; o[10] = pc[PC_PAIRS]; o[11] = pc[PC_WANT];
+    o[12] = pc[PC_TOT]; o[13] = serial;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  __threadfence_system();
+  pc[PC_SERIAL] = serial + 1;
+  pc[PC_MISS] = 0;
+  pc[PC_SEEN] = 0;
+}
+
 // Flags in the y word of a contact work item (see k_items).
 constexpr uint32_t ITEM_FIRST = 0x80000000u, ITEM_SPHERE = 0x40000000u, ITEM_BODY = 0x3fffffffu;
 
@@ -1130,10 +1226,13 @@ __global__ void k_color_hist(int n, const int *color, int *hist, int *numColors)
 // body whose colour is at or above the cap takes the last allowed colour,
 // which may put two bodies of one force in one colour (reported in bit 32 of
 // the overflow word).
-__global__ void k_color_cap(int n, int *color, int cap, int *overflow) {
+// seen (or null) receives the colour count before the cap.
+__global__ void k_color_cap(int n, int *color, int cap, int *overflow, int *seen) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (color[i] >= cap) {
+  const int c = color[i];
+  if (seen && c >= 0) atomicMax(seen, c + 1);
+  if (c >= cap) {
     color[i] = cap - 1;
     atomicOr(overflow, 32);
   }

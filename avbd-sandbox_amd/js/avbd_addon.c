@@ -142,7 +142,7 @@ static void finalize_ctx(napi_env env, void *data, void *hint) {
   }
 }
 
-/* create({device, precision, team}) -> external (GpuBackend.init, backend.js:337-469) */
+/* create({device, precision, team, manifoldCap, pairCap, sceneSolve, graphs}) -> external (GpuBackend.init, backend.js:337-469) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -157,6 +157,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.manifold_cap = (int32_t)get_num(env, argv[0], "manifoldCap", 0);
     cfg.pair_cap = (int32_t)get_num(env, argv[0], "pairCap", 0);
     cfg.scene_solve = (int32_t)get_num(env, argv[0], "sceneSolve", 0);
+    cfg.graphs = (int32_t)get_num(env, argv[0], "graphs", 0);
   }
   avbd_ctx *ctx = NULL;
   int rc = avbd_create(&ctx, &cfg);

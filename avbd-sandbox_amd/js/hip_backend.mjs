@@ -13,6 +13,8 @@ const require = createRequire(import.meta.url);
 const addon = require(join(here, 'avbd.node'));
 
 export const PRECISION = { exact: 0, f64: 1, mixed: 2 };
+// avbd_config.graphs
+export const GRAPHS = { solve: 0, step: 1, none: -1 };
 const PHASE_NAMES = ['broad', 'narrow', 'topology', 'colour', 'solve', 'total'];
 export const MAX_COLORS = 256;
 
@@ -43,7 +45,13 @@ export function bodyLayout(n) {
 }
 
 export class HipBackend {
-  constructor({ precision = 'mixed', team = 0, device = 0, manifoldCap = 0, pairCap = 0 } = {}) {
+  // graphs: 'solve' replays each step's solve loop as a graph (the step reads
+  // three counts on the host); 'step' replays whole steps with no host read
+  // inside them (the reference's step never waits either, backend.js:1606-1611),
+  // reporting a capacity overflow on a later step; 'none' launches directly.
+  constructor({ precision = 'mixed', team = 0, device = 0, manifoldCap = 0, pairCap = 0, graphs = 'solve' } = {}) {
+    if (!(graphs in GRAPHS)) throw new Error(`graphs must be one of ${Object.keys(GRAPHS).join(', ')}`);
+    this.graphs = graphs;
     this.precision = precision;
     this.team = team;
     this.device = device;
@@ -93,7 +101,7 @@ export class HipBackend {
     if (!this.ctx) {
       this.ctx = addon.create({
         device: this.device, precision: PRECISION[this.precision], team: this.team,
-        manifoldCap: this.manifoldCap, pairCap: this.pairCap,
+        manifoldCap: this.manifoldCap, pairCap: this.pairCap, graphs: GRAPHS[this.graphs],
       });
       this.adapterInfo = 'AMD Instinct MI355X (gfx950, HIP)';
     }

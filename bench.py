@@ -165,6 +165,14 @@ def settle_and_strike(gpu, spec):
     gpu.write_state(st["pos"], st["quat"], st["vel"], st["angvel"], st["prev_vel"])
 
 
+def graphs_cfg(args):
+    """avbd_config.graphs for the command line's choice."""
+    import avbd
+    if args.no_graphs:
+        return avbd.GRAPHS_NONE
+    return {"step": avbd.GRAPHS_STEP, "solve": avbd.GRAPHS_SOLVE, "none": avbd.GRAPHS_NONE}[args.graphs]
+
+
 def exact_leg(args, scene, params, spec, timed_end):
     """The same workload in the exact precision (f64, bit-identical to the
     oracle: the error-free line) over the same steps: its throughput, and the
@@ -173,7 +181,7 @@ def exact_leg(args, scene, params, spec, timed_end):
     dynamic bodies)."""
     import avbd
     import numpy as np
-    ex = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    ex = avbd.HipBackend(precision=avbd.PREC_EXACT, graphs=graphs_cfg(args))
     ex.pack(scene, params)
     if spec is not None:
         settle_and_strike(ex, spec)
@@ -205,14 +213,17 @@ def contact_law_leg(args, scene, params, spec, prec):
     steps after the same set-up: its throughput and the contact counts it
     carries (the skin adds speculative contacts)."""
     import avbd
-    g = avbd.HipBackend(precision=prec, team=args.team, graphs=-1 if args.no_graphs else 0)
+    g = avbd.HipBackend(precision=prec, team=args.team, graphs=graphs_cfg(args))
     g.pack(scene, dict(params, contactModel=1))
     if spec is not None:
         settle_and_strike(g, spec)
-    g.step(max(args.warmup - 1, 0))
-    tune = g.autotune() if args.team == 0 and prec != avbd.PREC_EXACT else None
-    if tune is None and args.warmup > 0:
+    tune = None
+    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 2:
+        g.step(args.warmup - 2)
+        tune = g.autotune()
         g.step(1)
+    else:
+        g.step(args.warmup)
     g.idle()
     t0 = time.perf_counter()
     g.step(args.steps)
@@ -308,7 +319,10 @@ def main():
                     help="CPU baseline: oracle steps until about this many seconds of CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-graphs", action="store_true", help="launch the solve loop directly, without graph replay")
+    ap.add_argument("--graphs", default="step", choices=["step", "solve", "none"],
+                    help="graph replay: whole steps with no host read inside a step (step), each step's solve loop "
+                         "after host reads of its counts (solve), or direct launches (none)")
+    ap.add_argument("--no-graphs", action="store_true", help="the same as --graphs none")
     ap.add_argument("--no-exact", action="store_true",
                     help="pyramid: skip the exact-precision leg (its throughput and the pose error against it)")
     ap.add_argument("--no-launch", action="store_true",
@@ -338,7 +352,7 @@ def main():
     if args.contact_model == "gpu":
         params = dict(params, contactModel=1)
         desc += ", the GPU backend's contact law"
-    gpu = avbd.HipBackend(precision=prec, team=args.team, graphs=-1 if args.no_graphs else 0)
+    gpu = avbd.HipBackend(precision=prec, team=args.team, graphs=graphs_cfg(args))
     gpu.pack(scene, params)
     iters = params["iterations"]
     # The pyramid configuration settles for its first steps and is then struck
@@ -353,12 +367,15 @@ def main():
         print(f"[bench] settled {settle} steps and launched the sphere at the apex: "
               f"{time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    gpu.step(max(args.warmup - 1, 0))
     tune = None
-    if args.team == 0 and prec != avbd.PREC_EXACT:
-        tune = gpu.autotune()  # one warm-up step that also picks the team shape
-    elif args.warmup > 0:
+    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 2:
+        # the last warm-up step but one also picks the team shape; the last
+        # captures the replayed graphs for it
+        gpu.step(args.warmup - 2)
+        tune = gpu.autotune()
         gpu.step(1)
+    else:
+        gpu.step(args.warmup)
     gpu.idle()
     print(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, team {tune}", file=sys.stderr,
           flush=True)
@@ -366,7 +383,7 @@ def main():
     nd = c["dynamic_bodies"]
 
     start = gpu.read_state() if not args.no_cpu else None  # where the CPU baseline resumes
-    # timed region: plain stepping (the solve loop replays a captured graph)
+    # timed region: plain stepping (replayed graphs: whole steps by default)
     t0 = time.perf_counter()
     gpu.step(args.steps)
     gpu.idle()
@@ -420,6 +437,7 @@ def main():
                          f"timed steps {settle + args.warmup + 1}-{settle + args.warmup + args.steps} after pack"),
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
             "precision": args.precision, "team": team, "color_order": args.order, "contact_law": args.contact_model,
+            "graphs": "none" if args.no_graphs else args.graphs,
             "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],
             "manifolds": c["manifolds"], "contacts": c["contacts"], "joints": c["joints"], "colors": c["colors"],

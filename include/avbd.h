@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 4
+#define AVBD_ABI_VERSION 5
 
 /* Error codes. */
 #define AVBD_OK 0
@@ -72,8 +72,11 @@ typedef struct avbd_config {
                               avbd_repack grows it to 1.25x what was asked */
   int32_t pair_cap;        /* max broad-phase pairs (0: 32 per body); grows
                               the same way                                  */
-  int32_t graphs;          /* solve loop as a replayed hipGraph: 0 auto (on
-                              unless profiling), -1 never                   */
+  int32_t graphs;          /* replayed hipGraphs: 0 auto (the solve loop of
+                              each step, unless profiling); AVBD_GRAPHS_STEP
+                              (1) whole steps, broad phase to velocities,
+                              with no host read inside a step (see
+                              avbd_step); -1 never                          */
   int32_t adjoint_small;   /* largest dynamic-body count whose adjoint step
                               is reversed in one workgroup (0: 64; -1:
                               never, always one kernel per phase)           */
@@ -90,6 +93,9 @@ typedef struct avbd_config {
                               -1: never, one launch per colour and
                               iteration)                                    */
 } avbd_config;
+
+/* avbd_config.graphs: whole steps as replayed graphs. */
+#define AVBD_GRAPHS_STEP 1
 
 /* Solver parameters (Solver.defaultParams, solver.js:92-236; GPU uniform
  * block GU_FIELDS, layout.js:223-252). */
@@ -241,7 +247,23 @@ int avbd_pack(avbd_ctx *ctx, const avbd_scene *scene);
 int avbd_repack(avbd_ctx *ctx, const avbd_scene *scene, const int32_t *old_index);
 
 /* --- stepping: GpuBackend.step (backend.js:1393-1620); avbd_sync is idle()
- * (backend.js:1804-1818). */
+ * (backend.js:1804-1818).
+ * With avbd_config.graphs = AVBD_GRAPHS_STEP, every step after the first
+ * since a pack (which reads its counts on the host and sets the launch
+ * shapes) is one replayed graph that takes every count from device memory,
+ * as the reference's step takes its dispatch sizes from indirect buffers
+ * (backend.js:1496-1535): no host read, at most 4 steps in flight. The
+ * colouring then runs a fixed number of rounds and the sweep a fixed number
+ * of colours per iteration: the caller's limits (avbd_set_coloring_limits,
+ * GpuBackend.colorRounds / encColors) or, without them, one colour and four
+ * rounds above what the last step read needed. A step that needed more
+ * reports it (bit 32 of avbd_counters.overflow, as a limited colouring does)
+ * and the following steps launch more. A step that exceeds the pair or
+ * manifold capacity drops the excess (bits 1, 2 of the overflow word, as the
+ * reference's steps do); the next avbd_step returns AVBD_E_CAPACITY and the
+ * next avbd_pack or avbd_repack grows the capacity. Every other call first
+ * waits for the steps in flight. Profiling, recording for the adjoint, an
+ * uploaded colouring and the autotuner use steps with host reads. */
 int avbd_step(avbd_ctx *ctx, int32_t nsteps);
 int avbd_sync(avbd_ctx *ctx);
 
