@@ -25,7 +25,7 @@ SYMBOLS = (
     "avbd_set_params", "avbd_pack", "avbd_step", "avbd_sync", "avbd_read_state", "avbd_write_state",
     "avbd_read_pairs", "avbd_read_coloring", "avbd_read_forces", "avbd_read_manifold",
     "avbd_read_counters", "avbd_set_profiling", "avbd_read_profile", "avbd_read_kernel_stats",
-    "avbd_autotune", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
+    "avbd_autotune", "avbd_read_launch_shape", "avbd_set_launch_shape", "avbd_adjoint_forward", "avbd_adjoint_backward", "avbd_set_body_order",
     "avbd_upload_coloring", "avbd_export_poses", "avbd_export_poses_static", "avbd_pose_source", "avbd_repack",
     "avbd_read_joints",
     "avbd_read_step_start", "avbd_set_joint_anchor", "avbd_read_manifolds", "avbd_set_coloring_limits",
@@ -111,6 +111,8 @@ def lib(path=None):
         "avbd_read_profile": (C.c_int, [ctx, P(C.c_float), i32, pi]),
         "avbd_read_kernel_stats": (C.c_int, [ctx, i32, pd, P(C.c_int64), pd]),
         "avbd_autotune": (C.c_int, [ctx, pi, P(C.c_float)]),
+        "avbd_read_launch_shape": (C.c_int, [ctx, pi, pi, P(C.c_float)]),
+        "avbd_set_launch_shape": (C.c_int, [ctx, i32, i32]),
         "avbd_adjoint_forward": (C.c_int, [ctx, i32, i32]),
         "avbd_adjoint_backward": (C.c_int, [ctx, pd, pd, pd]),
         "avbd_set_body_order": (C.c_int, [ctx, pi, i32]),

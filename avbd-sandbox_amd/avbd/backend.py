@@ -389,6 +389,23 @@ class HipBackend:
         self._ck(self.L.avbd_autotune(self.ctx, C.byref(t), ms))
         return t.value, dict(zip((1, 4, 8, 16, 32, 64), [float(x) for x in ms]))
 
+    def launch_shape(self):
+        """(team, block, trials): lanes per body and per workgroup in use, and
+        the last autotune's times {(team, block): ms} (tried pairs only)."""
+        t, b = C.c_int32(), C.c_int32()
+        g = (C.c_float * 24)()
+        self._ck(self.L.avbd_read_launch_shape(self.ctx, C.byref(t), C.byref(b), g))
+        trials = {}
+        for i, tm in enumerate((1, 4, 8, 16, 32, 64)):
+            for j, blk in enumerate((64, 128, 256, 512)):
+                if g[i * 4 + j] > 0:
+                    trials[(tm, blk)] = float(g[i * 4 + j])
+        return t.value, b.value, trials
+
+    def set_launch_shape(self, team, block):
+        """Fix lanes per body and per workgroup (see launch_shape)."""
+        self._ck(self.L.avbd_set_launch_shape(self.ctx, int(team), int(block)))
+
     # --- adjoint (no counterpart in the reference)
     def adjoint_forward(self, steps, ckpt_every=10):
         """Step `steps` times keeping checkpoints every `ckpt_every` steps."""

@@ -174,6 +174,7 @@ struct avbd_ctx {
   int sceneSolveCfg = 0;     // avbd_config.scene_solve
   bool sceneSolveOK = false; // the packed batch qualifies (2+ scenes, small, joints grouped)
   int sceneTeam = 0;         // lanes per body in it (0: from team)
+  int sceneBlock = 0;        // lanes per scene (0: 256)
   DBuf<int> countRev, startRev, countNewRev, startNewRev;
   DBuf<int2> pairs, newPairs;
   int nPairs = 0, nNew = 0;
@@ -236,6 +237,10 @@ struct avbd_ctx {
   // autotuner
   bool tuneRequest = false;
   float tuneMs[6] = {0};
+  // every trial of the last autotune: [team 1, 4, 8, 16, 32, 64][workgroup
+  // 64, 128, 256, 512] in ms (0: not tried)
+  float tuneGrid[6][4] = {};
+  int sweepBlock = 256;  // workgroup of the per-colour sweep (tuned with the team)
   DBuf<char> Xbk, DQbk;
   int numColors = 0, colorRounds = 0;
   DBuf<int> itemCnt, itemStart, itemJoints, chist, coffs, ctotal;
@@ -630,10 +635,10 @@ struct avbd_ctx {
   // the average). A captured graph is replayed on later steps whose colours
   // may have grown: it gets a quarter more teams (and teams stride over any
   // excess, so a larger colour is still swept whole).
-  int primalBlocks(int c, int tm) const {
+  int primalBlocks(int c, int tm, int blk = 256) const {
     const long cnt = std::max(colorCount(c), 1L);
     const long teams = graphCapturing ? cnt + cnt / 4 + 64 : cnt;
-    return std::max(1, std::min(blocks(teams * tm), 8192));
+    return std::max(1, std::min(blocks(teams * tm, blk), 8192 * (256 / blk)));
   }
   int dualBlocks() const { return std::max(1, std::min(blocks(manCap, 128), 2048)); }
 
@@ -642,8 +647,10 @@ struct avbd_ctx {
                     double dualAlpha) {
     if (!graphCapturing && colorCount(c) <= 0) return;
     const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
+    // the tuned workgroup; the one-lane exact sweep keeps 256
+    const int blk = EXACT ? 256 : sweepBlock;
     timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal<R, EXACT, TEAM, true> : k_primal<R, EXACT, TEAM, false>,
-                dim3(primalBlocks(c, TEAM)), dim3(256), c,
+                dim3(primalBlocks(c, TEAM, blk)), dim3(blk), c,
                 (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
   }
 
@@ -746,9 +753,22 @@ struct avbd_ctx {
     const int t = sceneTeam > 0 ? sceneTeam : 4;
     return t <= 4 ? 4 : t <= 8 ? 8 : 16;
   }
+  // Workgroup of the one-launch batch solve (lanes per scene): the tuned
+  // size, else 256. With TEAM lanes per body, a workgroup updates
+  // BLOCK / TEAM bodies of a colour at once (the colour tile); a scene whose
+  // colour holds more takes several passes.
+  int sceneBlockShape() const { return sceneBlock == 128 || sceneBlock == 512 ? sceneBlock : 256; }
+
+  template <class R, int BLOCK>
+  static auto sceneKernel(int tm, bool law) {
+    auto k = law ? k_solve_scenes<R, false, 16, true, BLOCK> : k_solve_scenes<R, false, 16, false, BLOCK>;
+    if (tm == 4) k = law ? k_solve_scenes<R, false, 4, true, BLOCK> : k_solve_scenes<R, false, 4, false, BLOCK>;
+    if (tm == 8) k = law ? k_solve_scenes<R, false, 8, true, BLOCK> : k_solve_scenes<R, false, 8, false, BLOCK>;
+    return k;
+  }
 
   template <class R, bool EXACT>
-  void launchScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, int tm) {
+  void launchScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P, int tm, int blk) {
     SceneSolve Q;
     Q.sceneStart = sceneStart.p; Q.sceneJointStart = sceneJointStart.p; Q.cmeta = cmeta.p; Q.cwork = cwork.p;
     Q.V = V.p; Q.joints = joints.p; Q.iterations = prm.iterations; Q.alpha = prm.alpha;
@@ -760,19 +780,21 @@ struct avbd_ctx {
         bytes += double(prm.iterations) * double(hcbytes[c]) + double(prm.iterations - 1) * double(hcbytes[MAX_COLORS + c]);
     R *dq = (R *)DQ.p;
     if constexpr (EXACT) {
-      timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_solve_scenes<R, true, 1, true> : k_solve_scenes<R, true, 1, false>,
-                  dim3(S), dim3(256), Q, sa, mb, dq, P);
+      // the exact team's LDS rows are sized for 256 lanes
+      timedLaunch(AVBD_K_PRIMAL, bytes,
+                  gl() ? k_solve_scenes<R, true, 1, true, 256> : k_solve_scenes<R, true, 1, false, 256>, dim3(S),
+                  dim3(256), Q, sa, mb, dq, P);
     } else {
-      auto k = gl() ? k_solve_scenes<R, false, 16, true> : k_solve_scenes<R, false, 16, false>;
-      if (tm == 4) k = gl() ? k_solve_scenes<R, false, 4, true> : k_solve_scenes<R, false, 4, false>;
-      if (tm == 8) k = gl() ? k_solve_scenes<R, false, 8, true> : k_solve_scenes<R, false, 8, false>;
-      timedLaunch(AVBD_K_PRIMAL, bytes, k, dim3(S), dim3(256), Q, sa, mb, dq, P);
+      const auto k = blk == 128 ? sceneKernel<R, 128>(tm, gl())
+                     : blk == 512 ? sceneKernel<R, 512>(tm, gl())
+                                  : sceneKernel<R, 256>(tm, gl());
+      timedLaunch(AVBD_K_PRIMAL, bytes, k, dim3(S), dim3(blk == 128 || blk == 512 ? blk : 256), Q, sa, mb, dq, P);
     }
   }
 
   template <class R, bool EXACT>
   void enqueueScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
-    launchScenes<R, EXACT>(sa, mb, P, sceneTeamShape());
+    launchScenes<R, EXACT>(sa, mb, P, sceneTeamShape(), sceneBlockShape());
     if (graphCapturing || nLive > 0)
       timedLaunch(AVBD_K_DUAL, profiling ? dualBytes() : 0.0, gl() ? k_dual<R, EXACT, true> : k_dual<R, EXACT, false>,
                   dim3(dualBlocks()), dim3(128), (const int *)(scal.p + 2), (const int *)live[cur].p, mb,
@@ -780,12 +802,14 @@ struct avbd_ctx {
   }
 
   // The autotuner for the one-launch batch solve: the whole solve of this
-  // step timed for each team shape (lanes per body: 4, 8, 16; the tile of
-  // bodies a workgroup updates at once is 256 / lanes), restoring everything
-  // it writes between trials; the fastest is kept.
+  // step timed for each team shape (lanes per body: 4, 8, 16) and workgroup
+  // (lanes per scene: 128, 256, 512; the tile of bodies a workgroup updates
+  // at once is their ratio), restoring everything it writes between trials;
+  // the fastest pair is kept.
   template <class R>
   void autotuneScenes(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
     static const int cands[3] = {4, 8, 16};
+    static const int blks[3] = {128, 256, 512};
     const size_t xb = sizeof(double) * size_t(N) * XS, vb = sizeof(double) * size_t(N) * VS,
                  qb = size_t(N) * 8 * rsize(), jb = sizeof(JointDev) * size_t(J);
     const size_t nT = size_t(std::max(lastNTot, 1)), hb = nT * MAXC * HOT * rsize(), sb = nT * MAXC * sizeof(int32_t);
@@ -808,27 +832,31 @@ struct avbd_ctx {
     hipEvent_t a = takeEvent(), b = takeEvent();
     const bool prof = profiling;
     profiling = false;
-    int best = sceneTeamShape();
+    int best = sceneTeamShape(), bestBlk = sceneBlockShape();
     float bestMs = 1e30f;
     for (int k = 0; k < 6; k++) tuneMs[k] = 0.0f;
-    for (int k = 0; k < 3; k++) {
-      HIPCHECK(hipEventRecord(a, stream));
-      launchScenes<R, false>(sa, mb, P, cands[k]);
-      HIPCHECK(hipEventRecord(b, stream));
-      HIPCHECK(hipEventSynchronize(b));
-      float ms = 0;
-      HIPCHECK(hipEventElapsedTime(&ms, a, b));
-      tuneMs[k + 1] = ms;
-      if (ms < bestMs) { bestMs = ms; best = cands[k]; }
-      restore();
-    }
+    std::memset(tuneGrid, 0, sizeof(tuneGrid));
+    for (int k = 0; k < 3; k++)
+      for (int g = 0; g < 3; g++) {
+        HIPCHECK(hipEventRecord(a, stream));
+        launchScenes<R, false>(sa, mb, P, cands[k], blks[g]);
+        HIPCHECK(hipEventRecord(b, stream));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        tuneGrid[k + 1][g + 1] = ms;
+        if (tuneMs[k + 1] == 0.0f || ms < tuneMs[k + 1]) tuneMs[k + 1] = ms;
+        if (ms < bestMs) { bestMs = ms; best = cands[k]; bestBlk = blks[g]; }
+        restore();
+      }
     HIPCHECK(hipStreamSynchronize(stream));
     bX.release(); bX0.release(); bY.release(); bV.release(); bQ.release(); bJ.release(); bH.release(); bS.release();
     profiling = prof;
     evPool.push_back(a);
     evPool.push_back(b);
-    if (best != sceneTeam) dropGraphs();
+    if (best != sceneTeam || bestBlk != sceneBlock) dropGraphs();
     sceneTeam = best;
+    sceneBlock = bestBlk;
   }
 
   void dropGraphs() {
@@ -901,9 +929,12 @@ struct avbd_ctx {
   // One primal sweep (every colour, first iteration's alpha) per team shape,
   // timed with events; the body state it changes (X, DQ) is restored after
   // each trial. Keeps the fastest shape.
+  // Workgroups of 64, 128 and 256 lanes are tried with each team shape (a
+  // colour's teams are spread over the CUs in workgroup-sized pieces).
   template <class R>
   void autotuneT(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
     static const int cands[6] = {1, 4, 8, 16, 32, 64};
+    static const int blks[3] = {64, 128, 256};
     const size_t xb = sizeof(double) * size_t(N) * XS, qb = size_t(N) * 8 * rsize();
     Xbk.reserve(xb);
     DQbk.reserve(qb);
@@ -912,25 +943,34 @@ struct avbd_ctx {
     hipEvent_t a = takeEvent(), b = takeEvent();
     const bool prof = profiling;
     profiling = false;
-    int best = team;
+    int best = team, bestBlk = sweepBlock;
     float bestMs = 1e30f;
+    const int keepBlk = sweepBlock;
+    std::memset(tuneGrid, 0, sizeof(tuneGrid));
     for (int k = 0; k < 6; k++) {
-      HIPCHECK(hipEventRecord(a, stream));
-      for (int c = 0; c < numColors; c++) primal<R, false>(c, sa, mb, P, prm.alpha, cands[k], 0, prm.alpha);
-      HIPCHECK(hipEventRecord(b, stream));
-      HIPCHECK(hipEventSynchronize(b));
-      float ms = 0;
-      HIPCHECK(hipEventElapsedTime(&ms, a, b));
-      tuneMs[k] = ms;
-      if (ms < bestMs) { bestMs = ms; best = cands[k]; }
-      HIPCHECK(hipMemcpyAsync(X.p, Xbk.p, xb, hipMemcpyDeviceToDevice, stream));
-      HIPCHECK(hipMemcpyAsync(DQ.p, DQbk.p, qb, hipMemcpyDeviceToDevice, stream));
+      tuneMs[k] = 0.0f;
+      for (int g = 0; g < 3; g++) {
+        sweepBlock = blks[g];
+        HIPCHECK(hipEventRecord(a, stream));
+        for (int c = 0; c < numColors; c++) primal<R, false>(c, sa, mb, P, prm.alpha, cands[k], 0, prm.alpha);
+        HIPCHECK(hipEventRecord(b, stream));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        tuneGrid[k][g] = ms;
+        if (tuneMs[k] == 0.0f || ms < tuneMs[k]) tuneMs[k] = ms;
+        if (ms < bestMs) { bestMs = ms; best = cands[k]; bestBlk = blks[g]; }
+        HIPCHECK(hipMemcpyAsync(X.p, Xbk.p, xb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(DQ.p, DQbk.p, qb, hipMemcpyDeviceToDevice, stream));
+      }
     }
+    sweepBlock = keepBlk;
     profiling = prof;
     evPool.push_back(a);
     evPool.push_back(b);
-    if (best != team) dropGraphs();
+    if (best != team || bestBlk != sweepBlock) dropGraphs();
     team = best;
+    sweepBlock = bestBlk;
   }
 
   // ---------------------------------------------------------------- step
@@ -2588,6 +2628,35 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
     ctx->step();
     if (team) *team = ctx->useScenes() ? ctx->sceneTeamShape() : ctx->team;
     if (ms) for (int k = 0; k < 6; k++) ms[k] = ctx->tuneMs[k];
+  });
+}
+
+int avbd_read_launch_shape(avbd_ctx *ctx, int32_t *team, int32_t *block, float *grid) {
+  return guarded(ctx, [&] {
+    const bool scenes = ctx->packed && ctx->useScenes();
+    if (team) *team = scenes ? ctx->sceneTeamShape() : ctx->team;
+    if (block) *block = scenes ? ctx->sceneBlockShape() : ctx->sweepBlock;
+    if (grid) std::memcpy(grid, ctx->tuneGrid, sizeof(ctx->tuneGrid));
+  });
+}
+
+int avbd_set_launch_shape(avbd_ctx *ctx, int32_t team, int32_t block) {
+  return guarded(ctx, [&] {
+    if (ctx->prec == AVBD_PREC_EXACT) throw ApiError(AVBD_E_STATE, "the exact precision has fixed launch shapes");
+    if (ctx->packed && ctx->useScenes()) {
+      if ((team != 4 && team != 8 && team != 16) || (block != 128 && block != 256 && block != 512))
+        throw ApiError(AVBD_E_INVALID, "a batch solved in one launch takes team 4, 8 or 16 and block 128, 256 or 512");
+      ctx->sceneTeam = team;
+      ctx->sceneBlock = block;
+    } else {
+      if ((team != 1 && team != 4 && team != 8 && team != 16 && team != 32 && team != 64) ||
+          (block != 64 && block != 128 && block != 256))
+        throw ApiError(AVBD_E_INVALID, "team must be 1, 4, 8, 16, 32 or 64 and block 64, 128 or 256");
+      ctx->team = team;
+      ctx->sweepBlock = block;
+    }
+    ctx->dropAdjoint();
+    ctx->dropGraphs();  // captured with the old shapes
   });
 }
 

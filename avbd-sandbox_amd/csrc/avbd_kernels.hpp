@@ -1953,23 +1953,25 @@ __device__ __forceinline__ int firstBodyAtLeast(const int4 *cwork, int lo, int h
   return lo;
 }
 
-template <class R, bool EXACT, int TEAM, bool GL>
-__global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S, ManBuf<R> mb, R *DQ,
+// BLOCK: the workgroup (lanes per scene); BLOCK / TEAM bodies of a colour
+// are updated at once (the colour tile).
+template <class R, bool EXACT, int TEAM, bool GL, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_solve_scenes(SceneSolve Q, SolveArgs S, ManBuf<R> mb, R *DQ,
                                                      StepParams P) {
   constexpr int T = EXACT ? XT_TEAM : TEAM;
-  __shared__ double rows[EXACT ? 256 / XT_TEAM : 1][EXACT ? XT_TEAM * XT_ROW : 1];
+  __shared__ double rows[EXACT ? BLOCK / XT_TEAM : 1][EXACT ? XT_TEAM * XT_ROW : 1];
   __shared__ int rlo[MAX_COLORS], rhi[MAX_COLORS];
   const int sc = blockIdx.x, tid = threadIdx.x;
   const int n0 = Q.sceneStart[sc], n1 = Q.sceneStart[sc + 1];
   const int j0 = Q.sceneJointStart[sc], j1 = Q.sceneJointStart[sc + 1];
   const int lane = tid % T, team = tid / T;
-  constexpr int nteams = 256 / T;
+  constexpr int nteams = BLOCK / T;
   double *X0 = const_cast<double *>(S.X0), *Y = const_cast<double *>(S.Y);
-  for (int i = n0 + tid; i < n1; i += 256) bodyInitOne<R>(i, S.props, S.X, X0, Y, Q.V, DQ, P);
+  for (int i = n0 + tid; i < n1; i += BLOCK) bodyInitOne<R>(i, S.props, S.X, X0, Y, Q.V, DQ, P);
   // the scene's run of each colour's work list, found once per step
   const int ncol = Q.cmeta[4];
   const int *cst = Q.cmeta + 8;
-  for (int c = tid; c < ncol; c += 256) {
+  for (int c = tid; c < ncol; c += BLOCK) {
     const int lo = firstBodyAtLeast(Q.cwork, cst[c], cst[c + 1], n0);
     rlo[c] = lo;
     rhi[c] = firstBodyAtLeast(Q.cwork, lo, cst[c + 1], n1);
@@ -1996,10 +1998,10 @@ __global__ void __launch_bounds__(256) k_solve_scenes(SceneSolve Q, SolveArgs S,
       }
       __syncthreads();
     }
-    for (int j = j0 + tid; j < j1; j += 256) jointDualOne(Q.joints[j], S.X, P, Q.alpha);
+    for (int j = j0 + tid; j < j1; j += BLOCK) jointDualOne(Q.joints[j], S.X, P, Q.alpha);
     __syncthreads();
   }
-  for (int i = n0 + tid; i < n1; i += 256) velocityOne(i, S.props, S.X, S.X0, Q.V, P.dt);
+  for (int i = n0 + tid; i < n1; i += BLOCK) velocityOne(i, S.props, S.X, S.X0, Q.V, P.dt);
 }
 
 // Dual update of every live manifold (manifold.js:447-491); the live count

@@ -414,6 +414,7 @@ def main():
     avg_ms = pms / max(plaunch, 1)
     achieved = (pbytes / max(plaunch, 1)) / (avg_ms * 1e-3) / 1e9 if pms > 0 else 0.0
     team = tune[0] if tune else args.team
+    shape = gpu.launch_shape()
     kern = {"exact": ("k_primal_xt",), "f64": ("k_primal<double",), "mixed": ("k_primal<float",)}[args.precision]
     if args.workload == "batched":
         kern = ("k_solve_scenes<" + ("double" if args.precision != "mixed" else "float"),)
@@ -436,7 +437,8 @@ def main():
                                          f"apex at 20 m/s; " if settle else "") +
                          f"timed steps {settle + args.warmup + 1}-{settle + args.warmup + args.steps} after pack"),
             "bodies": len(scene["bodies"]), "dynamic_bodies": nd, "iterations": iters,
-            "precision": args.precision, "team": team, "color_order": args.order, "contact_law": args.contact_model,
+            "precision": args.precision, "team": team, "workgroup": shape[1], "color_order": args.order,
+            "contact_law": args.contact_model,
             "graphs": "none" if args.no_graphs else args.graphs,
             "autotune_ms_per_sweep": tune[1] if tune else None,
             "color_rounds": c["color_rounds"],

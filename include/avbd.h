@@ -394,6 +394,23 @@ int avbd_read_kernel_stats(avbd_ctx *ctx, int32_t kernel, double *ms_total, int6
  * AVBD_PREC_EXACT, whose assembly order is fixed. */
 int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
 
+/* Launch shape (no counterpart in the reference). avbd_autotune also times
+ * each team shape at several workgroup sizes: the per-colour sweep at 64, 128
+ * and 256 lanes per workgroup, the one-launch batch solve at 128, 256 and 512
+ * lanes per scene (with TEAM lanes per body a scene's workgroup updates
+ * block / TEAM bodies of a colour at once), and keeps the fastest pair; `ms`
+ * of avbd_autotune then holds each team shape's best time.
+ * avbd_read_launch_shape returns the pair in use (`team`: lanes per body,
+ * `block`: lanes per workgroup of the solve's launches) and, in `grid` (may
+ * be NULL; 24 floats), every trial of the last autotune as
+ * [team 1, 4, 8, 16, 32, 64][workgroup 64, 128, 256, 512] in ms, 0 where not
+ * tried. avbd_set_launch_shape fixes the pair (team in {1, 4, 8, 16, 32, 64},
+ * block in {64, 128, 256} for the per-colour sweep; for a batch solved in one
+ * launch, team in {4, 8, 16} and block in {128, 256, 512}); results do not
+ * depend on the workgroup. AVBD_PREC_EXACT keeps its fixed shapes. */
+int avbd_read_launch_shape(avbd_ctx *ctx, int32_t *team, int32_t *block, float *grid);
+int avbd_set_launch_shape(avbd_ctx *ctx, int32_t team, int32_t block);
+
 /* Verification hook: the claim order of the greedy colouring, and so the
  * sweep's schedule (the reference CPU engine's Solver.bodyOrder,
  * solver.js:225-233 and 419-422, reproduces a colour schedule from the

@@ -226,6 +226,62 @@ def test_autotune_picks_the_fastest_team():
     assert err.value.code == -4  # the exact mode has one team shape
 
 
+def test_workgroup_size_changes_no_bits():
+    # a body's update is one team's work whichever workgroup holds the team:
+    # the per-colour sweep at 64, 128 and 256 lanes per workgroup, and the
+    # one-launch batch at 128, 256 and 512 lanes per scene, give the same bits
+    scene, params, _ = avbd.scenes.pyramid(layers=8)
+    runs = []
+    for blk in (64, 128, 256):
+        g = avbd.HipBackend(precision=avbd.PREC_MIXED, team=16)
+        g.pack(scene, params)
+        g.set_launch_shape(16, blk)
+        g.step(8)
+        assert g.launch_shape()[:2] == (16, blk)
+        runs.append(g.read_state())
+    for st in runs[1:]:
+        assert np.array_equal(st["pos"], runs[0]["pos"]) and np.array_equal(st["quat"], runs[0]["quat"])
+    bscene, bparams, _ = avbd.scenes.batched(num_scenes=12)
+    runs = []
+    for blk in (128, 256, 512):
+        g = avbd.HipBackend(precision=avbd.PREC_MIXED)
+        g.pack(bscene, bparams)
+        g.set_launch_shape(4, blk)
+        g.step(6)
+        assert g.launch_shape()[:2] == (4, blk)
+        runs.append(g.read_state())
+    for st in runs[1:]:
+        assert np.array_equal(st["pos"], runs[0]["pos"]) and np.array_equal(st["quat"], runs[0]["quat"])
+    with pytest.raises(AvbdError) as err:
+        g.set_launch_shape(4, 64)  # not a batch workgroup
+    assert err.value.code == -1
+
+
+def test_autotune_tries_workgroup_sizes():
+    scene, params, _ = avbd.scenes.pyramid(layers=8)
+    a = avbd.HipBackend(precision=avbd.PREC_MIXED)
+    a.pack(scene, params)
+    a.step(2)
+    team, ms = a.autotune()
+    t, blk, trials = a.launch_shape()
+    print(f"autotune: team {t}, block {blk}, trials {trials}")
+    assert t == team and blk in (64, 128, 256)
+    assert set(trials) == {(tm, b) for tm in (1, 4, 8, 16, 32, 64) for b in (64, 128, 256)}
+    assert trials[(t, blk)] == min(trials.values())
+    assert all(ms[tm] == min(trials[(tm, b)] for b in (64, 128, 256)) for tm in ms)
+    bscene, bparams, _ = avbd.scenes.batched(num_scenes=16)
+    b = avbd.HipBackend(precision=avbd.PREC_MIXED)
+    b.pack(bscene, bparams)
+    b.step(2)
+    b.autotune()
+    t, blk, trials = b.launch_shape()
+    print(f"batch autotune: team {t}, block {blk}, trials {trials}")
+    assert set(trials) == {(tm, g) for tm in (4, 8, 16) for g in (128, 256, 512)}
+    assert trials[(t, blk)] == min(trials.values())
+    b.step(3)
+    assert np.isfinite(b.read_state()["pos"]).all()
+
+
 def test_autotune_trials_restore_the_state():
     # spheres resting on the ground apart from each other: one contact item
     # per body, so every team shape sums the same terms exactly and a run
