@@ -232,8 +232,11 @@ __device__ __forceinline__ void insertDesc(int2 *list, int base, int cnt, int ca
 // scene's large bodies L < x, sorted by y descending (the brute-force order
 // i = N-1..1, j = i-1..0 of broadphase.js:236-243 restricted to one i).
 //
-// One pass with a team of 32 lanes per body: lanes 0-26 scan one neighbour
-// cell each, lanes 27-31 the scene's large bodies. The accepted pairs are
+// One pass with a team of 32 lanes per body: the 27 neighbour cells' bucket
+// entries are numbered as one list (a team prefix sum over their counts) and
+// the lanes take its entries in turn, so that a crowded cell is shared by
+// the whole team rather than left to one lane; then the scene's large bodies
+// likewise. The accepted pairs are
 // ranked by y in LDS and written to the body's slot of a scratch list (at most
 // BP_CAP entries; a body with more is listed again, serially, when the lists
 // are emitted). The counts go to countRev / countNewRev (indexed n-1-x so an
@@ -245,6 +248,7 @@ __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
   __shared__ int sy[TEAMS][BP_CAP];
   __shared__ int scount[TEAMS], snew[TEAMS];
   __shared__ uint32_t sadj[TEAMS][ADJ_LDS];
+  __shared__ int sbs[TEAMS][27], spre[TEAMS][27];  // each neighbour cell's bucket start and first list index
   const int tm = threadIdx.x / BP_TEAM, lane = threadIdx.x % BP_TEAM;
   const int x = blockIdx.x * TEAMS + tm;
   if (lane == 0) { scount[tm] = 0; snew[tm] = 0; }
@@ -261,8 +265,26 @@ __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
         sadj[tm][k] = (uint32_t(a.adjOther[e0 + k]) & 0x3fffffffu) | (kind << 30);
       }
   }
-  __syncthreads();
   const bool active = x < a.n && !a.props[x].large;
+  // the neighbour cells' buckets: lane q < 27 looks up cell q, and a prefix
+  // sum over the team numbers their entries as one list
+  const int4 c = active ? a.cellOf[x] : make_int4(0, 0, 0, 0);
+  int cntq = 0;
+  if (active && lane < 27) {
+    const int qx = c.x + lane % 3 - 1, qy = c.y + (lane / 3) % 3 - 1, qz = c.z + lane / 9 - 1;
+    const uint32_t bkt = cellHash(c.w, qx, qy, qz) & a.mask;
+    sbs[tm][lane] = a.bucketStart[bkt];
+    cntq = a.bucketCnt[bkt];
+  }
+  int incl = cntq;
+#pragma unroll
+  for (int off = 1; off < BP_TEAM; off <<= 1) {
+    const int u = __shfl_up(incl, off, BP_TEAM);
+    if (lane >= off) incl += u;
+  }
+  if (lane < 27) spre[tm][lane] = incl - cntq;
+  const int total = __shfl(incl, 26, BP_TEAM);
+  __syncthreads();
   const bool inLds = alen <= ADJ_LDS;
   auto test = [&](int y) { return inLds ? pairTestList(a, x, y, sadj[tm], alen) : pairTest(a, x, y); };
   if (active) {
@@ -271,28 +293,29 @@ __global__ void __launch_bounds__(256) k_bp_team(BpArgs a) {
       if (t == 2) atomicAdd(&snew[tm], 1);
       if (k < BP_CAP) sy[tm][k] = y | (t == 2 ? BP_NEW : 0);
     };
-    const int4 c = a.cellOf[x];
-    if (lane < 27) {
-      const int qx = c.x + lane % 3 - 1, qy = c.y + (lane / 3) % 3 - 1, qz = c.z + lane / 9 - 1;
-      const uint32_t bkt = cellHash(c.w, qx, qy, qz) & a.mask;
-      const int s0 = a.bucketStart[bkt], s1 = s0 + a.bucketCnt[bkt];
-      for (int e = s0; e < s1; e++) {
-        const int y = a.entries[e];
-        if (y >= x) continue;
-        const int4 cy = a.entryCell[e];
-        if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
-        const int t = test(y);
-        if (t) accept(y, t);
-      }
-    } else {
-      const int sc = a.props[x].scene;
-      const int l0 = a.sceneLargeStart[sc], l1 = a.sceneLargeStart[sc + 1];
-      for (int e = l0 + lane - 27; e < l1; e += BP_TEAM - 27) {
-        const int y = a.sceneLarges[e];
-        if (y >= x) continue;
-        const int t = test(y);
-        if (t) accept(y, t);
-      }
+    for (int j = lane; j < total; j += BP_TEAM) {
+      // the cell holding list index j: the last one whose first index is at
+      // most j (cells with no entries share their successor's first index)
+      int q = 0;
+#pragma unroll
+      for (int step = 16; step > 0; step >>= 1)
+        if (q + step < 27 && spre[tm][q + step] <= j) q += step;
+      const int e = sbs[tm][q] + (j - spre[tm][q]);
+      const int qx = c.x + q % 3 - 1, qy = c.y + (q / 3) % 3 - 1, qz = c.z + q / 9 - 1;
+      const int y = a.entries[e];
+      if (y >= x) continue;
+      const int4 cy = a.entryCell[e];
+      if (cy.x != qx || cy.y != qy || cy.z != qz || cy.w != c.w) continue;
+      const int t = test(y);
+      if (t) accept(y, t);
+    }
+    const int sc = a.props[x].scene;
+    const int l0 = a.sceneLargeStart[sc], l1 = a.sceneLargeStart[sc + 1];
+    for (int e = l0 + lane; e < l1; e += BP_TEAM) {
+      const int y = a.sceneLarges[e];
+      if (y >= x) continue;
+      const int t = test(y);
+      if (t) accept(y, t);
     }
   }
   __syncthreads();
