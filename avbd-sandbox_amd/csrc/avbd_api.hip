@@ -859,7 +859,17 @@ struct avbd_ctx {
     sceneBlock = bestBlk;
   }
 
+  // Graph executables are destroyed only when no replay of them is pending:
+  // a replay still queued or running on the stream, or a tool's completion
+  // callback for its kernels, must not outlive the executable it came from.
+  void quiesceForGraphs() {
+    if (!stream || graphs.empty()) return;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) == hipSuccess && st == hipStreamCaptureStatusNone)
+      (void)hipStreamSynchronize(stream);
+  }
   void dropGraphs() {
+    quiesceForGraphs();
     for (auto &g : graphs) {
       (void)hipGraphExecDestroy(g.second.second);
       (void)hipGraphDestroy(g.second.first);
@@ -1164,6 +1174,7 @@ struct avbd_ctx {
     pipeTotEst = lastNTot;
   }
   void dropStepGraphs() {
+    quiesceForGraphs();
     for (auto it = graphs.begin(); it != graphs.end();) {
       if (it->first >= (1LL << 40)) {
         (void)hipGraphExecDestroy(it->second.second);
