@@ -277,6 +277,11 @@ struct avbd_ctx {
     }
   };
   std::vector<TapeStep> tape;
+  // The backward with segment re-runs reverses one segment on rstream while
+  // the segment before it is re-run on stream, into the other half of a
+  // double-buffered segment tape (default priority, like stream).
+  hipStream_t rstream = nullptr;
+  hipEvent_t evRerun = nullptr, evRev[2] = {nullptr, nullptr};
   TapeStep *rec = nullptr;  // the step being recorded, if any
   DBuf<int> srcSlot;        // per new contact: matched old slot | kept-anchor bit, or -1
   int lastNTot = 0;
@@ -427,6 +432,12 @@ struct avbd_ctx {
     pc.release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+    if (rstream) (void)hipStreamDestroy(rstream);
+    rstream = nullptr;
+    for (hipEvent_t *e : {&evRerun, &evRev[0], &evRev[1]}) {
+      if (*e) (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
   }
 
   size_t rsize() const { return prec == AVBD_PREC_MIXED ? sizeof(float) : sizeof(double); }
@@ -1519,7 +1530,7 @@ struct avbd_ctx {
 
   // The reverse of one recorded step; lk selects which contact-adjoint buffer
   // holds this step's slots.
-  void reverseStep(const TapeStep &T, int &lk) {
+  void reverseStep(const TapeStep &T, int &lk, hipStream_t rs) {
     const StepParams P = sp();
     const int I = prm.iterations;
     const AdjStep S = adjView(T);
@@ -1528,39 +1539,39 @@ struct avbd_ctx {
     if (nDynamic <= adjSmallMax) {
       // a small scene: the whole reverse step in one workgroup
       if (wide)
-        hipLaunchKernelGGL(k_adj_step_small<16>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, stream, N, T.nTot,
+        hipLaunchKernelGGL(k_adj_step_small<16>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, rs, N, T.nTot,
                            (const BodyProp *)props.p, S, A, I, P);
       else
-        hipLaunchKernelGGL(k_adj_step_small<4>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, stream, N, T.nTot,
+        hipLaunchKernelGGL(k_adj_step_small<4>, dim3(1), dim3(ADJ_SMALL_THREADS), 0, rs, N, T.nTot,
                            (const BodyProp *)props.p, S, A, I, P);
       HIPCHECK(hipGetLastError());
       lk = 1 - lk;
       return;
     }
-    hipLaunchKernelGGL(k_adj_velocity, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, I,
+    hipLaunchKernelGGL(k_adj_velocity, dim3(blocks(N)), dim3(256), 0, rs, N, (const BodyProp *)props.p, S, A, I,
                        prm.dt);
     const int gb = std::max(1, std::min(blocks(long(std::max(nDynamic, 1)) * (wide ? 16 : 4)), 2048));
     // each iteration's dual adjoint rides with its first colour in reverse
     for (int l = I - 1; l >= 0; l--) {
       for (int c = T.numColors - 1; c >= 0; c--) {
         if (wide)
-          hipLaunchKernelGGL(k_adj_primal<16>, dim3(gb), dim3(256), 0, stream, c, S, A, (const BodyProp *)props.p, l,
+          hipLaunchKernelGGL(k_adj_primal<16>, dim3(gb), dim3(256), 0, rs, c, S, A, (const BodyProp *)props.p, l,
                              P);
         else
-          hipLaunchKernelGGL(k_adj_primal<4>, dim3(gb), dim3(256), 0, stream, c, S, A, (const BodyProp *)props.p, l,
+          hipLaunchKernelGGL(k_adj_primal<4>, dim3(gb), dim3(256), 0, rs, c, S, A, (const BodyProp *)props.p, l,
                              P);
       }
     }
     if (T.nLive > 0)
-      hipLaunchKernelGGL(k_adj_narrow, dim3(blocks(T.nLive, 128)), dim3(128), 0, stream, S, A,
+      hipLaunchKernelGGL(k_adj_narrow, dim3(blocks(T.nLive, 128)), dim3(128), 0, rs, S, A,
                          (const BodyProp *)props.p, P);
-    hipLaunchKernelGGL(k_adj_init, dim3(blocks(N)), dim3(256), 0, stream, N, (const BodyProp *)props.p, S, A, P);
+    hipLaunchKernelGGL(k_adj_init, dim3(blocks(N)), dim3(256), 0, rs, N, (const BodyProp *)props.p, S, A, P);
     if (T.nTot) {
       const size_t slots = size_t(T.nTot) * MAXC;
-      HIPCHECK(hipMemsetAsync(aLK[lk].p, 0, sizeof(double) * slots * 6, stream));
-      HIPCHECK(hipMemsetAsync(aC0b.p, 0, sizeof(double) * slots * 3, stream));
-      HIPCHECK(hipMemsetAsync(aRWb.p, 0, sizeof(double) * slots * 6, stream));
-      HIPCHECK(hipMemsetAsync(aMUb.p, 0, sizeof(double) * T.nTot, stream));
+      HIPCHECK(hipMemsetAsync(aLK[lk].p, 0, sizeof(double) * slots * 6, rs));
+      HIPCHECK(hipMemsetAsync(aC0b.p, 0, sizeof(double) * slots * 3, rs));
+      HIPCHECK(hipMemsetAsync(aRWb.p, 0, sizeof(double) * slots * 6, rs));
+      HIPCHECK(hipMemsetAsync(aMUb.p, 0, sizeof(double) * T.nTot, rs));
     }
     HIPCHECK(hipGetLastError());
     lk = 1 - lk;
@@ -1588,20 +1599,43 @@ struct avbd_ctx {
     } reset{this};
     int lk = 0;
     if (tapeResident) {
-      for (int k = adjSteps - 1; k >= 0; k--) reverseStep(tape[size_t(k)], lk);
+      for (int k = adjSteps - 1; k >= 0; k--) reverseStep(tape[size_t(k)], lk, stream);
     }
-    if (!tapeResident && tape.size() < size_t(adjEvery)) tape.resize(size_t(adjEvery));
+    // Segment re-runs, pipelined: segment seg is re-run on stream into half
+    // seg % 2 of the segment tape, then reversed on rstream while segment
+    // seg - 1 is re-run into the other half (which the reverse of segment
+    // seg + 1 has finished reading). The reverse reads only the tape, the
+    // bodies' properties and the adjoint buffers, none of which a re-run
+    // writes; the segments are still reversed in order, last to first.
     const int nseg = tapeResident ? 0 : int(ckpts.size());
-    for (int seg = nseg - 1; seg >= 0; seg--) {
+    const size_t E = size_t(adjEvery);
+    if (nseg > 0 && tape.size() < 2 * E) tape.resize(2 * E);
+    auto rerun = [&](int seg) {
+      const size_t half = size_t(seg % 2) * E;
       restoreCheckpoint(ckpts[size_t(seg)]);
       const int len = std::min(adjEvery, adjSteps - seg * adjEvery);
       for (int k = 0; k < len; k++) {
-        rec = &tape[size_t(k)];
+        rec = &tape[half + size_t(k)];
         step();
       }
       rec = nullptr;
-      for (int k = len - 1; k >= 0; k--) reverseStep(tape[size_t(k)], lk);
+      HIPCHECK(hipEventRecord(evRerun, stream));
+    };
+    if (nseg > 0) rerun(nseg - 1);
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+      const size_t half = size_t(seg % 2) * E;
+      const int len = std::min(adjEvery, adjSteps - seg * adjEvery);
+      HIPCHECK(hipStreamWaitEvent(rstream, evRerun, 0));
+      for (int k = len - 1; k >= 0; k--) reverseStep(tape[half + size_t(k)], lk, rstream);
+      HIPCHECK(hipEventRecord(evRev[seg % 2], rstream));
+      if (seg > 0) {
+        // the half segment seg - 1 records into was last read by the reverse
+        // of segment seg + 1 (its buffers may be regrown on the host)
+        if (seg + 1 < nseg) HIPCHECK(hipEventSynchronize(evRev[(seg + 1) % 2]));
+        rerun(seg - 1);
+      }
     }
+    if (nseg > 0) HIPCHECK(hipStreamWaitEvent(stream, evRev[0], 0));
     restoreCheckpoint(endState);
     std::vector<double> mu(static_cast<size_t>(N)), vb(static_cast<size_t>(N) * VS);
     HIPCHECK(hipMemcpyAsync(mu.data(), aMu.p, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
@@ -1708,6 +1742,10 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.releaseAll();
   HIPCHECK(hipSetDevice(c.device));
   HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&c.rstream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&c.evRerun, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&c.evRev[0], hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&c.evRev[1], hipEventDisableTiming));
   HIPCHECK(hipHostMalloc((void **)&c.hscal, sizeof(int) * 16, hipHostMallocDefault));
   HIPCHECK(hipHostMalloc((void **)&c.pipeRing, sizeof(int) * c.PIPE_SLOTS * PIPE_STRIDE, hipHostMallocDefault));
   std::memset(c.pipeRing, 0, sizeof(int) * c.PIPE_SLOTS * PIPE_STRIDE);
