@@ -133,6 +133,10 @@ struct avbd_ctx {
   // exact precision: a team of lanes per body with the additions in the
   // reference's order (k_primal_xt), or one lane per body (config team 1)
   bool exactTeam = true;
+  // workgroup of the exact team sweep (avbd_set_launch_shape): 64 lanes, two
+  // bodies, by default (the 51k pyramid: 8.99 ms per step against 9.38 at 128
+  // and 9.85 at 256)
+  int exactBlock = 64;
   int simds = 1024;  // SIMDs of the device (4 per CU), read at creation
   int manCapCfg = 0, pairCapCfg = 0;
   int needPairs = 0, needMan = 0;  // sizes an overflow asked for, for the next pack
@@ -692,8 +696,11 @@ struct avbd_ctx {
       // a team of lanes per body, the additions kept in the reference's
       // order; bodies with joints then take the one-lane sweep
       const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
-      timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal_xt<true> : k_primal_xt<false>,
-                  dim3(primalBlocks(c, XT_TEAM)), dim3(256), c,
+      const int blk = exactBlock;
+      auto xk = blk == 64    ? (gl() ? k_primal_xt<true, 64> : k_primal_xt<false, 64>)
+                : blk == 128 ? (gl() ? k_primal_xt<true, 128> : k_primal_xt<false, 128>)
+                             : (gl() ? k_primal_xt<true, 256> : k_primal_xt<false, 256>);
+      timedLaunch(AVBD_K_PRIMAL, bytes, xk, dim3(primalBlocks(c, XT_TEAM, blk)), dim3(blk), c,
                   (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (double *)DQ.p, P, alpha, dual,
                   dualAlpha);
       if (J > 0) {
@@ -2683,16 +2690,21 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms) {
 int avbd_read_launch_shape(avbd_ctx *ctx, int32_t *team, int32_t *block, float *grid) {
   return guarded(ctx, [&] {
     const bool scenes = ctx->packed && ctx->useScenes();
-    if (team) *team = scenes ? ctx->sceneTeamShape() : ctx->team;
-    if (block) *block = scenes ? ctx->sceneBlockShape() : ctx->sweepBlock;
+    const bool xt = ctx->prec == AVBD_PREC_EXACT;
+    if (team) *team = scenes ? ctx->sceneTeamShape() : xt ? (ctx->exactTeam ? XT_TEAM : 1) : ctx->team;
+    if (block) *block = scenes ? ctx->sceneBlockShape() : xt ? ctx->exactBlock : ctx->sweepBlock;
     if (grid) std::memcpy(grid, ctx->tuneGrid, sizeof(ctx->tuneGrid));
   });
 }
 
 int avbd_set_launch_shape(avbd_ctx *ctx, int32_t team, int32_t block) {
   return guarded(ctx, [&] {
-    if (ctx->prec == AVBD_PREC_EXACT) throw ApiError(AVBD_E_STATE, "the exact precision has fixed launch shapes");
-    if (ctx->packed && ctx->useScenes()) {
+    if (ctx->prec == AVBD_PREC_EXACT) {
+      // the exact team (XT_TEAM lanes per body) is fixed; its workgroup is not
+      if (team != XT_TEAM || (block != 64 && block != 128 && block != 256))
+        throw ApiError(AVBD_E_INVALID, "the exact precision takes team 32 and block 64, 128 or 256");
+      ctx->exactBlock = block;
+    } else if (ctx->packed && ctx->useScenes()) {
       if ((team != 4 && team != 8 && team != 16) || (block != 128 && block != 256 && block != 512))
         throw ApiError(AVBD_E_INVALID, "a batch solved in one launch takes team 4, 8 or 16 and block 128, 256 or 512");
       ctx->sceneTeam = team;

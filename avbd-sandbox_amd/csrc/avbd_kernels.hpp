@@ -1908,11 +1908,12 @@ __device__ __forceinline__ void primalBodyXT(const int4 w, const int lane, doubl
   waveSync();  // row 0 is the next body's
 }
 
-template <bool GL>
-__global__ void __launch_bounds__(256, 2) k_primal_xt(int c, const int *cstart, const int4 *cwork, SolveArgs S,
+// BLOCK lanes per workgroup (BLOCK / XT_TEAM bodies at once).
+template <bool GL, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_primal_xt(int c, const int *cstart, const int4 *cwork, SolveArgs S,
                                                    ManBuf<double> mb, double *DQ, StepParams P, double alpha,
                                                    int dual, double dualAlpha) {
-  __shared__ double rows[256 / XT_TEAM][XT_TEAM * XT_ROW];
+  __shared__ double rows[BLOCK / XT_TEAM][XT_TEAM * XT_ROW];
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = gt % XT_TEAM, teams = gridDim.x * blockDim.x / XT_TEAM;
   double *L = rows[threadIdx.x / XT_TEAM];

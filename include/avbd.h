@@ -407,7 +407,9 @@ int avbd_autotune(avbd_ctx *ctx, int32_t *team, float *ms);
  * tried. avbd_set_launch_shape fixes the pair (team in {1, 4, 8, 16, 32, 64},
  * block in {64, 128, 256} for the per-colour sweep; for a batch solved in one
  * launch, team in {4, 8, 16} and block in {128, 256, 512}); results do not
- * depend on the workgroup. AVBD_PREC_EXACT keeps its fixed shapes. */
+ * depend on the workgroup. AVBD_PREC_EXACT keeps its team (32 lanes per
+ * body, 1 with avbd_config.team = 1) and takes block 64, 128 or 256 for its
+ * team sweep (avbd_set_launch_shape(ctx, 32, block)). */
 int avbd_read_launch_shape(avbd_ctx *ctx, int32_t *team, int32_t *block, float *grid);
 int avbd_set_launch_shape(avbd_ctx *ctx, int32_t team, int32_t block);
 

@@ -241,6 +241,17 @@ def test_workgroup_size_changes_no_bits():
         runs.append(g.read_state())
     for st in runs[1:]:
         assert np.array_equal(st["pos"], runs[0]["pos"]) and np.array_equal(st["quat"], runs[0]["quat"])
+    # the exact team sweep at each workgroup: the oracle's bits
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR)
+    ref.step(8)
+    for blk in (64, 128, 256):
+        g = avbd.HipBackend(precision=avbd.PREC_EXACT)
+        g.pack(scene, params)
+        g.set_launch_shape(32, blk)
+        g.step(8)
+        assert g.launch_shape()[:2] == (32, blk)
+        st = g.read_state()
+        assert np.array_equal(st["pos"], ref.state()[:, :3]) and np.array_equal(st["quat"], ref.state()[:, 3:7])
     bscene, bparams, _ = avbd.scenes.batched(num_scenes=12)
     runs = []
     for blk in (128, 256, 512):
