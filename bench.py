@@ -250,7 +250,7 @@ def host_cpu():
     return {"model": model, "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
 
 
-def adjoint_leg():
+def adjoint_leg(graphs=0):
     """Forward and reverse timing of the ramp benchmark: 64 boxes (0.5 m, rho
     500) on a 30-degree ramp, 120 steps, 10 iterations, f64, checkpoint every
     10 steps; gradients of a loss of the final positions with respect to each
@@ -262,8 +262,8 @@ def adjoint_leg():
     scene, params, meta = avbd.scenes.ramp(64)
     target = np.array([b["pos"] for b in scene["bodies"]])
 
-    ctxs = {0: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=0),
-            -1: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=-1)}
+    ctxs = {0: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=0, graphs=graphs),
+            -1: avbd.HipBackend(precision=avbd.PREC_F64, adjoint_tape_mb=-1, graphs=graphs)}
 
     def adjoint(tape_mb):
         # one context per mode across the rounds, as an optimisation loop
@@ -282,7 +282,7 @@ def adjoint_leg():
 
     out = None
     for _ in range(2):  # the first round warms up the code paths
-        f = avbd.HipBackend(precision=avbd.PREC_F64)
+        f = avbd.HipBackend(precision=avbd.PREC_F64, graphs=graphs)
         f.pack(scene, params)
         t0 = time.perf_counter()
         f.step(meta["steps"])
@@ -486,7 +486,8 @@ def main():
     if args.workload == "pyramid" and args.contact_model == "cpu" and not args.no_contact_law:
         line["contact_law_gpu"] = contact_law_leg(args, scene, params, spec, prec)
     if not args.no_adjoint and args.workload == "pyramid":
-        line["adjoint"] = adjoint_leg()
+        # direct launches there too when the command asks for them (as under a profiler)
+        line["adjoint"] = adjoint_leg(avbd.GRAPHS_NONE if graphs_cfg(args) == avbd.GRAPHS_NONE else 0)
     if not args.no_cpu:
         cpu_rate, sample = cpu_sample(args, scene, params, meta, start)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "body-iterations/s", "cores": 1, "kind": "port",
