@@ -730,6 +730,53 @@ static napi_value js_read_inertial_target(napi_env env, napi_callback_info info)
   return NULL;
 }
 
+/* autotune(ctx) -> {team, block, trials: Float32Array(24)}: one step that
+ * times the solve's team and workgroup shapes and keeps the fastest
+ * (avbd_autotune, avbd_read_launch_shape; trials[t * 4 + w] for team
+ * 1, 4, 8, 16, 32, 64 and workgroup 64, 128, 256, 512, 0 where not tried) */
+static napi_value js_autotune(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1] = {NULL};
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int32_t team = 0, block = 0;
+  int rc = avbd_autotune(ctx, &team, NULL);
+  if (rc) return throw_avbd(env, ctx, rc);
+  napi_value ab, arr, o, v;
+  void *data = NULL;
+  CHECK(env, napi_create_arraybuffer(env, 24 * sizeof(float), &data, &ab));
+  rc = avbd_read_launch_shape(ctx, &team, &block, (float *)data);
+  if (rc) return throw_avbd(env, ctx, rc);
+  CHECK(env, napi_create_typedarray(env, napi_float32_array, 24, ab, 0, &arr));
+  CHECK(env, napi_create_object(env, &o));
+  CHECK(env, napi_create_int32(env, team, &v));
+  CHECK(env, napi_set_named_property(env, o, "team", v));
+  CHECK(env, napi_create_int32(env, block, &v));
+  CHECK(env, napi_set_named_property(env, o, "block", v));
+  CHECK(env, napi_set_named_property(env, o, "trials", arr));
+  return o;
+}
+
+/* setLaunchShape(ctx, team, block): fix lanes per body and per workgroup
+ * (avbd_set_launch_shape) */
+static napi_value js_set_launch_shape(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3] = {NULL, NULL, NULL};
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  int32_t team = 0, block = 0;
+  if (argc < 3 || napi_get_value_int32(env, argv[1], &team) != napi_ok ||
+      napi_get_value_int32(env, argv[2], &block) != napi_ok) {
+    napi_throw_type_error(env, NULL, "setLaunchShape(ctx, team, block) takes two integers");
+    return NULL;
+  }
+  avbd_ctx *ctx = get_ctx(env, argv[0]);
+  if (!ctx) return NULL;
+  int rc = avbd_set_launch_shape(ctx, team, block);
+  if (rc) return throw_avbd(env, ctx, rc);
+  return NULL;
+}
+
 static napi_value js_abi_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -761,6 +808,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readManifolds", NULL, js_read_manifolds, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setColoringLimits", NULL, js_set_coloring_limits, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readInertialTarget", NULL, js_read_inertial_target, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"autotune", NULL, js_autotune, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setLaunchShape", NULL, js_set_launch_shape, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

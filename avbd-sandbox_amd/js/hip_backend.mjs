@@ -484,6 +484,22 @@ export class HipBackend {
     addon.setBodyOrder(this.ctx, rank ? Int32Array.from(rank) : null);
   }
 
+  // --- launch shapes (no counterpart in the reference): one step that times
+  // the solve's team (lanes per body) and workgroup shapes on the step's own
+  // data and keeps the fastest; trials maps "team/block" to ms.
+  autotune() {
+    const r = addon.autotune(this.ctx);
+    const trials = {};
+    [1, 4, 8, 16, 32, 64].forEach((t, i) => [64, 128, 256, 512].forEach((b, j) => {
+      if (r.trials[i * 4 + j] > 0) trials[`${t}/${b}`] = r.trials[i * 4 + j];
+    }));
+    return { team: r.team, block: r.block, trials };
+  }
+
+  setLaunchShape(team, block) {
+    addon.setLaunchShape(this.ctx, team, block);
+  }
+
   // --- metrics and profiling (backend.js:309-324, 1836-1891)
   get lastStats() {
     return this.ctx ? addon.readCounters(this.ctx) : null;

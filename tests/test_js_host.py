@@ -19,7 +19,7 @@ NODE = shutil.which("node")
 needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
 EXPORTS = ["abiVersion", "create", "setParams", "pack", "step", "sync", "readState", "writeState", "exportPoses",
-           "readColoring", "readPairs", "readCounters", "setProfiling", "readProfile"]
+           "readColoring", "readPairs", "readCounters", "setProfiling", "readProfile", "autotune", "setLaunchShape"]
 
 
 @needs_node
@@ -209,3 +209,22 @@ def test_js_verification_readbacks_match_the_oracle(tmp_path):
     dyn = np.array([b["density"] > 0 for b in scene["bodies"]])
     assert np.array_equal(forced[dyn], np.arange(dyn.sum()))
     assert max(d["released"]["colorOf"]) < dyn.sum() - 1
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_host_autotunes_and_takes_a_launch_shape(tmp_path):
+    # HipBackend.autotune() times every team and workgroup shape on the
+    # step's own data and keeps the fastest; setLaunchShape fixes one and
+    # rejects a workgroup the sweep does not take
+    import avbd
+    scene, params, _ = avbd.scenes.pyramid(layers=6)
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"scene": scene, "params": params}))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_autotune.mjs"), str(path)],
+                       capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    t = d["tuned"]
+    assert len(t["trials"]) == 18 and t["trials"][f"{t['team']}/{t['block']}"] == min(t["trials"].values())
+    assert d["rejected"] and d["finite"] and d["steps"] == 5
