@@ -211,6 +211,13 @@ struct avbd_ctx {
   // colouring for the following steps
   bool useRank = false, useUpload = false;
   DBuf<int> claimRank, uploadColor;
+  // the colouring in one launch (k_color_dag): the dynamic bodies in claim
+  // order (built on the host for the order it was built for), each body's
+  // published result, and the host's copies it is built from
+  DBuf<int> corder, dagState;
+  int corderFor = -1, nDynOrder = 0;
+  std::vector<int> hsceneStart, hrank;
+  bool dagColoring = true;  // off after a step reported it unfinished
   int nDynamic = 0;
 
   // solve-loop graphs, keyed by colour count and manifold buffer parity
@@ -410,6 +417,7 @@ struct avbd_ctx {
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
     claimRank.release(); uploadColor.release();
+    corder.release(); dagState.release(); corderFor = -1; nDynOrder = 0; dagColoring = true;
     // the tape's buffers are kept for the next adjoint forward (an
     // optimisation loop packs and records the same scene again); only the
     // record of which steps they hold is dropped
@@ -550,13 +558,87 @@ struct avbd_ctx {
   // could have broken): one round then confirms the fixed point instead of a
   // batch re-deriving it. Pairs the broad phase offers that make no contact
   // do not change the graph.
+  // The colour buckets of the colour array (colour count in cmeta[4], starts
+  // from cmeta[8]), the sweep's work entries and the first-visitor flags.
+  void bucketize(bool dev = false) {
+    zero(cmeta.p + 4, 1, dev);
+    const int nb = blocks(N);
+    hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
+    hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
+    hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
+    hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
+                       cbodies.p, itemStart.p, itemJoints.p, cwork.p);
+    hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
+                       items.p);
+  }
+
+  // The dynamic bodies sorted by the claim order of claimsBefore (index;
+  // lowbias32 of the index within the scene, ties to the lower index; a
+  // caller's rank, ties to the lower index), for k_color_dag.
+  void ensureClaimOrder(int order) {
+    if (corderFor == order && corder.p) return;
+    std::vector<int> ids;
+    ids.reserve(size_t(N));
+    for (int i = 0; i < N; i++)
+      if (hprops[size_t(i)].mass > 0) ids.push_back(i);
+    if (order != AVBD_ORDER_INDEX) {
+      std::vector<int64_t> key(size_t(N), 0);
+      for (int i : ids) {
+        if (order == AVBD_ORDER_CUSTOM) {
+          key[size_t(i)] = hrank[size_t(i)];
+        } else {
+          uint32_t x = uint32_t(i - hsceneStart[size_t(hprops[size_t(i)].scene)]);
+          x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+          key[size_t(i)] = int64_t(x);
+        }
+      }
+      std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return key[size_t(a)] < key[size_t(b)]; });
+    }
+    corder.reserve(std::max<size_t>(ids.size(), 1));
+    if (!ids.empty())
+      HIPCHECK(hipMemcpy(corder.p, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice));
+    nDynOrder = int(ids.size());
+    corderFor = order;
+  }
+  // The colouring in one launch: state reset, the claim-order pass and the
+  // report (cmeta[1]: every body coloured; cmeta[3]: rounds); no host read.
+  void colorDag(int order) {
+    dagState.reserve(size_t(N));
+    hipLaunchKernelGGL(k_color_dag_init, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, dagState.p, cmeta.p);
+    const int waves = std::max(1, (nDynOrder + 63) / 64);
+    if (nDynOrder > 0)
+      hipLaunchKernelGGL(k_color_dag, dim3(blocks(long(waves) * 64)), dim3(256), 0, stream, nDynOrder,
+                         (const int *)corder.p, (const BodyProp *)props.p, (const int *)adjStart.p,
+                         (const int *)adjOther.p, color[0].p, dagState.p, order, (const int *)sceneStart.p,
+                         (const int *)claimRank.p, cmeta.p, scal.p + 5, 2 * N + 4096);
+    hipLaunchKernelGGL(k_color_dag_close, dim3(1), dim3(1), 0, stream, nDynOrder, cmeta.p);
+  }
+  bool dagUsable(bool uploaded, bool confirm) const {
+    return dagColoring && !uploaded && !confirm && roundCap <= 0 && colorCap <= 0;
+  }
+
   void colorize(bool graphSame) {
     HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
     const int maxRounds = N + 2;
     const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
     bool uploaded = useUpload;
     const bool confirm = graphSame && !uploaded && colorFor == order;
-    for (int launched = 0;;) {
+    // the colouring in one launch; the relaxation rounds below take over if
+    // it reports a body it could not colour
+    bool dagDone = false;
+    if (dagUsable(uploaded, confirm)) {
+      ensureClaimOrder(order);
+      colorDag(order);
+      bucketize();
+      HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      dagDone = hmeta[1] != 0;
+      if (!dagDone) {
+        dagColoring = false;
+        HIPCHECK(hipMemsetAsync(cmeta.p, 0, sizeof(int) * 8, stream));
+      }
+    }
+    for (int launched = 0; !dagDone;) {
       if (uploaded) {
         // the uploaded colouring, if it is proper for this step's forces
         hipLaunchKernelGGL(k_color_install, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, uploadColor.p,
@@ -576,15 +658,7 @@ struct avbd_ctx {
       if (colorCap > 0)
         hipLaunchKernelGGL(k_color_cap, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, colorCap, scal.p + 5,
                            (int *)nullptr);
-      HIPCHECK(hipMemsetAsync(cmeta.p + 4, 0, sizeof(int), stream));
-      const int nb = blocks(N);
-      hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
-      hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
-      hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
-      hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
-                         cbodies.p, itemStart.p, itemJoints.p, cwork.p);
-      hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
-                         items.p);
+      bucketize();
       HIPCHECK(hipMemcpyAsync(hmeta.data(), cmeta.p, sizeof(int) * hmeta.size(), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
       if (uploaded && hmeta[5]) {
@@ -603,7 +677,7 @@ struct avbd_ctx {
       if (launched > maxRounds) throw ApiError(AVBD_E_STATE, "colouring did not converge");
     }
     colorRounds = hmeta[3];
-    if (!uploaded && !confirm && roundCap <= 0) colorBatch = std::min(64, std::max(4, colorRounds + 2));
+    if (!dagDone && !uploaded && !confirm && roundCap <= 0) colorBatch = std::min(64, std::max(4, colorRounds + 2));
     const bool limited = (roundCap > 0 && !hmeta[1]) || colorCap > 0;
     colorFor = uploaded || limited ? -1 : order;
     numColors = hmeta[4];
@@ -616,23 +690,21 @@ struct avbd_ctx {
   void colorizeDev(int E, int R, bool ownRounds, bool ownCap) {
     zero(cmeta.p, 8, true);
     const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
-    for (int r = 0; r < R; r++)
-      hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjOther.p,
-                         color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
+    if (dagUsable(useUpload, false) && corderFor == order) {
+      // the colouring in one launch (its claim order built before capture)
+      colorDag(order);
+    } else {
+      for (int r = 0; r < R; r++)
+        hipLaunchKernelGGL(k_color_round, dim3(blocks(N)), dim3(256), 0, stream, N, props.p, adjStart.p, adjOther.p,
+                           color[0].p, cmeta.p, order, sceneStart.p, scal.p + 5, claimRank.p);
+    }
     hipLaunchKernelGGL(k_color_cap, dim3(blocks(N)), dim3(256), 0, stream, N, color[0].p, E, scal.p + 5,
                        pc.p + PC_SEEN);
     hipLaunchKernelGGL(k_pipe_color_check, dim3(1), dim3(1), 0, stream, (const int *)cmeta.p, pc.p, scal.p + 5, E,
                        ownRounds ? 1 : 0, ownCap ? 1 : 0);
-    zero(cmeta.p + 4, 1, true);
-    const int nb = blocks(N);
-    hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
-    hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
-    hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
-    hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
-                       cbodies.p, itemStart.p, itemJoints.p, cwork.p);
-    hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
-                       items.p);
+    bucketize(true);
   }
+
 
   // bodies of colour c at the last colouring the host read (an estimate for
   // a graph's launch shapes; 0 for a colour it did not see)
@@ -1215,7 +1287,9 @@ struct avbd_ctx {
     return std::max(1, std::min(MAX_COLORS, want));
   }
   int pipeRoundCount() const {
-    if (roundCap > 0) return roundCap;
+    // a step graph holds one launch per round: a caller's limit above the
+    // N + 2 rounds any colouring needs launches no more than those
+    if (roundCap > 0) return std::min(roundCap, N + 2);
     return std::min(N + 2, std::max({8, pipeRounds, colorRounds + 4}));
   }
 
@@ -1230,7 +1304,12 @@ struct avbd_ctx {
       needMan = std::max(needMan, o[11]);
       pipeCapacity = true;
     }
-    if (miss & 1) pipeRounds = std::min(N + 2, 2 * pipeRoundCount());
+    if (miss & 1) {
+      // relaxation rounds short of the fixed point, or the one-launch
+      // colouring unfinished: more rounds, and rounds from now on
+      pipeRounds = std::min(N + 2, 2 * pipeRoundCount());
+      dagColoring = false;
+    }
     if (miss & 2) pipeColors = std::min(MAX_COLORS, o[9] + 1);
     bool stale = false;
     if (long(o[12]) > narrowEstimate()) {
@@ -1262,7 +1341,11 @@ struct avbd_ctx {
                                       "dropped, bits 1 and 2 of the overflow counter); repack to grow it");
     }
     const int E = pipeColorCount(), rounds = pipeRoundCount();
-    const long long key = (1LL << 40) + (long long)E * (1LL << 24) + (long long)rounds * 4 + cur;
+    // the colouring in one launch needs its claim order built outside capture
+    const int order = useRank ? AVBD_ORDER_CUSTOM : prm.color_order;
+    const bool dag = dagUsable(useUpload, false);
+    if (dag) ensureClaimOrder(order);
+    const long long key = (1LL << 40) + (long long)E * (1LL << 24) + (long long)rounds * 4 + (dag ? 2 : 0) + cur;
     auto it = graphs.find(key);
     if (it == graphs.end()) {
       const int cur0 = cur, nc0 = numColors;
@@ -1879,6 +1962,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   HIPCHECK(hipMemcpy(c.sceneLargeStart.p, sls.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
   if (!slist.empty()) HIPCHECK(hipMemcpy(c.sceneLarges.p, slist.data(), sizeof(int) * slist.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c.sceneStart.p, sceneStart.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
+  c.hsceneStart = sceneStart;
   if (!largeList.empty())
     HIPCHECK(hipMemcpy(c.largeList.p, largeList.data(), sizeof(int) * largeList.size(), hipMemcpyHostToDevice));
 
@@ -2734,6 +2818,8 @@ int avbd_set_body_order(avbd_ctx *ctx, const int32_t *rank, int32_t n) {
     HIPCHECK(hipSetDevice(ctx->device));
     HIPCHECK(hipMemcpyAsync(ctx->claimRank.p, rank, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->hrank.assign(rank, rank + n);
+    ctx->corderFor = -1;  // the one-launch colouring's order follows the ranks
     ctx->useRank = true;
   });
 }

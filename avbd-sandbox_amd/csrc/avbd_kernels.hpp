@@ -1124,6 +1124,114 @@ __global__ void k_color_round(int n, const BodyProp *props, const int *adjStart,
 // An uploaded colouring, checked each step: every dynamic body has a colour
 // in range (else flag 1) and no force joins two bodies of one colour (else
 // flag 2), which the colour-parallel sweep requires.
+// ------------------------------------------------- colouring in one launch
+// The same greedy colouring computed in one launch instead of relaxation
+// rounds: a body takes the smallest colour no claims-before neighbour holds
+// once every such neighbour holds its final colour. state[i] publishes body
+// i's result, depth << 9 | colour (depth: 1 + the largest depth among its
+// claims-before neighbours, the relaxation rounds it takes to settle), or
+// DAG_OPEN until then; a static body is DAG_STATIC (it claims nothing).
+constexpr int DAG_OPEN = -1, DAG_STATIC = -2;
+
+__global__ void k_color_dag_init(int n, const BodyProp *props, int *state, int *ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) state[i] = props[i].mass > 0 ? DAG_OPEN : DAG_STATIC;
+  if (i == 0) { ctl[0] = 0; ctl[2] = 0; ctl[3] = 0; ctl[6] = 0; }
+}
+
+// Waves take the dynamic bodies in claim order (corder), 64 at a time from
+// one counter (ctl[2]). A body waits only for bodies before it in that order:
+// those of earlier chunks, which waves already running hold, and those of its
+// own chunk. No lane spins alone: each pass over the chunk colours every lane
+// whose claims-before neighbours are all final, so a chunk finishes in at
+// most as many passes as its longest chain, and the launch completes whatever
+// the number of waves resident. A wave that has made no progress for spinCap
+// passes stops and leaves its open bodies uncoloured (ctl[0] counts the
+// coloured bodies; k_color_dag_close reports the colouring unfinished).
+// state words are read and written with device-scope relaxed atomics: each
+// carries its whole payload, so no other ordering is needed.
+__global__ void __launch_bounds__(256) k_color_dag(int nDyn, const int *corder, const BodyProp *props,
+                                                   const int *adjStart, const int *adjOther, int *color, int *state,
+                                                   int order, const int *sceneStart, const int *rank, int *ctl,
+                                                   int *overflow, int spinCap) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int k0 = 0;
+    if (lane == 0) k0 = atomicAdd(&ctl[2], 64);
+    k0 = __shfl(k0, 0);
+    if (k0 >= nDyn) break;
+    const int k = k0 + lane;
+    const int i = k < nDyn ? corder[k] : -1;
+    bool open = i >= 0;
+    const int base = open ? sceneStart[props[i].scene] : 0;
+    const int e0 = open ? adjStart[i] : 0, e1 = open ? adjStart[i + 1] : 0;
+    int idle = 0, coloured = 0, maxDepth = 0;
+    while (__ballot(open) != 0ull) {
+      bool progress = false;
+      if (open) {
+        unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
+        int depth = 0;
+        bool ready = true;
+        constexpr int CH = 8;
+        for (int e = e0; e < e1 && ready; e += CH) {
+          int j[CH], st[CH];
+#pragma unroll
+          for (int q = 0; q < CH; q++) j[q] = e + q < e1 ? adjOther[e + q] : -1;
+#pragma unroll
+          for (int q = 0; q < CH; q++) {
+            if (j[q] >= 0 && !claimsBefore(j[q], i, order, base, rank)) j[q] = -1;
+            st[q] = j[q] >= 0 ? __hip_atomic_load(&state[j[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : DAG_STATIC;
+          }
+#pragma unroll
+          for (int q = 0; q < CH; q++) {
+            if (st[q] == DAG_OPEN) ready = false;
+            if (st[q] < 0) continue;
+            const int c = st[q] & 511;
+            used[c >> 6] |= 1ull << (c & 63);
+            depth = max(depth, st[q] >> 9);
+          }
+        }
+        if (ready) {
+          int c = MAX_COLORS;
+          for (int w = 0; w < MAX_COLORS / 64; w++)
+            if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
+          if (c >= MAX_COLORS) { atomicOr(overflow, 4); c = MAX_COLORS - 1; }
+          color[i] = c;
+          const int d = min(depth + 1, (1 << 21) - 1);
+          __hip_atomic_store(&state[i], (d << 9) | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          maxDepth = max(maxDepth, d);
+          coloured++;
+          open = false;
+          progress = true;
+        }
+      }
+      idle = __ballot(progress) != 0ull ? 0 : idle + 1;
+      if (idle > spinCap) {
+        // no progress for spinCap passes: every wave stops (ctl[6]), and
+        // k_color_dag_close reports the colouring unfinished
+        if (lane == 0) __hip_atomic_store(&ctl[6], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      if (idle > 0 && __hip_atomic_load(&ctl[6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+    if (coloured) {
+      atomicAdd(&ctl[0], coloured);
+      atomicMax(&ctl[3], maxDepth);
+    }
+    if (__hip_atomic_load(&ctl[6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+  }
+}
+
+// ctl[1] = 1 when every dynamic body was coloured (else the relaxation rounds
+// take over); ctl[3] holds the rounds the relaxation would take.
+__global__ void k_color_dag_close(int nDyn, int *ctl) {
+  ctl[1] = ctl[0] == nDyn && !ctl[6] ? 1 : 0;
+  ctl[0] = 0;
+  ctl[2] = 0;
+  ctl[6] = 0;
+}
+
 __global__ void k_color_validate(int n, const BodyProp *props, const int *adjStart, const int *adjOther,
                                  const int *color, int *flag, int *overflow) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -209,7 +209,9 @@ typedef struct avbd_counters {
   int32_t contacts;        /* contacts in them                              */
   int32_t joints;          /* live joints                                   */
   int32_t colors;          /* colours of the last step                      */
-  int32_t color_rounds;    /* fixed-point rounds the colouring took         */
+  int32_t color_rounds;    /* the colouring's rounds: relaxation rounds it
+                              took, or for the one-launch colouring its
+                              longest chain of claims-before neighbours     */
   int32_t overflow;        /* bit 1 pairs, 2 manifolds (layout.js:174), 4
                               colours, 8 work items, 16 an uploaded
                               colouring was not proper for a step, 32 the

@@ -143,3 +143,38 @@ def test_coloring_limits():
     # the inertial target is readable and finite
     y, qy = g.read_inertial_target()
     assert np.isfinite(y).all() and np.abs(np.linalg.norm(qy, axis=1) - 1).max() < 1e-9
+
+
+@pytest.mark.parametrize("order", [0, 1, 2], ids=["index", "priority", "rank"])
+def test_one_launch_colouring_equals_the_relaxation_rounds(order):
+    # the colouring in one launch (bodies in claim order, each coloured once
+    # its claims-before neighbours are) is the greedy colouring the relaxation
+    # rounds reach: forcing the rounds (a round limit far above the rounds
+    # needed) gives the same colours at every step, as does the oracle
+    scene, params, _ = avbd.scenes.pyramid(layers=10)
+    params = dict(params, colorOrder=min(order, 1))
+    n = len(scene["bodies"])
+    rank = np.random.default_rng(7).permutation(n).astype(np.int32)
+    dag = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    rounds = avbd.HipBackend(precision=avbd.PREC_EXACT)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=min(order, 1))
+    for g in (dag, rounds):
+        g.pack(scene, params)
+        if order == 2:
+            g.set_body_order(rank)
+    rounds.set_coloring_limits(rounds=1 << 20)
+    for s in range(1, 9):
+        dag.step(1)
+        rounds.step(1)
+        ref.step(1)
+        cd, kd = dag.read_coloring()
+        cr, kr = rounds.read_coloring()
+        assert kd == kr and np.array_equal(cd, cr), f"step {s}: the two colourings differ"
+        if order < 2:
+            co, ko = ref.last_coloring()
+            assert kd == ko and np.array_equal(cd, co), f"step {s}: differs from the oracle"
+        # the one launch reports its longest claims-before chain; the rounds,
+        # relaxing in place from the last step's colours, often need fewer
+        assert dag.counters()["color_rounds"] >= 1 and rounds.counters()["color_rounds"] >= 1
+        assert dag.counters()["overflow"] == 0
+    assert np.array_equal(dag.read_state()["pos"], rounds.read_state()["pos"])
