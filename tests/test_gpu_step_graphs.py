@@ -217,3 +217,19 @@ def test_capacity_overflow_is_reported_by_a_later_step():
     g.repack(scene, list(range(len(bodies))))
     g.step(4)
     assert g.counters()["overflow"] == 0
+
+
+def test_a_huge_round_limit_launches_at_most_n_plus_two_rounds():
+    # a step graph holds one launch per relaxation round: a caller's limit far
+    # above what any colouring needs is held to N + 2 rounds (the graph would
+    # otherwise hold a million launches); the colouring is then the greedy one
+    scene = avbd.scenes.pyramid(layers=4, speed=0.0)[0]
+    params = dict(gravity=-9.81, iterations=10, colorOrder=1)
+    g = avbd.HipBackend(precision=avbd.PREC_EXACT, graphs=avbd.GRAPHS_STEP)
+    g.pack(scene, params)
+    g.set_coloring_limits(rounds=1 << 20)
+    g.step(6)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=1)
+    ref.step(6)
+    assert np.array_equal(state_of(g), oracle_state(ref))
+    assert g.counters()["overflow"] == 0
