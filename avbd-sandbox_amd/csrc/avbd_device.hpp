@@ -723,6 +723,9 @@ struct Ldlt6 {
 };
 
 __device__ __forceinline__ void ldlt6(const double *aLin, const double *aAng, const double *aCross, Ldlt6 &f) {
+  // multiply-adds contracted: the throughput precisions and the adjoint use
+  // this factorisation, never the exact mode (solve6 keeps its roundings)
+#pragma clang fp contract(fast)
   const double A11 = aLin[0];
   const double A21 = aLin[3], A22 = aLin[4];
   const double A31 = aLin[6], A32 = aLin[7], A33 = aLin[8];
@@ -752,6 +755,7 @@ __device__ __forceinline__ void ldlt6(const double *aLin, const double *aAng, co
 
 __device__ __forceinline__ void ldlt6Solve(const Ldlt6 &f, const double *bLin, const double *bAng, double *xLin,
                                            double *xAng) {
+#pragma clang fp contract(fast)
   const double y1 = bLin[0];
   const double y2 = bLin[1] - f.L21 * y1;
   const double y3 = bLin[2] - f.L31 * y1 - f.L32 * y2;
@@ -777,6 +781,7 @@ __device__ __forceinline__ void solve6f(const double *aLin, const double *aAng, 
 
 // Eq. 21 quaternion update with the normalisation as one reciprocal.
 __device__ __forceinline__ void qintf(double *o, const double *q, const double *dw) {
+#pragma clang fp contract(fast)
   const double qx = q[0], qy = q[1], qz = q[2], qw = q[3];
   const double wx = dw[0], wy = dw[1], wz = dw[2];
   const double rx = qx + (wx * qw + wy * qz - wz * qy) * 0.5;
