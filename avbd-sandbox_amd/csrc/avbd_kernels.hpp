@@ -911,6 +911,123 @@ __device__ void jointStamp(Sys<double> &s, const JointDev &j, bool isA, const do
   }
 }
 
+// springStamp for the throughput precisions: the same sums with the outer
+// products written out, one reciprocal of the length, P = (I - n n^T) / len
+// symmetric so that Gla = -(P S) = (S P)^T = Gal^T, Gaa = -(Gal S) plus the
+// anchor term, and multiply-adds contracted.
+__device__ void springStampF(Sys<double> &s, const JointDev &j, bool isA, const double *X) {
+#pragma clang fp contract(fast)
+  if (j.broken) return;
+  double d[3], n[3], C;
+  if (!springEval(j, X, d, n, C)) return;
+  const double k = j.kLin[0], fo = k * C;
+  double w[3], wn[3];
+  if (isA) qrot(w, X + size_t(j.a) * XS + 3, j.rA);
+  else qrot(w, X + size_t(j.b) * XS + 3, j.rB);
+  cross3(wn, w, n);
+  const double sg = isA ? 1.0 : -1.0;
+  double jl[3], ja[3];
+  for (int e = 0; e < 3; e++) { jl[e] = sg * n[e]; ja[e] = sg * wn[e]; }
+  for (int q = 0; q < 6; q++) {
+    const int r = LT_R[q], c = LT_C[q];
+    s.lin[q] += jl[r] * jl[c] * k;
+    s.ang[q] += ja[r] * ja[c] * k;
+  }
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) s.cross[r * 3 + c] += ja[r] * jl[c] * k;
+  for (int e = 0; e < 3; e++) { s.rl[e] += jl[e] * fo; s.ra[e] += ja[e] * fo; }
+  // geometric stiffness of Equation 17, lumped by column norms
+  const double il = 1.0 / xsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double P[9], Sk[9], Gal[9], Gaa[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) P[r * 3 + c] = ((r == c ? 1.0 : 0.0) - n[r] * n[c]) * il;
+  Sk[0] = 0; Sk[1] = -w[2]; Sk[2] = w[1]; Sk[3] = w[2]; Sk[4] = 0; Sk[5] = -w[0]; Sk[6] = -w[1]; Sk[7] = w[0]; Sk[8] = 0;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      Gal[r * 3 + c] = Sk[r * 3 + 0] * P[c] + Sk[r * 3 + 1] * P[3 + c] + Sk[r * 3 + 2] * P[6 + c];
+  const double ndw = n[0] * w[0] + n[1] * w[1] + n[2] * w[2];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      Gaa[r * 3 + c] = -(Gal[r * 3 + 0] * Sk[c] + Gal[r * 3 + 1] * Sk[3 + c] + Gal[r * 3 + 2] * Sk[6 + c]) +
+                       sg * (w[r] * n[c] - (r == c ? ndw : 0.0));
+  const double af = xabs(fo);
+  const int diag[3] = {0, 2, 5};
+  for (int c = 0; c < 3; c++) {
+    double lc = 0, ac = 0;
+    for (int r = 0; r < 3; r++) {
+      lc += P[r * 3 + c] * P[r * 3 + c] + Gal[r * 3 + c] * Gal[r * 3 + c];
+      ac += Gal[c * 3 + r] * Gal[c * 3 + r] + Gaa[r * 3 + c] * Gaa[r * 3 + c];
+    }
+    s.lin[diag[c]] += xsqrt(lc) * af;
+    s.ang[diag[c]] += xsqrt(ac) * af;
+  }
+}
+
+// jointStamp for the throughput precisions: the same sums with the
+// identity and diagonal factors of joint.js:162-253 multiplied out and
+// multiply-adds contracted (the exact mode keeps jointStamp's order). With
+// S = [rW]x, the anchor's cross-product matrix, and sigma = -1 for body A,
+// +1 for body B: J_lin = -sigma I, J_ang = sigma S, so
+//   Lin += K, Ang += S^T K S, Cross += J_ang^T K J_lin = -S^T K,
+//   rl += -sigma F, ra += -sigma (rW x F),
+// and the geometric term H = -(r.F) I + r F^T with r = -sigma rW (lumped by
+// column norms into Ang's diagonal). The angular row: J = s I (s = +-the
+// torque arm), Ang += s^2 K, ra += s F. Springs take springStampF.
+__device__ void jointStampF(Sys<double> &s, const JointDev &j, bool isA, const double *X, double alpha) {
+#pragma clang fp contract(fast)
+  if (j.kind == AVBD_JOINT_SPRING) {
+    springStampF(s, j, isA, X);
+    return;
+  }
+  const double INF = __builtin_inf();
+  const double *kl = j.kLin;
+  if (kl[0] * kl[0] + kl[1] * kl[1] + kl[2] * kl[2] > 0) {
+    double C[3], F[3], w[3];
+    jointLin(j, X, C);
+    if (j.stiffLin == INF)
+      for (int k = 0; k < 3; k++) C[k] += j.C0Lin[k] * -alpha;
+    for (int k = 0; k < 3; k++) F[k] = kl[k] * C[k] + j.lamLin[k];
+    if (isA) qrot(w, X + size_t(j.a) * XS + 3, j.rA);
+    else qrot(w, X + size_t(j.b) * XS + 3, j.rB);
+    const double sig = isA ? -1.0 : 1.0;
+    s.lin[0] += kl[0]; s.lin[2] += kl[1]; s.lin[5] += kl[2];
+    s.ang[0] += kl[1] * w[2] * w[2] + kl[2] * w[1] * w[1];
+    s.ang[1] += -(kl[2] * w[0] * w[1]);
+    s.ang[2] += kl[0] * w[2] * w[2] + kl[2] * w[0] * w[0];
+    s.ang[3] += -(kl[1] * w[0] * w[2]);
+    s.ang[4] += -(kl[0] * w[1] * w[2]);
+    s.ang[5] += kl[0] * w[1] * w[1] + kl[1] * w[0] * w[0];
+    s.cross[1] += -(w[2] * kl[1]); s.cross[2] += w[1] * kl[2];
+    s.cross[3] += w[2] * kl[0];    s.cross[5] += -(w[0] * kl[2]);
+    s.cross[6] += -(w[1] * kl[0]); s.cross[7] += w[0] * kl[1];
+    double rxF[3];
+    cross3(rxF, w, F);
+    for (int k = 0; k < 3; k++) {
+      s.rl[k] += -sig * F[k];
+      s.ra[k] += -sig * rxF[k];
+    }
+    // geometric stiffness (joint.js:224-234): columns of H = -(r.F) I + r F^T
+    const double r0 = -sig * w[0], r1 = -sig * w[1], r2 = -sig * w[2];
+    const double rF = r0 * F[0] + r1 * F[1] + r2 * F[2];
+    const int diag[3] = {0, 2, 5};
+    for (int c = 0; c < 3; c++) {
+      const double h0 = r0 * F[c] - (c == 0 ? rF : 0.0), h1 = r1 * F[c] - (c == 1 ? rF : 0.0),
+                   h2 = r2 * F[c] - (c == 2 ? rF : 0.0);
+      s.ang[diag[c]] += xsqrt(h0 * h0 + h1 * h1 + h2 * h2);
+    }
+  }
+  const double *ka = j.kAng;
+  if (ka[0] * ka[0] + ka[1] * ka[1] + ka[2] * ka[2] > 0) {
+    double C[3];
+    jointAng(j, X, C);
+    if (j.stiffAng == INF)
+      for (int k = 0; k < 3; k++) C[k] += j.C0Ang[k] * -alpha;
+    const double sg = isA ? j.torqueArm : -j.torqueArm, sg2 = sg * sg;
+    s.ang[0] += sg2 * ka[0]; s.ang[2] += sg2 * ka[1]; s.ang[5] += sg2 * ka[2];
+    for (int k = 0; k < 3; k++) s.ra[k] += sg * (ka[k] * C[k] + j.lamAng[k]);
+  }
+}
+
 // Joint.updateDual (joint.js:255-317)
 __device__ __forceinline__ void jointDualOne(JointDev &j, const double *X, const StepParams &P, double alpha) {
   if (j.broken == 2) return;
@@ -1795,7 +1912,7 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
       if (lane != 0) s.zero();
       for (int t = b0 + lane; t < b1; t += TEAM) {
         const uint2 itw = S.items[t];
-        if (itw.x & 0x80000000u) jointStamp(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
+        if (itw.x & 0x80000000u) jointStampF(s, S.joints[int((itw.x & 0x7fffffffu) >> 1)], itw.x & 1u, S.X, alpha);
       }
       if constexpr (TEAM > 1) {
 #pragma unroll
