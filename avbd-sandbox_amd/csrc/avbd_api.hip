@@ -523,12 +523,14 @@ struct avbd_ctx {
     const int gf = dev ? strideBlocks(nf) : blocks(nf), gr = dev ? strideBlocks(2 * nf) : blocks(2 * nf);
     if (nf > 0)
       hipLaunchKernelGGL(k_adj<0>, dim3(gf), dim3(256), 0, stream, nLive, nLiveDev, J, live[cur].p, head,
-                         joints.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
+                         joints.p, (const BodyProp *)props.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p,
+                         adjOwner.p);
     scan(deg.p, adjStart.p, N, adjStart.p + N);
     HIPCHECK(hipMemcpyAsync(adjCursor.p, adjStart.p, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
     if (nf > 0) {
       hipLaunchKernelGGL(k_adj<1>, dim3(gf), dim3(256), 0, stream, nLive, nLiveDev, J, live[cur].p, head,
-                         joints.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p, adjOwner.p);
+                         joints.p, (const BodyProp *)props.p, deg.p, adjCursor.p, adjKeyFill.p, adjOtherFill.p,
+                         adjOwner.p);
       // at most two entries per force
       hipLaunchKernelGGL(k_adj_rank, dim3(gr), dim3(256), 0, stream, int(2 * nf), N, props.p, adjStart.p,
                          (const int *)adjOwner.p, (const int *)adjKeyFill.p, (const int *)adjOtherFill.p, adjKey.p,

@@ -1089,30 +1089,61 @@ __global__ void k_live_compact(int nTot, const int *nTotDev, const int *keep, co
 // Per-body force lists. Key = j for joint j, J + k for live manifold k, so
 // ascending keys are the force list's creation order. nLiveDev (or null): the
 // live count as produced on the device; the threads stride.
+// ctr[idx] += 1 for every lane with inc, returning each lane's old value
+// (its slot). Lanes with agg share one atomic per distinct idx in the wave
+// (a static body, the ground or a wall, meets hundreds of forces, and an
+// atomic per force on its one counter serialises); the others add alone.
+// Every lane of the wave calls it.
+__device__ __forceinline__ int waveSlot(int *ctr, int idx, bool inc, bool agg) {
+  int r = 0;
+  if (inc && !agg) r = atomicAdd(&ctr[idx], 1);
+  unsigned long long pending = __ballot(inc && agg);
+  const int lane = __lane_id();
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lidx = __shfl(idx, leader);
+    const bool mine = inc && agg && idx == lidx;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lidx], __popcll(same));
+    base = __shfl(base, leader);
+    if (mine) r = base + __popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return r;
+}
+
+// Force lists: PASS 0 counts each body's forces (deg), PASS 1 places them
+// (cursor from the scan of deg): one thread per force (joints first, then the
+// live manifolds), its key and other body at a slot of each of its bodies'
+// lists. The threads stride in whole-grid rounds, so a wave's lanes take
+// part in every round together.
 template <int PASS>
 __global__ void k_adj(int nLive, const int *nLiveDev, int J, const int *live, const int4 *head,
-                      const JointDev *joints, int *deg, int *cursor, int *adjKey, int *adjOther, int *owner) {
+                      const JointDev *joints, const BodyProp *props, int *deg, int *cursor, int *adjKey,
+                      int *adjOther, int *owner) {
   if (nLiveDev) nLive = *nLiveDev;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < J + nLive; t += gridDim.x * blockDim.x) {
-    int a, b;
-    const int key = t;
-    if (t < J) {
-      if (joints[t].broken == 2) continue;
-      a = joints[t].a; b = joints[t].b;
-    } else {
-      const int4 h = head[live[t - J]];
-      a = h.x; b = h.y;
-    }
-    if (PASS == 0) {
-      if (a >= 0) atomicAdd(&deg[a], 1);
-      atomicAdd(&deg[b], 1);
-    } else {
-      if (a >= 0) {
-        const int p = atomicAdd(&cursor[a], 1);
-        adjKey[p] = key; adjOther[p] = b; owner[p] = a;
+  const int total = J + nLive, stride = gridDim.x * blockDim.x;
+  for (int t0 = blockIdx.x * blockDim.x; t0 < total; t0 += stride) {
+    const int t = t0 + int(threadIdx.x);
+    int a = -1, b = -1;
+    bool valid = t < total;
+    if (valid) {
+      if (t < J) {
+        if (joints[t].broken == 2) valid = false;
+        a = joints[t].a; b = joints[t].b;
+      } else {
+        const int4 h = head[live[t - J]];
+        a = h.x; b = h.y;
       }
-      const int p = atomicAdd(&cursor[b], 1);
-      adjKey[p] = key; adjOther[p] = a; owner[p] = b;
+    }
+    const bool useA = valid && a >= 0, useB = valid;
+    const bool statA = useA && !(props[a].mass > 0), statB = useB && !(props[b].mass > 0);
+    const int pa = waveSlot(PASS == 0 ? deg : cursor, a, useA, statA);
+    const int pb = waveSlot(PASS == 0 ? deg : cursor, b, useB, statB);
+    if (PASS == 1) {
+      if (useA) { adjKey[pa] = t; adjOther[pa] = b; owner[pa] = a; }
+      if (useB) { adjKey[pb] = t; adjOther[pb] = a; owner[pb] = b; }
     }
   }
 }
