@@ -1314,17 +1314,20 @@ __global__ void __launch_bounds__(256) k_color_dag(int nDyn, const int *corder, 
     const int base = open ? sceneStart[props[i].scene] : 0;
     const int e0 = open ? adjStart[i] : 0, e1 = open ? adjStart[i + 1] : 0;
     int idle = 0, coloured = 0, maxDepth = 0;
+    // each lane's scan of its neighbours resumes at the first chunk not yet
+    // all final (the colours and depths of a chunk's final neighbours are
+    // merged again when the chunk is read again: the merges are idempotent)
+    unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
+    int depth = 0, ePos = e0;
     while (__ballot(open) != 0ull) {
       bool progress = false;
       if (open) {
-        unsigned long long used[MAX_COLORS / 64] = {0, 0, 0, 0};
-        int depth = 0;
         bool ready = true;
         constexpr int CH = 8;
-        for (int e = e0; e < e1 && ready; e += CH) {
+        for (; ePos < e1 && ready; ePos += CH) {
           int j[CH], st[CH];
 #pragma unroll
-          for (int q = 0; q < CH; q++) j[q] = e + q < e1 ? adjOther[e + q] : -1;
+          for (int q = 0; q < CH; q++) j[q] = ePos + q < e1 ? adjOther[ePos + q] : -1;
 #pragma unroll
           for (int q = 0; q < CH; q++) {
             if (j[q] >= 0 && !claimsBefore(j[q], i, order, base, rank)) j[q] = -1;
@@ -1339,6 +1342,7 @@ __global__ void __launch_bounds__(256) k_color_dag(int nDyn, const int *corder, 
             used[c >> 6] |= 1ull << (c & 63);
             depth = max(depth, st[q] >> 9);
           }
+          if (!ready) break;  // this chunk again on the next pass
         }
         if (ready) {
           int c = MAX_COLORS;
