@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 5
+#define AVBD_ABI_VERSION 6
 
 /* Error codes. */
 #define AVBD_OK 0

@@ -29,7 +29,7 @@ def test_abi_version_and_defaults():
     import avbd
     from avbd import _lib as L
     lib = avbd.lib()
-    assert lib.avbd_abi_version() == 5
+    assert lib.avbd_abi_version() == 6
     p = L.Params()
     lib.avbd_default_params(ctypes.byref(p))
     # solver.js:92-236

@@ -29,7 +29,7 @@ def test_addon_loads_and_exports():
                           "console.log(JSON.stringify({k:Object.keys(a),v:a.abiVersion()}))"],
                          capture_output=True, text=True, check=True).stdout
     d = json.loads(out)
-    assert d["v"] == 5
+    assert d["v"] == 6
     assert set(EXPORTS) <= set(d["k"])
 
 
@@ -62,7 +62,7 @@ def test_backend_module_imports():
     r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, cwd=ROOT)
     assert r.returncode == 0, r.stderr
     # a clean flush is a no-op that says so (backend.js:1911-1915)
-    assert r.stdout.split() == ["function"] * len(methods) + ["true", "true", "true", "false", "5"]
+    assert r.stdout.split() == ["function"] * len(methods) + ["true", "true", "true", "false", "6"]
 
 
 @needs_node
