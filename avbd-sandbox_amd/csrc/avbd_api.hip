@@ -198,8 +198,11 @@ struct avbd_ctx {
   DBuf<int> color[2];
   int colorCur = 0;
   DBuf<int> cstart, cbodies, cmeta;
-  DBuf<int4> cwork;
-  std::vector<int> hcstart, hmeta = std::vector<int>(8 + MAX_COLORS + 1);
+  DBuf<int4> cwork, hwork;  // each colour's bodies; its heavy bodies (k_color_scatter)
+  std::vector<int> hcstart, hheavy, hmeta = std::vector<int>(CMETA_N);
+  // the heavy-body split of the team sweeps (k_primal): on unless a caller
+  // measures without it
+  bool heavySplit = true;
   int colorBatch = 16;  // relaxation rounds launched between convergence checks
   // limits a host may set (GpuBackend.colorRounds / encColors): relaxation
   // rounds per step and colours per step; 0 = none (the exact greedy fixed point)
@@ -242,7 +245,7 @@ struct avbd_ctx {
   int pipeColors = 0, pipeRounds = 0;  // colour launches and relaxation rounds per graph (0: derived)
   // the counts a step graph's launch shapes are sized for (bodies per colour,
   // manifolds); a step that outgrows them has the graphs captured again
-  std::vector<long> pipeEst;
+  std::vector<long> pipeEst, pipeEstH;  // colour and heavy-body counts the step graphs are shaped for
   long pipeTotEst = 0;
   bool pipeCapturing = false;
   // autotuner
@@ -413,7 +416,7 @@ struct avbd_ctx {
     store[0].release(); store[1].release(); live[0].release(); live[1].release(); keep.release(); keepPos.release();
     joints.release(); deg.release(); adjStart.release(); adjCursor.release(); adjKey.release(); adjOther.release();
     adjKeyFill.release(); adjOtherFill.release(); adjOwner.release();
-    color[0].release(); color[1].release(); cstart.release(); cbodies.release(); cwork.release();
+    color[0].release(); color[1].release(); cstart.release(); cbodies.release(); cwork.release(); hwork.release();
     itemCnt.release(); itemStart.release(); items.release(); scal.release();
     itemJoints.release(); chist.release(); coffs.release(); ctotal.release();
     claimRank.release(); uploadColor.release();
@@ -567,9 +570,10 @@ struct avbd_ctx {
     const int nb = blocks(N);
     hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(256), 0, stream, N, color[0].p, chist.p, cmeta.p + 4);
     hipLaunchKernelGGL(k_color_offsets, dim3(MAX_COLORS), dim3(256), 0, stream, nb, chist.p, coffs.p, ctotal.p);
-    hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8);
+    hipLaunchKernelGGL(k_color_starts, dim3(1), dim3(MAX_COLORS), 0, stream, ctotal.p, cmeta.p + 8,
+                       cmeta.p + CMETA_HEAVY, (const int *)(itemStart.p + N), nDynamic, heavyMin());
     hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(256), 0, stream, N, color[0].p, coffs.p, cmeta.p + 8,
-                       cbodies.p, itemStart.p, itemJoints.p, cwork.p);
+                       cbodies.p, itemStart.p, itemJoints.p, cwork.p, cmeta.p + CMETA_HEAVY, hwork.p);
     hipLaunchKernelGGL(k_item_flags, dim3(blocks(long(N) * 8)), dim3(256), 0, stream, N, color[0].p, itemStart.p,
                        items.p);
   }
@@ -684,6 +688,7 @@ struct avbd_ctx {
     colorFor = uploaded || limited ? -1 : order;
     numColors = hmeta[4];
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
+    hheavy.assign(hmeta.begin() + CMETA_HEAVY, hmeta.begin() + CMETA_HEAVY + MAX_COLORS);
   }
 
   // The colouring inside a step captured whole: R relaxation rounds (those
@@ -729,6 +734,26 @@ struct avbd_ctx {
     const long teams = graphCapturing ? cnt + cnt / 4 + 64 : cnt;
     return std::max(1, std::min(blocks(teams * tm, blk), 8192 * (256 / blk)));
   }
+  // heavy bodies of colour c at the last colouring the host read
+  long heavyCount(int c) const {
+    if (pipeCapturing) return c >= 0 && c < int(pipeEstH.size()) ? pipeEstH[c] : 0;
+    return c >= 0 && c < int(hheavy.size()) ? hheavy[c] : 0;
+  }
+  // blocks of wide teams (tw lanes) for colour c's heavy bodies: at least
+  // one, which finds the count on the device and strides over it
+  int heavyBlocks(int c, int tw, int blk) const {
+    const long cnt = heavyCount(c);
+    const long teams = graphCapturing ? cnt + cnt / 4 + 8 : cnt;
+    return std::max(1, std::min(blocks(teams * tw, blk), 1024 * (256 / blk)));
+  }
+  // the least item count that makes a body heavy (the device raises it to
+  // twice the mean): twice the team, so that a body swept by the usual team
+  // holds at most two items per lane; 0 (no split) for the exact precision
+  // and teams too narrow or too wide to split
+  int heavyMin() const {
+    if (!heavySplit || prec == AVBD_PREC_EXACT || team < 4 || team * HEAVY_WIDEN > 64) return 0;
+    return 2 * team;
+  }
   int dualBlocks() const { return std::max(1, std::min(blocks(manCap, 128), 2048)); }
 
   template <class R, bool EXACT, int TEAM>
@@ -738,9 +763,13 @@ struct avbd_ctx {
     const double bytes = profiling ? double(hcbytes[c]) + (dual ? double(hcbytes[MAX_COLORS + c]) : 0.0) : 0.0;
     // the tuned workgroup; the one-lane exact sweep keeps 256
     const int blk = EXACT ? 256 : sweepBlock;
+    // the team modes' heavy bodies go to blocks of their own, first
+    constexpr bool split = heavySplitTeam<R, EXACT, TEAM>();
+    const int light = primalBlocks(c, TEAM, blk), wide = split ? heavyBlocks(c, HEAVY_WIDEN * TEAM, blk) : 0;
     timedLaunch(AVBD_K_PRIMAL, bytes, gl() ? k_primal<R, EXACT, TEAM, true> : k_primal<R, EXACT, TEAM, false>,
-                dim3(primalBlocks(c, TEAM, blk)), dim3(blk), c,
-                (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sa, mb, (R *)DQ.p, P, alpha, dual, dualAlpha);
+                dim3(light + wide), dim3(blk), c, (const int *)(cmeta.p + 8), (const int4 *)cwork.p,
+                (const int *)(cmeta.p + CMETA_HEAVY), (const int4 *)hwork.p, wide, sa, mb, (R *)DQ.p, P, alpha,
+                dual, dualAlpha);
   }
 
   // Team shape for colour c: a colour too small to give every SIMD two waves
@@ -780,10 +809,11 @@ struct avbd_ctx {
       if (J > 0) {
         SolveArgs sj = sa;
         sj.jointBodiesOnly = 1;
+        const int grid = primalBlocks(c, 1);
         timedLaunch(AVBD_K_PRIMAL, 0.0, gl() ? k_primal<double, true, 1, true> : k_primal<double, true, 1, false>,
-                    dim3(primalBlocks(c, 1)), dim3(256), c,
-                    (const int *)(cmeta.p + 8), (const int4 *)cwork.p, sj, mb, (double *)DQ.p, P, alpha, dual,
-                    dualAlpha);
+                    dim3(grid), dim3(256), c, (const int *)(cmeta.p + 8), (const int4 *)cwork.p,
+                    (const int *)(cmeta.p + CMETA_HEAVY), (const int4 *)hwork.p, 0, sj, mb, (double *)DQ.p, P,
+                    alpha, dual, dualAlpha);
       }
       return;
     }
@@ -1263,6 +1293,8 @@ struct avbd_ctx {
   void pipeShapesFromHost() {
     pipeEst.assign(MAX_COLORS, 0);
     for (int c = 0; c < numColors; c++) pipeEst[c] = hcstart[c + 1] - hcstart[c];
+    pipeEstH.assign(MAX_COLORS, 0);
+    for (int c = 0; c < std::min(numColors, int(hheavy.size())); c++) pipeEstH[c] = hheavy[c];
     pipeTotEst = lastNTot;
   }
   void dropStepGraphs() {
@@ -1319,10 +1351,16 @@ struct avbd_ctx {
       stale = true;
     }
     if (pipeEst.size() < size_t(MAX_COLORS)) pipeEst.resize(MAX_COLORS, 0);
+    if (pipeEstH.size() < size_t(MAX_COLORS)) pipeEstH.resize(MAX_COLORS, 0);
     for (int c = 0; c < std::min(o[5], PIPE_COLORS); c++) {
       const long cnt = o[17 + c] - o[16 + c], est = pipeEst[c];
       if (cnt > est + est / 4 + 64) {
         pipeEst[c] = cnt;
+        stale = true;
+      }
+      const long hv = o[64 + c], hest = pipeEstH[c];
+      if (hv > hest + hest / 4 + 8) {
+        pipeEstH[c] = hv;
         stale = true;
       }
     }
@@ -1406,6 +1444,7 @@ struct avbd_ctx {
     numColors = hmeta[4];
     colorRounds = hmeta[3];
     hcstart.assign(hmeta.begin() + 8, hmeta.begin() + 8 + numColors + 1);
+    hheavy.assign(hmeta.begin() + CMETA_HEAVY, hmeta.begin() + CMETA_HEAVY + MAX_COLORS);
     colorFor = -1;  // relax from these colours on a step with host reads
     pipeUnsettled = false;
   }
@@ -1846,6 +1885,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.pipeReady = c.pipeUnsettled = c.pipeCapacity = false;
   c.pipeColors = c.pipeRounds = 0;
   c.pipeEst.clear();
+  c.pipeEstH.clear();
   c.pipeTotEst = 0;
   if (s.num_joints < 0) throw ApiError(AVBD_E_INVALID, "num_joints must not be negative");
   if (s.num_joints > 0 && (!s.joint_a || !s.joint_b || !s.joint_ra || !s.joint_rb || !s.joint_stiff_lin ||
@@ -2057,6 +2097,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   c.cstart.reserve(MAX_COLORS + 1);
   c.cbodies.reserve(N);
   c.cwork.reserve(N);
+  c.hwork.reserve(N);
   c.itemCnt.reserve(N); c.itemStart.reserve(N + 1); c.itemJoints.reserve(N);
   c.chist.reserve(size_t(blocks(N)) * MAX_COLORS); c.coffs.reserve(size_t(blocks(N)) * MAX_COLORS);
   c.ctotal.reserve(MAX_COLORS);
@@ -2069,7 +2110,7 @@ void packScene(avbd_ctx &c, const avbd_scene &s) {
   // step replayed whole are scanned at capacity)
   c.scanTmp[0].reserve(size_t(std::max({c.manCap, int(c.tableMask) + 1, N + 1}) + 1023) / 1024 + 1);
   c.cbytes.reserve(2 * MAX_COLORS);
-  c.cmeta.reserve(8 + MAX_COLORS + 1);
+  c.cmeta.reserve(CMETA_N);
   HIPCHECK(hipMemset(c.scal.p, 0, sizeof(int) * 16));
   HIPCHECK(hipMemset(c.pc.p, 0, sizeof(int) * PC_N));
   c.evs.resize(AVBD_NUM_PHASES);

@@ -1188,6 +1188,10 @@ __global__ void k_adj_rank(int maxEntries, int n, const BodyProp *props, const i
 // result is the unique fixed point of colour(i) = mex(colours of earlier
 // neighbours); relaxation reaches it from any start, including last step's.
 constexpr int MAX_COLORS = 256;
+// the colouring's counters (cmeta): [0..2] change counters, [3] rounds, [4]
+// colours, [8..8+MAX_COLORS] colour starts, then each colour's heavy-body
+// count and the heavy threshold (k_color_starts)
+constexpr int CMETA_HEAVY = 8 + MAX_COLORS + 1, CMETA_N = CMETA_HEAVY + MAX_COLORS + 1;
 
 __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
   x ^= x >> 16;
@@ -1458,14 +1462,16 @@ __global__ void k_pipe_color_check(const int *cmeta, int *pc, int *overflow, int
 
 // The step's counters into the host ring (pinned host memory, PIPE_STRIDE
 // ints per step, slot = step serial mod slots): [0..13] counts, misses and
-// the serial, [16..16+PIPE_COLORS] the first colour starts (the host sizes
-// the next graphs' launches from them); this step's misses are reset.
-constexpr int PIPE_STRIDE = 64, PIPE_COLORS = 32;
+// the serial, [16..16+PIPE_COLORS] the first colour starts and
+// [64..64+PIPE_COLORS) their heavy-body counts (the host sizes the next
+// graphs' launches from them); this step's misses are reset.
+constexpr int PIPE_STRIDE = 96, PIPE_COLORS = 32;
 __global__ void k_pipe_stats(const int *scal, const int *cmeta, int *pc, int *ring, int slots) {
   const int serial = pc[PC_SERIAL];
   int *o = ring + (serial % slots) * PIPE_STRIDE;
   const int t = threadIdx.x;
   if (t <= PIPE_COLORS) o[16 + t] = cmeta[8 + t];
+  if (t < PIPE_COLORS) o[64 + t] = cmeta[CMETA_HEAVY + t];
   if (t == 0) {
     o[0] = scal[0]; o[1] = scal[1]; o[2] = scal[2]; o[3] = scal[8]; o[4] = scal[5];
     o[5] = cmeta[4]; o[6] = cmeta[3]; o[7] = cmeta[1];
@@ -1548,10 +1554,20 @@ __global__ void k_color_offsets(int nb, const int *hist, int *offs, int *ctotal)
 }
 
 // Colour starts: exclusive scan of the MAX_COLORS totals (one block).
-__global__ void k_color_starts(const int *ctotal, int *cstart) {
+// It also clears the heavy-body counts of each colour and sets the heavy
+// threshold heavy[MAX_COLORS]: a body with more work items than
+// max(hmin, twice the mean per dynamic body) is swept by a team twice as
+// wide (k_primal); hmin = 0 turns the split off.
+__global__ void k_color_starts(const int *ctotal, int *cstart, int *heavy, const int *itemTotal, int nDyn,
+                               int hmin) {
   __shared__ int sh[MAX_COLORS];
   const int c = threadIdx.x;
   const int v = ctotal[c];
+  heavy[c] = 0;
+  if (c == 0) {
+    const long mean2 = (2L * itemTotal[0] + nDyn - 1) / max(nDyn, 1);
+    heavy[MAX_COLORS] = hmin > 0 ? int(max(long(hmin), mean2)) : 0;
+  }
   sh[c] = v;
   __syncthreads();
   for (int off = 1; off < MAX_COLORS; off <<= 1) {
@@ -1569,9 +1585,13 @@ __global__ void k_color_starts(const int *ctotal, int *cstart) {
 // which is all a primal team needs to find its work.
 // Within a colour the bodies stay in index order (a body's place in its block
 // is the number of lower-indexed bodies of its colour there), so the bodies of
-// one scene of a batch form one run of each colour's list. 256 threads.
+// one scene of a batch form one run of each colour's list. A body above the
+// heavy threshold is also listed in hwork, from its colour's start on, in no
+// particular order (the bodies of one colour share no force, so the order in
+// which they are swept does not change a bit). 256 threads.
 __global__ void k_color_scatter(int n, const int *color, const int *offs, const int *cstart, int *cbodies,
-                                const int *itemStart, const int *itemJoints, int4 *cwork) {
+                                const int *itemStart, const int *itemJoints, int4 *cwork, int *heavy,
+                                int4 *hwork) {
   __shared__ int base[MAX_COLORS];
   __shared__ int col[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1584,8 +1604,11 @@ __global__ void k_color_scatter(int n, const int *color, const int *offs, const 
   int rank = 0;
   for (int j = 0; j < int(threadIdx.x); j++) rank += col[j] == c;
   const int pos = base[c] + rank;
+  const int4 w = make_int4(i, itemStart[i], itemStart[i + 1], itemJoints[i]);
   cbodies[pos] = i;
-  cwork[pos] = make_int4(i, itemStart[i], itemStart[i + 1], itemJoints[i]);
+  cwork[pos] = w;
+  const int H = heavy[MAX_COLORS];
+  if (H > 0 && w.z - w.y > H) hwork[cstart[c] + atomicAdd(&heavy[c], 1)] = w;
 }
 
 // Mark each contact item whose body comes first in the colour order (the other
@@ -1987,16 +2010,46 @@ __device__ __forceinline__ void primalBody(const int4 w, const int lane, const S
   d[3] = R(dq[0]); d[4] = R(dq[1]); d[5] = R(dq[2]);
 }
 
+// One colour of the sweep, a team of TEAM lanes per body. In the team modes
+// the colour's heavy bodies, those listed in hwork (their count in
+// heavy[c], the threshold in heavy[MAX_COLORS]), are swept by the first
+// wideBlocks blocks with teams of HEAVY_WIDEN x TEAM lanes, so that a body
+// with several times the usual contacts does not hold up the launch (their
+// blocks go first, as the longest work should start first); the other blocks
+// sweep the other bodies. Without the split wideBlocks is 0 and the blocks
+// sweep every body of the colour.
+constexpr int HEAVY_WIDEN = 2;
+template <class R, bool EXACT, int TEAM>
+constexpr bool heavySplitTeam() { return !EXACT && TEAM >= 4 && TEAM * HEAVY_WIDEN <= 64; }
+
 template <class R, bool EXACT, int TEAM, bool GL>
-__global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, const int4 *cwork, SolveArgs S,
-                                                ManBuf<R> mb, R *DQ, StepParams P, double alpha, int dual,
-                                                double dualAlpha) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = gt % TEAM, teams = gridDim.x * blockDim.x / TEAM;
-  const int c0 = cstart[c], cnt = cstart[c + 1] - c0;
+__global__ void __launch_bounds__(256, 2) k_primal(int c, const int *cstart, const int4 *cwork, const int *heavy,
+                                                const int4 *hwork, int wideBlocks, SolveArgs S, ManBuf<R> mb,
+                                                R *DQ, StepParams P, double alpha, int dual, double dualAlpha) {
+  constexpr bool SPLIT = heavySplitTeam<R, EXACT, TEAM>();
+  const int c0 = cstart[c];
+  if constexpr (SPLIT) {
+    if (int(blockIdx.x) < wideBlocks) {
+      constexpr int TW = HEAVY_WIDEN * TEAM;
+      const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+      const int lane = gt % TW, teams = wideBlocks * blockDim.x / TW;
+      const int cnt = heavy[c];
+      for (int team = gt / TW; team < cnt; team += teams) {
+        const int4 w = hwork[c0 + team];
+        if (S.jointBodiesOnly && w.w == 0) continue;
+        primalBody<R, EXACT, TW, GL>(w, lane, S, mb, DQ, P, alpha, dual, dualAlpha);
+      }
+      return;
+    }
+  }
+  const int H = SPLIT ? heavy[MAX_COLORS] : 0;
+  const int gt = (int(blockIdx.x) - wideBlocks) * blockDim.x + threadIdx.x;
+  const int lane = gt % TEAM, teams = (int(gridDim.x) - wideBlocks) * blockDim.x / TEAM;
+  const int cnt = cstart[c + 1] - c0;
   for (int team = gt / TEAM; team < cnt; team += teams) {
     const int4 w = cwork[c0 + team];
     if (S.jointBodiesOnly && w.w == 0) continue;
+    if (SPLIT && H > 0 && w.z - w.y > H) continue;  // swept by the wide teams
     primalBody<R, EXACT, TEAM, GL>(w, lane, S, mb, DQ, P, alpha, dual, dualAlpha);
   }
 }
