@@ -1100,7 +1100,13 @@ struct avbd_ctx {
     profiling = prof;
     evPool.push_back(a);
     evPool.push_back(b);
-    if (best != team || bestBlk != sweepBlock) dropGraphs();
+    if (best != team || bestBlk != sweepBlock) {
+      dropGraphs();
+      // the heavy-body threshold follows the team: the next step reads the
+      // colours' heavy-body counts on the host, which the step graphs
+      // captured after it are shaped for
+      pipeReady = false;
+    }
     team = best;
     sweepBlock = bestBlk;
   }
@@ -2842,6 +2848,7 @@ int avbd_set_launch_shape(avbd_ctx *ctx, int32_t team, int32_t block) {
         throw ApiError(AVBD_E_INVALID, "team must be 1, 4, 8, 16, 32 or 64 and block 64, 128 or 256");
       ctx->team = team;
       ctx->sweepBlock = block;
+      ctx->pipeReady = false;  // the heavy-body counts under the new team: read on the next step
     }
     ctx->dropAdjoint();
     ctx->dropGraphs();  // captured with the old shapes

@@ -218,10 +218,10 @@ def contact_law_leg(args, scene, params, spec, prec):
     if spec is not None:
         settle_and_strike(g, spec)
     tune = None
-    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 2:
-        g.step(args.warmup - 2)
+    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 3:
+        g.step(args.warmup - 3)
         tune = g.autotune()
-        g.step(1)
+        g.step(2)
     else:
         g.step(args.warmup)
     g.idle()
@@ -311,6 +311,9 @@ def main():
                          "(15k-particle spring cloth)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f64", "exact"])
     ap.add_argument("--team", type=int, default=0, help="lanes per body; 0 = autotune")
+    ap.add_argument("--workgroup", type=int, default=0,
+                    help="with --team: the sweep's workgroup (64, 128 or 256; batched scenes 128, 256 or 512); "
+                         "0 = the library's default")
     ap.add_argument("--order", default="priority", choices=["priority", "index"])
     ap.add_argument("--layers", type=int, default=53)
     ap.add_argument("--bodies", type=int, default=50000, help="rain: body count")
@@ -368,13 +371,16 @@ def main():
               f"{time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     tune = None
-    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 2:
-        # the last warm-up step but one also picks the team shape; the last
-        # captures the replayed graphs for it
-        gpu.step(args.warmup - 2)
+    if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 3:
+        # the last warm-up steps but two also pick the team shape; the next
+        # reads the counts the replayed graphs are shaped for under it, and
+        # the last captures them
+        gpu.step(args.warmup - 3)
         tune = gpu.autotune()
-        gpu.step(1)
+        gpu.step(2)
     else:
+        if args.team and args.workgroup:
+            gpu.set_launch_shape(args.team, args.workgroup)
         gpu.step(args.warmup)
     gpu.idle()
     print(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, team {tune}", file=sys.stderr,
