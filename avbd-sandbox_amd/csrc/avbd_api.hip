@@ -1063,6 +1063,15 @@ struct avbd_ctx {
   // each trial. Keeps the fastest shape.
   // Workgroups of 64, 128 and 256 lanes are tried with each team shape (a
   // colour's teams are spread over the CUs in workgroup-sized pieces).
+  // each colour's heavy-body count from the last bucketize, for the launch
+  // shapes of direct launches
+  void readHeavy() {
+    hheavy.resize(MAX_COLORS);
+    HIPCHECK(hipMemcpyAsync(hheavy.data(), cmeta.p + CMETA_HEAVY, sizeof(int) * MAX_COLORS, hipMemcpyDeviceToHost,
+                            stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
   template <class R>
   void autotuneT(const SolveArgs &sa, ManBuf<R> mb, const StepParams &P) {
     static const int cands[6] = {1, 4, 8, 16, 32, 64};
@@ -1079,8 +1088,13 @@ struct avbd_ctx {
     float bestMs = 1e30f;
     const int keepBlk = sweepBlock;
     std::memset(tuneGrid, 0, sizeof(tuneGrid));
+    const int keepTeam = team;
     for (int k = 0; k < 6; k++) {
       tuneMs[k] = 0.0f;
+      // the heavy-body lists this team would sweep with
+      team = cands[k];
+      bucketize();
+      readHeavy();
       for (int g = 0; g < 3; g++) {
         sweepBlock = blks[g];
         HIPCHECK(hipEventRecord(a, stream));
@@ -1097,9 +1111,15 @@ struct avbd_ctx {
       }
     }
     sweepBlock = keepBlk;
+    team = keepTeam;
     profiling = prof;
     evPool.push_back(a);
     evPool.push_back(b);
+    // the step's solve then runs with the chosen team's lists
+    std::swap(team, best);
+    bucketize();
+    readHeavy();
+    std::swap(team, best);
     if (best != team || bestBlk != sweepBlock) {
       dropGraphs();
       // the heavy-body threshold follows the team: the next step reads the
@@ -1297,11 +1317,15 @@ struct avbd_ctx {
   }
   // the shapes of the next step graphs from the counts the host last read
   void pipeShapesFromHost() {
-    pipeEst.assign(MAX_COLORS, 0);
-    for (int c = 0; c < numColors; c++) pipeEst[c] = hcstart[c + 1] - hcstart[c];
-    pipeEstH.assign(MAX_COLORS, 0);
-    for (int c = 0; c < std::min(numColors, int(hheavy.size())); c++) pipeEstH[c] = hheavy[c];
-    pipeTotEst = lastNTot;
+    // never below what earlier steps' graphs were shaped for: colour sizes
+    // move from step to step, and every shape that falls short has the step
+    // graphs captured again
+    pipeEst.resize(MAX_COLORS, 0);
+    for (int c = 0; c < numColors; c++) pipeEst[c] = std::max<long>(pipeEst[c], hcstart[c + 1] - hcstart[c]);
+    pipeEstH.resize(MAX_COLORS, 0);
+    for (int c = 0; c < std::min(numColors, int(hheavy.size())); c++)
+      pipeEstH[c] = std::max<long>(pipeEstH[c], hheavy[c]);
+    pipeTotEst = std::max<long>(pipeTotEst, lastNTot);
   }
   void dropStepGraphs() {
     quiesceForGraphs();

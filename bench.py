@@ -219,9 +219,10 @@ def contact_law_leg(args, scene, params, spec, prec):
         settle_and_strike(g, spec)
     tune = None
     if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 3:
-        g.step(args.warmup - 3)
+        after = max(2, min(12, args.warmup // 2))
+        g.step(args.warmup - after - 1)
         tune = g.autotune()
-        g.step(2)
+        g.step(after)
     else:
         g.step(args.warmup)
     g.idle()
@@ -372,12 +373,14 @@ def main():
     t0 = time.perf_counter()
     tune = None
     if args.team == 0 and prec != avbd.PREC_EXACT and args.warmup >= 3:
-        # the last warm-up steps but two also pick the team shape; the next
-        # reads the counts the replayed graphs are shaped for under it, and
-        # the last captures them
-        gpu.step(args.warmup - 3)
+        # a warm-up step picks the team shape; the next reads the counts the
+        # replayed graphs are shaped for under it, and the rest capture the
+        # step graphs (one per colour count and buffer parity the steps meet)
+        # before the timed steps
+        after = max(2, min(12, args.warmup // 2))
+        gpu.step(args.warmup - after - 1)
         tune = gpu.autotune()
-        gpu.step(2)
+        gpu.step(after)
     else:
         if args.team and args.workgroup:
             gpu.set_launch_shape(args.team, args.workgroup)
