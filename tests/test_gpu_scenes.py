@@ -125,6 +125,31 @@ def test_heavy_body_sweep_tracks_oracle(team, precision):
         assert after < 0.02
 
 
+def test_team_change_under_step_graphs_keeps_the_heavy_body_sweep():
+    # f64 teams, whole steps replayed as graphs: 16 lanes per body (the plank
+    # swept by 32), then 4 (the plank by 8) for the rest. The change of team
+    # moves the heavy threshold; the steps after it read the new heavy-body
+    # counts before their graphs are captured. The poses stay on the oracle's.
+    scene, params = plank_on_cubes()
+    gpu = avbd.HipBackend(precision=avbd.PREC_F64, team=16, graphs=avbd.GRAPHS_STEP)
+    gpu.pack(scene, params)
+    ref = pyoracle.Oracle(scene, params, schedule=pyoracle.SCHED_COLOR, color_order=0)
+    worst = 0.0
+    for chunk, team in ((6, None), (1, 4), (13, None)):
+        if team:
+            gpu.set_launch_shape(team, 64)
+        gpu.step(chunk)
+        ref.step(chunk)
+        st = gpu.read_state()
+        o = ref.state()
+        worst = max(worst, np.abs(st["pos"] - o[:, :3]).max(), np.abs(st["quat"] - o[:, 3:7]).max())
+        gk, gf = gpu.read_forces()
+        rk, rf = oracle_forces(ref)
+        assert np.array_equal(gk, rk) and np.array_equal(gf, rf)
+    assert gpu.launch_shape()[0] == 4
+    assert worst < 1e-12, worst
+
+
 @pytest.mark.parametrize("precision", [avbd.PREC_MIXED, avbd.PREC_F64], ids=["mixed", "f64"])
 @pytest.mark.parametrize("name", SCENES)
 def test_variant_tracks_oracle(name, precision):
